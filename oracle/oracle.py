@@ -1,0 +1,352 @@
+"""ctypes access to the oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is a plain-C restatement of minipoa-rs's POA path
+(src/align.rs, src/graph.rs, src/lib.rs). Only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg import this module, and only as the checker.
+
+`params` arguments are duck-typed: any object with the attributes of
+src/lib.rs:47-71 `Params` (match_score, mismatch, gap_open, gap_ext, b, f,
+adaptive, seeding, k, w, progressive, threads, win_band, win_max,
+mm_filter_ratio, max_occ, tr_dist, result, verbose, alphabet, matrix,
+is_rna) plus an optional `scalar_dp` flag.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from collections import namedtuple
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_poa.so"
+
+AlignedBase = namedtuple("AlignedBase", "node base aligned")
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("match_score", C.c_int32), ("mismatch", C.c_int32),
+        ("gap_open", C.c_int32), ("gap_ext", C.c_int32),
+        ("b", C.c_uint64), ("f", C.c_uint64),
+        ("adaptive", C.c_int32), ("seeding", C.c_int32),
+        ("k", C.c_uint64), ("w", C.c_uint64),
+        ("progressive", C.c_int32),
+        ("threads", C.c_uint64), ("win_band", C.c_uint64), ("win_max", C.c_uint64),
+        ("mm_filter_ratio", C.c_double),
+        ("max_occ", C.c_uint64), ("tr_dist", C.c_uint64),
+        ("result", C.c_uint8), ("verbose", C.c_uint8),
+        ("is_rna", C.c_int32), ("scalar_dp", C.c_int32),
+        ("alphabet_len", C.c_uint32),
+        ("alphabet", C.POINTER(C.c_uint8)),
+        ("matrix", C.POINTER(C.c_int32)),
+    ]
+
+
+class _Base(C.Structure):
+    _fields_ = [("node", C.c_int64), ("base", C.c_int64), ("aligned", C.c_int64)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER(_Params)
+        G = C.c_void_p
+        B = C.POINTER(_Base)
+        sz = C.c_size_t
+        L.o_last_error.restype = C.c_char_p
+        L.og_new.argtypes = [P]; L.og_new.restype = G
+        L.og_free.argtypes = [G]
+        L.og_add_node.argtypes = [G, sz]; L.og_add_node.restype = sz
+        L.og_add_edge.argtypes = [G, sz, sz]
+        L.og_add_path.argtypes = [G, sz, B, sz]; L.og_add_path.restype = C.c_int
+        L.og_toposort.argtypes = [G, C.c_int]; L.og_toposort.restype = C.c_int
+        L.og_n_nodes.argtypes = [G]; L.og_n_nodes.restype = sz
+        L.og_node_info.argtypes = [G, sz, C.POINTER(C.c_uint64)]
+        L.og_node_aligned.argtypes = [G, sz, sz]; L.og_node_aligned.restype = C.c_int64
+        L.og_node_edges.argtypes = [G, sz] + [C.POINTER(C.c_uint64)] * 4
+        L.og_rank_len.argtypes = [G]; L.og_rank_len.restype = sz
+        L.og_rank.argtypes = [G, C.POINTER(C.c_uint64)]
+        L.og_n_paths.argtypes = [G]; L.og_n_paths.restype = sz
+        L.og_consensus.argtypes = [G, P, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64), sz]
+        L.og_consensus.restype = sz
+        L.og_msa.argtypes = [G, P, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), sz,
+                             C.POINTER(C.c_void_p)]
+        L.og_msa.restype = C.c_int
+        L.og_gfa.argtypes = [G, P, C.POINTER(C.c_char_p), sz, C.POINTER(C.c_void_p)]
+        L.og_gfa.restype = C.c_int
+        L.o_free.argtypes = [C.c_void_p]
+        L.o_encode.argtypes = [P, C.c_uint8]; L.o_encode.restype = sz
+        L.o_validate.argtypes = [P]; L.o_validate.restype = C.c_int
+        L.o_insertion_prefix.argtypes = [C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                         C.c_int32, C.POINTER(C.c_int32)]
+        L.o_i32_scores_safe.argtypes = [P, G, sz, sz, sz]; L.o_i32_scores_safe.restype = C.c_int
+        for name in ("o_poa", "o_poa_inner", "o_poa_simd_i32", "o_poa_scalar_i64"):
+            fn = getattr(L, name)
+            fn.argtypes = [P, G, sz, sz, C.c_char_p, sz, C.c_int, sz, B, sz, C.POINTER(sz)]
+            fn.restype = C.c_int
+        L.o_align.argtypes = [P, G, C.c_char_p, sz, B, sz, C.POINTER(sz)]
+        L.o_align.restype = C.c_int
+        L.o_cells.restype = C.c_uint64
+        L.o_last_score.restype = C.c_int64
+        L.o_arena_capacity.restype = sz
+        L.o_align_sequences.argtypes = [P, C.POINTER(C.c_char_p), C.POINTER(sz), sz]
+        L.o_align_sequences.restype = G
+        L.o_extend_graph.argtypes = [P, G, C.POINTER(C.c_char_p), C.POINTER(sz), sz, sz]
+        L.o_extend_graph.restype = C.c_int
+        L.o_batch_run.argtypes = [P, sz, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                  C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]
+        L.o_batch_run.restype = C.c_uint64
+        _lib = L
+    return _lib
+
+
+def _err():
+    return OracleError(lib().o_last_error().decode())
+
+
+class _P:
+    """Keeps the ctypes buffers of one params conversion alive."""
+
+    def __init__(self, p):
+        alphabet = bytes(p.alphabet)
+        matrix = list(p.matrix)
+        self.alpha = (C.c_uint8 * max(1, len(alphabet)))(*alphabet)
+        self.mat = (C.c_int32 * max(1, len(matrix)))(*matrix)
+        valid_dims = len(matrix) == len(alphabet) * len(alphabet)
+        self.s = _Params(
+            int(p.match_score), int(p.mismatch), int(p.gap_open), int(p.gap_ext),
+            int(p.b), int(p.f), int(bool(p.adaptive)), int(bool(p.seeding)),
+            int(p.k), int(p.w), int(bool(p.progressive)),
+            int(p.threads), int(p.win_band), int(p.win_max), float(p.mm_filter_ratio),
+            int(p.max_occ), int(p.tr_dist), int(p.result), int(p.verbose),
+            int(bool(getattr(p, "is_rna", False))), int(bool(getattr(p, "scalar_dp", False))),
+            len(alphabet) if valid_dims else 0,
+            C.cast(self.alpha, C.POINTER(C.c_uint8)),
+            C.cast(self.mat, C.POINTER(C.c_int32)) if valid_dims and matrix else None,
+        )
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def encode(params, base):
+    return lib().o_encode(_P(params).ref, base if isinstance(base, int) else ord(base))
+
+
+def validate(params):
+    if lib().o_validate(_P(params).ref):
+        raise _err()
+
+
+class Graph:
+    """src/graph.rs:20-331 through the oracle."""
+
+    def __init__(self, params=None, _handle=None):
+        self._h = _handle if _handle is not None else lib().og_new(_P(params).ref)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().og_free(self._h)
+            self._h = None
+
+    def add_node(self, base):
+        return lib().og_add_node(self._h, base)
+
+    def add_edge(self, a, b):
+        lib().og_add_edge(self._h, a, b)
+
+    def add_path(self, seq_id, path):
+        arr = (_Base * max(1, len(path)))()
+        for k, it in enumerate(path):
+            arr[k] = _Base(-1 if it[0] is None else it[0], it[1], -1 if it[2] is None else it[2])
+        if lib().og_add_path(self._h, seq_id, arr, len(path)):
+            raise _err()
+
+    def toposort(self, grouped=False):
+        if lib().og_toposort(self._h, int(grouped)):
+            raise _err()
+
+    @property
+    def n_nodes(self):
+        return lib().og_n_nodes(self._h)
+
+    def node(self, i):
+        out = (C.c_uint64 * 6)()
+        lib().og_node_info(self._h, i, out)
+        rank, parent, base, coverage, nin, nout = list(out)
+        a = (C.c_uint64 * max(1, nin))(); aw = (C.c_uint64 * max(1, nin))()
+        b = (C.c_uint64 * max(1, nout))(); bw = (C.c_uint64 * max(1, nout))()
+        lib().og_node_edges(self._h, i, a, aw, b, bw)
+        return dict(id=i, rank=rank, parent=parent, base=base, coverage=coverage,
+                    incoming=list(a)[:nin], in_weights=list(aw)[:nin],
+                    outgoing=list(b)[:nout], out_weights=list(bw)[:nout])
+
+    def aligned(self, i, base):
+        v = lib().og_node_aligned(self._h, i, base)
+        return None if v < 0 else v
+
+    @property
+    def rank(self):
+        n = lib().og_rank_len(self._h)
+        out = (C.c_uint64 * max(1, n))()
+        lib().og_rank(self._h, out)
+        return list(out)[:n]
+
+    def consensus(self, params):
+        n = self.n_nodes
+        seq = (C.c_uint8 * n)(); ids = (C.c_uint64 * n)(); cov = (C.c_uint64 * n)()
+        k = lib().og_consensus(self._h, _P(params).ref, seq, ids, cov, n)
+        return bytes(seq)[:k], list(ids)[:k], list(cov)[:k]
+
+    def msa(self, params, seqs):
+        names = (C.c_char_p * max(1, len(seqs)))(*[s.name.encode() for s in seqs])
+        comments = (C.c_char_p * max(1, len(seqs)))(*[s.comment.encode() for s in seqs])
+        out = C.c_void_p()
+        if lib().og_msa(self._h, _P(params).ref, names, comments, len(seqs), C.byref(out)):
+            raise _err()
+        text = C.string_at(out.value).decode()
+        lib().o_free(out)
+        return text
+
+    def gfa(self, params, seqs):
+        names = (C.c_char_p * max(1, len(seqs)))(*[s.name.encode() for s in seqs])
+        out = C.c_void_p()
+        if lib().og_gfa(self._h, _P(params).ref, names, len(seqs), C.byref(out)):
+            raise _err()
+        text = C.string_at(out.value).decode()
+        lib().o_free(out)
+        return text
+
+
+def _path_call(fn, params, graph, *args, qlen_hint=0):
+    cap = qlen_hint + 16
+    while True:
+        buf = (_Base * cap)()
+        n = C.c_size_t(0)
+        if fn(*args, buf, cap, C.byref(n)):
+            raise _err()
+        if n.value <= cap:
+            return [AlignedBase(None if b.node < 0 else b.node, b.base,
+                                None if b.aligned < 0 else b.aligned) for b in buf[:n.value]]
+        cap = n.value + 16
+
+
+def align(params, graph, query):
+    q = bytes(query)
+    P = _P(params)
+    return _path_call(lib().o_align, params, graph, P.ref, graph._h, q, len(q), qlen_hint=len(q))
+
+
+def _poa(name, params, graph, beg, end, query, adaptive, band):
+    q = bytes(query)
+    P = _P(params)
+    return _path_call(getattr(lib(), name), params, graph, P.ref, graph._h, beg, end, q, len(q),
+                      int(bool(adaptive)), band, qlen_hint=len(q))
+
+
+def poa(params, graph, beg, end, query, adaptive, band):
+    return _poa("o_poa", params, graph, beg, end, query, adaptive, band)
+
+
+def poa_inner(params, graph, beg, end, query, adaptive, band):
+    return _poa("o_poa_inner", params, graph, beg, end, query, adaptive, band)
+
+
+def poa_simd_i32(params, graph, beg, end, query, adaptive, band):
+    return _poa("o_poa_simd_i32", params, graph, beg, end, query, adaptive, band)
+
+
+def poa_scalar_i64(params, graph, beg, end, query, adaptive, band):
+    return _poa("o_poa_scalar_i64", params, graph, beg, end, query, adaptive, band)
+
+
+def result_or_error(fn, *args):
+    """Ok(path) / Err(message) as a comparable tuple, like Rust's Result."""
+    try:
+        return ("ok", fn(*args))
+    except OracleError as e:
+        return ("err", str(e))
+
+
+def i32_scores_safe(params, graph, beg, end, qlen):
+    return bool(lib().o_i32_scores_safe(_P(params).ref, graph._h, beg, end, qlen))
+
+
+def insertion_prefix(md, carry, open_, ext):
+    a = (C.c_int32 * 8)(*md)
+    out = (C.c_int32 * 8)()
+    lib().o_insertion_prefix(a, carry, open_, ext, out)
+    return list(out)
+
+
+def cells():
+    return lib().o_cells()
+
+
+def last_score():
+    """M at (n-1, m-1) of the calling thread's last DP fill."""
+    return lib().o_last_score()
+
+
+def reset_cells():
+    lib().o_reset_cells()
+
+
+def arena_capacity():
+    return lib().o_arena_capacity()
+
+
+def arena_clear():
+    lib().o_arena_clear()
+
+
+def align_sequences(params, seqs):
+    raw = [bytes(s.seq) for s in seqs]
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_size_t * max(1, len(raw)))(*[len(r) for r in raw])
+    h = lib().o_align_sequences(_P(params).ref, arr, lens, len(raw))
+    if not h:
+        raise _err()
+    return Graph(_handle=h)
+
+
+def extend_graph(params, graph, seqs, existing):
+    raw = [bytes(s.seq) for s in seqs]
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_size_t * max(1, len(raw)))(*[len(r) for r in raw])
+    if lib().o_extend_graph(_P(params).ref, graph._h, arr, lens, len(raw), existing):
+        raise _err()
+
+
+def batch_run(params, job_off, read_off, bases, nthreads):
+    """Job-parallel progressive POA + consensus/MSA hash per job (numpy arrays)."""
+    import numpy as np
+
+    job_off = np.ascontiguousarray(job_off, dtype=np.uint64)
+    read_off = np.ascontiguousarray(read_off, dtype=np.uint64)
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    njobs = len(job_off) - 1
+    out = np.zeros(njobs, dtype=np.uint64)
+    u64p = C.POINTER(C.c_uint64)
+    cells_ = lib().o_batch_run(
+        _P(params).ref, njobs, job_off.ctypes.data_as(u64p), read_off.ctypes.data_as(u64p),
+        bases.ctypes.data_as(C.c_void_p), int(nthreads), out.ctypes.data_as(u64p))
+    return int(cells_), out

@@ -1,0 +1,1430 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle. A plain-C restatement of the
+ * reference's POA hot path (minipoa-rs src/align.rs:16-1347), the graph it
+ * reads and writes (src/graph.rs:30-331) and the library driver
+ * (src/lib.rs:109-204). It exists to check the HIP product path and to time
+ * the reference's algorithm on host cores; the product never links it.
+ *
+ * Pinning (what fixes each part independently of this file):
+ *   insertion prefix      serial recurrence I<-max(I+e, md+o) (tests)
+ *   DP optimum            textbook Gotoh DP + path rescoring on the
+ *                         reference's LCG seeds (tests/test_oracle_align.py)
+ *   branched optimum      brute force over all DAG paths (tiny graphs)
+ *   i32 kernel == i64     cross-implementation equality incl. errors
+ *   graph/consensus       reference KATs (src/graph.rs:344-392)
+ *   pipeline bytes        reference KATs (tests/io_cli.rs:421-451, regressions)
+ *   MSA/GFA structure     invariants of tests/avx512.rs:27-91
+ * Parity unpinned: exact traceback tie-breaks, adaptive band evolution,
+ * head/tail support ties and toposort order beyond what the KATs reach are
+ * pinned only by this restatement being line-faithful.
+ */
+#include "poa_oracle.h"
+
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdatomic.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* src/align.rs:16-23, 216-220 */
+#define NEG (-(INT64_C(1) << 50))
+#define WIDTH 8
+#define MATCH 1u
+#define DELETE 2u
+#define INSERT 4u
+#define ALL (MATCH | DELETE | INSERT)
+#define NEG32 (-(INT32_C(1) << 29))
+#define SCORE_LIMIT (UINT64_C(1) << 26)
+#define SHIFT_FILL (NEG32 - 2 * (int32_t)SCORE_LIMIT)
+
+static __thread char g_err[512];
+static __thread uint64_t g_cells;
+static __thread int64_t g_last_score;
+
+static int fail(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return 1;
+}
+const char *o_last_error(void) { return g_err; }
+uint64_t o_cells(void) { return g_cells; }
+int64_t o_last_score(void) { return g_last_score; }
+void o_reset_cells(void) { g_cells = 0; }
+void o_free(void *ptr) { free(ptr); }
+
+static void *xmalloc(size_t n) {
+    void *p = malloc(n ? n : 1);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+static void *xcalloc(size_t n, size_t s) {
+    void *p = calloc(n ? n : 1, s ? s : 1);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+static void *xrealloc(void *p, size_t n) {
+    p = realloc(p, n ? n : 1);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return p;
+}
+
+/* Wrapping i32 arithmetic: wide::i32x8 lanes add with two's-complement wrap. */
+static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static inline int32_t max32(int32_t a, int32_t b) { return a > b ? a : b; }
+static inline int64_t max64(int64_t a, int64_t b) { return a > b ? a : b; }
+static inline int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+typedef struct {
+    size_t *v;
+    size_t n, cap;
+} vecz;
+static void vz_push(vecz *a, size_t x) {
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 4;
+        a->v = xrealloc(a->v, a->cap * sizeof(size_t));
+    }
+    a->v[a->n++] = x;
+}
+
+typedef struct {
+    char *s;
+    size_t n, cap;
+} sbuf;
+static void sb_putn(sbuf *b, const char *s, size_t n) {
+    if (b->n + n + 1 > b->cap) {
+        size_t c = b->cap ? b->cap : 256;
+        while (c < b->n + n + 1) c *= 2;
+        b->s = xrealloc(b->s, c);
+        b->cap = c;
+    }
+    memcpy(b->s + b->n, s, n);
+    b->n += n;
+    b->s[b->n] = 0;
+}
+static void sb_puts(sbuf *b, const char *s) { sb_putn(b, s, strlen(s)); }
+static void sb_putc(sbuf *b, char c) { sb_putn(b, &c, 1); }
+static void sb_printf(sbuf *b, const char *fmt, ...) {
+    char tmp[128];
+    va_list ap;
+    va_start(ap, fmt);
+    int k = vsnprintf(tmp, sizeof tmp, fmt, ap);
+    va_end(ap);
+    sb_putn(b, tmp, (size_t)k);
+}
+
+typedef struct {
+    ob_base *v;
+    size_t n, cap;
+} obvec;
+static void ob_push(obvec *a, int64_t node, int64_t base, int64_t aligned) {
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 64;
+        a->v = xrealloc(a->v, a->cap * sizeof(ob_base));
+    }
+    a->v[a->n].node = node;
+    a->v[a->n].base = base;
+    a->v[a->n].aligned = aligned;
+    a->n++;
+}
+
+/* ---------------- Params (src/lib.rs:109-173) ---------------- */
+
+static size_t p_alpha(const op_params *p) { return p->alphabet_len; }
+
+/* src/lib.rs:110-126 */
+size_t o_encode(const op_params *p, uint8_t base) {
+    uint8_t b = (base >= 'a' && base <= 'z') ? (uint8_t)(base - 32) : base;
+    if (p->alphabet_len <= 5 && b == 'U') b = 'T';
+    for (size_t i = 0; i < p->alphabet_len; i++)
+        if (p->alphabet[i] == b) return i;
+    for (size_t i = 0; i < p->alphabet_len; i++)
+        if (p->alphabet[i] == 'N') return i;
+    return p->alphabet_len - 1;
+}
+/* src/lib.rs:127-134 */
+static uint8_t o_decode(const op_params *p, size_t base) {
+    uint8_t c = p->alphabet[base];
+    if (p->is_rna && p->alphabet_len <= 5 && c == 'T') return 'U';
+    return c;
+}
+/* src/lib.rs:135-137 */
+static inline int32_t o_score(const op_params *p, size_t a, size_t b) {
+    return p->matrix[a * p->alphabet_len + b];
+}
+
+/* src/lib.rs:149-173 */
+int o_validate(const op_params *p) {
+    size_t n = p->alphabet_len;
+    if (n == 0 || n > 256 || p->matrix == NULL) return fail("invalid scoring matrix dimensions");
+    if (p->threads == 0) return fail("thread count must be positive");
+    if (!(p->k >= 1 && p->k <= 31) || !(p->w >= 1 && p->w < 32768))
+        return fail("k must be 1..31 and minimizer window must be 1..32767");
+    if (p->gap_open > 0 || p->gap_ext > 0) return fail("gap penalties must be zero or negative");
+    if (p->result > 2 || p->verbose > 2) return fail("result format and verbosity must be 0..2");
+    double r = p->mm_filter_ratio;
+    if (!(r == r) || r - r != 0.0 || !(r >= 0.0 && r <= 1.0))
+        return fail("minimizer filter ratio must be between 0 and 1");
+    if (p->win_max == 0) return fail("window maximum must be positive");
+    return 0;
+}
+
+/* ---------------- Graph (src/graph.rs:6-331) ---------------- */
+
+typedef struct {
+    size_t id, rank, parent, base, coverage;
+    vecz incoming, in_w, outgoing, out_w;
+    int64_t *aligned;
+} onode;
+
+typedef struct {
+    size_t seq_id;
+    vecz ids;
+} opath;
+
+struct og_graph {
+    onode *nodes;
+    size_t n_nodes, cap_nodes;
+    vecz rank;
+    opath *paths;
+    size_t n_paths, cap_paths;
+    size_t alphabet_size;
+};
+
+/* src/graph.rs:42-61 */
+size_t og_add_node(og_graph *g, size_t base) {
+    size_t id = g->n_nodes;
+    if (g->n_nodes == g->cap_nodes) {
+        g->cap_nodes = g->cap_nodes ? 2 * g->cap_nodes : 64;
+        g->nodes = xrealloc(g->nodes, g->cap_nodes * sizeof(onode));
+    }
+    onode *nd = &g->nodes[id];
+    memset(nd, 0, sizeof *nd);
+    nd->id = id;
+    nd->rank = 0;
+    nd->parent = id;
+    nd->base = base;
+    nd->aligned = xmalloc(g->alphabet_size * sizeof(int64_t));
+    for (size_t a = 0; a < g->alphabet_size; a++) nd->aligned[a] = -1;
+    nd->aligned[base] = (int64_t)id;
+    g->n_nodes++;
+    return id;
+}
+
+/* src/graph.rs:30-40 */
+og_graph *og_new(const op_params *p) {
+    og_graph *g = xcalloc(1, sizeof *g);
+    g->alphabet_size = p->alphabet_len;
+    og_add_node(g, o_encode(p, 'N'));
+    og_add_node(g, o_encode(p, 'N'));
+    return g;
+}
+
+void og_free(og_graph *g) {
+    if (!g) return;
+    for (size_t i = 0; i < g->n_nodes; i++) {
+        onode *nd = &g->nodes[i];
+        free(nd->incoming.v);
+        free(nd->in_w.v);
+        free(nd->outgoing.v);
+        free(nd->out_w.v);
+        free(nd->aligned);
+    }
+    for (size_t i = 0; i < g->n_paths; i++) free(g->paths[i].ids.v);
+    free(g->nodes);
+    free(g->paths);
+    free(g->rank.v);
+    free(g);
+}
+
+/* src/graph.rs:63-80 */
+void og_add_edge(og_graph *g, size_t from, size_t to) {
+    if (from == to) return;
+    onode *f = &g->nodes[from], *t = &g->nodes[to];
+    size_t i;
+    for (i = 0; i < f->outgoing.n; i++)
+        if (f->outgoing.v[i] == to) break;
+    if (i < f->outgoing.n) {
+        f->out_w.v[i] += 1;
+    } else {
+        vz_push(&f->outgoing, to);
+        vz_push(&f->out_w, 1);
+    }
+    for (i = 0; i < t->incoming.n; i++)
+        if (t->incoming.v[i] == from) break;
+    if (i < t->incoming.n) {
+        t->in_w.v[i] += 1;
+    } else {
+        vz_push(&t->incoming, from);
+        vz_push(&t->in_w, 1);
+    }
+    t->coverage += 1;
+}
+
+/* src/graph.rs:82-122 */
+int og_add_path(og_graph *g, size_t seq_id, const ob_base *aln, size_t len) {
+    for (size_t k = 0; k < len; k++) {
+        if (aln[k].base < 0 || (uint64_t)aln[k].base >= g->alphabet_size)
+            return fail("alignment base outside alphabet");
+        if ((aln[k].node >= 0 && (uint64_t)aln[k].node >= g->n_nodes) ||
+            (aln[k].aligned >= 0 && (uint64_t)aln[k].aligned >= g->n_nodes))
+            return fail("alignment references an unknown node");
+    }
+    int has_prev = 0;
+    size_t previous = 0;
+    vecz path = {0};
+    for (size_t k = 0; k < len; k++) {
+        const ob_base *it = &aln[k];
+        size_t id;
+        size_t base = (size_t)it->base;
+        if (it->node >= 0) {
+            id = (size_t)it->node;
+        } else if (it->aligned >= 0) {
+            size_t parent = g->nodes[(size_t)it->aligned].parent;
+            if (g->nodes[parent].aligned[base] >= 0) {
+                id = (size_t)g->nodes[parent].aligned[base];
+            } else {
+                id = og_add_node(g, base);
+                g->nodes[id].parent = parent;
+                g->nodes[parent].aligned[base] = (int64_t)id;
+            }
+        } else {
+            id = og_add_node(g, base);
+        }
+        if (has_prev) og_add_edge(g, previous, id);
+        previous = id;
+        has_prev = 1;
+        if (id >= 2) vz_push(&path, id);
+    }
+    og_add_edge(g, has_prev ? previous : 0, 1);
+    if (g->n_paths == g->cap_paths) {
+        g->cap_paths = g->cap_paths ? 2 * g->cap_paths : 16;
+        g->paths = xrealloc(g->paths, g->cap_paths * sizeof(opath));
+    }
+    g->paths[g->n_paths].seq_id = seq_id;
+    g->paths[g->n_paths].ids = path;
+    g->n_paths++;
+    return 0;
+}
+
+/* src/graph.rs:124-178 -- LIFO Kahn order; `grouped` merges column groups. */
+int og_toposort(og_graph *g, int grouped) {
+    size_t nn = g->n_nodes;
+    size_t *degree = xcalloc(nn, sizeof(size_t));
+    vecz stack = {0};
+    for (size_t id = 0; id < nn; id++) {
+        onode *n = &g->nodes[id];
+        if (grouped && n->parent != n->id) continue;
+        if (grouped) {
+            size_t s = 0;
+            for (size_t a = 0; a < g->alphabet_size; a++)
+                if (n->aligned[a] >= 0) s += g->nodes[(size_t)n->aligned[a]].incoming.n;
+            degree[id] = s;
+        } else {
+            degree[id] = n->incoming.n;
+        }
+        if (degree[id] == 0) vz_push(&stack, id);
+    }
+    g->rank.n = 0;
+    size_t visited = 0;
+    int rc = 0;
+    while (stack.n > 0) {
+        size_t id = stack.v[--stack.n];
+        size_t rank = g->rank.n;
+        vz_push(&g->rank, id);
+        size_t members[256];
+        size_t nm = 0;
+        if (grouped) {
+            for (size_t a = 0; a < g->alphabet_size; a++)
+                if (g->nodes[id].aligned[a] >= 0) members[nm++] = (size_t)g->nodes[id].aligned[a];
+        } else {
+            members[nm++] = id;
+        }
+        for (size_t mi = 0; mi < nm; mi++) {
+            onode *mem = &g->nodes[members[mi]];
+            mem->rank = rank;
+            visited++;
+            for (size_t e = 0; e < mem->outgoing.n; e++) {
+                size_t next = mem->outgoing.v[e];
+                if (grouped) next = g->nodes[next].parent;
+                if (degree[next] == 0) {
+                    rc = fail("invalid graph column dependencies");
+                    goto done;
+                }
+                degree[next]--;
+                if (degree[next] == 0) vz_push(&stack, next);
+            }
+        }
+    }
+    if (visited != nn) rc = fail("graph contains a cycle (possibly between alignment columns)");
+done:
+    free(degree);
+    free(stack.v);
+    return rc;
+}
+
+size_t og_n_nodes(const og_graph *g) { return g->n_nodes; }
+void og_node_info(const og_graph *g, size_t id, uint64_t out6[6]) {
+    const onode *n = &g->nodes[id];
+    out6[0] = n->rank;
+    out6[1] = n->parent;
+    out6[2] = n->base;
+    out6[3] = n->coverage;
+    out6[4] = n->incoming.n;
+    out6[5] = n->outgoing.n;
+}
+int64_t og_node_aligned(const og_graph *g, size_t id, size_t base) {
+    return g->nodes[id].aligned[base];
+}
+void og_node_edges(const og_graph *g, size_t id, uint64_t *in_ids, uint64_t *in_w,
+                   uint64_t *out_ids, uint64_t *out_w) {
+    const onode *n = &g->nodes[id];
+    for (size_t k = 0; k < n->incoming.n; k++) {
+        in_ids[k] = n->incoming.v[k];
+        in_w[k] = n->in_w.v[k];
+    }
+    for (size_t k = 0; k < n->outgoing.n; k++) {
+        out_ids[k] = n->outgoing.v[k];
+        out_w[k] = n->out_w.v[k];
+    }
+}
+size_t og_rank_len(const og_graph *g) { return g->rank.n; }
+void og_rank(const og_graph *g, uint64_t *out) {
+    for (size_t k = 0; k < g->rank.n; k++) out[k] = g->rank.v[k];
+}
+size_t og_n_paths(const og_graph *g) { return g->n_paths; }
+
+/* src/graph.rs:180-237 -- heaviest incoming edge, ties by path score then
+ * by lower incoming index; FIFO Kahn from the source, stop at the sink. */
+size_t og_consensus(const og_graph *g, const op_params *p, uint8_t *seq, uint64_t *ids_out,
+                    uint64_t *cov, size_t cap) {
+    size_t nn = g->n_nodes;
+    size_t *degree = xmalloc(nn * sizeof(size_t));
+    int64_t *previous = xmalloc(nn * sizeof(int64_t));
+    size_t *score = xcalloc(nn, sizeof(size_t));
+    size_t *queue = xmalloc(nn * sizeof(size_t));
+    size_t qh = 0, qt = 0;
+    for (size_t i = 0; i < nn; i++) {
+        degree[i] = g->nodes[i].incoming.n;
+        previous[i] = -1;
+    }
+    queue[qt++] = 0;
+    while (qh < qt) {
+        size_t id = queue[qh++];
+        const onode *node = &g->nodes[id];
+        if (id != 0 && node->incoming.n > 0) {
+            size_t bi = 0;
+            for (size_t k = 1; k < node->incoming.n; k++) {
+                size_t a = node->incoming.v[bi], b = node->incoming.v[k];
+                /* max_by keeps the accumulator only when it compares Greater. */
+                int acc_greater;
+                if (node->in_w.v[bi] != node->in_w.v[k])
+                    acc_greater = node->in_w.v[bi] > node->in_w.v[k];
+                else if (score[a] != score[b])
+                    acc_greater = score[a] > score[b];
+                else
+                    acc_greater = k > bi;
+                if (!acc_greater) bi = k;
+            }
+            size_t pr = node->incoming.v[bi];
+            previous[id] = (int64_t)pr;
+            score[id] = score[pr] + node->in_w.v[bi];
+        }
+        if (id == 1) break;
+        for (size_t e = 0; e < node->outgoing.n; e++) {
+            size_t next = node->outgoing.v[e];
+            degree[next]--;
+            if (degree[next] == 0) queue[qt++] = next;
+        }
+    }
+    vecz ids = {0};
+    int64_t cur = previous[1];
+    while (cur >= 0) {
+        if (cur == 0) break;
+        vz_push(&ids, (size_t)cur);
+        cur = previous[cur];
+    }
+    size_t len = ids.n;
+    for (size_t k = 0; k < len && k < cap; k++) {
+        size_t id = ids.v[len - 1 - k];
+        const onode *n = &g->nodes[id];
+        if (seq) seq[k] = o_decode(p, n->base);
+        if (ids_out) ids_out[k] = id;
+        if (cov) {
+            size_t s = 0;
+            for (size_t e = 0; e < n->out_w.n; e++) s += n->out_w.v[e];
+            cov[k] = n->coverage > s ? n->coverage : s;
+        }
+    }
+    free(ids.v);
+    free(degree);
+    free(previous);
+    free(score);
+    free(queue);
+    return len;
+}
+
+/* src/graph.rs:255-279 */
+static int msa_grouped(const og_graph *g, const op_params *p, const char *const *names,
+                       const char *const *comments, size_t nseq, sbuf *out) {
+    size_t width = g->rank.n >= 2 ? g->rank.n - 2 : 0;
+    char *row = xmalloc(width + 1);
+    for (size_t pi = 0; pi < g->n_paths; pi++) {
+        const opath *pa = &g->paths[pi];
+        if (pa->seq_id >= nseq) {
+            free(row);
+            return fail("unknown sequence ID in graph path");
+        }
+        memset(row, '-', width);
+        row[width] = 0;
+        for (size_t k = 0; k < pa->ids.n; k++) {
+            size_t id = pa->ids.v[k];
+            size_t rank = g->nodes[g->nodes[id].parent].rank;
+            if (rank == 0 || rank > width) {
+                free(row);
+                return fail("graph path is outside MSA columns");
+            }
+            row[rank - 1] = (char)o_decode(p, g->nodes[id].base);
+        }
+        sb_putc(out, '>');
+        sb_puts(out, names[pa->seq_id]);
+        sb_putc(out, ' ');
+        sb_puts(out, comments ? comments[pa->seq_id] : "");
+        sb_putc(out, '\n');
+        sb_putn(out, row, width);
+        sb_putc(out, '\n');
+    }
+    free(row);
+    return 0;
+}
+
+/* src/graph.rs:239-253 -- grouped toposort, render, restore ranks. */
+int og_msa(og_graph *g, const op_params *p, const char *const *names,
+           const char *const *comments, size_t nseq, char **out) {
+    sbuf b = {0};
+    sb_puts(&b, "");
+    if (g->n_paths == 0) {
+        *out = b.s;
+        return 0;
+    }
+    size_t *rank = xmalloc((g->rank.n + 1) * sizeof(size_t));
+    size_t nrank = g->rank.n;
+    memcpy(rank, g->rank.v, nrank * sizeof(size_t));
+    size_t *node_ranks = xmalloc(g->n_nodes * sizeof(size_t));
+    for (size_t i = 0; i < g->n_nodes; i++) node_ranks[i] = g->nodes[i].rank;
+    int rc = og_toposort(g, 1);
+    if (rc == 0) rc = msa_grouped(g, p, names, comments, nseq, &b);
+    g->rank.n = 0;
+    for (size_t k = 0; k < nrank; k++) vz_push(&g->rank, rank[k]);
+    for (size_t i = 0; i < g->n_nodes; i++) g->nodes[i].rank = node_ranks[i];
+    free(rank);
+    free(node_ranks);
+    if (rc) {
+        free(b.s);
+        *out = NULL;
+        return rc;
+    }
+    *out = b.s;
+    return 0;
+}
+
+/* src/graph.rs:281-331 */
+int og_gfa(const og_graph *g, const op_params *p, const char *const *names, size_t nseq,
+           char **out) {
+    sbuf b = {0};
+    sb_puts(&b, "");
+    if (g->n_nodes <= 2 && g->n_paths == 0) {
+        *out = b.s;
+        return 0;
+    }
+    sb_puts(&b, "H\tVN:Z:1.0\n");
+    size_t nn = g->n_nodes;
+    size_t *degree = xmalloc(nn * sizeof(size_t));
+    size_t *queue = xmalloc(nn * sizeof(size_t));
+    size_t qh = 0, qt = 0;
+    for (size_t i = 0; i < nn; i++) degree[i] = g->nodes[i].incoming.n;
+    queue[qt++] = 0;
+    while (qh < qt) {
+        size_t id = queue[qh++];
+        if (id == 1) break;
+        const onode *node = &g->nodes[id];
+        if (id >= 2) {
+            sb_printf(&b, "S\t%zu\t%c\n", id - 1, (char)o_decode(p, node->base));
+            for (size_t e = 0; e < node->incoming.n; e++) {
+                size_t prev = node->incoming.v[e];
+                if (prev != 0) sb_printf(&b, "L\t%zu\t+\t%zu\t+\t0M\n", prev - 1, id - 1);
+            }
+        }
+        for (size_t e = 0; e < node->outgoing.n; e++) {
+            size_t next = node->outgoing.v[e];
+            degree[next]--;
+            if (degree[next] == 0) queue[qt++] = next;
+        }
+    }
+    free(degree);
+    free(queue);
+    for (size_t pi = 0; pi < g->n_paths; pi++) {
+        const opath *pa = &g->paths[pi];
+        if (pa->seq_id >= nseq) {
+            free(b.s);
+            *out = NULL;
+            return fail("unknown sequence ID in graph path");
+        }
+        sb_puts(&b, "P\t");
+        sb_puts(&b, names[pa->seq_id]);
+        sb_putc(&b, '\t');
+        if (pa->ids.n == 0) sb_putc(&b, '*');
+        for (size_t k = 0; k < pa->ids.n; k++) {
+            if (k > 0) sb_putc(&b, ',');
+            sb_printf(&b, "%zu+", pa->ids.v[k] - 1);
+        }
+        sb_puts(&b, "\t*\n");
+    }
+    *out = b.s;
+    return 0;
+}
+
+/* ---------------- DP common setup (src/align.rs:598-688, 913-1011) ------- */
+
+typedef struct {
+    size_t beg, end, n, m, padded, unknown;
+    size_t *seq;
+    int adaptive;
+    int64_t band;
+    int64_t *head, *tail, *support;
+    int64_t *peak, *streak, *peak_left, *peak_right, *offset_left, *offset_right, *extra;
+} dpsetup;
+
+static void setup_free(dpsetup *s) {
+    free(s->seq);
+    free(s->head);
+    free(s->tail);
+    free(s->support);
+    free(s->peak);
+    free(s->streak);
+    free(s->peak_left);
+    free(s->peak_right);
+    free(s->offset_left);
+    free(s->offset_right);
+    free(s->extra);
+    memset(s, 0, sizeof *s);
+}
+
+/* Anchor checks, encoded/padded query, band, and the head/tail/support
+ * most-supported-path distances. src/align.rs:598-688 == 921-1011. */
+static int dp_setup(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                    const uint8_t *q, size_t qlen, int adaptive, size_t band_b, dpsetup *s) {
+    memset(s, 0, sizeof *s);
+    if (beg_id >= g->n_nodes || end_id >= g->n_nodes) return fail("alignment anchor is not in graph");
+    size_t beg = g->nodes[beg_id].rank, end = g->nodes[end_id].rank;
+    if (end <= beg || end >= g->rank.n) return fail("alignment anchors are not topologically ordered");
+    size_t n = end - beg + 1;
+    size_t m = qlen + 2;
+    if (n > (size_t)INT32_MAX || m > (size_t)INT32_MAX)
+        return fail("alignment exceeds coordinate limits");
+    s->beg = beg;
+    s->end = end;
+    s->n = n;
+    s->m = m;
+    s->unknown = o_encode(p, 'N');
+    s->padded = (m / WIDTH + 1) * WIDTH;
+    s->seq = xmalloc(s->padded * sizeof(size_t));
+    s->seq[0] = s->unknown;
+    for (size_t k = 0; k < qlen; k++) s->seq[k + 1] = o_encode(p, q[k]);
+    for (size_t k = qlen + 1; k < s->padded; k++) s->seq[k] = s->unknown;
+    s->adaptive = adaptive || p->adaptive;
+    uint64_t mdivf = p->f ? (uint64_t)m / p->f : 0;
+    uint64_t bw = (uint64_t)band_b + mdivf;
+    if (bw < (uint64_t)band_b) bw = UINT64_MAX; /* saturating_add */
+    if (bw > (uint64_t)INT32_MAX) bw = INT32_MAX;
+    s->band = (int64_t)bw;
+    s->head = xmalloc(n * sizeof(int64_t));
+    s->tail = xmalloc(n * sizeof(int64_t));
+    s->support = xcalloc(n, sizeof(int64_t));
+    for (size_t i = 0; i < n; i++) s->head[i] = s->tail[i] = NEG;
+    s->head[0] = 0;
+    for (size_t i = 1; i < n; i++) {
+        const onode *node = &g->nodes[g->rank.v[beg + i]];
+        int have = 0;
+        size_t bp = 0, bw2 = 0;
+        for (size_t k = 0; k < node->incoming.n; k++) {
+            size_t rank = g->nodes[node->incoming.v[k]].rank;
+            if (rank < beg || rank >= beg + i) continue;
+            size_t pr = rank - beg;
+            if (s->head[pr] == NEG) continue;
+            size_t w = node->in_w.v[k];
+            if (!have || w > bw2 || (w == bw2 && s->support[pr] > s->support[bp])) {
+                have = 1;
+                bp = pr;
+                bw2 = w;
+            }
+        }
+        if (have) {
+            s->head[i] = s->head[bp] + 1;
+            s->support[i] = s->support[bp] + (int64_t)bw2;
+        }
+    }
+    s->tail[n - 1] = 1;
+    for (size_t ii = n - 1; ii-- > 0;) {
+        size_t i = ii;
+        const onode *node = &g->nodes[g->rank.v[beg + i]];
+        int have = 0;
+        size_t bs = 0, bw2 = 0;
+        for (size_t k = 0; k < node->outgoing.n; k++) {
+            size_t rank = g->nodes[node->outgoing.v[k]].rank;
+            if (rank <= beg + i || rank > end) continue;
+            size_t sc = rank - beg;
+            if (s->tail[sc] == NEG) continue;
+            size_t w = node->out_w.v[k];
+            if (!have || w > bw2 || (w == bw2 && s->support[sc] > s->support[bs])) {
+                have = 1;
+                bs = sc;
+                bw2 = w;
+            }
+        }
+        if (have) {
+            s->tail[i] = s->tail[bs] + 1;
+            s->support[i] = s->support[bs] + (int64_t)bw2;
+        }
+    }
+    if (s->head[n - 1] == NEG) {
+        setup_free(s);
+        return fail("no graph path connects alignment anchors");
+    }
+    s->peak = xcalloc(n, sizeof(int64_t));
+    s->streak = xcalloc(n, sizeof(int64_t));
+    s->peak_left = xmalloc(n * sizeof(int64_t));
+    s->peak_right = xcalloc(n, sizeof(int64_t));
+    s->offset_left = xcalloc(n, sizeof(int64_t));
+    s->offset_right = xcalloc(n, sizeof(int64_t));
+    s->extra = xcalloc(n, sizeof(int64_t));
+    for (size_t i = 0; i < n; i++) s->peak_left[i] = (int64_t)m;
+    return 0;
+}
+
+/* Row band, src/align.rs:711-734 == 1017-1040. */
+static int row_band(const op_params *p, const dpsetup *s, size_t i, size_t *start, size_t *stop) {
+    int64_t m = (int64_t)s->m, left, right;
+    if (p->f == 0) {
+        left = 0;
+        right = m;
+    } else if (s->adaptive) {
+        int64_t a = min64(min64(s->peak_left[i], s->head[i] + s->offset_left[i]), m - s->tail[i]);
+        int64_t b = max64(max64(s->peak_right[i], s->head[i] + s->offset_right[i]), m - s->tail[i]);
+        left = max64(a - s->band - s->extra[i], 0);
+        right = min64(b + s->band + s->extra[i], m);
+    } else {
+        left = max64(min64(s->head[i], m - s->tail[i]) - s->band, 0);
+        right = min64(max64(s->head[i], m - s->tail[i]) + s->band, m);
+    }
+    if (left > right || right < 0) return fail("alignment band does not intersect graph path");
+    *start = (size_t)left / WIDTH * WIDTH;
+    *stop = ((size_t)right / WIDTH + 1) * WIDTH;
+    return 0;
+}
+
+/* Adaptive band bookkeeping after row i, src/align.rs:829-897 == 1088-1146.
+ * `row_m` are the row's M scores as i64; argmax is the first maximum. */
+static void adaptive_update(const og_graph *g, dpsetup *s, size_t i, size_t start,
+                            int64_t maximum_col) {
+    const onode *node = &g->nodes[g->rank.v[s->beg + i]];
+    int64_t maximum = maximum_col;
+    s->peak[i] = maximum;
+    int64_t max_streak = 0;
+    int is_match = 0;
+    (void)start;
+    if (node->base == s->seq[(size_t)maximum]) {
+        for (size_t k = 0; k < node->incoming.n; k++) {
+            size_t rank = g->nodes[node->incoming.v[k]].rank;
+            if (rank < s->beg || rank >= s->beg + i) continue;
+            size_t pr = rank - s->beg;
+            if (s->head[pr] == NEG) continue;
+            max_streak = max64(max_streak, s->streak[pr]);
+            if (s->peak[pr] + 1 == maximum) {
+                s->streak[i] = max64(s->streak[i], s->streak[pr] + 1);
+                is_match = 1;
+            }
+        }
+    }
+    if (!is_match && max_streak < 30) s->extra[i] = min64(s->extra[i] + max_streak + 1, 5 * s->band);
+    if (s->streak[i] >= 30) {
+        s->offset_left[i] = maximum - s->head[i];
+        s->offset_right[i] = maximum - s->head[i];
+        s->extra[i] = 0;
+        for (size_t k = 0; k < node->outgoing.n; k++) {
+            size_t rank = g->nodes[node->outgoing.v[k]].rank;
+            if (rank > s->beg + i && rank <= s->end) {
+                size_t sc = rank - s->beg;
+                s->offset_left[sc] = s->offset_left[i];
+                s->offset_right[sc] = s->offset_right[i];
+            }
+        }
+    }
+    for (size_t k = 0; k < node->outgoing.n; k++) {
+        size_t rank = g->nodes[node->outgoing.v[k]].rank;
+        if (rank <= s->beg + i || rank > s->end) continue;
+        size_t sc = rank - s->beg;
+        s->offset_left[sc] = min64(s->offset_left[sc], s->offset_left[i]);
+        s->offset_right[sc] = max64(s->offset_right[sc], s->offset_right[i]);
+        s->peak_left[sc] = min64(s->peak_left[sc], maximum);
+        s->peak_right[sc] = max64(s->peak_right[sc], maximum + 1);
+        s->extra[sc] = s->extra[i] != 0 ? max64(s->extra[sc], s->extra[i]) : 0;
+    }
+}
+
+/* ---------------- Traceback (src/align.rs:276-353, 1164-1347) ------------ */
+
+typedef struct omat omat;
+struct omat {
+    int (*contains)(const omat *, size_t r, size_t c);
+    int64_t (*m)(const omat *, size_t r, size_t c);
+    int64_t (*d)(const omat *, size_t r, size_t c);
+    int64_t (*ins)(const omat *, size_t r, size_t c);
+    int64_t negative;
+    /* i64 rows (src/align.rs:25-77) */
+    size_t *r_start, *r_len;
+    int64_t **r_m, **r_d, **r_i;
+    /* i32 arena rows (src/align.rs:228-274) */
+    const int32_t *data;
+    size_t *f_start, *f_len, *f_m, *f_d, *f_i;
+};
+
+static int s_contains(const omat *x, size_t r, size_t c) {
+    return c >= x->r_start[r] && c - x->r_start[r] < x->r_len[r];
+}
+static int64_t s_m(const omat *x, size_t r, size_t c) {
+    return s_contains(x, r, c) ? x->r_m[r][c - x->r_start[r]] : NEG;
+}
+static int64_t s_d(const omat *x, size_t r, size_t c) {
+    return s_contains(x, r, c) ? x->r_d[r][c - x->r_start[r]] : NEG;
+}
+static int64_t s_i(const omat *x, size_t r, size_t c) {
+    return s_contains(x, r, c) ? x->r_i[r][c - x->r_start[r]] : NEG;
+}
+static int f_contains(const omat *x, size_t r, size_t c) {
+    return c >= x->f_start[r] && c - x->f_start[r] < x->f_len[r];
+}
+static int64_t f_m(const omat *x, size_t r, size_t c) {
+    return f_contains(x, r, c) ? (int64_t)x->data[x->f_m[r] + c - x->f_start[r]] : (int64_t)NEG32;
+}
+static int64_t f_d(const omat *x, size_t r, size_t c) {
+    return f_contains(x, r, c) ? (int64_t)x->data[x->f_d[r] + c - x->f_start[r]] : (int64_t)NEG32;
+}
+static int64_t f_i(const omat *x, size_t r, size_t c) {
+    return f_contains(x, r, c) ? (int64_t)x->data[x->f_i[r] + c - x->f_start[r]] : (int64_t)NEG32;
+}
+
+/* src/align.rs:1315-1347 */
+static void push_diagonal(obvec *res, const og_graph *g, size_t i, size_t n, size_t j, size_t m,
+                          size_t query_base, size_t node_id, size_t node_base) {
+    if (i == n - 1 && j != m - 1) {
+        ob_push(res, -1, (int64_t)query_base, -1);
+    } else if (node_base == query_base) {
+        ob_push(res, (int64_t)node_id, (int64_t)g->nodes[node_id].base, -1);
+    } else {
+        size_t parent = g->nodes[node_id].parent;
+        ob_push(res, g->nodes[parent].aligned[query_base], (int64_t)query_base, (int64_t)parent);
+    }
+}
+
+/* src/align.rs:1201-1225 `choose`: max in_weight (strict >, first wins) among
+ * in-range predecessors whose state equals target - penalty. */
+static int64_t tb_choose(const og_graph *g, const dpsetup *s, const omat *x, const onode *node,
+                         size_t i, size_t j, unsigned state, int64_t penalty, size_t pj) {
+    int64_t best = -1;
+    size_t weight = 0;
+    for (size_t k = 0; k < node->incoming.n; k++) {
+        size_t rank = g->nodes[node->incoming.v[k]].rank;
+        if (rank < s->beg || rank >= s->beg + i) continue;
+        size_t pr = rank - s->beg;
+        if (s->head[pr] == NEG || !x->contains(x, pr, pj)) continue;
+        int64_t source = state == DELETE ? x->d(x, pr, pj) : x->m(x, pr, pj);
+        int64_t target = state == MATCH ? x->m(x, i, j) : x->d(x, i, j);
+        if (target == source + penalty && (best < 0 || node->in_w.v[k] > weight)) {
+            best = (int64_t)k;
+            weight = node->in_w.v[k];
+        }
+    }
+    return best;
+}
+
+/* src/align.rs:1164-1313 */
+static int traceback(const op_params *p, const og_graph *g, const dpsetup *s, const omat *x,
+                     int64_t gap_open, int64_t gap_ext, obvec *result) {
+    size_t n = s->n, m = s->m, beg = s->beg;
+    size_t i = n - 1, j = m - 1;
+    unsigned op = ALL;
+    size_t steps = 0;
+    result->n = 0;
+    while (i > 0 || j > 0) {
+        steps++;
+        if (steps > n + m) return fail("alignment traceback failed to advance");
+        if (!x->contains(x, i, j) || x->m(x, i, j) <= x->negative / 2)
+            return fail("alignment traceback left active band");
+        const onode *node = &g->nodes[g->rank.v[beg + i]];
+        size_t base = i == n - 1 ? o_encode(p, 'N') : node->base;
+        int64_t score = o_score(p, base, s->seq[j]);
+        int64_t pref = -1;
+        if ((op & MATCH) && j > 0 && base == s->seq[j])
+            pref = tb_choose(g, s, x, node, i, j, MATCH, score, j - 1);
+        if (pref >= 0) {
+            size_t tot = 0;
+            for (size_t k = 0; k < node->in_w.n; k++) tot += node->in_w.v[k];
+            if (!(node->in_w.v[pref] >= tot / 10)) pref = -1;
+        }
+        if (pref >= 0) {
+            push_diagonal(result, g, i, n, j, m, s->seq[j], node->id, base);
+            i = g->nodes[node->incoming.v[pref]].rank - beg;
+            j -= 1;
+            op = ALL;
+            continue;
+        }
+        if ((op & DELETE) && (op == DELETE || x->m(x, i, j) == x->d(x, i, j))) {
+            int64_t k = tb_choose(g, s, x, node, i, j, INSERT, gap_open, j);
+            if (k >= 0) {
+                i = g->nodes[node->incoming.v[k]].rank - beg;
+                op = p->gap_open == 0 ? ALL : (MATCH | INSERT);
+                continue;
+            }
+            k = tb_choose(g, s, x, node, i, j, DELETE, gap_ext, j);
+            if (k >= 0) {
+                i = g->nodes[node->incoming.v[k]].rank - beg;
+                op = DELETE;
+                continue;
+            }
+        }
+        if ((op & INSERT) && j > 0 && x->contains(x, i, j - 1) &&
+            (op == INSERT || x->m(x, i, j) == x->ins(x, i, j))) {
+            if (x->ins(x, i, j) == x->m(x, i, j - 1) + gap_open) {
+                ob_push(result, -1, (int64_t)s->seq[j], -1);
+                j -= 1;
+                op = p->gap_open == 0 ? ALL : (MATCH | DELETE);
+                continue;
+            }
+            if (x->ins(x, i, j) == x->ins(x, i, j - 1) + gap_ext) {
+                ob_push(result, -1, (int64_t)s->seq[j], -1);
+                j -= 1;
+                op = INSERT;
+                continue;
+            }
+        }
+        if ((op & MATCH) && j > 0) {
+            int64_t k = tb_choose(g, s, x, node, i, j, MATCH, score, j - 1);
+            if (k >= 0) {
+                push_diagonal(result, g, i, n, j, m, s->seq[j], node->id, base);
+                i = g->nodes[node->incoming.v[k]].rank - beg;
+                j -= 1;
+                op = ALL;
+                continue;
+            }
+        }
+        return fail("alignment traceback has no predecessor at graph row %zu, query column %zu", i, j);
+    }
+    size_t start = g->rank.v[beg];
+    ob_push(result, (int64_t)start, (int64_t)g->nodes[start].base, -1);
+    /* reverse, then pop the last element */
+    for (size_t a = 0, b = result->n - 1; a < b; a++, b--) {
+        ob_base t = result->v[a];
+        result->v[a] = result->v[b];
+        result->v[b] = t;
+    }
+    result->n -= 1;
+    return 0;
+}
+
+static int emit(const obvec *res, ob_base *out, size_t cap, size_t *out_len) {
+    *out_len = res->n;
+    for (size_t k = 0; k < res->n && k < cap; k++) out[k] = res->v[k];
+    return 0;
+}
+
+/* ---------------- i64 scalar kernel (src/align.rs:913-1162) ------------- */
+
+int o_poa_scalar_i64(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                     const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                     size_t cap, size_t *out_len) {
+    dpsetup s;
+    if (dp_setup(p, g, beg_id, end_id, q, qlen, adaptive, band_b, &s)) return 1;
+    size_t n = s.n, beg = s.beg;
+    int64_t gap_ext = p->gap_ext;
+    int64_t gap_open = (int64_t)p->gap_open + gap_ext;
+    omat x;
+    memset(&x, 0, sizeof x);
+    x.contains = s_contains;
+    x.m = s_m;
+    x.d = s_d;
+    x.ins = s_i;
+    x.negative = NEG;
+    x.r_start = xcalloc(n, sizeof(size_t));
+    x.r_len = xcalloc(n, sizeof(size_t));
+    x.r_m = xcalloc(n, sizeof(int64_t *));
+    x.r_d = xcalloc(n, sizeof(int64_t *));
+    x.r_i = xcalloc(n, sizeof(int64_t *));
+    int rc = 0;
+    obvec res = {0};
+    for (size_t i = 0; i < n; i++) {
+        if (s.head[i] == NEG || s.tail[i] == NEG) continue;
+        size_t start, stop;
+        if (row_band(p, &s, i, &start, &stop)) {
+            rc = 1;
+            goto out;
+        }
+        size_t len = stop - start;
+        int64_t *rm = xmalloc(len * sizeof(int64_t)), *rd = xmalloc(len * sizeof(int64_t)),
+                *ri = xmalloc(len * sizeof(int64_t));
+        for (size_t k = 0; k < len; k++) rm[k] = rd[k] = ri[k] = NEG;
+        x.r_start[i] = start;
+        x.r_len[i] = len;
+        x.r_m[i] = rm;
+        x.r_d[i] = rd;
+        x.r_i[i] = ri;
+        g_cells += len;
+        if (i == 0) {
+            if (!s_contains(&x, 0, 0)) {
+                rc = fail("alignment band excludes source");
+                goto out;
+            }
+            rm[0] = o_score(p, s.unknown, s.unknown);
+            for (size_t j = 1; j < len; j++) {
+                ri[j] = max64(ri[j - 1] + gap_ext, rm[j - 1] + gap_open);
+                rm[j] = max64(rm[j], ri[j]);
+            }
+            continue;
+        }
+        const onode *node = &g->nodes[g->rank.v[beg + i]];
+        size_t base = i == n - 1 ? s.unknown : node->base;
+        for (size_t j = start, local = 0; j < stop; j++, local++) {
+            size_t block = j / WIDTH * WIDTH;
+            int64_t diagonal = NEG, deletion = NEG;
+            for (size_t k = 0; k < node->incoming.n; k++) {
+                size_t rank = g->nodes[node->incoming.v[k]].rank;
+                if (rank < beg || rank >= beg + i) continue;
+                size_t pr = rank - beg;
+                if (s.head[pr] == NEG) continue;
+                if (!s_contains(&x, pr, block)) continue;
+                if (j > 0) diagonal = max64(diagonal, s_m(&x, pr, j - 1));
+                deletion = max64(max64(deletion, s_d(&x, pr, j) + gap_ext), s_m(&x, pr, j) + gap_open);
+            }
+            rd[local] = deletion;
+            int64_t md = max64(diagonal + (int64_t)o_score(p, base, s.seq[j]), deletion);
+            rm[local] = max64(md, ri[local]);
+            if (local + 1 < len) ri[local + 1] = max64(ri[local] + gap_ext, md + gap_open);
+        }
+        if (p->f != 0 && s.adaptive) {
+            size_t argmax = 0;
+            for (size_t j = 1; j < len; j++)
+                if (rm[j] > rm[argmax]) argmax = j;
+            adaptive_update(g, &s, i, start, (int64_t)(start + argmax));
+        }
+    }
+    g_last_score = s_m(&x, n - 1, s.m - 1);
+    rc = traceback(p, g, &s, &x, gap_open, gap_ext, &res);
+    if (!rc) emit(&res, out, cap, out_len);
+out:
+    for (size_t i = 0; i < n; i++) {
+        free(x.r_m[i]);
+        free(x.r_d[i]);
+        free(x.r_i[i]);
+    }
+    free(x.r_start);
+    free(x.r_len);
+    free(x.r_m);
+    free(x.r_d);
+    free(x.r_i);
+    free(res.v);
+    setup_free(&s);
+    return rc;
+}
+
+/* ---------------- i32 8-lane kernel (src/align.rs:222-911) -------------- */
+
+/* src/align.rs:222-226 worker-local arena, retained across calls. */
+static __thread int32_t *t_arena;
+static __thread size_t t_arena_len, t_arena_cap;
+size_t o_arena_capacity(void) { return t_arena_cap; }
+void o_arena_clear(void) {
+    free(t_arena);
+    t_arena = NULL;
+    t_arena_len = t_arena_cap = 0;
+}
+
+/* src/align.rs:459-491: three shift-max stages padded with SHIFT_FILL. */
+void o_insertion_prefix(const int32_t md[8], int32_t carry, int32_t open, int32_t ext,
+                        int32_t out[8]) {
+    int32_t v[8], t[8];
+    for (int l = 0; l < 8; l++) v[l] = wadd(md[l], open);
+    for (int sh = 1; sh <= 4; sh *= 2) {
+        int32_t e = (int32_t)((uint32_t)ext * (uint32_t)sh);
+        for (int l = 0; l < 8; l++) t[l] = wadd(l >= sh ? v[l - sh] : SHIFT_FILL, e);
+        for (int l = 0; l < 8; l++) v[l] = max32(v[l], t[l]);
+    }
+    for (int l = 0; l < 8; l++) {
+        int32_t c = wadd(carry, (int32_t)((uint32_t)ext * (uint32_t)(l + 1)));
+        out[l] = max32(v[l], c);
+    }
+}
+
+/* src/align.rs:493-529 */
+int o_i32_scores_safe(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                      size_t query_len) {
+    if (p->scalar_dp || beg_id >= g->n_nodes || end_id >= g->n_nodes) return 0;
+    size_t re = g->nodes[end_id].rank, rb = g->nodes[beg_id].rank;
+    if (re < rb) return 0;
+    uint64_t rows = re - rb;
+    uint64_t steps = rows + query_len;
+    if (steps < rows) return 0;
+    if (steps + WIDTH * 2 + 4 < steps) return 0;
+    steps += WIDTH * 2 + 4;
+    int64_t opening = (int64_t)p->gap_open + (int64_t)p->gap_ext;
+    uint64_t largest = 0;
+    size_t nm = (size_t)p->alphabet_len * p->alphabet_len;
+    for (size_t k = 0; k < nm; k++) {
+        int64_t v = p->matrix[k];
+        uint64_t a = (uint64_t)(v < 0 ? -v : v);
+        if (a > largest) largest = a;
+    }
+    uint64_t a = (uint64_t)(opening < 0 ? -opening : opening);
+    if (a > largest) largest = a;
+    int64_t ge = p->gap_ext;
+    a = (uint64_t)(ge < 0 ? -ge : ge);
+    if (a > largest) largest = a;
+    if (largest != 0 && steps > UINT64_MAX / largest) return 0;
+    return largest * steps <= SCORE_LIMIT;
+}
+
+static void arena_reserve(size_t need) {
+    if (need > t_arena_len) {
+        if (need > t_arena_cap) {
+            size_t c = t_arena_cap ? t_arena_cap : 1024;
+            while (c < need) c *= 2;
+            t_arena = xrealloc(t_arena, c * sizeof(int32_t));
+            t_arena_cap = c;
+        }
+        for (size_t k = t_arena_len; k < need; k++) t_arena[k] = NEG32;
+        t_arena_len = need;
+    }
+}
+
+/* src/align.rs:587-911 */
+int o_poa_simd_i32(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                   const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                   size_t cap, size_t *out_len) {
+    dpsetup s;
+    if (dp_setup(p, g, beg_id, end_id, q, qlen, adaptive, band_b, &s)) return 1;
+    size_t n = s.n, beg = s.beg, padded = s.padded;
+    int32_t gap_ext = p->gap_ext;
+    int32_t gap_open = p->gap_open + gap_ext;
+    size_t alpha = p_alpha(p);
+    int32_t *profiles = xmalloc(alpha * padded * sizeof(int32_t));
+    for (size_t a = 0; a < alpha; a++)
+        for (size_t c = 0; c < padded; c++) profiles[a * padded + c] = o_score(p, a, s.seq[c]);
+    omat x;
+    memset(&x, 0, sizeof x);
+    x.contains = f_contains;
+    x.m = f_m;
+    x.d = f_d;
+    x.ins = f_i;
+    x.negative = NEG32;
+    x.f_start = xcalloc(n, sizeof(size_t));
+    x.f_len = xcalloc(n, sizeof(size_t));
+    x.f_m = xcalloc(n, sizeof(size_t));
+    x.f_d = xcalloc(n, sizeof(size_t));
+    x.f_i = xcalloc(n, sizeof(size_t));
+    size_t cursor = 0;
+    int rc = 0;
+    obvec res = {0};
+    for (size_t i = 0; i < n; i++) {
+        if (s.head[i] == NEG || s.tail[i] == NEG) continue;
+        size_t start, stop;
+        if (row_band(p, &s, i, &start, &stop)) {
+            rc = 1;
+            goto out;
+        }
+        /* FastMatrix::allocate, src/align.rs:235-273: [8 pad][M][8 pad][D][I] */
+        size_t len = stop - start;
+        size_t size = len * 3 + WIDTH * 2;
+        size_t rstop = cursor + size;
+        arena_reserve(rstop);
+        int32_t *data = t_arena;
+        size_t mo = cursor + WIDTH, dof = cursor + WIDTH * 2 + len, io = cursor + WIDTH * 2 + len * 2;
+        for (size_t k = cursor; k < dof + len; k++) data[k] = NEG32;
+        cursor = rstop;
+        x.f_start[i] = start;
+        x.f_len[i] = len;
+        x.f_m[i] = mo;
+        x.f_d[i] = dof;
+        x.f_i[i] = io;
+        g_cells += len;
+        if (i == 0) {
+            if (start != 0) {
+                rc = fail("alignment band excludes source");
+                goto out;
+            }
+            data[mo] = o_score(p, s.unknown, s.unknown);
+            data[io] = NEG32;
+            for (size_t j = 1; j < len; j++) {
+                int32_t ins = max32(wadd(data[io + j - 1], gap_ext), wadd(data[mo + j - 1], gap_open));
+                data[io + j] = ins;
+                data[mo + j] = max32(data[mo + j], ins);
+            }
+            continue;
+        }
+        const onode *node = &g->nodes[g->rank.v[beg + i]];
+        for (size_t k = 0; k < node->incoming.n; k++) {
+            size_t rank = g->nodes[node->incoming.v[k]].rank;
+            if (rank < beg || rank >= beg + i) continue;
+            size_t pr = rank - beg;
+            if (s.head[pr] == NEG) continue;
+            size_t ps = x.f_start[pr], pl = x.f_len[pr], pm = x.f_m[pr], pd = x.f_d[pr];
+            size_t ov_start = start > ps ? start : ps;
+            size_t ov_stop = stop < ps + pl ? stop : ps + pl;
+            for (size_t j = ov_start; j < ov_stop; j += WIDTH) {
+                size_t local = j - start, prior = j - ps;
+                for (int l = 0; l < WIDTH; l++) {
+                    int32_t diagonal = data[pm + prior - 1 + l];
+                    int32_t prev_m = data[pm + prior + l];
+                    int32_t prev_d = data[pd + prior + l];
+                    int32_t mm = max32(data[mo + local + l], diagonal);
+                    int32_t dd = max32(max32(data[dof + local + l], wadd(prev_d, gap_ext)),
+                                       wadd(prev_m, gap_open));
+                    data[mo + local + l] = mm;
+                    data[dof + local + l] = dd;
+                }
+            }
+        }
+        size_t base = i == n - 1 ? s.unknown : node->base;
+        const int32_t *profile = profiles + base * padded;
+        int32_t carry = NEG32;
+        for (size_t local = 0; local < len; local += WIDTH) {
+            int32_t md[8], next[8];
+            for (int l = 0; l < 8; l++) {
+                int32_t diagonal = wadd(data[mo + local + l], profile[start + local + l]);
+                md[l] = max32(diagonal, data[dof + local + l]);
+            }
+            o_insertion_prefix(md, carry, gap_open, gap_ext, next);
+            int32_t current[8];
+            current[0] = carry;
+            for (int l = 1; l < 8; l++) current[l] = next[l - 1];
+            for (int l = 0; l < 8; l++) {
+                data[io + local + l] = current[l];
+                data[mo + local + l] = max32(md[l], current[l]);
+            }
+            carry = next[7];
+        }
+        if (p->f != 0 && s.adaptive) {
+            /* fold over whole 8-lane chunks from splat(NEG32), then the first
+             * position holding the maximum (rows are multiples of eight). */
+            int32_t best = NEG32;
+            for (size_t c = 0; c < len / WIDTH * WIDTH; c++) best = max32(best, data[mo + c]);
+            size_t argmax = 0;
+            while (argmax + 1 < len && data[mo + argmax] != best) argmax++;
+            adaptive_update(g, &s, i, start, (int64_t)(start + argmax));
+        }
+    }
+    x.data = t_arena;
+    g_last_score = f_m(&x, n - 1, s.m - 1);
+    rc = traceback(p, g, &s, &x, gap_open, gap_ext, &res);
+    if (!rc) emit(&res, out, cap, out_len);
+out:
+    free(profiles);
+    free(x.f_start);
+    free(x.f_len);
+    free(x.f_m);
+    free(x.f_d);
+    free(x.f_i);
+    free(res.v);
+    setup_free(&s);
+    return rc;
+}
+
+/* ---------------- Dispatch and driver (src/align.rs:81-211, lib.rs) ----- */
+
+/* src/align.rs:531-545 */
+int o_poa_inner(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                size_t cap, size_t *out_len) {
+    if (o_i32_scores_safe(p, g, beg_id, end_id, qlen))
+        return o_poa_simd_i32(p, g, beg_id, end_id, q, qlen, adaptive, band_b, out, cap, out_len);
+    return o_poa_scalar_i64(p, g, beg_id, end_id, q, qlen, adaptive, band_b, out, cap, out_len);
+}
+
+/* src/align.rs:192-211 */
+int o_poa(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+          const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out, size_t cap,
+          size_t *out_len) {
+    int rc = o_poa_inner(p, g, beg_id, end_id, q, qlen, adaptive, band_b, out, cap, out_len);
+    if (rc && p->f != 0) {
+        op_params full = *p;
+        full.f = 0;
+        rc = o_poa_inner(&full, g, beg_id, end_id, q, qlen, 0, band_b, out, cap, out_len);
+    }
+    return rc;
+}
+
+/* src/align.rs:81-98 (empty graph and unseeded branches). */
+int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen, ob_base *out,
+            size_t cap, size_t *out_len) {
+    if (g->n_nodes <= 2) {
+        *out_len = qlen + 1;
+        if (cap > 0) {
+            out[0].node = 0;
+            out[0].base = (int64_t)o_encode(p, 'N');
+            out[0].aligned = -1;
+        }
+        for (size_t k = 0; k < qlen && k + 1 < cap; k++) {
+            out[k + 1].node = -1;
+            out[k + 1].base = (int64_t)o_encode(p, q[k]);
+            out[k + 1].aligned = -1;
+        }
+        return 0;
+    }
+    if (!p->seeding) return o_poa(p, g, 0, 1, q, qlen, p->adaptive, p->b, out, cap, out_len);
+    return fail("minimizer seeding is not restated in the oracle");
+}
+
+/* src/lib.rs:184-204 (input order). */
+int o_extend_graph(const op_params *p, og_graph *g, const uint8_t *const *seqs,
+                   const size_t *lens, size_t nseq, size_t existing) {
+    if (p->progressive && existing == 0) return fail("guide order (-p) is not restated in the oracle");
+    size_t cap = 0;
+    ob_base *buf = NULL;
+    for (size_t id = existing; id < nseq; id++) {
+        size_t need = lens[id] + 8;
+        if (need > cap) {
+            cap = need * 2;
+            buf = xrealloc(buf, cap * sizeof(ob_base));
+        }
+        size_t len = 0;
+        if (o_align(p, g, seqs[id], lens[id], buf, cap, &len)) {
+            free(buf);
+            return 1;
+        }
+        if (len > cap) {
+            free(buf);
+            return fail("oracle path buffer overflow");
+        }
+        if (og_add_path(g, id, buf, len) || og_toposort(g, 0)) {
+            free(buf);
+            return 1;
+        }
+    }
+    free(buf);
+    return 0;
+}
+
+/* src/lib.rs:177-182 */
+og_graph *o_align_sequences(const op_params *p, const uint8_t *const *seqs,
+                            const size_t *lens, size_t nseq) {
+    if (o_validate(p)) return NULL;
+    og_graph *g = og_new(p);
+    if (o_extend_graph(p, g, seqs, lens, nseq, 0)) {
+        og_free(g);
+        return NULL;
+    }
+    return g;
+}
+
+/* ---------------- Job-parallel CPU baseline driver ---------------------- */
+
+static uint64_t fnv1a(uint64_t h, const void *data, size_t n) {
+    const uint8_t *b = data;
+    for (size_t k = 0; k < n; k++) {
+        h ^= b[k];
+        h *= UINT64_C(1099511628211);
+    }
+    return h;
+}
+
+typedef struct {
+    const op_params *p;
+    size_t njobs;
+    const uint64_t *job_off, *read_off;
+    const uint8_t *bases;
+    uint64_t *out_hash;
+    atomic_size_t next;
+    atomic_uint_fast64_t cells;
+} batch_ctx;
+
+static uint64_t run_job(const op_params *p, const batch_ctx *c, size_t job) {
+    size_t r0 = c->job_off[job], r1 = c->job_off[job + 1], nr = r1 - r0;
+    const uint8_t **seqs = xmalloc((nr + 1) * sizeof(uint8_t *));
+    size_t *lens = xmalloc((nr + 1) * sizeof(size_t));
+    char **names = xmalloc((nr + 1) * sizeof(char *));
+    for (size_t k = 0; k < nr; k++) {
+        seqs[k] = c->bases + c->read_off[r0 + k];
+        lens[k] = c->read_off[r0 + k + 1] - c->read_off[r0 + k];
+        names[k] = xmalloc(32);
+        snprintf(names[k], 32, "read%zu", k);
+    }
+    uint64_t h = 0;
+    og_graph *g = o_align_sequences(p, seqs, lens, nr);
+    if (g) {
+        size_t total = 0;
+        for (size_t k = 0; k < nr; k++) total += lens[k];
+        uint8_t *cons = xmalloc(g->n_nodes + 1);
+        size_t cl = og_consensus(g, p, cons, NULL, NULL, g->n_nodes);
+        char *msa = NULL;
+        if (og_msa(g, p, (const char *const *)names, NULL, nr, &msa) == 0) {
+            h = UINT64_C(14695981039346656037);
+            const char *hdr = ">consensus sequence\n";
+            h = fnv1a(h, hdr, strlen(hdr));
+            h = fnv1a(h, cons, cl);
+            h = fnv1a(h, "\n", 1);
+            h = fnv1a(h, msa, strlen(msa));
+            if (h == 0) h = 1;
+        }
+        free(msa);
+        free(cons);
+        og_free(g);
+        (void)total;
+    }
+    for (size_t k = 0; k < nr; k++) free(names[k]);
+    free(names);
+    free(seqs);
+    free(lens);
+    return h;
+}
+
+static void *batch_worker(void *arg) {
+    batch_ctx *c = arg;
+    g_cells = 0;
+    for (;;) {
+        size_t job = atomic_fetch_add(&c->next, 1);
+        if (job >= c->njobs) break;
+        c->out_hash[job] = run_job(c->p, c, job);
+    }
+    atomic_fetch_add(&c->cells, g_cells);
+    o_arena_clear();
+    return NULL;
+}
+
+uint64_t o_batch_run(const op_params *p, size_t njobs, const uint64_t *job_off,
+                     const uint64_t *read_off, const uint8_t *bases, int nthreads,
+                     uint64_t *out_hash) {
+    batch_ctx c;
+    c.p = p;
+    c.njobs = njobs;
+    c.job_off = job_off;
+    c.read_off = read_off;
+    c.bases = bases;
+    c.out_hash = out_hash;
+    atomic_init(&c.next, 0);
+    atomic_init(&c.cells, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = xmalloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, batch_worker, &c);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(th);
+    return (uint64_t)atomic_load(&c.cells);
+}

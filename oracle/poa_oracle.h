@@ -1,0 +1,139 @@
+/*
+ * TEST INFRASTRUCTURE ONLY. A plain-C restatement of the reference's
+ * sequence-to-DAG POA path (minipoa-rs: src/align.rs, src/graph.rs,
+ * src/lib.rs). Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it, and only as the checker. The product library
+ * never links or calls it.
+ *
+ * Every function names the reference file:line it restates.
+ */
+#ifndef POA_ORACLE_H
+#define POA_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/lib.rs:47-71 Params (every field the path and validate() read). */
+typedef struct {
+    int32_t match_score, mismatch, gap_open, gap_ext;
+    uint64_t b, f;
+    int32_t adaptive, seeding;
+    uint64_t k, w;
+    int32_t progressive;
+    uint64_t threads, win_band, win_max;
+    double mm_filter_ratio;
+    uint64_t max_occ, tr_dist;
+    uint8_t result, verbose;
+    int32_t is_rna;
+    /* Cargo feature `scalar-dp` (Cargo.toml:19-25): force the i64 kernel. */
+    int32_t scalar_dp;
+    uint32_t alphabet_len;
+    const uint8_t *alphabet;
+    const int32_t *matrix; /* alphabet_len * alphabet_len, row-major a*n+b */
+} op_params;
+
+/* src/lib.rs:140-145 AlignedBase; -1 encodes None. */
+typedef struct {
+    int64_t node;
+    int64_t base;
+    int64_t aligned;
+} ob_base;
+
+typedef struct og_graph og_graph;
+
+const char *o_last_error(void);
+
+/* src/graph.rs:30-178 */
+og_graph *og_new(const op_params *p);
+void og_free(og_graph *g);
+size_t og_add_node(og_graph *g, size_t base);
+void og_add_edge(og_graph *g, size_t from, size_t to);
+int og_add_path(og_graph *g, size_t seq_id, const ob_base *aln, size_t len);
+int og_toposort(og_graph *g, int grouped);
+size_t og_n_nodes(const og_graph *g);
+/* Node fields: rank, parent, base, coverage, n_in, n_out. */
+void og_node_info(const og_graph *g, size_t id, uint64_t out6[6]);
+int64_t og_node_aligned(const og_graph *g, size_t id, size_t base);
+void og_node_edges(const og_graph *g, size_t id, uint64_t *in_ids, uint64_t *in_w,
+                   uint64_t *out_ids, uint64_t *out_w);
+size_t og_rank_len(const og_graph *g);
+void og_rank(const og_graph *g, uint64_t *out);
+size_t og_n_paths(const og_graph *g);
+/* src/graph.rs:181-331; strings are malloc'd, release with o_free. */
+size_t og_consensus(const og_graph *g, const op_params *p, uint8_t *seq, uint64_t *ids,
+                    uint64_t *cov, size_t cap);
+int og_msa(og_graph *g, const op_params *p, const char *const *names,
+           const char *const *comments, size_t nseq, char **out);
+int og_gfa(const og_graph *g, const op_params *p, const char *const *names, size_t nseq,
+           char **out);
+void o_free(void *ptr);
+
+/* src/lib.rs:109-138 */
+size_t o_encode(const op_params *p, uint8_t base);
+int o_validate(const op_params *p);
+
+/* src/align.rs:459-491: out = I[j+1..j+8] given md[0..8] and carry I[j]. */
+void o_insertion_prefix(const int32_t md[8], int32_t carry, int32_t open, int32_t ext,
+                        int32_t out[8]);
+/* src/align.rs:493-529 */
+int o_i32_scores_safe(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                      size_t query_len);
+
+/*
+ * All alignment entry points write up to `cap` AlignedBase records to `out`,
+ * set *out_len (the full length, even if it exceeds cap) and return 0, or
+ * return 1 with the reference's error message in o_last_error().
+ */
+/* src/align.rs:81-188 (unseeded and empty-graph branches). */
+int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen, ob_base *out,
+            size_t cap, size_t *out_len);
+/* src/align.rs:192-211 */
+int o_poa(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+          const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out, size_t cap,
+          size_t *out_len);
+/* src/align.rs:531-545 */
+int o_poa_inner(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                size_t cap, size_t *out_len);
+/* src/align.rs:547-567 + 587-911 (8-lane i32 kernel). */
+int o_poa_simd_i32(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                   const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                   size_t cap, size_t *out_len);
+/* src/align.rs:913-1162 (i64 scalar kernel). */
+int o_poa_scalar_i64(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
+                     const uint8_t *q, size_t qlen, int adaptive, size_t band_b, ob_base *out,
+                     size_t cap, size_t *out_len);
+
+/* Cells (sum of computed row lengths) of DP calls on this thread since reset. */
+uint64_t o_cells(void);
+/* M at (n-1, m-1) of this thread's last DP fill (the global optimum). */
+int64_t o_last_score(void);
+void o_reset_cells(void);
+/* Capacity of the calling thread's i32 arena (src/align.rs:222-226). */
+size_t o_arena_capacity(void);
+void o_arena_clear(void);
+
+/* src/lib.rs:177-204 align_sequences (in input order; -p is not restated). */
+og_graph *o_align_sequences(const op_params *p, const uint8_t *const *seqs,
+                            const size_t *lens, size_t nseq);
+/* Continue progressive alignment of seqs[existing..] into g (src/lib.rs:184-204). */
+int o_extend_graph(const op_params *p, og_graph *g, const uint8_t *const *seqs,
+                   const size_t *lens, size_t nseq, size_t existing);
+
+/*
+ * Job-parallel CPU driver for the throughput baseline: job i aligns reads
+ * read_off[job_off[i]..job_off[i+1]] progressively and renders consensus and
+ * MSA (reads named read<k>, empty comments). out_hash[i] is the FNV-1a 64 of
+ * the consensus FASTA followed by the MSA text, or 0 on error. Returns cells.
+ */
+uint64_t o_batch_run(const op_params *p, size_t njobs, const uint64_t *job_off,
+                     const uint64_t *read_off, const uint8_t *bases, int nthreads,
+                     uint64_t *out_hash);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
