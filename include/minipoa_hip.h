@@ -1,0 +1,215 @@
+/*
+ * minipoa_hip -- MI355X (gfx950) backend for minipoa-rs's affine-gap
+ * sequence-to-DAG alignment DP, as a C ABI.
+ *
+ * Plain pointers, sizes and status codes only: no C++ types, no exceptions
+ * and no GPU types cross this boundary, so a Rust `-sys` crate can bind it
+ * (minipoa itself is `#![forbid(unsafe_code)]`, src/lib.rs:2). Every
+ * function returns MPOA_OK (0) or a status; the message of the calling
+ * thread's last failure is mpoa_last_error().
+ *
+ * usize fields of the reference are uint64_t here; Option<usize> is int64_t
+ * with -1 for None.
+ */
+#ifndef MINIPOA_HIP_H
+#define MINIPOA_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. Each DP error has the reference's message (src/align.rs),
+ * so `poa`'s banded -> full-DP retry fires on exactly the same cases. */
+typedef enum {
+    MPOA_OK = 0,
+    MPOA_ERR_ANCHOR_NOT_IN_GRAPH = 1,     /* align.rs:597-598 */
+    MPOA_ERR_ANCHORS_NOT_ORDERED = 2,     /* align.rs:602-603 */
+    MPOA_ERR_COORDINATE_LIMITS = 3,       /* align.rs:607-608 */
+    MPOA_ERR_NO_PATH = 4,                 /* align.rs:679-680 */
+    MPOA_ERR_BAND_NO_INTERSECT = 5,       /* align.rs:730-731 */
+    MPOA_ERR_BAND_EXCLUDES_SOURCE = 6,    /* align.rs:738-739 */
+    MPOA_ERR_TRACEBACK_LEFT_BAND = 7,     /* align.rs:1190-1191 */
+    MPOA_ERR_TRACEBACK_NO_ADVANCE = 8,    /* align.rs:1184-1185 */
+    MPOA_ERR_TRACEBACK_NO_PREDECESSOR = 9,/* align.rs:1304-1306 */
+    MPOA_ERR_ALLOC = 10,                  /* host or device allocation */
+    MPOA_ERR_INVALID_PARAMS = 11,         /* Params::validate, lib.rs:149-173 */
+    MPOA_ERR_GRAPH = 12,                  /* add_path / toposort / msa / gfa errors */
+    MPOA_ERR_DEVICE = 13,                 /* HIP runtime error */
+    MPOA_ERR_NO_DEVICE = 14,              /* no gfx950 device / library not loadable */
+    MPOA_ERR_UNSUPPORTED = 15,            /* a reference feature outside this backend */
+    MPOA_ERR_BUFFER_TOO_SMALL = 16,       /* *out_len holds the size needed */
+    MPOA_ERR_INVALID_ARGUMENT = 17
+} mpoa_status;
+
+/* src/lib.rs:47-71 Params. `alphabet` and `matrix` are borrowed for the
+ * duration of each call; matrix is alphabet_len^2, row-major (a*n + b). */
+typedef struct {
+    int32_t match_score, mismatch, gap_open, gap_ext;
+    uint64_t b, f;
+    int32_t adaptive, seeding;
+    uint64_t k, w;
+    int32_t progressive;
+    uint64_t threads, win_band, win_max;
+    double mm_filter_ratio;
+    uint64_t max_occ, tr_dist;
+    uint8_t result, verbose;
+    int32_t is_rna;
+    /* Cargo feature `scalar-dp` (Cargo.toml:19-25): force the i64 kernel. */
+    int32_t scalar_dp;
+    uint32_t alphabet_len;
+    const uint8_t *alphabet;
+    const int32_t *matrix;
+} mpoa_params;
+
+/* src/lib.rs:140-145 AlignedBase. */
+typedef struct {
+    int64_t node;    /* -1 = None */
+    int64_t base;    /* alphabet index */
+    int64_t aligned; /* -1 = None */
+} mpoa_aligned_base;
+
+/* Read-only flat view of a graph::Graph (src/graph.rs:6-27), node-indexed.
+ * Adjacency lists keep the reference's insertion order: it decides every
+ * tie in the band heuristics and in traceback. */
+typedef struct {
+    uint64_t n_nodes;
+    uint64_t alphabet_len;
+    const uint64_t *rank;       /* nodes[i].rank */
+    const uint64_t *parent;     /* nodes[i].parent */
+    const uint64_t *base;       /* nodes[i].base */
+    const uint64_t *in_off;     /* n_nodes + 1 offsets into in_node/in_weight */
+    const uint64_t *in_node;    /* nodes[i].incoming */
+    const uint64_t *in_weight;  /* nodes[i].in_weights */
+    const uint64_t *out_off;    /* n_nodes + 1 offsets into out_node/out_weight */
+    const uint64_t *out_node;   /* nodes[i].outgoing */
+    const uint64_t *out_weight; /* nodes[i].out_weights */
+    const int64_t *aligned;     /* n_nodes * alphabet_len; nodes[i].aligned[b], -1 = None */
+    uint64_t n_order;           /* graph.rank.len() */
+    const uint64_t *order;      /* graph.rank: rank -> node id */
+} mpoa_graph_view;
+
+typedef struct mpoa_ctx mpoa_ctx;
+typedef struct mpoa_graph mpoa_graph;
+
+/* ---- backend selection (src/lib.rs:15-37 dp_backend) ---- */
+
+/* "hip-gfx950 (wave64 x i32)"; printed where the reference prints its
+ * `DP backend:` line (src/cli.rs:112-121). */
+const char *mpoa_backend_name(void);
+/* The calling thread's last failure message. */
+const char *mpoa_last_error(void);
+
+/* A context owns one device's stream, device arenas and staging buffers. It
+ * plays the part of the reference's per-worker DP_ARENA (src/align.rs:222-226):
+ * use one context per host thread. */
+int mpoa_ctx_create(int device, mpoa_ctx **out);
+void mpoa_ctx_destroy(mpoa_ctx *ctx);
+/* Bytes of device memory the context currently holds. */
+uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx);
+
+/* ---- drop-in DP entry points (src/align.rs) ---- */
+
+/* src/align.rs:192-211 `poa`: banded attempt, then an exact full-DP retry
+ * with f = 0 when the banded attempt fails and f != 0. Writes up to out_cap
+ * records; *out_len is the full length (MPOA_ERR_BUFFER_TOO_SMALL if it
+ * exceeds out_cap). *out_score (may be NULL) receives M at (n-1, m-1). */
+int mpoa_poa(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+             uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+             int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
+             uint64_t *out_len, int64_t *out_score);
+/* src/align.rs:531-545 `poa_inner`: one attempt, no retry; the i32 kernel
+ * under the score guard (src/align.rs:493-529), else the i64 kernel. */
+int mpoa_poa_inner(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                   uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                   int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
+                   uint64_t *out_len, int64_t *out_score);
+/* Force one kernel: score_bits 32 or 64 (the i32 fast path and the i64 path
+ * of src/align.rs:587-911 and 913-1162). The i32 kernel refuses inputs that
+ * fail the score guard (MPOA_ERR_INVALID_ARGUMENT). */
+int mpoa_poa_kernel(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                    uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                    int adaptive, uint64_t band_b, int score_bits, mpoa_aligned_base *out,
+                    uint64_t out_cap, uint64_t *out_len, int64_t *out_score);
+/* src/align.rs:493-529 */
+int mpoa_i32_scores_safe(const mpoa_params *params, const mpoa_graph_view *graph,
+                         uint64_t beg_id, uint64_t end_id, uint64_t query_len);
+/* src/align.rs:81-98 `align` (empty-graph shortcut and unseeded DP). Seeded
+ * alignment (params.seeding) returns MPOA_ERR_UNSUPPORTED. */
+int mpoa_align(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+               const uint8_t *query, uint64_t qlen, mpoa_aligned_base *out, uint64_t out_cap,
+               uint64_t *out_len);
+
+/* ---- library-owned graph (src/graph.rs), for callers without one ---- */
+
+mpoa_graph *mpoa_graph_new(const mpoa_params *params);                 /* graph.rs:30-40 */
+void mpoa_graph_free(mpoa_graph *g);
+uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base);            /* graph.rs:42-61 */
+void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to);   /* graph.rs:63-80 */
+int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *alignment,
+                        uint64_t len);                                 /* graph.rs:82-122 */
+int mpoa_graph_toposort(mpoa_graph *g, int grouped);                   /* graph.rs:124-178 */
+/* A view of the graph, valid until the graph is next modified or freed. */
+int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out);
+uint64_t mpoa_graph_n_paths(const mpoa_graph *g);
+/* graph.rs:181-237: writes up to cap bases (decoded) / node ids / coverage
+ * (any pointer may be NULL) and returns the consensus length. */
+uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, uint8_t *seq,
+                              uint64_t *ids, uint64_t *coverage, uint64_t cap);
+/* graph.rs:239-331: text is written to out (up to cap bytes, no terminator);
+ * *out_len is the full size. names/comments are indexed by sequence id. */
+int mpoa_graph_msa(mpoa_graph *g, const mpoa_params *params, const char *const *names,
+                   const char *const *comments, uint64_t nseq, char *out, uint64_t cap,
+                   uint64_t *out_len);
+int mpoa_graph_gfa(const mpoa_graph *g, const mpoa_params *params, const char *const *names,
+                   uint64_t nseq, char *out, uint64_t cap, uint64_t *out_len);
+
+/* src/lib.rs:149-173 */
+int mpoa_params_validate(const mpoa_params *params);
+/* src/lib.rs:177-204 `align_sequences` (input order) on the device. */
+int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
+                         const uint64_t *lens, uint64_t nseq, mpoa_graph **out);
+/* src/lib.rs:184-204 `extend_graph` from sequence index `existing` (the -i
+ * incremental mode continues a loaded graph this way). */
+int mpoa_extend_graph(mpoa_ctx *ctx, const mpoa_params *params, mpoa_graph *g,
+                      const uint8_t *const *seqs, const uint64_t *lens, uint64_t nseq,
+                      uint64_t existing);
+
+/* ---- batched progressive POA (additive; no reference counterpart) ----
+ * Job i aligns reads read_off[job_off[i]] .. read_off[job_off[i+1]] of
+ * `bases` progressively (src/lib.rs:184-204 per job); every DP of one
+ * progressive step of all jobs runs in one device launch. Results keep job
+ * order (the ordering contract of src/parallel.rs:46-67). */
+typedef struct {
+    uint64_t cells;        /* DP cells computed (sum of row band widths) */
+    uint64_t dp_calls;     /* DP alignments run */
+    uint64_t retries;      /* banded failures re-run as full DP */
+    uint64_t i64_calls;    /* alignments routed to the i64 kernel */
+    uint64_t pred_cells;   /* sum over rows and predecessors of band overlaps */
+    uint64_t fill_launches;/* fill kernel launches */
+    double device_ms;      /* fill + traceback kernel time (HIP events) */
+    double fill_ms;        /* fill kernel time alone (HIP events, summed) */
+    double host_ms;        /* host preparation + fusion time */
+    double wall_ms;
+} mpoa_batch_stats;
+
+typedef struct mpoa_batch mpoa_batch;
+/* Runs all jobs to completion. names: reads are named read<k> within a job. */
+int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
+                   const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
+                   mpoa_batch **out);
+void mpoa_batch_free(mpoa_batch *b);
+int mpoa_batch_stats_get(const mpoa_batch *b, mpoa_batch_stats *out);
+/* Per-job status (MPOA_OK or the job's first error). */
+int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job);
+/* Per-job rendered output: the `-r0` consensus FASTA (mode 0), the `-r1`
+ * MSA (mode 1) or the `-r2` GFA (mode 2), exactly as src/lib.rs:233-241
+ * would print them. */
+int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out, uint64_t cap,
+                          uint64_t *out_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,564 @@
+// C ABI of the gfx950 POA backend (include/minipoa_hip.h): the drop-in DP
+// entry points, the library-owned graph and the batched progressive engine.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "graph.hpp"
+#include "minipoa_hip.h"
+#include "plan.hpp"
+#include "runtime.hpp"
+
+using namespace mpoa;
+
+struct mpoa_ctx {
+    Ctx *c = nullptr;
+};
+struct mpoa_graph {
+    Graph g;
+    explicit mpoa_graph(const Params &p) : g(p) {}
+};
+struct mpoa_batch {
+    std::vector<int> status;
+    std::vector<std::string> err, out0, out1;
+    std::vector<std::unique_ptr<Graph>> graphs;
+    std::vector<uint32_t> nreads;
+    Params params;
+    mpoa_batch_stats stats{};
+};
+
+namespace {
+
+int fail(int status, const std::string &msg) {
+    set_error(msg);
+    return status;
+}
+
+// Reference-visible DP errors (src/align.rs returns Err for each of them), on
+// which `poa` retries with the exact full matrix.
+bool is_dp_error(int s) { return s >= MPOA_ERR_ANCHOR_NOT_IN_GRAPH && s <= MPOA_ERR_ALLOC; }
+
+template <class G>
+int poa_once(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end, const uint8_t *q,
+             uint64_t qlen, bool adaptive, uint64_t band_b, int force_bits,
+             std::vector<AlignedBase> &out, int64_t *score) {
+    JobPlan J;
+    plan_job(p, g, beg, end, q, qlen, adaptive, band_b, force_bits, J);
+    if (J.status != MPOA_OK) return fail(J.status, J.err);
+    std::vector<JobPlan *> plans{&J};
+    std::vector<DevResult> res;
+    std::vector<uint32_t> tb;
+    std::vector<uint64_t> off;
+    int rc = run_plans(ctx, p, plans, res, tb, off, nullptr);
+    if (rc != MPOA_OK) return rc;
+    std::string msg;
+    int s = device_status(res[0], msg);
+    if (s != MPOA_OK) return fail(s, msg);
+    assemble(g, J, tb.data() + off[0], out);
+    if (score) *score = res[0].score;
+    return MPOA_OK;
+}
+
+// src/align.rs:192-211
+template <class G>
+int poa_retry(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end,
+              const uint8_t *q, uint64_t qlen, bool adaptive, uint64_t band_b,
+              std::vector<AlignedBase> &out, int64_t *score) {
+    int rc = poa_once(ctx, p, g, beg, end, q, qlen, adaptive, band_b, 0, out, score);
+    if (rc != MPOA_OK && p.f != 0 && is_dp_error(rc)) {
+        Params full = p;
+        full.f = 0;
+        rc = poa_once(ctx, full, g, beg, end, q, qlen, false, band_b, 0, out, score);
+    }
+    return rc;
+}
+
+// src/align.rs:81-98
+template <class G>
+int align_one(Ctx &ctx, const Params &p, const G &g, const uint8_t *q, uint64_t qlen,
+              std::vector<AlignedBase> &out) {
+    if (g.size() <= 2) {
+        out.clear();
+        out.push_back(AlignedBase{0, int64_t(p.encode('N')), -1});
+        for (uint64_t k = 0; k < qlen; k++) out.push_back(AlignedBase{-1, int64_t(p.encode(q[k])), -1});
+        return MPOA_OK;
+    }
+    if (p.seeding)
+        return fail(MPOA_ERR_UNSUPPORTED, "minimizer-seeded windows are not implemented by this backend");
+    return poa_retry(ctx, p, g, 0, 1, q, qlen, p.adaptive, p.b, out, nullptr);
+}
+
+int emit(const std::vector<AlignedBase> &v, mpoa_aligned_base *out, uint64_t cap, uint64_t *len) {
+    if (len) *len = v.size();
+    if (v.size() > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "output buffer is too small");
+    if (!v.empty()) memcpy(out, v.data(), v.size() * sizeof(AlignedBase));
+    return MPOA_OK;
+}
+
+int emit_text(const std::string &s, char *out, uint64_t cap, uint64_t *len) {
+    if (len) *len = s.size();
+    if (s.size() > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "output buffer is too small");
+    if (!s.empty()) memcpy(out, s.data(), s.size());
+    return MPOA_OK;
+}
+
+bool check_params(const mpoa_params *params, Params &p, int &rc) {
+    if (!params) {
+        rc = fail(MPOA_ERR_INVALID_ARGUMENT, "params is NULL");
+        return false;
+    }
+    p = Params::from_c(*params);
+    std::string v = p.validate();
+    if (!v.empty()) {
+        rc = fail(MPOA_ERR_INVALID_PARAMS, v);
+        return false;
+    }
+    return true;
+}
+
+bool view_ok(const mpoa_graph_view *g, const Params &p, int &rc) {
+    if (!g || !g->rank || !g->base || !g->in_off || !g->out_off || !g->order || !g->parent ||
+        !g->aligned) {
+        rc = fail(MPOA_ERR_INVALID_ARGUMENT, "incomplete graph view");
+        return false;
+    }
+    if (g->alphabet_len != p.alpha()) {
+        rc = fail(MPOA_ERR_INVALID_ARGUMENT, "graph alphabet differs from params");
+        return false;
+    }
+    return true;
+}
+
+std::vector<std::string> read_names(size_t n) {
+    std::vector<std::string> v(n);
+    for (size_t k = 0; k < n; k++) v[k] = "read" + std::to_string(k);
+    return v;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpoa_backend_name(void) { return "hip-gfx950 (wave64 x i32)"; }
+const char *mpoa_last_error(void) { return last_error(); }
+
+int mpoa_ctx_create(int device, mpoa_ctx **out) {
+    if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    Ctx *c = nullptr;
+    int rc = ctx_create(device, &c);
+    if (rc != MPOA_OK) return rc;
+    *out = new mpoa_ctx{c};
+    return MPOA_OK;
+}
+
+void mpoa_ctx_destroy(mpoa_ctx *ctx) {
+    if (!ctx) return;
+    delete ctx->c;
+    delete ctx;
+}
+
+uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) {
+    if (!ctx) return 0;
+    const Ctx &c = *ctx->c;
+    return c.d_in.cap + c.d_res.cap + c.d_meta.cap + c.d_adapt.cap + c.d_tb.cap + c.d_arena.cap;
+}
+
+int mpoa_params_validate(const mpoa_params *params) {
+    Params p;
+    int rc = MPOA_OK;
+    check_params(params, p, rc);
+    return rc;
+}
+
+int mpoa_i32_scores_safe(const mpoa_params *params, const mpoa_graph_view *graph,
+                         uint64_t beg_id, uint64_t end_id, uint64_t query_len) {
+    if (!params || !graph) return 0;
+    Params p = Params::from_c(*params);
+    return i32_scores_safe(p, ViewAcc{*graph}, beg_id, end_id, query_len) ? 1 : 0;
+}
+
+static int poa_entry(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                     uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                     int adaptive, uint64_t band_b, int mode, mpoa_aligned_base *out,
+                     uint64_t out_cap, uint64_t *out_len, int64_t *out_score) {
+    if (!ctx || (!query && qlen)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc) || !view_ok(graph, p, rc)) return rc;
+    ViewAcc g{*graph};
+    std::vector<AlignedBase> v;
+    if (mode == 0) {
+        rc = poa_retry(*ctx->c, p, g, beg_id, end_id, query, qlen, adaptive != 0, band_b, v, out_score);
+    } else if (mode == 1) {
+        rc = poa_once(*ctx->c, p, g, beg_id, end_id, query, qlen, adaptive != 0, band_b, 0, v, out_score);
+    } else {
+        if (mode == 32 && !i32_scores_safe(p, g, beg_id, end_id, qlen))
+            return fail(MPOA_ERR_INVALID_ARGUMENT, "scores exceed the i32 guard");
+        rc = poa_once(*ctx->c, p, g, beg_id, end_id, query, qlen, adaptive != 0, band_b, mode, v,
+                      out_score);
+    }
+    if (rc != MPOA_OK) return rc;
+    return emit(v, out, out_cap, out_len);
+}
+
+int mpoa_poa(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+             uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen, int adaptive,
+             uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap, uint64_t *out_len,
+             int64_t *out_score) {
+    return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, 0, out,
+                     out_cap, out_len, out_score);
+}
+
+int mpoa_poa_inner(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                   uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                   int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
+                   uint64_t *out_len, int64_t *out_score) {
+    return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, 1, out,
+                     out_cap, out_len, out_score);
+}
+
+int mpoa_poa_kernel(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                    uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                    int adaptive, uint64_t band_b, int score_bits, mpoa_aligned_base *out,
+                    uint64_t out_cap, uint64_t *out_len, int64_t *out_score) {
+    if (score_bits != 32 && score_bits != 64)
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "score_bits must be 32 or 64");
+    return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, score_bits,
+                     out, out_cap, out_len, out_score);
+}
+
+int mpoa_align(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+               const uint8_t *query, uint64_t qlen, mpoa_aligned_base *out, uint64_t out_cap,
+               uint64_t *out_len) {
+    if (!ctx || (!query && qlen)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc) || !view_ok(graph, p, rc)) return rc;
+    std::vector<AlignedBase> v;
+    rc = align_one(*ctx->c, p, ViewAcc{*graph}, query, qlen, v);
+    if (rc != MPOA_OK) return rc;
+    return emit(v, out, out_cap, out_len);
+}
+
+// ---- library-owned graph ----
+
+mpoa_graph *mpoa_graph_new(const mpoa_params *params) {
+    if (!params) return nullptr;
+    Params p = Params::from_c(*params);
+    if (p.alpha() == 0) {
+        set_error("invalid scoring matrix dimensions");
+        return nullptr;
+    }
+    return new mpoa_graph(p);
+}
+
+void mpoa_graph_free(mpoa_graph *g) { delete g; }
+
+uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base) { return g->g.add_node(uint32_t(base)); }
+
+void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) {
+    g->g.add_edge(uint32_t(from), uint32_t(to));
+}
+
+int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *a, uint64_t len) {
+    std::string e = g->g.add_path(seq_id, a, len);
+    return e.empty() ? MPOA_OK : fail(MPOA_ERR_GRAPH, e);
+}
+
+int mpoa_graph_toposort(mpoa_graph *g, int grouped) {
+    std::string e = g->g.toposort(grouped != 0);
+    return e.empty() ? MPOA_OK : fail(MPOA_ERR_GRAPH, e);
+}
+
+int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out) {
+    if (!g || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = g->g.view();
+    return MPOA_OK;
+}
+
+uint64_t mpoa_graph_n_paths(const mpoa_graph *g) { return g->g.paths().size(); }
+
+uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, uint8_t *seq,
+                              uint64_t *ids, uint64_t *coverage, uint64_t cap) {
+    Params p = Params::from_c(*params);
+    std::vector<uint32_t> v;
+    g->g.consensus(v);
+    for (size_t k = 0; k < v.size() && k < cap; k++) {
+        const Graph::Node &n = g->g.node(v[k]);
+        if (seq) seq[k] = p.decode(n.base);
+        if (ids) ids[k] = v[k];
+        if (coverage) {
+            uint64_t s = 0;
+            for (uint32_t w : n.out_w) s += w;
+            coverage[k] = std::max<uint64_t>(n.coverage, s);
+        }
+    }
+    return v.size();
+}
+
+int mpoa_graph_msa(mpoa_graph *g, const mpoa_params *params, const char *const *names,
+                   const char *const *comments, uint64_t nseq, char *out, uint64_t cap,
+                   uint64_t *out_len) {
+    Params p = Params::from_c(*params);
+    std::vector<std::string> nm(nseq), cm(nseq);
+    for (uint64_t k = 0; k < nseq; k++) {
+        nm[k] = names && names[k] ? names[k] : "";
+        cm[k] = comments && comments[k] ? comments[k] : "";
+    }
+    std::string text;
+    std::string e = g->g.msa(p, nm, cm, text);
+    if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+    return emit_text(text, out, cap, out_len);
+}
+
+int mpoa_graph_gfa(const mpoa_graph *g, const mpoa_params *params, const char *const *names,
+                   uint64_t nseq, char *out, uint64_t cap, uint64_t *out_len) {
+    Params p = Params::from_c(*params);
+    std::vector<std::string> nm(nseq);
+    for (uint64_t k = 0; k < nseq; k++) nm[k] = names && names[k] ? names[k] : "";
+    std::string text;
+    std::string e = g->g.gfa(p, nm, text);
+    if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+    return emit_text(text, out, cap, out_len);
+}
+
+int mpoa_extend_graph(mpoa_ctx *ctx, const mpoa_params *params, mpoa_graph *g,
+                      const uint8_t *const *seqs, const uint64_t *lens, uint64_t nseq,
+                      uint64_t existing) {
+    if (!ctx || !g) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    if (p.progressive && existing == 0)
+        return fail(MPOA_ERR_UNSUPPORTED, "guide-tree order (-p) is not implemented by this backend");
+    std::vector<AlignedBase> path;
+    for (uint64_t id = existing; id < nseq; id++) {
+        rc = align_one(*ctx->c, p, GraphAcc{g->g}, seqs[id], lens[id], path);
+        if (rc != MPOA_OK) return rc;
+        std::string e = g->g.add_path(id, path.data(), path.size());
+        if (e.empty()) e = g->g.toposort(false);
+        if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+    }
+    return MPOA_OK;
+}
+
+int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
+                         const uint64_t *lens, uint64_t nseq, mpoa_graph **out) {
+    if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    auto *g = new mpoa_graph(p);
+    rc = mpoa_extend_graph(ctx, params, g, seqs, lens, nseq, 0);
+    if (rc != MPOA_OK) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return MPOA_OK;
+}
+
+// ---- batched progressive POA ----
+
+int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
+                   const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
+                   mpoa_batch **out) {
+    if (!ctx || !out || (njobs && (!job_off || !read_off || !bases)))
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    if (p.seeding || p.progressive)
+        return fail(MPOA_ERR_UNSUPPORTED, "seeded or guide-ordered batches are not implemented");
+    auto t_all = std::chrono::steady_clock::now();
+    Ctx &c = *ctx->c;
+    auto B = std::make_unique<mpoa_batch>();
+    B->params = p;
+    B->status.assign(njobs, MPOA_OK);
+    B->err.assign(njobs, "");
+    B->out0.assign(njobs, "");
+    B->out1.assign(njobs, "");
+    B->graphs.resize(njobs);
+    B->nreads.resize(njobs);
+    uint32_t max_reads = 0;
+    for (uint64_t j = 0; j < njobs; j++) {
+        B->graphs[j] = std::make_unique<Graph>(p);
+        B->nreads[j] = uint32_t(job_off[j + 1] - job_off[j]);
+        max_reads = std::max(max_reads, B->nreads[j]);
+    }
+    Params full = p;
+    full.f = 0;
+    RunStats st;
+    double host_ms = 0;
+    std::vector<uint64_t> active;
+    std::vector<JobPlan> plans;
+    std::vector<JobPlan *> dp;
+    std::vector<std::vector<AlignedBase>> paths(njobs);
+    for (uint32_t s = 0; s < max_reads; s++) {
+        auto t0 = std::chrono::steady_clock::now();
+        active.clear();
+        for (uint64_t j = 0; j < njobs; j++)
+            if (s < B->nreads[j] && B->status[j] == MPOA_OK) active.push_back(j);
+        plans.assign(active.size(), JobPlan());
+        std::vector<char> shortcut(active.size(), 0);
+        c.pool->parallel_for(active.size(), [&](size_t k) {
+            uint64_t j = active[k];
+            uint64_t r = job_off[j] + s;
+            const uint8_t *q = bases + read_off[r];
+            uint64_t qlen = read_off[r + 1] - read_off[r];
+            Graph &g = *B->graphs[j];
+            if (g.size() <= 2) {
+                shortcut[k] = 1;
+                auto &v = paths[j];
+                v.clear();
+                v.push_back(AlignedBase{0, int64_t(p.encode('N')), -1});
+                for (uint64_t t = 0; t < qlen; t++) v.push_back(AlignedBase{-1, int64_t(p.encode(q[t])), -1});
+                return;
+            }
+            plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, plans[k]);
+        });
+        dp.clear();
+        std::vector<size_t> dp_k;
+        for (size_t k = 0; k < active.size(); k++)
+            if (!shortcut[k]) {
+                dp.push_back(&plans[k]);
+                dp_k.push_back(k);
+            }
+        host_ms += ms_since(t0);
+        std::vector<DevResult> res;
+        std::vector<uint32_t> tb;
+        std::vector<uint64_t> tb_off;
+        rc = run_plans(c, p, dp, res, tb, tb_off, &st);
+        if (rc != MPOA_OK) return rc;
+        // `poa` retry (src/align.rs:202-207): failed banded jobs re-run with f = 0.
+        std::vector<size_t> retry;
+        for (size_t t = 0; t < dp.size(); t++) {
+            bool failed = dp[t]->status != MPOA_OK || res[t].status != kOk;
+            if (failed && p.f != 0) retry.push_back(t);
+        }
+        std::vector<JobPlan> rplans(retry.size());
+        std::vector<DevResult> rres;
+        std::vector<uint32_t> rtb;
+        std::vector<uint64_t> rtb_off;
+        if (!retry.empty()) {
+            auto t1 = std::chrono::steady_clock::now();
+            c.pool->parallel_for(retry.size(), [&](size_t u) {
+                size_t k = dp_k[retry[u]];
+                uint64_t j = active[k];
+                uint64_t r = job_off[j] + s;
+                plan_job(full, GraphAcc{*B->graphs[j]}, 0, 1, bases + read_off[r],
+                         read_off[r + 1] - read_off[r], false, p.b, 0, rplans[u]);
+            });
+            host_ms += ms_since(t1);
+            std::vector<JobPlan *> rp;
+            for (auto &x : rplans) rp.push_back(&x);
+            rc = run_plans(c, full, rp, rres, rtb, rtb_off, &st);
+            if (rc != MPOA_OK) return rc;
+            st.retries += retry.size();
+        }
+        std::vector<int> retry_of(dp.size(), -1);
+        for (size_t u = 0; u < retry.size(); u++) retry_of[retry[u]] = int(u);
+        auto t2 = std::chrono::steady_clock::now();
+        c.pool->parallel_for(active.size(), [&](size_t k) {
+            uint64_t j = active[k];
+            Graph &g = *B->graphs[j];
+            if (!shortcut[k]) {
+                size_t t = size_t(std::lower_bound(dp_k.begin(), dp_k.end(), k) - dp_k.begin());
+                const JobPlan *J = dp[t];
+                const DevResult *R = &res[t];
+                const uint32_t *T = J->status == MPOA_OK ? tb.data() + tb_off[t] : nullptr;
+                if (retry_of[t] >= 0) {
+                    int u = retry_of[t];
+                    J = &rplans[size_t(u)];
+                    R = &rres[size_t(u)];
+                    T = J->status == MPOA_OK ? rtb.data() + rtb_off[size_t(u)] : nullptr;
+                }
+                if (J->status != MPOA_OK) {
+                    B->status[j] = J->status;
+                    B->err[j] = J->err;
+                    return;
+                }
+                std::string msg;
+                int sst = device_status(*R, msg);
+                if (sst != MPOA_OK) {
+                    B->status[j] = sst;
+                    B->err[j] = msg;
+                    return;
+                }
+                assemble(GraphAcc{g}, *J, T, paths[j]);
+            }
+            std::string e = g.add_path(s, paths[j].data(), paths[j].size());
+            if (e.empty()) e = g.toposort(false);
+            if (!e.empty()) {
+                B->status[j] = MPOA_ERR_GRAPH;
+                B->err[j] = e;
+            }
+        });
+        host_ms += ms_since(t2);
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    c.pool->parallel_for(njobs, [&](size_t j) {
+        if (B->status[j] != MPOA_OK) return;
+        Graph &g = *B->graphs[j];
+        B->out0[j] = ">consensus sequence\n" + g.consensus_bytes(p) + "\n";
+        std::vector<std::string> names = read_names(B->nreads[j]), comments(B->nreads[j]);
+        std::string e = g.msa(p, names, comments, B->out1[j]);
+        if (!e.empty()) {
+            B->status[j] = MPOA_ERR_GRAPH;
+            B->err[j] = e;
+        }
+    });
+    host_ms += ms_since(t3);
+    B->stats.cells = st.cells;
+    B->stats.dp_calls = st.dp_calls;
+    B->stats.retries = st.retries;
+    B->stats.i64_calls = st.i64_calls;
+    B->stats.pred_cells = st.pred_cells;
+    B->stats.fill_launches = st.fill_launches;
+    B->stats.device_ms = st.device_ms;
+    B->stats.fill_ms = st.fill_ms;
+    B->stats.host_ms = host_ms;
+    B->stats.wall_ms = ms_since(t_all);
+    *out = B.release();
+    return MPOA_OK;
+}
+
+void mpoa_batch_free(mpoa_batch *b) { delete b; }
+
+int mpoa_batch_stats_get(const mpoa_batch *b, mpoa_batch_stats *out) {
+    if (!b || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = b->stats;
+    return MPOA_OK;
+}
+
+int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job) {
+    if (!b || job >= b->status.size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "job out of range");
+    if (b->status[job] != MPOA_OK) set_error(b->err[job]);
+    return b->status[job];
+}
+
+int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out, uint64_t cap,
+                          uint64_t *out_len) {
+    if (!b || job >= b->status.size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "job out of range");
+    if (b->status[job] != MPOA_OK) return fail(b->status[job], b->err[job]);
+    if (mode == 0) return emit_text(b->out0[job], out, cap, out_len);
+    if (mode == 1) return emit_text(b->out1[job], out, cap, out_len);
+    if (mode == 2) {
+        std::string text;
+        std::string e = b->graphs[job]->gfa(b->params, read_names(b->nreads[job]), text);
+        if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+        return emit_text(text, out, cap, out_len);
+    }
+    return fail(MPOA_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+}
+
+}  // extern "C"

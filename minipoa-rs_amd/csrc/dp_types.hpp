@@ -1,0 +1,89 @@
+// Device-side job layout shared by the host planner and the HIP kernels.
+//
+// One DP job = one `poa_inner` call (src/align.rs:531-545): a graph rank range
+// [beg, end] (n rows, in rank order) against one query (m = qlen + 2 columns).
+// Everything the kernels read is flattened into pools in rank order: per-row
+// records, the row's in-range predecessors (in `incoming` order, which decides
+// traceback ties) and in-range successors (adaptive band propagation).
+#pragma once
+#include <cstdint>
+
+namespace mpoa {
+
+constexpr int32_t kNeg32 = -(1 << 29);               // src/align.rs:216
+constexpr int64_t kNeg64 = -(int64_t(1) << 50);      // src/align.rs:16
+constexpr int32_t kHeadNone = INT32_MIN;             // head/tail == NEG
+constexpr uint32_t kTbInsert = 0xFFFFFFFFu;          // traceback column was an insertion
+
+enum JobFlags : uint32_t {
+    kFull = 1u << 0,        // params.f == 0: every row spans [0, m]
+    kAdaptive = 1u << 1,    // f != 0 and (adaptive || params.adaptive)
+    kGapOpenZero = 1u << 2, // params.gap_open == 0 (traceback state rule)
+};
+
+// Per row (n + 1 entries per job; entry n carries the end offsets).
+struct RowIn {
+    uint32_t pred_beg;  // into the job's predecessor list
+    uint32_t succ_beg;  // into the job's successor list
+    int32_t head;       // most-supported distance from beg (kHeadNone = NEG)
+    int32_t tail;       // most-supported distance to end (kHeadNone = NEG)
+};
+
+struct RowAux {
+    uint32_t thresh;    // sum(in_weights) / 10: preferred-match filter
+    uint32_t base;      // node base (alphabet index)
+};
+
+struct PredE {
+    uint32_t row;       // predecessor row (rank - beg), head != NEG
+    uint32_t weight;    // in_weight of that edge
+};
+
+// Row geometry written by the fill kernel, read by traceback.
+struct RowMeta {
+    uint32_t start;     // first column (multiple of 8)
+    uint32_t len;       // width (multiple of 8); 0 for skipped rows
+    uint64_t off;       // element offset of [M len][D len][I len] in the arena
+};
+
+struct DevJob {
+    uint64_t rows;      // RowIn/RowAux index of row 0
+    uint64_t preds;     // PredE index of the job's first entry
+    uint64_t succs;     // successor index
+    uint64_t seq;       // byte offset of the encoded, padded query
+    uint64_t matrix;    // int32 index of the alpha x alpha matrix
+    uint64_t arena;     // element offset of the job's arena
+    uint64_t arena_cap; // elements available
+    uint64_t rowmeta;   // RowMeta index of row 0
+    uint64_t adapt;     // int64 index of the 7 x n adaptive state
+    uint64_t tb;        // uint32 index of the m-entry traceback record
+    uint32_t n, m, padded, alpha;
+    int64_t band;
+    int32_t o, e;       // gap_open + gap_ext, gap_ext (src/align.rs:621-622)
+    uint32_t unknown;   // encode('N')
+    int32_t score_nn;   // score(N, N)
+    uint32_t flags;
+    uint32_t pad_;
+};
+
+enum DevStatus : int32_t {
+    kOk = 0,
+    kBandNoIntersect = 5,
+    kBandExcludesSource = 6,
+    kTbLeftBand = 7,
+    kTbNoAdvance = 8,
+    kTbNoPredecessor = 9,
+    kArenaOverflow = 100,
+};
+
+struct DevResult {
+    int32_t status;
+    uint32_t err_i, err_j;
+    uint32_t pad_;
+    int64_t score;      // M at (n-1, m-1)
+    uint64_t cells;     // sum of computed row widths
+    uint64_t arena_used;
+    uint64_t pred_cells;  // sum over rows and predecessors of the band overlap
+};
+
+}  // namespace mpoa

@@ -1,0 +1,395 @@
+// Host POA graph: fusion, LIFO-Kahn ranking and rendering with the exact
+// ordering rules of minipoa-rs src/graph.rs (cited per function).
+#include "graph.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+
+namespace mpoa {
+
+Params Params::from_c(const mpoa_params &c) {
+    Params p;
+    p.match_score = c.match_score;
+    p.mismatch = c.mismatch;
+    p.gap_open = c.gap_open;
+    p.gap_ext = c.gap_ext;
+    p.b = c.b;
+    p.f = c.f;
+    p.adaptive = c.adaptive != 0;
+    p.seeding = c.seeding != 0;
+    p.progressive = c.progressive != 0;
+    p.k = c.k;
+    p.w = c.w;
+    p.threads = c.threads;
+    p.win_band = c.win_band;
+    p.win_max = c.win_max;
+    p.mm_filter_ratio = c.mm_filter_ratio;
+    p.max_occ = c.max_occ;
+    p.tr_dist = c.tr_dist;
+    p.result = c.result;
+    p.verbose = c.verbose;
+    p.is_rna = c.is_rna != 0;
+    p.scalar_dp = c.scalar_dp != 0;
+    if (c.alphabet && c.alphabet_len) p.alphabet.assign(c.alphabet, c.alphabet + c.alphabet_len);
+    p.matrix_valid = c.matrix != nullptr || c.alphabet_len == 0;
+    if (c.matrix && c.alphabet_len)
+        p.matrix.assign(c.matrix, c.matrix + size_t(c.alphabet_len) * c.alphabet_len);
+    return p;
+}
+
+uint32_t Params::encode(uint8_t c) const {
+    if (c >= 'a' && c <= 'z') c = uint8_t(c - 'a' + 'A');
+    if (alphabet.size() <= 5 && c == 'U') c = 'T';
+    for (size_t i = 0; i < alphabet.size(); i++)
+        if (alphabet[i] == c) return uint32_t(i);
+    for (size_t i = 0; i < alphabet.size(); i++)
+        if (alphabet[i] == 'N') return uint32_t(i);
+    return uint32_t(alphabet.size() - 1);
+}
+
+uint8_t Params::decode(uint32_t b) const {
+    uint8_t c = alphabet[b];
+    return (is_rna && alphabet.size() <= 5 && c == 'T') ? uint8_t('U') : c;
+}
+
+std::string Params::validate() const {
+    size_t n = alphabet.size();
+    if (n == 0 || n > 256 || !matrix_valid || matrix.size() != n * n)
+        return "invalid scoring matrix dimensions";
+    if (threads == 0) return "thread count must be positive";
+    if (!(k >= 1 && k <= 31) || !(w >= 1 && w < 32768))
+        return "k must be 1..31 and minimizer window must be 1..32767";
+    if (gap_open > 0 || gap_ext > 0) return "gap penalties must be zero or negative";
+    if (result > 2 || verbose > 2) return "result format and verbosity must be 0..2";
+    if (!std::isfinite(mm_filter_ratio) || mm_filter_ratio < 0.0 || mm_filter_ratio > 1.0)
+        return "minimizer filter ratio must be between 0 and 1";
+    if (win_max == 0) return "window maximum must be positive";
+    return {};
+}
+
+// graph.rs:30-40: source (id 0) and sink (id 1), both base N.
+Graph::Graph(const Params &p) : alpha_(p.alpha()) {
+    uint32_t n = p.alpha() ? p.encode('N') : 0;
+    add_node(n);
+    add_node(n);
+}
+
+uint32_t Graph::add_node(uint32_t base) {
+    uint32_t id = uint32_t(nodes_.size());
+    Node nd;
+    nd.parent = id;
+    nd.base = base;
+    nodes_.push_back(std::move(nd));
+    aligned_.resize(aligned_.size() + alpha_, -1);
+    if (alpha_) aligned_[size_t(id) * alpha_ + base] = id;
+    return id;
+}
+
+// graph.rs:63-80: self edges ignored; repeated edges add weight.
+void Graph::add_edge(uint32_t from, uint32_t to) {
+    if (from == to) return;
+    Node &f = nodes_[from];
+    auto it = std::find(f.out.begin(), f.out.end(), to);
+    if (it != f.out.end()) {
+        f.out_w[size_t(it - f.out.begin())] += 1;
+    } else {
+        f.out.push_back(to);
+        f.out_w.push_back(1);
+    }
+    Node &t = nodes_[to];
+    auto jt = std::find(t.in.begin(), t.in.end(), from);
+    if (jt != t.in.end()) {
+        t.in_w[size_t(jt - t.in.begin())] += 1;
+    } else {
+        t.in.push_back(from);
+        t.in_w.push_back(1);
+    }
+    t.coverage += 1;
+}
+
+// graph.rs:82-122: reuse the column node of `aligned`'s parent for the base
+// or create it; the path runs from the source and ends with an edge to the sink.
+std::string Graph::add_path(uint64_t seq_id, const AlignedBase *a, size_t len) {
+    for (size_t k = 0; k < len; k++) {
+        if (a[k].base < 0 || uint64_t(a[k].base) >= alpha_) return "alignment base outside alphabet";
+        if ((a[k].node >= 0 && uint64_t(a[k].node) >= nodes_.size()) ||
+            (a[k].aligned >= 0 && uint64_t(a[k].aligned) >= nodes_.size()))
+            return "alignment references an unknown node";
+    }
+    bool has_prev = false;
+    uint32_t prev = 0;
+    std::vector<uint32_t> path;
+    path.reserve(len);
+    for (size_t k = 0; k < len; k++) {
+        uint32_t base = uint32_t(a[k].base);
+        uint32_t id;
+        if (a[k].node >= 0) {
+            id = uint32_t(a[k].node);
+        } else if (a[k].aligned >= 0) {
+            uint32_t parent = nodes_[size_t(a[k].aligned)].parent;
+            int64_t col = aligned_[size_t(parent) * alpha_ + base];
+            if (col >= 0) {
+                id = uint32_t(col);
+            } else {
+                id = add_node(base);
+                nodes_[id].parent = parent;
+                aligned_[size_t(parent) * alpha_ + base] = id;
+            }
+        } else {
+            id = add_node(base);
+        }
+        if (has_prev) add_edge(prev, id);
+        prev = id;
+        has_prev = true;
+        if (id >= 2) path.push_back(id);
+    }
+    add_edge(has_prev ? prev : 0, 1);
+    paths_.emplace_back(seq_id, std::move(path));
+    return {};
+}
+
+// graph.rs:124-178: Kahn's algorithm with a LIFO stack seeded in id order.
+// `grouped` ranks whole column groups (parents) for MSA rendering.
+std::string Graph::toposort(bool grouped) {
+    const size_t nn = nodes_.size();
+    std::vector<uint32_t> degree(nn, 0);
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    for (size_t id = 0; id < nn; id++) {
+        const Node &n = nodes_[id];
+        if (grouped && n.parent != id) continue;
+        if (grouped) {
+            uint32_t s = 0;
+            for (size_t b = 0; b < alpha_; b++) {
+                int64_t mbr = aligned_[id * alpha_ + b];
+                if (mbr >= 0) s += uint32_t(nodes_[size_t(mbr)].in.size());
+            }
+            degree[id] = s;
+        } else {
+            degree[id] = uint32_t(n.in.size());
+        }
+        if (degree[id] == 0) stack.push_back(uint32_t(id));
+    }
+    rank_.clear();
+    rank_.reserve(nn);
+    size_t visited = 0;
+    uint32_t single[1];
+    std::vector<uint32_t> members;
+    while (!stack.empty()) {
+        uint32_t id = stack.back();
+        stack.pop_back();
+        uint32_t rank = uint32_t(rank_.size());
+        rank_.push_back(id);
+        const uint32_t *mb;
+        size_t nm;
+        if (grouped) {
+            members.clear();
+            for (size_t b = 0; b < alpha_; b++) {
+                int64_t mbr = aligned_[size_t(id) * alpha_ + b];
+                if (mbr >= 0) members.push_back(uint32_t(mbr));
+            }
+            mb = members.data();
+            nm = members.size();
+        } else {
+            single[0] = id;
+            mb = single;
+            nm = 1;
+        }
+        for (size_t k = 0; k < nm; k++) {
+            Node &mem = nodes_[mb[k]];
+            mem.rank = rank;
+            visited++;
+            for (uint32_t next : mem.out) {
+                if (grouped) next = nodes_[next].parent;
+                if (degree[next] == 0) return "invalid graph column dependencies";
+                if (--degree[next] == 0) stack.push_back(next);
+            }
+        }
+    }
+    if (visited != nn) return "graph contains a cycle (possibly between alignment columns)";
+    return {};
+}
+
+// graph.rs:180-237: FIFO Kahn from the source; each node keeps the incoming
+// edge of largest weight, then largest accumulated score, then lowest index.
+void Graph::consensus(std::vector<uint32_t> &ids) const {
+    const size_t nn = nodes_.size();
+    std::vector<uint32_t> degree(nn);
+    std::vector<int64_t> previous(nn, -1);
+    std::vector<uint64_t> score(nn, 0);
+    std::vector<uint32_t> queue;
+    queue.reserve(nn);
+    for (size_t i = 0; i < nn; i++) degree[i] = uint32_t(nodes_[i].in.size());
+    queue.push_back(0);
+    for (size_t qh = 0; qh < queue.size(); qh++) {
+        uint32_t id = queue[qh];
+        const Node &node = nodes_[id];
+        if (id != 0 && !node.in.empty()) {
+            size_t best = 0;
+            for (size_t k = 1; k < node.in.size(); k++) {
+                uint32_t wb = node.in_w[best], wk = node.in_w[k];
+                uint64_t sb = score[node.in[best]], sk = score[node.in[k]];
+                bool keep = wb != wk ? wb > wk : (sb != sk ? sb > sk : true);
+                if (!keep) best = k;
+            }
+            previous[id] = node.in[best];
+            score[id] = score[node.in[best]] + node.in_w[best];
+        }
+        if (id == 1) break;
+        for (uint32_t next : node.out)
+            if (--degree[next] == 0) queue.push_back(next);
+    }
+    ids.clear();
+    for (int64_t cur = previous[1]; cur > 0; cur = previous[size_t(cur)]) ids.push_back(uint32_t(cur));
+    std::reverse(ids.begin(), ids.end());
+}
+
+std::string Graph::consensus_bytes(const Params &p) const {
+    std::vector<uint32_t> ids;
+    consensus(ids);
+    std::string s(ids.size(), 'N');
+    for (size_t k = 0; k < ids.size(); k++) s[k] = char(p.decode(nodes_[ids[k]].base));
+    return s;
+}
+
+// graph.rs:239-279: rank column groups, render one row per path, then restore
+// the per-node ranks and rank order whatever the outcome.
+std::string Graph::msa(const Params &p, const std::vector<std::string> &names,
+                       const std::vector<std::string> &comments, std::string &out) {
+    out.clear();
+    if (paths_.empty()) return {};
+    std::vector<uint32_t> saved_order = rank_;
+    std::vector<uint32_t> saved_rank(nodes_.size());
+    for (size_t i = 0; i < nodes_.size(); i++) saved_rank[i] = nodes_[i].rank;
+    std::string err = toposort(true);
+    if (err.empty()) {
+        size_t width = rank_.size() >= 2 ? rank_.size() - 2 : 0;
+        std::string row;
+        for (const auto &pa : paths_) {
+            if (pa.first >= names.size()) {
+                err = "unknown sequence ID in graph path";
+                break;
+            }
+            row.assign(width, '-');
+            bool bad = false;
+            for (uint32_t id : pa.second) {
+                uint32_t r = nodes_[nodes_[id].parent].rank;
+                if (r == 0 || r > width) {
+                    bad = true;
+                    break;
+                }
+                row[r - 1] = char(p.decode(nodes_[id].base));
+            }
+            if (bad) {
+                err = "graph path is outside MSA columns";
+                break;
+            }
+            out += '>';
+            out += names[pa.first];
+            out += ' ';
+            if (pa.first < comments.size()) out += comments[pa.first];
+            out += '\n';
+            out += row;
+            out += '\n';
+        }
+    }
+    rank_ = std::move(saved_order);
+    for (size_t i = 0; i < nodes_.size(); i++) nodes_[i].rank = saved_rank[i];
+    if (!err.empty()) out.clear();
+    return err;
+}
+
+// graph.rs:281-331: segments and links in FIFO-Kahn order, then one P line
+// per path; GFA ids are node ids minus one.
+std::string Graph::gfa(const Params &p, const std::vector<std::string> &names,
+                       std::string &out) const {
+    out.clear();
+    if (nodes_.size() <= 2 && paths_.empty()) return {};
+    out = "H\tVN:Z:1.0\n";
+    const size_t nn = nodes_.size();
+    std::vector<uint32_t> degree(nn);
+    std::vector<uint32_t> queue;
+    queue.reserve(nn);
+    for (size_t i = 0; i < nn; i++) degree[i] = uint32_t(nodes_[i].in.size());
+    queue.push_back(0);
+    char buf[96];
+    for (size_t qh = 0; qh < queue.size(); qh++) {
+        uint32_t id = queue[qh];
+        if (id == 1) break;
+        const Node &node = nodes_[id];
+        if (id >= 2) {
+            int k = snprintf(buf, sizeof buf, "S\t%u\t%c\n", id - 1, char(p.decode(node.base)));
+            out.append(buf, size_t(k));
+            for (uint32_t prev : node.in) {
+                if (prev == 0) continue;
+                k = snprintf(buf, sizeof buf, "L\t%u\t+\t%u\t+\t0M\n", prev - 1, id - 1);
+                out.append(buf, size_t(k));
+            }
+        }
+        for (uint32_t next : node.out)
+            if (--degree[next] == 0) queue.push_back(next);
+    }
+    for (const auto &pa : paths_) {
+        if (pa.first >= names.size()) {
+            out.clear();
+            return "unknown sequence ID in graph path";
+        }
+        out += "P\t";
+        out += names[pa.first];
+        out += '\t';
+        if (pa.second.empty()) out += '*';
+        for (size_t k = 0; k < pa.second.size(); k++) {
+            if (k) out += ',';
+            int n = snprintf(buf, sizeof buf, "%u+", pa.second[k] - 1);
+            out.append(buf, size_t(n));
+        }
+        out += "\t*\n";
+    }
+    return {};
+}
+
+mpoa_graph_view Graph::view() {
+    const size_t nn = nodes_.size();
+    v_rank_.resize(nn);
+    v_parent_.resize(nn);
+    v_base_.resize(nn);
+    v_in_off_.assign(nn + 1, 0);
+    v_out_off_.assign(nn + 1, 0);
+    v_in_.clear();
+    v_in_w_.clear();
+    v_out_.clear();
+    v_out_w_.clear();
+    for (size_t i = 0; i < nn; i++) {
+        const Node &n = nodes_[i];
+        v_rank_[i] = n.rank;
+        v_parent_[i] = n.parent;
+        v_base_[i] = n.base;
+        v_in_.insert(v_in_.end(), n.in.begin(), n.in.end());
+        v_in_w_.insert(v_in_w_.end(), n.in_w.begin(), n.in_w.end());
+        v_out_.insert(v_out_.end(), n.out.begin(), n.out.end());
+        v_out_w_.insert(v_out_w_.end(), n.out_w.begin(), n.out_w.end());
+        v_in_off_[i + 1] = v_in_.size();
+        v_out_off_[i + 1] = v_out_.size();
+    }
+    v_order_.assign(rank_.begin(), rank_.end());
+    mpoa_graph_view v{};
+    v.n_nodes = nn;
+    v.alphabet_len = alpha_;
+    v.rank = v_rank_.data();
+    v.parent = v_parent_.data();
+    v.base = v_base_.data();
+    v.in_off = v_in_off_.data();
+    v.in_node = v_in_.data();
+    v.in_weight = v_in_w_.data();
+    v.out_off = v_out_off_.data();
+    v.out_node = v_out_.data();
+    v.out_weight = v_out_w_.data();
+    v.aligned = aligned_.data();
+    v.n_order = v_order_.size();
+    v.order = v_order_.data();
+    return v;
+}
+
+}  // namespace mpoa

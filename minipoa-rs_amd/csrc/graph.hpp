@@ -1,0 +1,89 @@
+// Host-side POA graph and parameters: the data the device DP reads and the
+// fusion/rendering that consume its paths. Semantics follow minipoa-rs
+// src/graph.rs and src/lib.rs; adjacency order and ranks decide DP row order
+// and every tie, so they are kept exactly.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "minipoa_hip.h"
+
+namespace mpoa {
+
+// src/lib.rs:47-71 (the fields the DP path reads), owned copy.
+struct Params {
+    int32_t match_score = 2, mismatch = -4, gap_open = -4, gap_ext = -2;
+    uint64_t b = 100, f = 40;
+    bool adaptive = false, seeding = false, progressive = false;
+    uint64_t k = 19, w = 10, threads = 1, win_band = 500, win_max = 10000;
+    double mm_filter_ratio = 0.25;
+    uint64_t max_occ = 0, tr_dist = 1000;
+    uint8_t result = 0, verbose = 0;
+    bool is_rna = false, scalar_dp = false;
+    std::vector<uint8_t> alphabet;
+    std::vector<int32_t> matrix;
+    bool matrix_valid = true;
+
+    static Params from_c(const mpoa_params &p);
+    size_t alpha() const { return alphabet.size(); }
+    // src/lib.rs:110-126
+    uint32_t encode(uint8_t c) const;
+    // src/lib.rs:127-134
+    uint8_t decode(uint32_t b) const;
+    // src/lib.rs:135-137
+    int32_t score(uint32_t a, uint32_t b) const { return matrix[size_t(a) * alpha() + b]; }
+    // src/lib.rs:149-173; returns empty string when valid.
+    std::string validate() const;
+};
+
+// src/lib.rs:140-145 (node/aligned -1 = None).
+using AlignedBase = mpoa_aligned_base;
+
+// src/graph.rs:6-27. Node ids are dense indices; adjacency keeps insertion
+// order. `aligned` lives in one flat table of n_nodes x alphabet entries.
+class Graph {
+public:
+    explicit Graph(const Params &p);
+
+    struct Node {
+        uint32_t rank = 0, parent = 0, base = 0;
+        uint64_t coverage = 0;
+        std::vector<uint32_t> in, in_w, out, out_w;
+    };
+
+    size_t size() const { return nodes_.size(); }
+    const Node &node(size_t id) const { return nodes_[id]; }
+    int64_t aligned(size_t id, size_t b) const { return aligned_[id * alpha_ + b]; }
+    const std::vector<uint32_t> &order() const { return rank_; }
+    const std::vector<std::pair<uint64_t, std::vector<uint32_t>>> &paths() const { return paths_; }
+    size_t alpha() const { return alpha_; }
+
+    uint32_t add_node(uint32_t base);                                  // graph.rs:42-61
+    void add_edge(uint32_t from, uint32_t to);                         // graph.rs:63-80
+    std::string add_path(uint64_t seq_id, const AlignedBase *a, size_t len);  // graph.rs:82-122
+    std::string toposort(bool grouped);                                // graph.rs:124-178
+    // graph.rs:180-237: ids of the heaviest path (source/sink excluded).
+    void consensus(std::vector<uint32_t> &ids) const;
+    std::string consensus_bytes(const Params &p) const;
+    // graph.rs:239-279 (restores ranks afterwards, success or not).
+    std::string msa(const Params &p, const std::vector<std::string> &names,
+                    const std::vector<std::string> &comments, std::string &out);
+    // graph.rs:281-331
+    std::string gfa(const Params &p, const std::vector<std::string> &names, std::string &out) const;
+
+    // Flat node-indexed view (mpoa_graph_view) rebuilt on demand.
+    mpoa_graph_view view();
+
+private:
+    size_t alpha_;
+    std::vector<Node> nodes_;
+    std::vector<int64_t> aligned_;
+    std::vector<uint32_t> rank_;
+    std::vector<std::pair<uint64_t, std::vector<uint32_t>>> paths_;
+    // view buffers
+    std::vector<uint64_t> v_rank_, v_parent_, v_base_, v_in_off_, v_in_, v_in_w_, v_out_off_,
+        v_out_, v_out_w_, v_order_;
+};
+
+}  // namespace mpoa

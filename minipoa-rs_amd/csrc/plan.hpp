@@ -1,0 +1,297 @@
+// Host planning of one DP job: anchor checks, query encoding, band size,
+// the most-supported-path head/tail distances and the rank-ordered row pools
+// the device kernels read; plus assembly of the device traceback record into
+// AlignedBase records. Semantics: minipoa-rs src/align.rs (cited inline).
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dp_types.hpp"
+#include "graph.hpp"
+#include "minipoa_hip.h"
+
+namespace mpoa {
+
+// Graph accessors: the library-owned Graph and the caller's flat view expose
+// the same read-only surface to the planner.
+struct GraphAcc {
+    const Graph &g;
+    uint64_t size() const { return g.size(); }
+    uint64_t rank(uint64_t id) const { return g.node(id).rank; }
+    uint64_t order_size() const { return g.order().size(); }
+    uint64_t order(uint64_t r) const { return g.order()[r]; }
+    uint32_t base(uint64_t id) const { return g.node(id).base; }
+    uint64_t parent(uint64_t id) const { return g.node(id).parent; }
+    int64_t aligned(uint64_t id, uint64_t b) const { return g.aligned(id, b); }
+    template <class F> void for_in(uint64_t id, F &&f) const {
+        const auto &n = g.node(id);
+        for (size_t k = 0; k < n.in.size(); k++) f(uint64_t(n.in[k]), uint64_t(n.in_w[k]));
+    }
+    template <class F> void for_out(uint64_t id, F &&f) const {
+        const auto &n = g.node(id);
+        for (size_t k = 0; k < n.out.size(); k++) f(uint64_t(n.out[k]), uint64_t(n.out_w[k]));
+    }
+};
+
+struct ViewAcc {
+    const mpoa_graph_view &v;
+    uint64_t size() const { return v.n_nodes; }
+    uint64_t rank(uint64_t id) const { return v.rank[id]; }
+    uint64_t order_size() const { return v.n_order; }
+    uint64_t order(uint64_t r) const { return v.order[r]; }
+    uint32_t base(uint64_t id) const { return uint32_t(v.base[id]); }
+    uint64_t parent(uint64_t id) const { return v.parent[id]; }
+    int64_t aligned(uint64_t id, uint64_t b) const { return v.aligned[id * v.alphabet_len + b]; }
+    template <class F> void for_in(uint64_t id, F &&f) const {
+        for (uint64_t k = v.in_off[id]; k < v.in_off[id + 1]; k++) f(v.in_node[k], v.in_weight[k]);
+    }
+    template <class F> void for_out(uint64_t id, F &&f) const {
+        for (uint64_t k = v.out_off[id]; k < v.out_off[id + 1]; k++) f(v.out_node[k], v.out_weight[k]);
+    }
+};
+
+struct JobPlan {
+    int status = MPOA_OK;  // host-side error before any device work
+    std::string err;
+    uint32_t n = 0, m = 0, padded = 0, beg = 0;
+    bool use_i64 = false;
+    uint32_t flags = 0, unknown = 0;
+    int64_t band = 0;
+    int32_t o = 0, e = 0, score_nn = 0;
+    uint64_t static_cells = 0;  // cells of the rows under the static/full band
+    uint64_t arena_est = 0;     // elements requested for the arena
+    uint32_t max_width = 0;     // widest static row (chooses columns per lane)
+    std::vector<RowIn> rows;    // n + 1
+    std::vector<RowAux> aux;    // n
+    std::vector<PredE> preds;
+    std::vector<uint32_t> succs;
+    std::vector<uint8_t> seq;   // encoded query, padded with N
+    std::vector<uint32_t> row_node;
+};
+
+constexpr uint64_t kWidth = 8;
+constexpr uint64_t kScoreLimit = uint64_t(1) << 26;  // src/align.rs:217
+
+// src/align.rs:493-529: i32 states are lossless when every |score| times the
+// number of DP steps stays within 2^26.
+template <class G>
+bool i32_scores_safe(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id,
+                     uint64_t qlen) {
+    if (p.scalar_dp || beg_id >= g.size() || end_id >= g.size()) return false;
+    uint64_t re = g.rank(end_id), rb = g.rank(beg_id);
+    if (re < rb) return false;
+    uint64_t rows = re - rb, steps = rows + qlen;
+    if (steps < rows || steps + kWidth * 2 + 4 < steps) return false;
+    steps += kWidth * 2 + 4;
+    int64_t opening = int64_t(p.gap_open) + int64_t(p.gap_ext);
+    uint64_t largest = 0;
+    for (int32_t v : p.matrix) largest = std::max<uint64_t>(largest, uint64_t(std::llabs(int64_t(v))));
+    largest = std::max<uint64_t>(largest, uint64_t(std::llabs(opening)));
+    largest = std::max<uint64_t>(largest, uint64_t(std::llabs(int64_t(p.gap_ext))));
+    if (largest != 0 && steps > UINT64_MAX / largest) return false;
+    return largest * steps <= kScoreLimit;
+}
+
+// Static/full row band (src/align.rs:711-734 without the adaptive branch).
+inline bool static_band(const JobPlan &J, uint32_t i, uint64_t &start, uint64_t &stop) {
+    int64_t m = J.m, left, right;
+    if (J.flags & kFull) {
+        left = 0;
+        right = m;
+    } else {
+        int64_t h = J.rows[i].head, t = J.rows[i].tail;
+        left = std::max<int64_t>(std::min(h, m - t) - J.band, 0);
+        right = std::min<int64_t>(std::max(h, m - t) + J.band, m);
+    }
+    if (left > right || right < 0) return false;
+    start = uint64_t(left) / kWidth * kWidth;
+    stop = (uint64_t(right) / kWidth + 1) * kWidth;
+    return true;
+}
+
+// Plans one `poa_inner` call (src/align.rs:587-688 == 913-1011 up to the row
+// loop). force_bits: 0 = guard decides, 32 / 64 = that kernel.
+template <class G>
+void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, const uint8_t *q,
+              uint64_t qlen, bool adaptive, uint64_t band_b, int force_bits, JobPlan &J) {
+    J = JobPlan();
+    if (beg_id >= g.size() || end_id >= g.size()) {
+        J.status = MPOA_ERR_ANCHOR_NOT_IN_GRAPH;
+        J.err = "alignment anchor is not in graph";
+        return;
+    }
+    uint64_t beg = g.rank(beg_id), end = g.rank(end_id);
+    if (end <= beg || end >= g.order_size()) {
+        J.status = MPOA_ERR_ANCHORS_NOT_ORDERED;
+        J.err = "alignment anchors are not topologically ordered";
+        return;
+    }
+    uint64_t n = end - beg + 1, m = qlen + 2;
+    if (n > uint64_t(INT32_MAX) || m > uint64_t(INT32_MAX)) {
+        J.status = MPOA_ERR_COORDINATE_LIMITS;
+        J.err = "alignment exceeds coordinate limits";
+        return;
+    }
+    J.use_i64 = force_bits == 64 || (force_bits == 0 && !i32_scores_safe(p, g, beg_id, end_id, qlen));
+    J.n = uint32_t(n);
+    J.m = uint32_t(m);
+    J.beg = uint32_t(beg);
+    J.unknown = p.encode('N');
+    J.padded = uint32_t((m / kWidth + 1) * kWidth);
+    J.seq.assign(J.padded, uint8_t(J.unknown));
+    for (uint64_t k = 0; k < qlen; k++) J.seq[k + 1] = uint8_t(p.encode(q[k]));
+    bool adapt = adaptive || p.adaptive;
+    uint64_t mdivf = p.f ? m / p.f : 0;
+    uint64_t bw = band_b + mdivf;
+    if (bw < band_b) bw = UINT64_MAX;  // saturating_add
+    J.band = int64_t(std::min<uint64_t>(bw, uint64_t(INT32_MAX)));
+    J.e = p.gap_ext;
+    J.o = p.gap_open + p.gap_ext;
+    J.score_nn = p.score(J.unknown, J.unknown);
+    J.flags = (p.f == 0 ? kFull : 0) | ((p.f != 0 && adapt) ? kAdaptive : 0) |
+              (p.gap_open == 0 ? kGapOpenZero : 0);
+
+    // Most-supported-path distances (src/align.rs:623-678): heaviest edge,
+    // ties to larger support; `support` is shared by both passes.
+    std::vector<int64_t> head(n, INT64_MIN), tail(n, INT64_MIN), support(n, 0);
+    J.row_node.resize(n);
+    for (uint64_t i = 0; i < n; i++) J.row_node[i] = uint32_t(g.order(beg + i));
+    head[0] = 0;
+    for (uint64_t i = 1; i < n; i++) {
+        bool have = false;
+        uint64_t bp = 0, bwt = 0;
+        g.for_in(J.row_node[i], [&](uint64_t pred, uint64_t w) {
+            uint64_t r = g.rank(pred);
+            if (r < beg || r >= beg + i) return;
+            uint64_t pr = r - beg;
+            if (head[pr] == INT64_MIN) return;
+            if (!have || w > bwt || (w == bwt && support[pr] > support[bp])) {
+                have = true;
+                bp = pr;
+                bwt = w;
+            }
+        });
+        if (have) {
+            head[i] = head[bp] + 1;
+            support[i] = support[bp] + int64_t(bwt);
+        }
+    }
+    tail[n - 1] = 1;
+    for (uint64_t ii = n - 1; ii-- > 0;) {
+        bool have = false;
+        uint64_t bs = 0, bwt = 0;
+        g.for_out(J.row_node[ii], [&](uint64_t succ, uint64_t w) {
+            uint64_t r = g.rank(succ);
+            if (r <= beg + ii || r > end) return;
+            uint64_t sc = r - beg;
+            if (tail[sc] == INT64_MIN) return;
+            if (!have || w > bwt || (w == bwt && support[sc] > support[bs])) {
+                have = true;
+                bs = sc;
+                bwt = w;
+            }
+        });
+        if (have) {
+            tail[ii] = tail[bs] + 1;
+            support[ii] = support[bs] + int64_t(bwt);
+        }
+    }
+    if (head[n - 1] == INT64_MIN) {
+        J.status = MPOA_ERR_NO_PATH;
+        J.err = "no graph path connects alignment anchors";
+        return;
+    }
+
+    // Row pools: in-range predecessors with head != NEG (every consumer of a
+    // predecessor applies that filter), in-range successors for adaptive bands.
+    J.rows.resize(n + 1);
+    J.aux.resize(n);
+    J.preds.clear();
+    J.succs.clear();
+    for (uint64_t i = 0; i < n; i++) {
+        RowIn &r = J.rows[i];
+        r.pred_beg = uint32_t(J.preds.size());
+        r.succ_beg = uint32_t(J.succs.size());
+        r.head = head[i] == INT64_MIN ? kHeadNone : int32_t(head[i]);
+        r.tail = tail[i] == INT64_MIN ? kHeadNone : int32_t(tail[i]);
+        uint64_t id = J.row_node[i], tot = 0;
+        g.for_in(id, [&](uint64_t pred, uint64_t w) {
+            tot += w;
+            uint64_t rk = g.rank(pred);
+            if (rk < beg || rk >= beg + i) return;
+            uint64_t pr = rk - beg;
+            if (head[pr] == INT64_MIN) return;
+            J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX))});
+        });
+        if (J.flags & kAdaptive) {
+            g.for_out(id, [&](uint64_t succ, uint64_t) {
+                uint64_t rk = g.rank(succ);
+                if (rk <= beg + i || rk > end) return;
+                J.succs.push_back(uint32_t(rk - beg));
+            });
+        }
+        J.aux[i].thresh = uint32_t(std::min<uint64_t>(tot / 10, UINT32_MAX));
+        J.aux[i].base = g.base(id);
+    }
+    J.rows[n].pred_beg = uint32_t(J.preds.size());
+    J.rows[n].succ_beg = uint32_t(J.succs.size());
+    J.rows[n].head = J.rows[n].tail = kHeadNone;
+
+    // Arena: exact for static/full bands; adaptive rows start from the static
+    // width plus two band widths and grow to the full width on overflow.
+    uint64_t cells = 0, est = 0;
+    uint32_t maxw = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (J.rows[i].head == kHeadNone || J.rows[i].tail == kHeadNone) continue;
+        uint64_t s, t;
+        uint64_t w;
+        if (static_band(J, i, s, t))
+            w = t - s;
+        else
+            w = J.padded;
+        cells += w;
+        maxw = std::max<uint32_t>(maxw, uint32_t(w));
+        if (J.flags & kAdaptive)
+            est += std::min<uint64_t>(J.padded, w + 2 * uint64_t(J.band) + 16);
+        else
+            est += w;
+    }
+    J.static_cells = cells;
+    J.max_width = maxw;
+    J.arena_est = 3 * est;
+}
+
+// Full-width arena bound: every row is at most `padded` wide.
+inline uint64_t arena_full(const JobPlan &J) { return 3ull * J.n * J.padded; }
+
+// Device traceback record -> AlignedBase list (src/align.rs:1293-1347): the
+// record holds, per column j = 1 .. m-1, the row of the diagonal step that
+// consumed j or kTbInsert. Reversal then dropping the first traceback step
+// (column m-1) leaves [source, col 1, .., col m-2].
+template <class G>
+void assemble(const G &g, const JobPlan &J, const uint32_t *tb, std::vector<AlignedBase> &out) {
+    out.clear();
+    out.reserve(J.m);
+    uint64_t start = J.row_node[0];
+    out.push_back(AlignedBase{int64_t(start), int64_t(g.base(start)), -1});
+    for (uint32_t j = 1; j + 1 < J.m; j++) {
+        uint32_t qb = J.seq[j];
+        uint32_t i = tb[j];
+        if (i == kTbInsert || (i == J.n - 1 && j != J.m - 1)) {
+            out.push_back(AlignedBase{-1, int64_t(qb), -1});
+            continue;
+        }
+        uint64_t id = J.row_node[i];
+        uint32_t nb = g.base(id);
+        if (nb == qb) {
+            out.push_back(AlignedBase{int64_t(id), int64_t(nb), -1});
+        } else {
+            uint64_t parent = g.parent(id);
+            out.push_back(AlignedBase{g.aligned(parent, qb), int64_t(qb), int64_t(parent)});
+        }
+    }
+}
+
+}  // namespace mpoa

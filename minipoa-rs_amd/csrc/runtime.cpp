@@ -1,0 +1,394 @@
+// Device context, staging and launch orchestration for the DP kernels.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace mpoa {
+
+static thread_local std::string t_error;
+void set_error(const std::string &msg) { t_error = msg; }
+const char *last_error() { return t_error.c_str(); }
+
+// ---------------- thread pool ----------------
+
+ThreadPool::ThreadPool(unsigned n) {
+    for (unsigned k = 1; k < n; k++) workers_.emplace_back([this] { loop(); });
+}
+
+ThreadPool::~ThreadPool() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+}
+
+void ThreadPool::loop() {
+    size_t seen = 0;
+    for (;;) {
+        const std::function<void(size_t)> *fn;
+        size_t n;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
+            if (stop_) return;
+            seen = generation_;
+            fn = fn_;
+            n = n_;
+        }
+        for (size_t i; (i = next_.fetch_add(1)) < n;) (*fn)(i);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_cv_.notify_all();
+        }
+    }
+}
+
+void ThreadPool::parallel_for(size_t n, const std::function<void(size_t)> &f) {
+    if (n == 0) return;
+    if (workers_.empty() || n == 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        fn_ = &f;
+        n_ = n;
+        next_.store(0);
+        active_ = unsigned(workers_.size());
+        generation_++;
+    }
+    cv_.notify_all();
+    for (size_t i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return active_ == 0; });
+}
+
+unsigned default_threads() {
+    if (const char *s = getenv("MPOA_THREADS")) {
+        int v = atoi(s);
+        if (v > 0) return unsigned(v);
+    }
+    unsigned hw = std::thread::hardware_concurrency();
+    if (hw == 0) hw = 1;
+    return std::min(hw, 16u);
+}
+
+// ---------------- buffers ----------------
+
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = std::max(bytes, cap + cap / 4);
+    want = (want + 4095) & ~size_t(4095);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        cap = 0;
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return e;
+        }
+        cap = bytes;
+        return hipSuccess;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+hipError_t PinBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    size_t want = std::max(bytes, cap + cap / 4);
+    want = (want + 4095) & ~size_t(4095);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+void PinBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+Ctx::~Ctx() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    d_in.release();
+    d_res.release();
+    d_meta.release();
+    d_adapt.release();
+    d_tb.release();
+    d_arena.release();
+    h_in.release();
+    h_res.release();
+    h_tb.release();
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    delete pool;
+}
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
+            return MPOA_ERR_DEVICE;                                                   \
+        }                                                                             \
+    } while (0)
+
+int ctx_create(int device, Ctx **out) {
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device is available");
+        return MPOA_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        set_error("device index out of range");
+        return MPOA_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        return MPOA_ERR_NO_DEVICE;
+    }
+    Ctx *c = new Ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (auto &ev : c->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        set_error(std::string("HIP error: ") + hipGetErrorString(e));
+        delete c;
+        return MPOA_ERR_DEVICE;
+    }
+    c->pool = new ThreadPool(default_threads());
+    *out = c;
+    return MPOA_OK;
+}
+
+int device_status(const DevResult &r, std::string &msg) {
+    switch (r.status) {
+        case kOk: msg.clear(); return MPOA_OK;
+        case kBandNoIntersect: msg = "alignment band does not intersect graph path"; return MPOA_ERR_BAND_NO_INTERSECT;
+        case kBandExcludesSource: msg = "alignment band excludes source"; return MPOA_ERR_BAND_EXCLUDES_SOURCE;
+        case kTbLeftBand: msg = "alignment traceback left active band"; return MPOA_ERR_TRACEBACK_LEFT_BAND;
+        case kTbNoAdvance: msg = "alignment traceback failed to advance"; return MPOA_ERR_TRACEBACK_NO_ADVANCE;
+        case kTbNoPredecessor: {
+            char buf[128];
+            snprintf(buf, sizeof buf,
+                     "alignment traceback has no predecessor at graph row %u, query column %u",
+                     r.err_i, r.err_j);
+            msg = buf;
+            return MPOA_ERR_TRACEBACK_NO_PREDECESSOR;
+        }
+        case kArenaOverflow: msg = "alignment arena overflow"; return MPOA_ERR_ALLOC;
+        default: msg = "device DP returned an unknown status"; return MPOA_ERR_DEVICE;
+    }
+}
+
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// One launch over plans[idx[*]] that share the score width.
+static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
+                     const std::vector<size_t> &idx, bool i64, bool full_arena,
+                     std::vector<DevResult> &res, std::vector<uint32_t> &tb,
+                     const std::vector<uint64_t> &tb_off, RunStats *st) {
+    const size_t N = idx.size();
+    const size_t tsz = i64 ? 8 : 4;
+    struct Off {
+        uint64_t rows, preds, succs, seq, arena, cap, meta, adapt, tb;
+    };
+    std::vector<Off> off(N);
+    uint64_t rows = 0, aux = 0, preds = 0, succs = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
+             tbn = 0;
+    uint32_t maxw = 0;
+    for (size_t k = 0; k < N; k++) {
+        const JobPlan &J = *plans[idx[k]];
+        Off &o = off[k];
+        o.rows = rows;
+        o.preds = preds;
+        o.succs = succs;
+        o.seq = seq;
+        o.cap = full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64);
+        o.arena = arena;
+        o.meta = meta;
+        o.adapt = adapt;
+        o.tb = tbn;
+        rows += J.n + 1;
+        aux += J.n + 1;
+        preds += J.preds.size();
+        succs += J.succs.size();
+        seq += J.padded;
+        arena += align_up(o.cap, 64);
+        meta += J.n;
+        if (J.flags & kAdaptive) adapt += 7ull * J.n;
+        tbn += J.m;
+        maxw = std::max(maxw, J.max_width);
+    }
+    const size_t alpha = p.alpha();
+    size_t b_jobs = 0;
+    size_t b_rows = align_up(b_jobs + N * sizeof(DevJob), 256);
+    size_t b_aux = align_up(b_rows + rows * sizeof(RowIn), 256);
+    size_t b_preds = align_up(b_aux + aux * sizeof(RowAux), 256);
+    size_t b_succs = align_up(b_preds + preds * sizeof(PredE), 256);
+    size_t b_seq = align_up(b_succs + succs * sizeof(uint32_t), 256);
+    size_t b_mat = align_up(b_seq + seq, 256);
+    size_t b_end = align_up(b_mat + alpha * alpha * sizeof(int32_t), 256);
+
+    HIP_TRY(hipSetDevice(ctx.device));
+    HIP_TRY(ctx.h_in.ensure(b_end));
+    char *h = static_cast<char *>(ctx.h_in.p);
+    ctx.pool->parallel_for(N, [&](size_t k) {
+        const JobPlan &J = *plans[idx[k]];
+        const Off &o = off[k];
+        DevJob d{};
+        d.rows = o.rows;
+        d.preds = o.preds;
+        d.succs = o.succs;
+        d.seq = o.seq;
+        d.matrix = 0;
+        d.arena = o.arena;
+        d.arena_cap = o.cap;
+        d.rowmeta = o.meta;
+        d.adapt = o.adapt;
+        d.tb = o.tb;
+        d.n = J.n;
+        d.m = J.m;
+        d.padded = J.padded;
+        d.alpha = uint32_t(alpha);
+        d.band = J.band;
+        d.o = J.o;
+        d.e = J.e;
+        d.unknown = J.unknown;
+        d.score_nn = J.score_nn;
+        d.flags = J.flags;
+        memcpy(h + b_jobs + k * sizeof(DevJob), &d, sizeof d);
+        memcpy(h + b_rows + o.rows * sizeof(RowIn), J.rows.data(), J.rows.size() * sizeof(RowIn));
+        memcpy(h + b_aux + o.rows * sizeof(RowAux), J.aux.data(), J.aux.size() * sizeof(RowAux));
+        if (!J.preds.empty())
+            memcpy(h + b_preds + o.preds * sizeof(PredE), J.preds.data(), J.preds.size() * sizeof(PredE));
+        if (!J.succs.empty())
+            memcpy(h + b_succs + o.succs * sizeof(uint32_t), J.succs.data(),
+                   J.succs.size() * sizeof(uint32_t));
+        memcpy(h + b_seq + o.seq, J.seq.data(), J.seq.size());
+    });
+    memcpy(h + b_mat, p.matrix.data(), alpha * alpha * sizeof(int32_t));
+
+    HIP_TRY(ctx.d_in.ensure(b_end));
+    HIP_TRY(ctx.d_res.ensure(N * sizeof(DevResult)));
+    HIP_TRY(ctx.d_meta.ensure(std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
+    HIP_TRY(ctx.d_adapt.ensure(std::max<uint64_t>(adapt, 1) * sizeof(int64_t)));
+    HIP_TRY(ctx.d_tb.ensure(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+    HIP_TRY(ctx.d_arena.ensure(std::max<uint64_t>(arena, 1) * tsz));
+    HIP_TRY(ctx.h_res.ensure(N * sizeof(DevResult)));
+    HIP_TRY(ctx.h_tb.ensure(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+
+    char *d = static_cast<char *>(ctx.d_in.p);
+    DevPools P;
+    P.jobs = reinterpret_cast<const DevJob *>(d + b_jobs);
+    P.res = static_cast<DevResult *>(ctx.d_res.p);
+    P.rows = reinterpret_cast<const RowIn *>(d + b_rows);
+    P.aux = reinterpret_cast<const RowAux *>(d + b_aux);
+    P.preds = reinterpret_cast<const PredE *>(d + b_preds);
+    P.succs = reinterpret_cast<const uint32_t *>(d + b_succs);
+    P.seq = reinterpret_cast<const uint8_t *>(d + b_seq);
+    P.matrix = reinterpret_cast<const int32_t *>(d + b_mat);
+    P.arena = ctx.d_arena.p;
+    P.meta = static_cast<RowMeta *>(ctx.d_meta.p);
+    P.adapt = static_cast<int64_t *>(ctx.d_adapt.p);
+    P.tb = static_cast<uint32_t *>(ctx.d_tb.p);
+
+    const int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(maxw);
+    HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
+    HIP_TRY(hipEventRecord(ctx.ev[0], ctx.stream));
+    launch_fill(P, uint32_t(N), i64, cols, ctx.stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx.ev[1], ctx.stream));
+    launch_traceback(P, uint32_t(N), i64, ctx.stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx.ev[2], ctx.stream));
+    HIP_TRY(hipMemcpyAsync(ctx.h_res.p, P.res, N * sizeof(DevResult), hipMemcpyDeviceToHost,
+                           ctx.stream));
+    HIP_TRY(hipMemcpyAsync(ctx.h_tb.p, P.tb, tbn * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx.stream));
+    HIP_TRY(hipStreamSynchronize(ctx.stream));
+    float fill_ms = 0, all_ms = 0;
+    HIP_TRY(hipEventElapsedTime(&fill_ms, ctx.ev[0], ctx.ev[1]));
+    HIP_TRY(hipEventElapsedTime(&all_ms, ctx.ev[0], ctx.ev[2]));
+
+    const DevResult *hr = static_cast<const DevResult *>(ctx.h_res.p);
+    const uint32_t *ht = static_cast<const uint32_t *>(ctx.h_tb.p);
+    for (size_t k = 0; k < N; k++) res[idx[k]] = hr[k];
+    ctx.pool->parallel_for(N, [&](size_t k) {
+        const JobPlan &J = *plans[idx[k]];
+        if (hr[k].status == kOk)
+            memcpy(tb.data() + tb_off[idx[k]], ht + off[k].tb, size_t(J.m) * sizeof(uint32_t));
+    });
+    if (st) {
+        // Overflowed attempts are re-run with the full arena; count the cells
+        // of the run that completes.
+        st->fill_ms += fill_ms;
+        st->device_ms += all_ms;
+        st->fill_launches += 1;
+        for (size_t k = 0; k < N; k++) {
+            if (hr[k].status == kArenaOverflow) continue;
+            st->cells += hr[k].cells;
+            st->pred_cells += hr[k].pred_cells;
+            st->dp_calls += 1;
+            if (i64) st->i64_calls += 1;
+        }
+    }
+    return MPOA_OK;
+}
+
+int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
+              std::vector<DevResult> &res, std::vector<uint32_t> &tb,
+              std::vector<uint64_t> &tb_off, RunStats *st) {
+    const size_t N = plans.size();
+    res.assign(N, DevResult{});
+    tb_off.assign(N + 1, 0);
+    for (size_t k = 0; k < N; k++)
+        tb_off[k + 1] = tb_off[k] + (plans[k]->status == MPOA_OK ? plans[k]->m : 0);
+    tb.resize(tb_off[N]);
+    for (int pass = 0; pass < 2; pass++) {
+        for (int wide = 0; wide < 2; wide++) {
+            std::vector<size_t> idx;
+            for (size_t k = 0; k < N; k++) {
+                if (plans[k]->status != MPOA_OK || plans[k]->use_i64 != (wide == 1)) continue;
+                if (pass == 1 && res[k].status != kArenaOverflow) continue;
+                idx.push_back(k);
+            }
+            if (idx.empty()) continue;
+            int rc = run_group(ctx, p, plans, idx, wide == 1, pass == 1, res, tb, tb_off, st);
+            if (rc != MPOA_OK) return rc;
+        }
+    }
+    return MPOA_OK;
+}
+
+}  // namespace mpoa

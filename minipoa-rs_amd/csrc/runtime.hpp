@@ -1,0 +1,92 @@
+// Device context and DP orchestration: plans go up as one blob per launch,
+// the fill and traceback kernels run on the context's stream, and results
+// plus per-column traceback records come back for host assembly.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dp_types.hpp"
+#include "graph.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace mpoa {
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+// Fixed pool of host workers for per-job planning, assembly and fusion.
+class ThreadPool {
+public:
+    explicit ThreadPool(unsigned n);
+    ~ThreadPool();
+    unsigned size() const { return unsigned(workers_.size()) + 1; }
+    // Runs f(i) for i in [0, n) across the pool and the calling thread.
+    void parallel_for(size_t n, const std::function<void(size_t)> &f);
+
+private:
+    void loop();
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    size_t generation_ = 0;
+    unsigned active_ = 0;
+    bool stop_ = false;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+};
+
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+};
+
+struct RunStats {
+    uint64_t cells = 0, dp_calls = 0, retries = 0, i64_calls = 0, pred_cells = 0;
+    uint64_t fill_launches = 0;
+    double device_ms = 0, fill_ms = 0, host_ms = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    DevBuf d_in, d_res, d_meta, d_adapt, d_tb, d_arena;
+    PinBuf h_in, h_res, h_tb;
+    ThreadPool *pool = nullptr;
+    int force_cols = 0;  // 0: choose from the widest static row
+    ~Ctx();
+};
+
+int ctx_create(int device, Ctx **out);
+unsigned default_threads();
+
+// Runs every plan with status MPOA_OK on the device (i32 and i64 groups in
+// separate launches; arena overflows are re-run with the full-width bound).
+// Fills res[k] and tb (flattened, tb_off[k] .. tb_off[k] + m).
+int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
+              std::vector<DevResult> &res, std::vector<uint32_t> &tb,
+              std::vector<uint64_t> &tb_off, RunStats *stats);
+
+// Maps a failed DevResult to (status, message) as the reference reports it.
+int device_status(const DevResult &r, std::string &msg);
+
+}  // namespace mpoa

@@ -1,0 +1,493 @@
+"""ctypes binding of libminipoa_hip.so (include/minipoa_hip.h).
+
+Mirrors minipoa-rs's library surface for the DP path: `poa`/`poa_inner`
+(src/align.rs:192-211, 531-545), `align` (81-98), `align_sequences`
+(src/lib.rs:177-204), the graph (src/graph.rs) and the batched engine.
+There is no CPU fallback: every DP call runs the gfx950 kernels, and a missing
+library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from .params import AlignedBase, Params, Sequence, default_matrix, records
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "libminipoa_hip.so"
+
+STATUS = {
+    0: "MPOA_OK", 1: "MPOA_ERR_ANCHOR_NOT_IN_GRAPH", 2: "MPOA_ERR_ANCHORS_NOT_ORDERED",
+    3: "MPOA_ERR_COORDINATE_LIMITS", 4: "MPOA_ERR_NO_PATH", 5: "MPOA_ERR_BAND_NO_INTERSECT",
+    6: "MPOA_ERR_BAND_EXCLUDES_SOURCE", 7: "MPOA_ERR_TRACEBACK_LEFT_BAND",
+    8: "MPOA_ERR_TRACEBACK_NO_ADVANCE", 9: "MPOA_ERR_TRACEBACK_NO_PREDECESSOR",
+    10: "MPOA_ERR_ALLOC", 11: "MPOA_ERR_INVALID_PARAMS", 12: "MPOA_ERR_GRAPH",
+    13: "MPOA_ERR_DEVICE", 14: "MPOA_ERR_NO_DEVICE", 15: "MPOA_ERR_UNSUPPORTED",
+    16: "MPOA_ERR_BUFFER_TOO_SMALL", 17: "MPOA_ERR_INVALID_ARGUMENT",
+}
+BUFFER_TOO_SMALL = 16
+
+
+class HipPoaError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("match_score", C.c_int32), ("mismatch", C.c_int32),
+        ("gap_open", C.c_int32), ("gap_ext", C.c_int32),
+        ("b", C.c_uint64), ("f", C.c_uint64),
+        ("adaptive", C.c_int32), ("seeding", C.c_int32),
+        ("k", C.c_uint64), ("w", C.c_uint64),
+        ("progressive", C.c_int32),
+        ("threads", C.c_uint64), ("win_band", C.c_uint64), ("win_max", C.c_uint64),
+        ("mm_filter_ratio", C.c_double),
+        ("max_occ", C.c_uint64), ("tr_dist", C.c_uint64),
+        ("result", C.c_uint8), ("verbose", C.c_uint8),
+        ("is_rna", C.c_int32), ("scalar_dp", C.c_int32),
+        ("alphabet_len", C.c_uint32),
+        ("alphabet", C.POINTER(C.c_uint8)),
+        ("matrix", C.POINTER(C.c_int32)),
+    ]
+
+
+class _Base(C.Structure):
+    _fields_ = [("node", C.c_int64), ("base", C.c_int64), ("aligned", C.c_int64)]
+
+
+U64P = C.POINTER(C.c_uint64)
+
+
+class _View(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint64), ("alphabet_len", C.c_uint64),
+        ("rank", U64P), ("parent", U64P), ("base", U64P),
+        ("in_off", U64P), ("in_node", U64P), ("in_weight", U64P),
+        ("out_off", U64P), ("out_node", U64P), ("out_weight", U64P),
+        ("aligned", C.POINTER(C.c_int64)),
+        ("n_order", C.c_uint64), ("order", U64P),
+    ]
+
+
+class _Stats(C.Structure):
+    _fields_ = [
+        ("cells", C.c_uint64), ("dp_calls", C.c_uint64), ("retries", C.c_uint64),
+        ("i64_calls", C.c_uint64), ("pred_cells", C.c_uint64), ("fill_launches", C.c_uint64),
+        ("device_ms", C.c_double), ("fill_ms", C.c_double),
+        ("host_ms", C.c_double), ("wall_ms", C.c_double),
+    ]
+
+
+# Exported symbols of include/minipoa_hip.h with their ctypes signatures.
+_P = C.POINTER(_Params)
+_V = C.POINTER(_View)
+_B = C.POINTER(_Base)
+_u64 = C.c_uint64
+_vp = C.c_void_p
+SIGNATURES = {
+    "mpoa_backend_name": (C.c_char_p, []),
+    "mpoa_last_error": (C.c_char_p, []),
+    "mpoa_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "mpoa_ctx_destroy": (None, [_vp]),
+    "mpoa_ctx_device_bytes": (_u64, [_vp]),
+    "mpoa_poa": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64, _B, _u64,
+                           U64P, C.POINTER(C.c_int64)]),
+    "mpoa_poa_inner": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64, _B,
+                                 _u64, U64P, C.POINTER(C.c_int64)]),
+    "mpoa_poa_kernel": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64,
+                                  C.c_int, _B, _u64, U64P, C.POINTER(C.c_int64)]),
+    "mpoa_i32_scores_safe": (C.c_int, [_P, _V, _u64, _u64, _u64]),
+    "mpoa_align": (C.c_int, [_vp, _P, _V, C.c_char_p, _u64, _B, _u64, U64P]),
+    "mpoa_graph_new": (_vp, [_P]),
+    "mpoa_graph_free": (None, [_vp]),
+    "mpoa_graph_add_node": (_u64, [_vp, _u64]),
+    "mpoa_graph_add_edge": (None, [_vp, _u64, _u64]),
+    "mpoa_graph_add_path": (C.c_int, [_vp, _u64, _B, _u64]),
+    "mpoa_graph_toposort": (C.c_int, [_vp, C.c_int]),
+    "mpoa_graph_view_of": (C.c_int, [_vp, _V]),
+    "mpoa_graph_n_paths": (_u64, [_vp]),
+    "mpoa_graph_consensus": (_u64, [_vp, _P, C.POINTER(C.c_uint8), U64P, U64P, _u64]),
+    "mpoa_graph_msa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), _u64,
+                                 C.c_char_p, _u64, U64P]),
+    "mpoa_graph_gfa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), _u64, C.c_char_p, _u64, U64P]),
+    "mpoa_params_validate": (C.c_int, [_P]),
+    "mpoa_align_sequences": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), U64P, _u64,
+                                       C.POINTER(_vp)]),
+    "mpoa_extend_graph": (C.c_int, [_vp, _P, _vp, C.POINTER(C.c_char_p), U64P, _u64, _u64]),
+    "mpoa_batch_run": (C.c_int, [_vp, _P, _u64, U64P, U64P, _vp, C.POINTER(_vp)]),
+    "mpoa_batch_free": (None, [_vp]),
+    "mpoa_batch_stats_get": (C.c_int, [_vp, C.POINTER(_Stats)]),
+    "mpoa_batch_job_status": (C.c_int, [_vp, _u64]),
+    "mpoa_batch_job_output": (C.c_int, [_vp, _u64, C.c_int, C.c_char_p, _u64, U64P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HipPoaError(14, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().mpoa_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc != 0:
+        raise HipPoaError(rc, last_error())
+
+
+def backend_name():
+    return lib().mpoa_backend_name().decode()
+
+
+class CParams:
+    """Keeps the ctypes buffers of one Params conversion alive."""
+
+    def __init__(self, p: Params):
+        alphabet = bytes(p.alphabet)
+        matrix = list(p.matrix)
+        self.alpha = (C.c_uint8 * max(1, len(alphabet)))(*alphabet)
+        self.mat = (C.c_int32 * max(1, len(matrix)))(*matrix)
+        ok = len(matrix) == len(alphabet) ** 2 and len(matrix) > 0
+        self.s = _Params(
+            int(p.match_score), int(p.mismatch), int(p.gap_open), int(p.gap_ext),
+            int(p.b), int(p.f), int(bool(p.adaptive)), int(bool(p.seeding)),
+            int(p.k), int(p.w), int(bool(p.progressive)),
+            int(p.threads), int(p.win_band), int(p.win_max), float(p.mm_filter_ratio),
+            int(p.max_occ), int(p.tr_dist), int(p.result), int(p.verbose),
+            int(bool(p.is_rna)), int(bool(getattr(p, "scalar_dp", False))),
+            len(alphabet) if ok else 0,
+            C.cast(self.alpha, C.POINTER(C.c_uint8)),
+            C.cast(self.mat, C.POINTER(C.c_int32)) if ok else None,
+        )
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+class Context:
+    """One device context (stream, device arenas, staging, host workers)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().mpoa_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mpoa_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def device_bytes(self):
+        return lib().mpoa_ctx_device_bytes(self._h)
+
+
+class GraphView:
+    """A flat mpoa_graph_view built from node records (keeps arrays alive).
+
+    nodes: list of dicts with rank, parent, base, incoming, in_weights,
+    outgoing, out_weights; aligned: n x alphabet list (None entries allowed);
+    order: graph.rank (rank -> id)."""
+
+    def __init__(self, nodes, aligned, order, alphabet_len):
+        n = len(nodes)
+        u64 = C.c_uint64
+
+        def arr(vals, t=u64):
+            return (t * max(1, len(vals)))(*vals)
+
+        in_off, out_off, ins, inw, outs, outw = [0], [0], [], [], [], []
+        for nd in nodes:
+            ins += nd["incoming"]
+            inw += nd["in_weights"]
+            outs += nd["outgoing"]
+            outw += nd["out_weights"]
+            in_off.append(len(ins))
+            out_off.append(len(outs))
+        flat_al = []
+        for row in aligned:
+            flat_al += [-1 if x is None else x for x in row]
+        self._keep = [
+            arr([nd["rank"] for nd in nodes]), arr([nd["parent"] for nd in nodes]),
+            arr([nd["base"] for nd in nodes]), arr(in_off), arr(ins), arr(inw), arr(out_off),
+            arr(outs), arr(outw), arr(flat_al, C.c_int64), arr(order),
+        ]
+        k = self._keep
+        self.s = _View(n, alphabet_len, k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], k[8],
+                       k[9], len(order), k[10])
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def view_from_graph(og, alphabet_len):
+    """Flatten any graph object exposing node(i), aligned(i, b), rank and
+    n_nodes (as graph::Graph does) into a GraphView."""
+    nodes = [og.node(i) for i in range(og.n_nodes)]
+    aligned = [[og.aligned(i, b) for b in range(alphabet_len)] for i in range(og.n_nodes)]
+    return GraphView(nodes, aligned, og.rank, alphabet_len)
+
+
+def _to_bases(buf, n):
+    return [AlignedBase(None if b.node < 0 else b.node, b.base,
+                        None if b.aligned < 0 else b.aligned) for b in buf[:n]]
+
+
+def _path_call(fn, args, cap):
+    while True:
+        buf = (_Base * cap)()
+        n = C.c_uint64(0)
+        rc = fn(*args[0], buf, cap, C.byref(n), *args[1])
+        if rc == BUFFER_TOO_SMALL:
+            cap = n.value + 16
+            continue
+        _check(rc)
+        return _to_bases(buf, n.value)
+
+
+class Graph:
+    """Library-owned graph (src/graph.rs semantics)."""
+
+    def __init__(self, params: Params, _handle=None):
+        self._cp = CParams(params)
+        self._h = _handle if _handle is not None else lib().mpoa_graph_new(self._cp.ref)
+        if not self._h:
+            raise HipPoaError(11, last_error())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpoa_graph_free(self._h)
+            self._h = None
+
+    def add_node(self, base):
+        return lib().mpoa_graph_add_node(self._h, base)
+
+    def add_edge(self, a, b):
+        lib().mpoa_graph_add_edge(self._h, a, b)
+
+    def add_path(self, seq_id, path):
+        arr = (_Base * max(1, len(path)))()
+        for k, it in enumerate(path):
+            arr[k] = _Base(-1 if it[0] is None else it[0], it[1], -1 if it[2] is None else it[2])
+        _check(lib().mpoa_graph_add_path(self._h, seq_id, arr, len(path)))
+
+    def toposort(self, grouped=False):
+        _check(lib().mpoa_graph_toposort(self._h, int(grouped)))
+
+    def view(self):
+        v = _View()
+        _check(lib().mpoa_graph_view_of(self._h, C.byref(v)))
+        return v
+
+    @property
+    def n_nodes(self):
+        return self.view().n_nodes
+
+    def node(self, i):
+        v = self.view()
+        a, b = v.in_off[i], v.in_off[i + 1]
+        c, d = v.out_off[i], v.out_off[i + 1]
+        return dict(id=i, rank=v.rank[i], parent=v.parent[i], base=v.base[i],
+                    incoming=list(v.in_node[a:b]), in_weights=list(v.in_weight[a:b]),
+                    outgoing=list(v.out_node[c:d]), out_weights=list(v.out_weight[c:d]))
+
+    def aligned(self, i, base):
+        v = self.view()
+        x = v.aligned[i * v.alphabet_len + base]
+        return None if x < 0 else x
+
+    @property
+    def rank(self):
+        v = self.view()
+        return list(v.order[:v.n_order])
+
+    def consensus(self, params):
+        n = self.n_nodes
+        seq = (C.c_uint8 * max(1, n))()
+        ids = (C.c_uint64 * max(1, n))()
+        cov = (C.c_uint64 * max(1, n))()
+        cp = CParams(params)
+        k = lib().mpoa_graph_consensus(self._h, cp.ref, seq, ids, cov, n)
+        return bytes(seq)[:k], list(ids)[:k], list(cov)[:k]
+
+    def _text(self, fn, *args):
+        cap = 1 << 16
+        while True:
+            buf = C.create_string_buffer(cap)
+            n = C.c_uint64(0)
+            rc = fn(*args, buf, cap, C.byref(n))
+            if rc == BUFFER_TOO_SMALL:
+                cap = n.value + 1
+                continue
+            _check(rc)
+            return buf.raw[:n.value].decode()
+
+    def msa(self, params, seqs):
+        names = (C.c_char_p * max(1, len(seqs)))(*[s.name.encode() for s in seqs])
+        comments = (C.c_char_p * max(1, len(seqs)))(*[s.comment.encode() for s in seqs])
+        cp = CParams(params)
+        return self._text(lib().mpoa_graph_msa, self._h, cp.ref, names, comments, len(seqs))
+
+    def gfa(self, params, seqs):
+        names = (C.c_char_p * max(1, len(seqs)))(*[s.name.encode() for s in seqs])
+        cp = CParams(params)
+        return self._text(lib().mpoa_graph_gfa, self._h, cp.ref, names, len(seqs))
+
+
+def _as_view(graph):
+    if isinstance(graph, GraphView):
+        return graph.ref, graph
+    v = graph.view()
+    return C.byref(v), v
+
+
+def poa(ctx, params, graph, beg, end, query, adaptive, band, score=False):
+    """src/align.rs:192-211 on the device (banded, then full-DP retry)."""
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    q = bytes(query)
+    sc = C.c_int64(0)
+    out = _path_call(lib().mpoa_poa, ((ctx._h, cp.ref, vref, beg, end, q, len(q),
+                                       int(bool(adaptive)), band), (C.byref(sc),)), len(q) + 16)
+    return (out, sc.value) if score else out
+
+
+def poa_inner(ctx, params, graph, beg, end, query, adaptive, band, score=False):
+    """src/align.rs:531-545 on the device (guard picks i32 or i64)."""
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    q = bytes(query)
+    sc = C.c_int64(0)
+    out = _path_call(lib().mpoa_poa_inner, ((ctx._h, cp.ref, vref, beg, end, q, len(q),
+                                             int(bool(adaptive)), band), (C.byref(sc),)),
+                     len(q) + 16)
+    return (out, sc.value) if score else out
+
+
+def poa_kernel(ctx, params, graph, beg, end, query, adaptive, band, bits, score=False):
+    """One device kernel forced: bits 32 (i32 fast path) or 64 (i64)."""
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    q = bytes(query)
+    sc = C.c_int64(0)
+    out = _path_call(lib().mpoa_poa_kernel, ((ctx._h, cp.ref, vref, beg, end, q, len(q),
+                                              int(bool(adaptive)), band, bits), (C.byref(sc),)),
+                     len(q) + 16)
+    return (out, sc.value) if score else out
+
+
+def i32_scores_safe(params, graph, beg, end, qlen):
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    return bool(lib().mpoa_i32_scores_safe(cp.ref, vref, beg, end, qlen))
+
+
+def align(ctx, params, graph, query):
+    """src/align.rs:81-98 on the device."""
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    q = bytes(query)
+    return _path_call(lib().mpoa_align, ((ctx._h, cp.ref, vref, q, len(q)), ()), len(q) + 16)
+
+
+def result_or_error(fn, *args, **kw):
+    try:
+        return ("ok", fn(*args, **kw))
+    except HipPoaError as e:
+        return ("err", str(e))
+
+
+def validate(params):
+    _check(lib().mpoa_params_validate(CParams(params).ref))
+
+
+def align_sequences(ctx, params, seqs):
+    """src/lib.rs:177-182 with every DP on the device."""
+    cp = CParams(params)
+    raw = [bytes(s.seq) for s in seqs]
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_uint64 * max(1, len(raw)))(*[len(r) for r in raw])
+    h = C.c_void_p()
+    _check(lib().mpoa_align_sequences(ctx._h, cp.ref, arr, lens, len(raw), C.byref(h)))
+    return Graph(params, _handle=h)
+
+
+def extend_graph(ctx, params, graph, seqs, existing):
+    cp = CParams(params)
+    raw = [bytes(s.seq) for s in seqs]
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_uint64 * max(1, len(raw)))(*[len(r) for r in raw])
+    _check(lib().mpoa_extend_graph(ctx._h, cp.ref, graph._h, arr, lens, len(raw), existing))
+
+
+class BatchResult:
+    def __init__(self, handle, njobs):
+        self._h = handle
+        self.njobs = njobs
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpoa_batch_free(self._h)
+            self._h = None
+
+    def stats(self):
+        s = _Stats()
+        _check(lib().mpoa_batch_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _Stats._fields_}
+
+    def status(self, job):
+        return lib().mpoa_batch_job_status(self._h, job)
+
+    def output(self, job, mode):
+        cap = 1 << 16
+        while True:
+            buf = C.create_string_buffer(cap)
+            n = C.c_uint64(0)
+            rc = lib().mpoa_batch_job_output(self._h, job, mode, buf, cap, C.byref(n))
+            if rc == BUFFER_TOO_SMALL:
+                cap = n.value + 1
+                continue
+            _check(rc)
+            return buf.raw[:n.value].decode()
+
+
+def batch_run(ctx, params, job_off, read_off, bases):
+    """Batched progressive POA over numpy job/read offsets and a base buffer."""
+    import numpy as np
+
+    job_off = np.ascontiguousarray(job_off, dtype=np.uint64)
+    read_off = np.ascontiguousarray(read_off, dtype=np.uint64)
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    cp = CParams(params)
+    h = C.c_void_p()
+    _check(lib().mpoa_batch_run(ctx._h, cp.ref, len(job_off) - 1, job_off.ctypes.data_as(U64P),
+                                read_off.ctypes.data_as(U64P), bases.ctypes.data_as(C.c_void_p),
+                                C.byref(h)))
+    return BatchResult(h, len(job_off) - 1)
+
+
+__all__ = [
+    "AlignedBase", "Params", "Sequence", "records", "default_matrix", "HipPoaError", "Context",
+    "Graph", "GraphView", "view_from_graph", "poa", "poa_inner", "poa_kernel", "align",
+    "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
+    "validate", "result_or_error", "lib", "last_error", "SIGNATURES",
+]

@@ -1,0 +1,160 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+header declares, and its host graph (src/graph.rs semantics) matches the
+oracle byte for byte. No device compute here."""
+import random
+import re
+from pathlib import Path
+
+import pytest
+
+import minipoa_hip as mh
+from helpers import dna, validate_gfa, validate_msa
+from minipoa_hip import AlignedBase, Params, Sequence, records
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols():
+    text = (ROOT / "include" / "minipoa_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpoa_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = mh.lib()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(mh.SIGNATURES), set(syms) ^ set(mh.SIGNATURES)
+
+
+def test_backend_name_identifies_device_kernel():
+    """src/lib.rs:15-37 analogue: the backend string names the kernel width."""
+    name = mh.backend_name()
+    assert name.startswith("hip-gfx950") and "i32" in name
+
+
+def test_context_creation_fails_cleanly_without_device():
+    try:
+        ctx = mh.Context(0)
+    except mh.HipPoaError as e:
+        assert e.status in (13, 14)
+    else:
+        ctx.close()
+
+
+def test_params_validation_messages():
+    """src/lib.rs:149-173"""
+    mh.validate(Params())
+    cases = [
+        (Params(alphabet=b"", matrix=[]), "invalid scoring matrix dimensions"),
+        (Params(threads=0), "thread count must be positive"),
+        (Params(k=0), "k must be 1..31"),
+        (Params(gap_open=1), "gap penalties must be zero or negative"),
+        (Params(result=3), "result format and verbosity must be 0..2"),
+        (Params(mm_filter_ratio=1.5), "minimizer filter ratio"),
+        (Params(win_max=0), "window maximum must be positive"),
+    ]
+    for p, msg in cases:
+        with pytest.raises(mh.HipPoaError, match=re.escape(msg)):
+            mh.validate(p)
+
+
+def test_weights_and_consensus_ties_library_graph():
+    """src/graph.rs:344-380 on the library graph."""
+    p = Params()
+    g = mh.Graph(p)
+    B = AlignedBase
+    g.add_path(0, [B(0, 4, None), B(None, 0, None), B(None, 1, None)])
+    g.add_path(1, [B(0, 4, None), B(2, 0, None), B(None, 2, 3)])
+    g.add_path(2, [B(0, 4, None), B(2, 0, None), B(4, 2, None)])
+    g.toposort(False)
+    assert g.consensus(p)[0] == b"AG"
+    assert g.consensus(p)[2] == [3, 2]
+    assert g.aligned(3, 2) == 4
+
+
+def test_rejects_cycles_library_graph():
+    """src/graph.rs:381-392"""
+    p = Params()
+    g = mh.Graph(p)
+    a = g.add_node(0)
+    b = g.add_node(1)
+    g.add_edge(0, a)
+    g.add_edge(a, b)
+    g.add_edge(b, a)
+    g.add_edge(b, 1)
+    with pytest.raises(mh.HipPoaError, match="cycle"):
+        g.toposort(False)
+
+
+def test_add_path_rejects_bad_records():
+    """src/graph.rs:84-93"""
+    p = Params()
+    g = mh.Graph(p)
+    with pytest.raises(mh.HipPoaError, match="outside alphabet"):
+        g.add_path(0, [AlignedBase(0, 9, None)])
+    with pytest.raises(mh.HipPoaError, match="unknown node"):
+        g.add_path(0, [AlignedBase(7, 0, None)])
+
+
+def _same_graph(a, b, alpha):
+    assert a.n_nodes == b.n_nodes
+    assert a.rank == b.rank
+    for i in range(a.n_nodes):
+        na, nb = a.node(i), b.node(i)
+        for key in ("rank", "parent", "base", "incoming", "in_weights", "outgoing", "out_weights"):
+            assert na[key] == nb[key], (i, key)
+        for base in range(alpha):
+            assert a.aligned(i, base) == b.aligned(i, base)
+
+
+def test_library_graph_matches_oracle_fusion_and_rendering(oracle):
+    """The library graph fed the oracle's alignment paths reproduces the
+    oracle's nodes, edges, ranks, consensus, MSA and GFA exactly."""
+    rng = random.Random(20261)
+    for case in range(12):
+        p = Params(f=0 if case % 3 == 0 else 40, adaptive=case % 2 == 1, b=4 + case)
+        template = dna(30 + 7 * case, seed=1000 + case)
+        reads = [template]
+        for _ in range(5):
+            r = bytearray()
+            for c in template:
+                x = rng.random()
+                if x < 0.04:
+                    continue
+                r.append(rng.choice(b"ACGT") if x < 0.10 else c)
+                if rng.random() < 0.04:
+                    r.append(rng.choice(b"ACGT"))
+            reads.append(bytes(r))
+        seqs = records(reads)
+        og = oracle.Graph(p)
+        lg = mh.Graph(p)
+        for sid, s in enumerate(seqs):
+            path = oracle.align(p, og, s.seq)
+            og.add_path(sid, path)
+            og.toposort(False)
+            lg.add_path(sid, path)
+            lg.toposort(False)
+        _same_graph(og, lg, 5)
+        assert lg.consensus(p) == og.consensus(p)
+        assert lg.msa(p, seqs) == og.msa(p, seqs)
+        assert lg.gfa(p, seqs) == og.gfa(p, seqs)
+        # rendering the MSA restores ranks (tests/regressions.rs:15-34)
+        _same_graph(og, lg, 5)
+        validate_msa(lg.msa(p, seqs), seqs)
+        validate_gfa(lg.gfa(p, seqs), seqs)
+
+
+def test_library_graph_rna_decoding(oracle):
+    """tests/io_cli.rs:542-561: RNA input renders U."""
+    p = Params(is_rna=True)
+    og = oracle.Graph(p)
+    lg = mh.Graph(p)
+    path = oracle.align(p, og, b"acgu")
+    og.add_path(0, path)
+    lg.add_path(0, path)
+    og.toposort(False)
+    lg.toposort(False)
+    assert lg.consensus(p)[0] == og.consensus(p)[0] == b"ACGU"
