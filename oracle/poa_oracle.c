@@ -27,6 +27,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <immintrin.h>
+
 /* src/align.rs:16-23, 216-220 */
 #define NEG (-(INT64_C(1) << 50))
 #define WIDTH 8
@@ -1059,20 +1061,51 @@ void o_arena_clear(void) {
     t_arena_len = t_arena_cap = 0;
 }
 
-/* src/align.rs:459-491: three shift-max stages padded with SHIFT_FILL. */
+/* i32x8 lanes as AVX2 registers: lane-wise wrapping add and max, constant
+ * lane shifts (the `wide::i32x8` operations of src/align.rs:356-375). */
+typedef __m256i v8;
+static inline v8 ld8(const int32_t *p) { return _mm256_loadu_si256((const __m256i *)p); }
+static inline void st8(int32_t *p, v8 v) { _mm256_storeu_si256((__m256i *)p, v); }
+static inline v8 splat8(int32_t x) { return _mm256_set1_epi32(x); }
+static inline v8 add8(v8 a, v8 b) { return _mm256_add_epi32(a, b); }
+static inline v8 max8(v8 a, v8 b) { return _mm256_max_epi32(a, b); }
+/* [fill x k, a0 .. a(7-k)] */
+static inline v8 shift1(v8 a) {
+    v8 t = _mm256_permutevar8x32_epi32(a, _mm256_setr_epi32(0, 0, 1, 2, 3, 4, 5, 6));
+    return _mm256_blend_epi32(t, splat8(SHIFT_FILL), 0x01);
+}
+static inline v8 shift2(v8 a) {
+    v8 t = _mm256_permutevar8x32_epi32(a, _mm256_setr_epi32(0, 0, 0, 1, 2, 3, 4, 5));
+    return _mm256_blend_epi32(t, splat8(SHIFT_FILL), 0x03);
+}
+static inline v8 shift4(v8 a) {
+    v8 t = _mm256_permutevar8x32_epi32(a, _mm256_setr_epi32(0, 0, 0, 0, 0, 1, 2, 3));
+    return _mm256_blend_epi32(t, splat8(SHIFT_FILL), 0x0F);
+}
+
+/* src/align.rs:459-491: three shift-max stages padded with SHIFT_FILL, then
+ * the carry extended across the eight lanes. `ext_steps` = ext * (1..8). */
+static inline v8 insertion_prefix8(v8 md, int32_t carry, v8 open, v8 e1, v8 e2, v8 e4,
+                                   v8 ext_steps) {
+    v8 v = add8(md, open);
+    v = max8(v, add8(shift1(v), e1));
+    v = max8(v, add8(shift2(v), e2));
+    v = max8(v, add8(shift4(v), e4));
+    return max8(v, add8(splat8(carry), ext_steps));
+}
+
+static inline v8 ext_steps8(int32_t ext) {
+    int32_t t[8];
+    for (int l = 0; l < 8; l++) t[l] = (int32_t)((uint32_t)ext * (uint32_t)(l + 1));
+    return ld8(t);
+}
+
 void o_insertion_prefix(const int32_t md[8], int32_t carry, int32_t open, int32_t ext,
                         int32_t out[8]) {
-    int32_t v[8], t[8];
-    for (int l = 0; l < 8; l++) v[l] = wadd(md[l], open);
-    for (int sh = 1; sh <= 4; sh *= 2) {
-        int32_t e = (int32_t)((uint32_t)ext * (uint32_t)sh);
-        for (int l = 0; l < 8; l++) t[l] = wadd(l >= sh ? v[l - sh] : SHIFT_FILL, e);
-        for (int l = 0; l < 8; l++) v[l] = max32(v[l], t[l]);
-    }
-    for (int l = 0; l < 8; l++) {
-        int32_t c = wadd(carry, (int32_t)((uint32_t)ext * (uint32_t)(l + 1)));
-        out[l] = max32(v[l], c);
-    }
+    v8 r = insertion_prefix8(ld8(md), carry, splat8(open), splat8(ext),
+                             splat8((int32_t)((uint32_t)ext * 2u)),
+                             splat8((int32_t)((uint32_t)ext * 4u)), ext_steps8(ext));
+    st8(out, r);
 }
 
 /* src/align.rs:493-529 */
@@ -1125,6 +1158,9 @@ int o_poa_simd_i32(const op_params *p, const og_graph *g, size_t beg_id, size_t 
     size_t n = s.n, beg = s.beg, padded = s.padded;
     int32_t gap_ext = p->gap_ext;
     int32_t gap_open = p->gap_open + gap_ext;
+    const v8 ext = splat8(gap_ext), open = splat8(gap_open);
+    const v8 ext2 = splat8((int32_t)((uint32_t)gap_ext * 2u)), ext4 = splat8((int32_t)((uint32_t)gap_ext * 4u));
+    const v8 steps = ext_steps8(gap_ext);
     size_t alpha = p_alpha(p);
     int32_t *profiles = xmalloc(alpha * padded * sizeof(int32_t));
     for (size_t a = 0; a < alpha; a++)
@@ -1191,42 +1227,38 @@ int o_poa_simd_i32(const op_params *p, const og_graph *g, size_t beg_id, size_t 
             size_t ov_stop = stop < ps + pl ? stop : ps + pl;
             for (size_t j = ov_start; j < ov_stop; j += WIDTH) {
                 size_t local = j - start, prior = j - ps;
-                for (int l = 0; l < WIDTH; l++) {
-                    int32_t diagonal = data[pm + prior - 1 + l];
-                    int32_t prev_m = data[pm + prior + l];
-                    int32_t prev_d = data[pd + prior + l];
-                    int32_t mm = max32(data[mo + local + l], diagonal);
-                    int32_t dd = max32(max32(data[dof + local + l], wadd(prev_d, gap_ext)),
-                                       wadd(prev_m, gap_open));
-                    data[mo + local + l] = mm;
-                    data[dof + local + l] = dd;
-                }
+                v8 diagonal = ld8(data + pm + prior - 1);
+                v8 prev_m = ld8(data + pm + prior);
+                v8 prev_d = ld8(data + pd + prior);
+                v8 mm = max8(ld8(data + mo + local), diagonal);
+                v8 dd = max8(max8(ld8(data + dof + local), add8(prev_d, ext)), add8(prev_m, open));
+                st8(data + mo + local, mm);
+                st8(data + dof + local, dd);
             }
         }
         size_t base = i == n - 1 ? s.unknown : node->base;
         const int32_t *profile = profiles + base * padded;
         int32_t carry = NEG32;
         for (size_t local = 0; local < len; local += WIDTH) {
-            int32_t md[8], next[8];
-            for (int l = 0; l < 8; l++) {
-                int32_t diagonal = wadd(data[mo + local + l], profile[start + local + l]);
-                md[l] = max32(diagonal, data[dof + local + l]);
-            }
-            o_insertion_prefix(md, carry, gap_open, gap_ext, next);
-            int32_t current[8];
-            current[0] = carry;
-            for (int l = 1; l < 8; l++) current[l] = next[l - 1];
-            for (int l = 0; l < 8; l++) {
-                data[io + local + l] = current[l];
-                data[mo + local + l] = max32(md[l], current[l]);
-            }
-            carry = next[7];
+            v8 diagonal = add8(ld8(data + mo + local), ld8(profile + start + local));
+            v8 md = max8(diagonal, ld8(data + dof + local));
+            v8 next = insertion_prefix8(md, carry, open, ext, ext2, ext4, steps);
+            /* current = [carry, next0 .. next6] */
+            v8 current = _mm256_permutevar8x32_epi32(next, _mm256_setr_epi32(0, 0, 1, 2, 3, 4, 5, 6));
+            current = _mm256_blend_epi32(current, splat8(carry), 0x01);
+            st8(data + io + local, current);
+            st8(data + mo + local, max8(md, current));
+            carry = _mm256_extract_epi32(next, 7);
         }
         if (p->f != 0 && s.adaptive) {
             /* fold over whole 8-lane chunks from splat(NEG32), then the first
              * position holding the maximum (rows are multiples of eight). */
-            int32_t best = NEG32;
-            for (size_t c = 0; c < len / WIDTH * WIDTH; c++) best = max32(best, data[mo + c]);
+            v8 bv = splat8(NEG32);
+            for (size_t c = 0; c + WIDTH <= len; c += WIDTH) bv = max8(bv, ld8(data + mo + c));
+            int32_t lanes[8];
+            st8(lanes, bv);
+            int32_t best = lanes[0];
+            for (int l = 1; l < 8; l++) best = max32(best, lanes[l]);
             size_t argmax = 0;
             while (argmax + 1 < len && data[mo + argmax] != best) argmax++;
             adaptive_update(g, &s, i, start, (int64_t)(start + argmax));
