@@ -403,6 +403,7 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     std::vector<JobPlan> plans;
     std::vector<JobPlan *> dp;
     std::vector<std::vector<AlignedBase>> paths(njobs);
+    std::vector<uint64_t> prev_used(njobs, 0), prev_n(njobs, 0);
     for (uint32_t s = 0; s < max_reads; s++) {
         auto t0 = std::chrono::steady_clock::now();
         active.clear();
@@ -424,7 +425,15 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
                 for (uint64_t t = 0; t < qlen; t++) v.push_back(AlignedBase{-1, int64_t(p.encode(q[t])), -1});
                 return;
             }
-            plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, plans[k]);
+            JobPlan &J = plans[k];
+            plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, J);
+            // Adaptive bands: start from what the previous step's DP used,
+            // scaled to the grown graph, so arenas rarely overflow.
+            if (J.status == MPOA_OK && (J.flags & kAdaptive) && prev_n[j] > 0) {
+                uint64_t hint = prev_used[j] * J.n / prev_n[j];
+                hint += hint / 8 + 3ull * J.padded * 8;
+                J.arena_est = std::min(std::max(J.arena_est, hint), arena_full(J));
+            }
         });
         dp.clear();
         std::vector<size_t> dp_k;
@@ -487,6 +496,8 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
                     B->err[j] = J->err;
                     return;
                 }
+                prev_used[j] = R->arena_used;
+                prev_n[j] = J->n;
                 std::string msg;
                 int sst = device_status(*R, msg);
                 if (sst != MPOA_OK) {

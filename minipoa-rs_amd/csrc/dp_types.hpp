@@ -3,8 +3,7 @@
 // One DP job = one `poa_inner` call (src/align.rs:531-545): a graph rank range
 // [beg, end] (n rows, in rank order) against one query (m = qlen + 2 columns).
 // Everything the kernels read is flattened into pools in rank order: per-row
-// records, the row's in-range predecessors (in `incoming` order, which decides
-// traceback ties) and in-range successors (adaptive band propagation).
+// records and each row's in-range predecessors.
 #pragma once
 #include <cstdint>
 
@@ -21,12 +20,12 @@ enum JobFlags : uint32_t {
     kGapOpenZero = 1u << 2, // params.gap_open == 0 (traceback state rule)
 };
 
-// Per row (n + 1 entries per job; entry n carries the end offsets).
+// Per row (n + 1 entries per job; entry n carries the end offset).
 struct RowIn {
     uint32_t pred_beg;  // into the job's predecessor list
-    uint32_t succ_beg;  // into the job's successor list
     int32_t head;       // most-supported distance from beg (kHeadNone = NEG)
     int32_t tail;       // most-supported distance to end (kHeadNone = NEG)
+    uint32_t pad_;
 };
 
 struct RowAux {
@@ -34,9 +33,20 @@ struct RowAux {
     uint32_t base;      // node base (alphabet index)
 };
 
+// In-range predecessors of a row (head != NEG), sorted by row: the adaptive
+// band state is folded from them in processing order. `ord` is the index in
+// the node's `incoming` list, which breaks traceback weight ties.
 struct PredE {
-    uint32_t row;       // predecessor row (rank - beg), head != NEG
+    uint32_t row;       // predecessor row (rank - beg)
     uint32_t weight;    // in_weight of that edge
+    uint32_t ord;       // position in `incoming`
+};
+
+// Final adaptive band state of a computed row (src/align.rs:829-897): its
+// M argmax column, match streak, band offsets and extra width. Successors
+// fold these in rank order, which reproduces the reference's pushes.
+struct RowRec {
+    int64_t peak, streak, ofl, ofr, extra;
 };
 
 // Row geometry written by the fill kernel, read by traceback.
@@ -49,13 +59,12 @@ struct RowMeta {
 struct DevJob {
     uint64_t rows;      // RowIn/RowAux index of row 0
     uint64_t preds;     // PredE index of the job's first entry
-    uint64_t succs;     // successor index
     uint64_t seq;       // byte offset of the encoded, padded query
     uint64_t matrix;    // int32 index of the alpha x alpha matrix
     uint64_t arena;     // element offset of the job's arena
     uint64_t arena_cap; // elements available
     uint64_t rowmeta;   // RowMeta index of row 0
-    uint64_t adapt;     // int64 index of the 7 x n adaptive state
+    uint64_t adapt;     // RowRec index of row 0 (adaptive jobs)
     uint64_t tb;        // uint32 index of the m-entry traceback record
     uint32_t n, m, padded, alpha;
     int64_t band;

@@ -1,20 +1,29 @@
 // gfx950 kernels for the affine-gap sequence-to-DAG DP (minipoa-rs
 // src/align.rs:587-1347), integer max-plus only: no MFMA.
 //
-// Fill: one wave64 per job walks the rows in rank order. A row's band is
-// computed on the device (static, full or adaptive), then the wave covers it
-// in segments of 64*C columns, lane l owning C contiguous columns. Per lane:
-// predecessor max-plus (M <- max M_p[j-1], D <- max(D_p+e, M_p+o)), profile
-// add, and a serial in-lane insertion prefix; one Kogge-Stone max-plus scan
-// across the 64 lanes resolves the insertion carries exactly (integer
-// max-plus is associative, so any bracketing equals the serial recurrence
-// I[j+1] = max(I[j]+e, md[j]+o) that the reference pins in
-// src/align.rs:1568-1601). Adaptive bands take a wave first-argmax of M and
-// lane 0 updates the per-row band state (src/align.rs:829-897).
+// Fill: one wave64 per job walks the rows in rank order; a row is a serial
+// dependency of its predecessors, so the design goal is a short per-row
+// critical path with no global-memory round trip on it:
+//  * recent rows' M and D stay in a per-wave LDS byte ring; predecessors
+//    (85% within 4 ranks, all but a few within 16) read from there, older or
+//    evicted rows from the global arena;
+//  * row descriptors and predecessor lists arrive in 64-row blocks, loaded
+//    coalesced into LDS;
+//  * the adaptive band state is pulled at each row from its predecessors'
+//    final per-row records, folded in rank order (the order in which the
+//    reference pushes them to successors, src/align.rs:866-896), so no row
+//    writes another row's state;
+//  * the row is covered in segments of 64*C columns, lane l owning C
+//    contiguous columns; the insertion recurrence I[j+1] = max(I[j]+e,
+//    md[j]+o) runs serially inside a lane and across lanes as a DPP max-plus
+//    Kogge-Stone scan (integer max-plus is associative, so this equals the
+//    serial recurrence exactly, as src/align.rs:1568-1601 requires);
+//  * the first-argmax of M for adaptive bands is two DPP reductions.
+// M, D and I of every cell are written to the global arena for traceback.
 //
 // Traceback: one lane per job replays the weighted deterministic traceback
-// (src/align.rs:1164-1313) over the stored M/D/I and writes, per query
-// column, the row of its diagonal step or an insertion mark.
+// (src/align.rs:1164-1313) and writes, per query column, the row of its
+// diagonal step or an insertion mark.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -27,8 +36,8 @@ namespace mpoa {
 template <typename T> struct Lim;
 template <> struct Lim<int32_t> {
     static constexpr int32_t neg = kNeg32;
-    // "No contribution" for the scan; stays below every guarded state and
-    // above INT32_MIN after the bounded additions of one row.
+    // "No contribution" for scans: below every guarded state, and above
+    // INT32_MIN after the bounded additions of one row.
     static constexpr int32_t ninf = -(1 << 30) - (1 << 29) - (1 << 28);
 };
 template <> struct Lim<int64_t> {
@@ -41,61 +50,222 @@ template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b
 
 constexpr uint32_t kMatch = 1, kDelete = 2, kInsert = 4, kAll = 7;
 
+// DPP controls (GFX9 encoding).
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kBcast15 = 0x142, kBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// DPP move with `old` in lanes whose source is out of range or masked off.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int32_t dpp(int32_t old, int32_t v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(int32_t(old), int32_t(v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int64_t dpp(int64_t old, int64_t v) {
+    const int32_t lo = __builtin_amdgcn_update_dpp(int32_t(uint64_t(old)), int32_t(uint64_t(v)),
+                                                   CTRL, ROWMASK, 0xF, false);
+    const int32_t hi = __builtin_amdgcn_update_dpp(int32_t(uint64_t(old) >> 32),
+                                                   int32_t(uint64_t(v) >> 32), CTRL, ROWMASK,
+                                                   0xF, false);
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+__device__ __forceinline__ int32_t lane63(int32_t v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ uint32_t lane63(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_readlane(int32_t(v), 63));
+}
+__device__ __forceinline__ int64_t lane63(int64_t v) {
+    const int32_t lo = __builtin_amdgcn_readlane(int32_t(uint64_t(v)), 63);
+    const int32_t hi = __builtin_amdgcn_readlane(int32_t(uint64_t(v) >> 32), 63);
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+// Inclusive max-plus scan S_l = max_{j<=l} (v_j + (l-j)*a) over the wave.
+// d15/d31 = ((l&15)+1)*a and ((l&31)+1)*a: the decay from the broadcast lane.
+template <typename T>
+__device__ __forceinline__ T maxplus_scan(T v, T a, T d15, T d31) {
+    constexpr T NI = Lim<T>::ninf;
+    v = tmax(v, dpp<kRowShr1, 0xF>(NI, v) + a);
+    v = tmax(v, dpp<kRowShr2, 0xF>(NI, v) + 2 * a);
+    v = tmax(v, dpp<kRowShr4, 0xF>(NI, v) + 4 * a);
+    v = tmax(v, dpp<kRowShr8, 0xF>(NI, v) + 8 * a);
+    v = tmax(v, dpp<kBcast15, 0xA>(NI, v) + d15);
+    v = tmax(v, dpp<kBcast31, 0xC>(NI, v) + d31);
+    return v;
+}
+
+// Wave max / min reductions; the result is uniform.
+template <typename T> __device__ __forceinline__ T wave_max(T v, T lo) {
+    v = tmax(v, dpp<kRowShr1, 0xF>(lo, v));
+    v = tmax(v, dpp<kRowShr2, 0xF>(lo, v));
+    v = tmax(v, dpp<kRowShr4, 0xF>(lo, v));
+    v = tmax(v, dpp<kRowShr8, 0xF>(lo, v));
+    v = tmax(v, dpp<kBcast15, 0xA>(lo, v));
+    v = tmax(v, dpp<kBcast31, 0xC>(lo, v));
+    return lane63(v);
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+    constexpr uint32_t hi = 0xFFFFFFFFu;
+    v = tmin(v, dpp<kRowShr1, 0xF>(hi, v));
+    v = tmin(v, dpp<kRowShr2, 0xF>(hi, v));
+    v = tmin(v, dpp<kRowShr4, 0xF>(hi, v));
+    v = tmin(v, dpp<kRowShr8, 0xF>(hi, v));
+    v = tmin(v, dpp<kBcast15, 0xA>(hi, v));
+    v = tmin(v, dpp<kBcast31, 0xC>(hi, v));
+    return lane63(v);
+}
+
+// Per-wave LDS budget.
+constexpr uint32_t kRingBytes = 16384;  // recent rows' M and D
+constexpr uint32_t kBlk = 64;           // rows per descriptor block
+constexpr uint32_t kPredCache = 256;    // predecessor entries cached per block
+constexpr uint32_t kHist = 64;          // rows of metadata/records kept in LDS
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kMatLds = 640;       // matrices up to 25 x 25 live in LDS
+
+struct LMeta {
+    uint32_t start, len;
+    uint32_t slot;   // byte offset in the ring, or kNoSlot
+    uint32_t wpos;   // ring write position (monotonic bytes) of the slot
+};
+
+// Global-memory fallbacks of the LDS caches. Out of line so the compiler
+// cannot fold an LDS load and a global load into one load through a
+// selected generic pointer (the gfx950 backend rejects that form).
+__device__ __attribute__((noinline)) PredE pred_global(const PredE *pr, uint32_t q) { return pr[q]; }
+__device__ __attribute__((noinline)) RowMeta meta_global(const RowMeta *meta, uint32_t p) {
+    return meta[p];
+}
+__device__ __attribute__((noinline)) int32_t mat_global(const int32_t *mat, uint32_t k) {
+    return mat[k];
+}
+__device__ __attribute__((noinline)) RowRec rec_global(const RowRec *rec, uint32_t p) {
+    return rec[p];
+}
+
 template <typename T, int C>
 __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
+    __shared__ __align__(16) T ring[kRingBytes / sizeof(T)];  // element-indexed
+    __shared__ LMeta hmeta[kHist];
+    __shared__ RowRec hrec[kHist];
+    __shared__ RowIn drow[kBlk + 1];
+    __shared__ RowAux daux[kBlk];
+    __shared__ PredE dpred[kPredCache];
+    __shared__ int32_t smat[kMatLds];
+
     const DevJob J = P.jobs[blockIdx.x];
     const int lane = threadIdx.x;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
     const RowAux *aux = P.aux + J.rows;
     const PredE *pr = P.preds + J.preds;
-    const uint32_t *sc = P.succs + J.succs;
     const uint8_t *seq = P.seq + J.seq;
-    const int32_t *mat = P.matrix + J.matrix;
+    const int32_t *gmat = P.matrix + J.matrix;
     T *arena = reinterpret_cast<T *>(P.arena) + J.arena;
     RowMeta *meta = P.meta + J.rowmeta;
+    RowRec *grec = P.adapt + J.adapt;
     const bool adaptive = (J.flags & kAdaptive) != 0;
     const bool full = (J.flags & kFull) != 0;
-    int64_t *peak = P.adapt + J.adapt;
-    int64_t *streak = peak + n, *pkl = peak + 2 * size_t(n), *pkr = peak + 3 * size_t(n);
-    int64_t *ofl = peak + 4 * size_t(n), *ofr = peak + 5 * size_t(n), *extra = peak + 6 * size_t(n);
     const T e = T(J.e), o = T(J.o);
     const T a = T(C) * e;
-    constexpr T NEG = Lim<T>::neg, NINF = Lim<T>::ninf;
+    const T d15 = T((lane & 15) + 1) * a, d31 = T((lane & 31) + 1) * a;
+    constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
     const int64_t mm = int64_t(m);
-
-    if (adaptive) {
-        for (uint32_t r = lane; r < n; r += 64) {
-            peak[r] = 0;
-            streak[r] = 0;
-            pkl[r] = mm;
-            pkr[r] = 0;
-            ofl[r] = 0;
-            ofr[r] = 0;
-            extra[r] = 0;
-        }
-        __syncthreads();
-    }
+    const uint32_t alpha = J.alpha;
+    const bool mat_lds = alpha * alpha <= kMatLds;
+    if (mat_lds)
+        for (uint32_t k = lane; k < alpha * alpha; k += 64) smat[k] = gmat[k];
 
     uint64_t cursor = 0, cells = 0, pcells = 0;
+    uint32_t wpos = 0;      // ring bytes handed out so far
+    uint32_t pb0 = 0, pcnt = 0;
     int32_t status = kOk;
+
+    // Predecessor entry q: cached block entries, else the global list.
+    // (if/else rather than ?: on lvalues: a select between an LDS and a
+    // global address would go through a generic pointer.)
+    auto pred_at = [&](uint32_t q) -> PredE {
+        const uint32_t k = q - pb0;
+        if (k < pcnt) return dpred[k];
+        return pred_global(pr, q);
+    };
+    // Metadata of an earlier row p < i.
+    auto meta_of = [&](uint32_t p, uint32_t i) -> LMeta {
+        if (i - p < kHist) return hmeta[p % kHist];
+        const RowMeta g = meta_global(meta, p);
+        return LMeta{g.start, g.len, kNoSlot, 0};
+    };
+    auto rec_of = [&](uint32_t p, uint32_t i) -> RowRec {
+        if (i - p < kHist) return hrec[p % kHist];
+        return rec_global(grec, p);
+    };
+
     for (uint32_t i = 0; i < n; i++) {
-        const RowIn ri = rin[i];
+        if (i % kBlk == 0) {
+            __syncthreads();
+            const uint32_t cnt = tmin(kBlk, n - i);
+            for (uint32_t k = lane; k <= cnt; k += 64) drow[k] = rin[i + k];
+            for (uint32_t k = lane; k < cnt; k += 64) daux[k] = aux[i + k];
+            pb0 = rin[i].pred_beg;
+            pcnt = tmin(rin[i + cnt].pred_beg - pb0, kPredCache);
+            for (uint32_t k = lane; k < pcnt; k += 64) dpred[k] = pr[pb0 + k];
+            __syncthreads();
+        }
+        const RowIn ri = drow[i % kBlk];
+        const uint32_t pb = ri.pred_beg, pe = drow[i % kBlk + 1].pred_beg;
         if (ri.head == kHeadNone || ri.tail == kHeadNone) {
-            if (lane == 0) meta[i] = RowMeta{0, 0, 0};
+            // Skipped row (src/align.rs:708-710): no band, never pushes state.
+            if (lane == 0) {
+                hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0};
+                hrec[i % kHist] = RowRec{0, 0, 0, 0, 0};
+                meta[i] = RowMeta{0, 0, 0};
+                if (adaptive) grec[i] = RowRec{0, 0, 0, 0, 0};
+            }
+            __syncthreads();
             continue;
         }
+        const int64_t h = ri.head, t = ri.tail;
+
+        // Adaptive state of row i: the reference's pushes from its computed
+        // predecessors (rows >= 1), replayed in rank order.
+        int64_t ofl = 0, ofr = 0, pkl = mm, pkr = 0, extra = 0;
+        // Rows older than the LDS history are read back from global memory,
+        // where this wave wrote them earlier.
+        bool far = false;
+        #pragma clang loop vectorize(disable) interleave(disable)
+        for (uint32_t q = pb; q < pe; q++) far |= (i - pred_at(q).row) >= kHist;
+        if (far) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (adaptive) {
+            #pragma clang loop vectorize(disable) interleave(disable)
+            for (uint32_t q = pb; q < pe; q++) {
+                const uint32_t p = pred_at(q).row;
+                if (p == 0 || meta_of(p, i).len == 0) continue;
+                const RowRec r = rec_of(p, i);
+                if (r.streak >= 30) {
+                    ofl = r.ofl;
+                    ofr = r.ofr;
+                }
+                ofl = tmin(ofl, r.ofl);
+                ofr = tmax(ofr, r.ofr);
+                pkl = tmin(pkl, r.peak);
+                pkr = tmax(pkr, r.peak + 1);
+                extra = r.extra != 0 ? tmax(extra, r.extra) : 0;
+            }
+        }
+
         // Row band (src/align.rs:711-734).
         int64_t left, right;
-        const int64_t h = ri.head, t = ri.tail;
         if (full) {
             left = 0;
             right = mm;
         } else if (adaptive) {
-            int64_t lo = tmin(tmin(pkl[i], h + ofl[i]), mm - t);
-            int64_t hi = tmax(tmax(pkr[i], h + ofr[i]), mm - t);
-            left = tmax<int64_t>(lo - J.band - extra[i], 0);
-            right = tmin<int64_t>(hi + J.band + extra[i], mm);
+            const int64_t lo = tmin(tmin(pkl, h + ofl), mm - t);
+            const int64_t hi = tmax(tmax(pkr, h + ofr), mm - t);
+            left = tmax<int64_t>(lo - J.band - extra, 0);
+            right = tmin<int64_t>(hi + J.band + extra, mm);
         } else {
             left = tmax<int64_t>(tmin(h, mm - t) - J.band, 0);
             right = tmin<int64_t>(tmax(h, mm - t) + J.band, mm);
@@ -114,10 +284,39 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         const uint64_t off = cursor;
         cursor += 3ull * len;
         cells += len;
-        if (lane == 0) meta[i] = RowMeta{start, len, off};
         T *Mr = arena + off;
         T *Dr = Mr + len;
         T *Ir = Dr + len;
+
+        // Ring slot for this row's M and D.
+        const uint32_t bytes = 2u * len * uint32_t(sizeof(T));
+        uint32_t slot = kNoSlot, swpos = 0;
+        if (bytes <= kRingBytes / 2) {
+            uint32_t at = wpos % kRingBytes;
+            if (at + bytes > kRingBytes) {
+                wpos += kRingBytes - at;
+                at = 0;
+            }
+            slot = at;
+            swpos = wpos;
+            wpos += bytes;
+        }
+        // Ring element offsets of this row's M and D.
+        const uint32_t sm = (slot == kNoSlot ? 0u : slot) / uint32_t(sizeof(T));
+        const uint32_t sd = sm + len;
+        // Predecessors evicted from the ring (or never in it) also come back
+        // from global memory.
+        bool evicted = false;
+        #pragma clang loop vectorize(disable) interleave(disable)
+        for (uint32_t q = pb; q < pe; q++) {
+            const uint32_t p = pred_at(q).row;
+            if (i - p < kHist) {
+                const LMeta pm = hmeta[p % kHist];
+                evicted |= pm.len != 0 && (pm.slot == kNoSlot || wpos - pm.wpos > kRingBytes);
+            }
+        }
+        if (evicted) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (lane == 0) meta[i] = RowMeta{start, len, off};
 
         if (i == 0) {
             // Source row (src/align.rs:737-750): M[0] = score(N,N), then
@@ -128,28 +327,33 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             }
             if (lane == 0) {
                 T mprev = T(J.score_nn), iprev = NEG;
-                Mr[0] = mprev;
-                Dr[0] = NEG;
-                Ir[0] = NEG;
-                for (uint32_t j = 1; j < len; j++) {
-                    T ins = tmax(iprev + e, mprev + o);
-                    T mj = tmax(NEG, ins);
+                for (uint32_t j = 0; j < len; j++) {
+                    T mj = mprev, ins = iprev;
+                    if (j > 0) {
+                        ins = tmax(iprev + e, mprev + o);
+                        mj = tmax(NEG, ins);
+                    }
                     Mr[j] = mj;
                     Dr[j] = NEG;
                     Ir[j] = ins;
+                    if (slot != kNoSlot) {
+                        ring[sm + j] = mj;
+                        ring[sd + j] = NEG;
+                    }
                     iprev = ins;
                     mprev = mj;
                 }
+                hmeta[0] = LMeta{start, len, slot, swpos};
+                hrec[0] = RowRec{0, 0, 0, 0, 0};
+                if (adaptive) grec[0] = RowRec{0, 0, 0, 0, 0};
             }
             __syncthreads();
             continue;
         }
 
-        const uint32_t sbase = (i == n - 1) ? J.unknown : aux[i].base;
-        const int32_t *mrow = mat + size_t(sbase) * J.alpha;
-        const uint32_t pb = ri.pred_beg, pe = rin[i + 1].pred_beg;
+        const uint32_t sbase = (i == n - 1) ? J.unknown : daux[i % kBlk].base;
         T carry = NEG;
-        T best = NINF;
+        T best = NI;
         uint32_t bestc = 0xFFFFFFFFu;
         for (uint32_t seg = start; seg < stop; seg += 64u * C) {
             const uint32_t c0 = seg + uint32_t(lane) * C;
@@ -161,50 +365,64 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             }
             // Predecessor transitions on the band overlap (src/align.rs:751-797);
             // the diagonal into the predecessor's first column reads NEG.
+            #pragma clang loop vectorize(disable) interleave(disable)
             for (uint32_t q = pb; q < pe; q++) {
-                const PredE pq = pr[q];
-                const RowMeta pm = meta[pq.row];
-                const uint32_t ovs = start > pm.start ? start : pm.start;
-                const uint32_t ove = stop < pm.start + pm.len ? stop : pm.start + pm.len;
+                const uint32_t p = pred_at(q).row;
+                const LMeta pm = meta_of(p, i);
+                const uint32_t ovs = tmax(start, pm.start);
+                const uint32_t ove = tmin(stop, pm.start + pm.len);
                 if (ovs >= ove) continue;
                 if (seg == start) pcells += ove - ovs;
-                const T *PM = arena + pm.off;
-                const T *PD = PM + pm.len;
+                const bool in_ring = pm.slot != kNoSlot && wpos - pm.wpos <= kRingBytes;
+                if (in_ring) {
+                    const uint32_t pmo = pm.slot / uint32_t(sizeof(T));
+                    const uint32_t pdo = pmo + pm.len;
 #pragma unroll
-                for (int k = 0; k < C; k++) {
-                    const uint32_t col = c0 + k;
-                    if (col >= ovs && col < ove) {
-                        const uint32_t rel = col - pm.start;
-                        const T dg = rel > 0 ? PM[rel - 1] : NEG;
-                        const T pmv = PM[rel];
-                        const T pdv = PD[rel];
-                        Mv[k] = tmax(Mv[k], dg);
-                        Dv[k] = tmax(Dv[k], tmax(pdv + e, pmv + o));
+                    for (int k = 0; k < C; k++) {
+                        const uint32_t col = c0 + k;
+                        if (col >= ovs && col < ove) {
+                            const uint32_t rel = col - pm.start;
+                            const T dg = rel > 0 ? ring[pmo + rel - 1] : NEG;
+                            Mv[k] = tmax(Mv[k], dg);
+                            Dv[k] = tmax(Dv[k], tmax(ring[pdo + rel] + e, ring[pmo + rel] + o));
+                        }
+                    }
+                } else {
+                    const RowMeta gm = meta[p];
+                    const T *PM = arena + gm.off;
+                    const T *PD = PM + gm.len;
+#pragma unroll
+                    for (int k = 0; k < C; k++) {
+                        const uint32_t col = c0 + k;
+                        if (col >= ovs && col < ove) {
+                            const uint32_t rel = col - gm.start;
+                            const T dg = rel > 0 ? PM[rel - 1] : NEG;
+                            Mv[k] = tmax(Mv[k], dg);
+                            Dv[k] = tmax(Dv[k], tmax(PD[rel] + e, PM[rel] + o));
+                        }
                     }
                 }
             }
             // Profile + in-lane insertion prefix without carry.
             T md[C], Lk[C];
-            T L = NINF;
+            T L = NI;
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 const uint32_t col = c0 + k;
-                T v = NINF;
-                if (col < stop) v = tmax(Mv[k] + T(mrow[seq[col]]), Dv[k]);
+                T v = NI;
+                if (col < stop) {
+                    const uint32_t sidx = sbase * alpha + seq[col];
+                    const T prof = T(mat_lds ? smat[sidx] : mat_global(gmat, sidx));
+                    v = tmax(Mv[k] + prof, Dv[k]);
+                }
                 md[k] = v;
                 Lk[k] = L;
                 L = tmax(L + e, v + o);
             }
-            // Wave inclusive scan S_l = max_{j<=l} (L_j + (l-j)*C*e).
-            T S = L;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const T tv = __shfl_up(S, d, 64);
-                if (lane >= d) S = tmax(S, tv + T(d) * a);
-            }
-            const T Sp = __shfl_up(S, 1, 64);
-            T X = carry + T(lane) * a;
-            if (lane > 0) X = tmax(X, Sp);
+            // Carry into each lane: X_l = max(carry + l*a, S_{l-1}).
+            const T S = maxplus_scan<T>(L, a, d15, d31);
+            const T Sp = dpp<kWaveShr1, 0xF>(NI, S);
+            const T X = tmax(carry + T(lane) * a, Sp);
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 const uint32_t col = c0 + k;
@@ -215,73 +433,61 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                     Mr[rel] = Mk;
                     Dr[rel] = Dv[k];
                     Ir[rel] = Ik;
+                    if (slot != kNoSlot) {
+                        ring[sm + rel] = Mk;
+                        ring[sd + rel] = Dv[k];
+                    }
                     if (Mk > best) {
                         best = Mk;
                         bestc = col;
                     }
                 }
             }
-            const T cout = tmax(X + a, L);
-            carry = __shfl(cout, 63, 64);
+            carry = lane63(tmax(X + a, L));
+        }
+
+        RowRec rec{0, 0, ofl, ofr, extra};
+        if (adaptive) {
+            // First argmax of M over the row (src/align.rs:830-844), then the
+            // streak / extra / offset update of src/align.rs:845-865.
+            const T gmax = wave_max<T>(best, NI);
+            const uint32_t maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+            const int64_t maximum = int64_t(maxc);
+            int64_t streak = 0, max_streak = 0;
+            bool is_match = false;
+            if (daux[i % kBlk].base == seq[maxc]) {
+                #pragma clang loop vectorize(disable) interleave(disable)
+                for (uint32_t q = pb; q < pe; q++) {
+                    const uint32_t p = pred_at(q).row;
+                    const RowRec r = rec_of(p, i);
+                    max_streak = tmax(max_streak, r.streak);
+                    if (r.peak + 1 == maximum) {
+                        streak = tmax(streak, r.streak + 1);
+                        is_match = true;
+                    }
+                }
+            }
+            if (!is_match && max_streak < 30) extra = tmin(extra + max_streak + 1, 5 * J.band);
+            if (streak >= 30) {
+                ofl = maximum - h;
+                ofr = maximum - h;
+                extra = 0;
+            }
+            rec = RowRec{maximum, streak, ofl, ofr, extra};
+        }
+        if (lane == 0) {
+            hmeta[i % kHist] = LMeta{start, len, slot, swpos};
+            hrec[i % kHist] = rec;
+            if (adaptive) grec[i] = rec;
         }
         __syncthreads();
-
-        if (adaptive) {
-            // First argmax of M over the whole row (src/align.rs:830-844).
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                const T ob = __shfl_xor(best, d, 64);
-                const uint32_t oc = __shfl_xor(bestc, d, 64);
-                if (ob > best || (ob == best && oc < bestc)) {
-                    best = ob;
-                    bestc = oc;
-                }
-            }
-            if (lane == 0) {
-                const int64_t maximum = int64_t(bestc);
-                peak[i] = maximum;
-                int64_t max_streak = 0;
-                bool is_match = false;
-                if (aux[i].base == seq[bestc]) {
-                    for (uint32_t q = pb; q < pe; q++) {
-                        const uint32_t p = pr[q].row;
-                        max_streak = tmax(max_streak, streak[p]);
-                        if (peak[p] + 1 == maximum) {
-                            streak[i] = tmax(streak[i], streak[p] + 1);
-                            is_match = true;
-                        }
-                    }
-                }
-                if (!is_match && max_streak < 30)
-                    extra[i] = tmin(extra[i] + max_streak + 1, 5 * J.band);
-                const uint32_t sb = ri.succ_beg, se = rin[i + 1].succ_beg;
-                if (streak[i] >= 30) {
-                    ofl[i] = maximum - h;
-                    ofr[i] = maximum - h;
-                    extra[i] = 0;
-                    for (uint32_t q = sb; q < se; q++) {
-                        ofl[sc[q]] = ofl[i];
-                        ofr[sc[q]] = ofr[i];
-                    }
-                }
-                for (uint32_t q = sb; q < se; q++) {
-                    const uint32_t s = sc[q];
-                    ofl[s] = tmin(ofl[s], ofl[i]);
-                    ofr[s] = tmax(ofr[s], ofr[i]);
-                    pkl[s] = tmin(pkl[s], maximum);
-                    pkr[s] = tmax(pkr[s], maximum + 1);
-                    extra[s] = extra[i] != 0 ? tmax(extra[s], extra[i]) : 0;
-                }
-            }
-            __syncthreads();
-        }
     }
     if (lane == 0) {
         DevResult R{};
         R.status = status;
         R.cells = cells;
-        R.arena_used = cursor;
         R.pred_cells = pcells;
+        R.arena_used = cursor;
         R.score = int64_t(NEG);
         if (status == kOk) {
             const RowMeta rm = meta[n - 1];
@@ -339,12 +545,13 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         const uint32_t base = (i == n - 1) ? J.unknown : aux[i].base;
         const int64_t score = mat[size_t(base) * J.alpha + seq[j]];
         const uint32_t pb = rin[i].pred_beg, pe = rin[i + 1].pred_beg;
-        // `choose` (src/align.rs:1201-1225): heaviest in-range predecessor
-        // (strict >, first wins) whose state plus penalty equals the target.
+        // `choose` (src/align.rs:1201-1225): among in-range predecessors whose
+        // state plus penalty equals the target, the largest in_weight; ties go
+        // to the earliest in `incoming` order.
         auto choose = [&](uint32_t state, int64_t penalty, uint32_t pj) -> int64_t {
             const int64_t target = state == kMatch ? Mij : Dij;
             int64_t bestq = -1;
-            uint32_t w = 0;
+            uint32_t w = 0, bord = 0;
             for (uint32_t q = pb; q < pe; q++) {
                 const PredE pq = pr[q];
                 const RowMeta pm = meta[pq.row];
@@ -352,9 +559,11 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
                 const T *PM = arena + pm.off;
                 const int64_t src = state == kDelete ? int64_t(PM[pm.len + pj - pm.start])
                                                      : int64_t(PM[pj - pm.start]);
-                if (target == src + penalty && (bestq < 0 || pq.weight > w)) {
+                if (target == src + penalty &&
+                    (bestq < 0 || pq.weight > w || (pq.weight == w && pq.ord < bord))) {
                     bestq = q;
                     w = pq.weight;
+                    bord = pq.ord;
                 }
             }
             return bestq;

@@ -16,12 +16,11 @@ struct DevPools {
     const RowIn *rows;
     const RowAux *aux;
     const PredE *preds;
-    const uint32_t *succs;
     const uint8_t *seq;
     const int32_t *matrix;
     void *arena;
     RowMeta *meta;
-    int64_t *adapt;
+    RowRec *adapt;
     uint32_t *tb;
 };
 

@@ -66,7 +66,6 @@ struct JobPlan {
     std::vector<RowIn> rows;    // n + 1
     std::vector<RowAux> aux;    // n
     std::vector<PredE> preds;
-    std::vector<uint32_t> succs;
     std::vector<uint8_t> seq;   // encoded query, padded with N
     std::vector<uint32_t> row_node;
 };
@@ -205,39 +204,36 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
     }
 
     // Row pools: in-range predecessors with head != NEG (every consumer of a
-    // predecessor applies that filter), in-range successors for adaptive bands.
+    // predecessor applies that filter), sorted by row, each keeping its
+    // position in `incoming` for traceback ties.
     J.rows.resize(n + 1);
     J.aux.resize(n);
     J.preds.clear();
-    J.succs.clear();
     for (uint64_t i = 0; i < n; i++) {
         RowIn &r = J.rows[i];
         r.pred_beg = uint32_t(J.preds.size());
-        r.succ_beg = uint32_t(J.succs.size());
         r.head = head[i] == INT64_MIN ? kHeadNone : int32_t(head[i]);
         r.tail = tail[i] == INT64_MIN ? kHeadNone : int32_t(tail[i]);
+        r.pad_ = 0;
         uint64_t id = J.row_node[i], tot = 0;
+        uint32_t ord = 0;
         g.for_in(id, [&](uint64_t pred, uint64_t w) {
             tot += w;
+            uint32_t k = ord++;
             uint64_t rk = g.rank(pred);
             if (rk < beg || rk >= beg + i) return;
             uint64_t pr = rk - beg;
             if (head[pr] == INT64_MIN) return;
-            J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX))});
+            J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX)), k});
         });
-        if (J.flags & kAdaptive) {
-            g.for_out(id, [&](uint64_t succ, uint64_t) {
-                uint64_t rk = g.rank(succ);
-                if (rk <= beg + i || rk > end) return;
-                J.succs.push_back(uint32_t(rk - beg));
-            });
-        }
+        std::sort(J.preds.begin() + r.pred_beg, J.preds.end(),
+                  [](const PredE &x, const PredE &y) { return x.row < y.row; });
         J.aux[i].thresh = uint32_t(std::min<uint64_t>(tot / 10, UINT32_MAX));
         J.aux[i].base = g.base(id);
     }
     J.rows[n].pred_beg = uint32_t(J.preds.size());
-    J.rows[n].succ_beg = uint32_t(J.succs.size());
     J.rows[n].head = J.rows[n].tail = kHeadNone;
+    J.rows[n].pad_ = 0;
 
     // Arena: exact for static/full bands; adaptive rows start from the static
     // width plus two band widths and grow to the full width on overflow.

@@ -221,10 +221,10 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     const size_t N = idx.size();
     const size_t tsz = i64 ? 8 : 4;
     struct Off {
-        uint64_t rows, preds, succs, seq, arena, cap, meta, adapt, tb;
+        uint64_t rows, preds, seq, arena, cap, meta, adapt, tb;
     };
     std::vector<Off> off(N);
-    uint64_t rows = 0, aux = 0, preds = 0, succs = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
+    uint64_t rows = 0, aux = 0, preds = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
              tbn = 0;
     uint32_t maxw = 0;
     for (size_t k = 0; k < N; k++) {
@@ -232,7 +232,6 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         Off &o = off[k];
         o.rows = rows;
         o.preds = preds;
-        o.succs = succs;
         o.seq = seq;
         o.cap = full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64);
         o.arena = arena;
@@ -242,11 +241,10 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         rows += J.n + 1;
         aux += J.n + 1;
         preds += J.preds.size();
-        succs += J.succs.size();
         seq += J.padded;
         arena += align_up(o.cap, 64);
         meta += J.n;
-        if (J.flags & kAdaptive) adapt += 7ull * J.n;
+        if (J.flags & kAdaptive) adapt += J.n;
         tbn += J.m;
         maxw = std::max(maxw, J.max_width);
     }
@@ -255,8 +253,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     size_t b_rows = align_up(b_jobs + N * sizeof(DevJob), 256);
     size_t b_aux = align_up(b_rows + rows * sizeof(RowIn), 256);
     size_t b_preds = align_up(b_aux + aux * sizeof(RowAux), 256);
-    size_t b_succs = align_up(b_preds + preds * sizeof(PredE), 256);
-    size_t b_seq = align_up(b_succs + succs * sizeof(uint32_t), 256);
+    size_t b_seq = align_up(b_preds + preds * sizeof(PredE), 256);
     size_t b_mat = align_up(b_seq + seq, 256);
     size_t b_end = align_up(b_mat + alpha * alpha * sizeof(int32_t), 256);
 
@@ -269,7 +266,6 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         DevJob d{};
         d.rows = o.rows;
         d.preds = o.preds;
-        d.succs = o.succs;
         d.seq = o.seq;
         d.matrix = 0;
         d.arena = o.arena;
@@ -292,9 +288,6 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         memcpy(h + b_aux + o.rows * sizeof(RowAux), J.aux.data(), J.aux.size() * sizeof(RowAux));
         if (!J.preds.empty())
             memcpy(h + b_preds + o.preds * sizeof(PredE), J.preds.data(), J.preds.size() * sizeof(PredE));
-        if (!J.succs.empty())
-            memcpy(h + b_succs + o.succs * sizeof(uint32_t), J.succs.data(),
-                   J.succs.size() * sizeof(uint32_t));
         memcpy(h + b_seq + o.seq, J.seq.data(), J.seq.size());
     });
     memcpy(h + b_mat, p.matrix.data(), alpha * alpha * sizeof(int32_t));
@@ -302,7 +295,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     HIP_TRY(ctx.d_in.ensure(b_end));
     HIP_TRY(ctx.d_res.ensure(N * sizeof(DevResult)));
     HIP_TRY(ctx.d_meta.ensure(std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
-    HIP_TRY(ctx.d_adapt.ensure(std::max<uint64_t>(adapt, 1) * sizeof(int64_t)));
+    HIP_TRY(ctx.d_adapt.ensure(std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
     HIP_TRY(ctx.d_tb.ensure(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
     HIP_TRY(ctx.d_arena.ensure(std::max<uint64_t>(arena, 1) * tsz));
     HIP_TRY(ctx.h_res.ensure(N * sizeof(DevResult)));
@@ -315,12 +308,11 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     P.rows = reinterpret_cast<const RowIn *>(d + b_rows);
     P.aux = reinterpret_cast<const RowAux *>(d + b_aux);
     P.preds = reinterpret_cast<const PredE *>(d + b_preds);
-    P.succs = reinterpret_cast<const uint32_t *>(d + b_succs);
     P.seq = reinterpret_cast<const uint8_t *>(d + b_seq);
     P.matrix = reinterpret_cast<const int32_t *>(d + b_mat);
     P.arena = ctx.d_arena.p;
     P.meta = static_cast<RowMeta *>(ctx.d_meta.p);
-    P.adapt = static_cast<int64_t *>(ctx.d_adapt.p);
+    P.adapt = static_cast<RowRec *>(ctx.d_adapt.p);
     P.tb = static_cast<uint32_t *>(ctx.d_tb.p);
 
     const int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(maxw);
@@ -384,6 +376,11 @@ int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
                 idx.push_back(k);
             }
             if (idx.empty()) continue;
+            // Largest jobs first: workgroups dispatch in order, and a step
+            // lasts as long as its longest row chain.
+            std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+                return plans[x]->static_cells > plans[y]->static_cells;
+            });
             int rc = run_group(ctx, p, plans, idx, wide == 1, pass == 1, res, tb, tb_off, st);
             if (rc != MPOA_OK) return rc;
         }

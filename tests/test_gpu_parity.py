@@ -212,29 +212,25 @@ def test_device_anchor_errors_match_reference_messages(ctx, oracle):
 
 
 def test_device_narrow_band_failure_retries_full_dp(ctx, oracle):
-    """src/align.rs:192-211: a narrow band on indel reads fails and `poa`
-    re-runs the exact full matrix; the device reproduces both outcomes."""
-    rng = random.Random(77)
+    """src/align.rs:192-211: band 0 at vector and band edges leaves the band
+    (src/align.rs:1848-1857 cases); `poa` then re-runs the exact full matrix.
+    The device reproduces the failure and the retried result."""
     retried = 0
-    for case in range(40):
-        params = Params(b=rng.choice([0, 1]), f=40, adaptive=case % 2 == 1)
-        template = dna(60 + case, seed=500 + case)
-        og = linear(oracle, params, template)
-        q = bytearray(template)
-        at = rng.randrange(5, len(q) - 5)
-        if case % 3 == 0:
-            q[at:at] = bytes(rng.choice(b"ACGT") for _ in range(12))
-        else:
-            del q[at:at + 12]
-        q = bytes(q)
+    for length, params, reference, alternate, queries in cases_1080():
+        og = linear(oracle, params, reference)
+        og.add_path(1, oracle.poa_scalar_i64(params, og, 0, 1, alternate, False, 100))
+        og.toposort(False)
         view = gv(og, params)
-        inner = oracle.result_or_error(oracle.poa_inner, params, og, 0, 1, q, params.adaptive,
-                                       params.b)
-        retried += inner[0] == "err"
-        assert mh.result_or_error(mh.poa_inner, ctx, params, view, 0, 1, q, params.adaptive,
-                                  params.b) == inner
-        assert mh.poa(ctx, params, view, 0, 1, q, params.adaptive, params.b) == \
-            oracle.poa(params, og, 0, 1, q, params.adaptive, params.b)
+        for query in queries:
+            for band in (0, 1):
+                params.f, params.adaptive = 40, False
+                inner = oracle.result_or_error(oracle.poa_inner, params, og, 0, 1, query, False,
+                                               band)
+                retried += inner[0] == "err"
+                assert mh.result_or_error(mh.poa_inner, ctx, params, view, 0, 1, query, False,
+                                          band) == inner
+                assert mh.result_or_error(mh.poa, ctx, params, view, 0, 1, query, False, band) \
+                    == oracle.result_or_error(oracle.poa, params, og, 0, 1, query, False, band)
     assert retried > 0
 
 
