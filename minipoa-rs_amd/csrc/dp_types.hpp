@@ -61,8 +61,8 @@ struct DevJob {
     uint64_t preds;     // PredE index of the job's first entry
     uint64_t seq;       // byte offset of the encoded, padded query
     uint64_t matrix;    // int32 index of the alpha x alpha matrix
-    uint64_t arena;     // element offset of the job's arena
-    uint64_t arena_cap; // elements available
+    uint64_t arena;     // element offset of the job's own arena region
+    uint64_t arena_cap; // elements in that region (more come from the pool)
     uint64_t rowmeta;   // RowMeta index of row 0
     uint64_t adapt;     // RowRec index of row 0 (adaptive jobs)
     uint64_t tb;        // uint32 index of the m-entry traceback record

@@ -118,13 +118,17 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
     return lane63(v);
 }
 
-// Per-wave LDS budget.
+// Per-wave LDS budget (static part; the ring, query codes and score matrix
+// live in dynamic LDS sized per launch).
 constexpr uint32_t kRingBytes = 16384;  // recent rows' M and D
 constexpr uint32_t kBlk = 64;           // rows per descriptor block
 constexpr uint32_t kPredCache = 256;    // predecessor entries cached per block
 constexpr uint32_t kHist = 64;          // rows of metadata/records kept in LDS
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-constexpr uint32_t kMatLds = 640;       // matrices up to 25 x 25 live in LDS
+
+uint32_t fill_dynamic_lds(uint32_t seq_lds, uint32_t mat_lds) {
+    return kRingBytes + ((seq_lds + 15u) & ~15u) + mat_lds;
+}
 
 struct LMeta {
     uint32_t start, len;
@@ -134,7 +138,9 @@ struct LMeta {
 
 // Global-memory fallbacks of the LDS caches. Out of line so the compiler
 // cannot fold an LDS load and a global load into one load through a
-// selected generic pointer (the gfx950 backend rejects that form).
+// selected generic pointer (the gfx950 backend rejects that form). None of
+// them runs in the steady state: on gfx9 a global load waits for every
+// earlier global store of the wave, which would serialise rows on stores.
 __device__ __attribute__((noinline)) PredE pred_global(const PredE *pr, uint32_t q) { return pr[q]; }
 __device__ __attribute__((noinline)) RowMeta meta_global(const RowMeta *meta, uint32_t p) {
     return meta[p];
@@ -142,19 +148,52 @@ __device__ __attribute__((noinline)) RowMeta meta_global(const RowMeta *meta, ui
 __device__ __attribute__((noinline)) int32_t mat_global(const int32_t *mat, uint32_t k) {
     return mat[k];
 }
+__device__ __attribute__((noinline)) uint32_t seq_global(const uint8_t *seq, uint32_t k) {
+    return seq[k];
+}
 __device__ __attribute__((noinline)) RowRec rec_global(const RowRec *rec, uint32_t p) {
     return rec[p];
 }
 
+// Stores a lane's C contiguous values; whole runs use the widest aligned
+// vector store (row starts are 8-element aligned, lane runs C elements).
+template <typename T, int C>
+__device__ __forceinline__ void store_run(T *dst, const T (&v)[C], uint32_t valid) {
+    if (valid >= uint32_t(C)) {
+        if constexpr (sizeof(T) == 4 && C % 4 == 0) {
+#pragma unroll
+            for (int k = 0; k < C; k += 4)
+                *reinterpret_cast<int4 *>(dst + k) = make_int4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+        } else if constexpr (sizeof(T) == 4 && C % 2 == 0) {
+#pragma unroll
+            for (int k = 0; k < C; k += 2)
+                *reinterpret_cast<int2 *>(dst + k) = make_int2(v[k], v[k + 1]);
+        } else if constexpr (sizeof(T) == 8 && C % 2 == 0) {
+#pragma unroll
+            for (int k = 0; k < C; k += 2)
+                *reinterpret_cast<longlong2 *>(dst + k) = make_longlong2(v[k], v[k + 1]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < C; k++) dst[k] = v[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            if (uint32_t(k) < valid) dst[k] = v[k];
+    }
+}
+
 template <typename T, int C>
 __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
-    __shared__ __align__(16) T ring[kRingBytes / sizeof(T)];  // element-indexed
+    extern __shared__ __align__(16) char dyn_lds[];
     __shared__ LMeta hmeta[kHist];
     __shared__ RowRec hrec[kHist];
     __shared__ RowIn drow[kBlk + 1];
     __shared__ RowAux daux[kBlk];
     __shared__ PredE dpred[kPredCache];
-    __shared__ int32_t smat[kMatLds];
+    T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
+    uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
+    int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
 
     const DevJob J = P.jobs[blockIdx.x];
     const int lane = threadIdx.x;
@@ -164,7 +203,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const int32_t *gmat = P.matrix + J.matrix;
-    T *arena = reinterpret_cast<T *>(P.arena) + J.arena;
+    T *arena = reinterpret_cast<T *>(P.arena);  // RowMeta offsets are absolute
     RowMeta *meta = P.meta + J.rowmeta;
     RowRec *grec = P.adapt + J.adapt;
     const bool adaptive = (J.flags & kAdaptive) != 0;
@@ -175,24 +214,34 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
     const int64_t mm = int64_t(m);
     const uint32_t alpha = J.alpha;
-    const bool mat_lds = alpha * alpha <= kMatLds;
+    const bool mat_lds = P.mat_lds != 0;
+    const bool seq_lds = P.seq_lds >= J.padded;
     if (mat_lds)
         for (uint32_t k = lane; k < alpha * alpha; k += 64) smat[k] = gmat[k];
+    if (seq_lds)
+        for (uint32_t k = lane; k < J.padded; k += 64) sseq[k] = seq[k];
+    auto seq_at = [&](uint32_t k) -> uint32_t {
+        if (seq_lds) return sseq[k];
+        return seq_global(seq, k);
+    };
+    auto mat_at = [&](uint32_t k) -> int32_t {
+        if (mat_lds) return smat[k];
+        return mat_global(gmat, k);
+    };
 
-    uint64_t cursor = 0, cells = 0, pcells = 0;
+    uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region
+    uint64_t used = 0, cells = 0, pcells = 0;
     uint32_t wpos = 0;      // ring bytes handed out so far
     uint32_t pb0 = 0, pcnt = 0;
     int32_t status = kOk;
 
     // Predecessor entry q: cached block entries, else the global list.
-    // (if/else rather than ?: on lvalues: a select between an LDS and a
-    // global address would go through a generic pointer.)
     auto pred_at = [&](uint32_t q) -> PredE {
         const uint32_t k = q - pb0;
         if (k < pcnt) return dpred[k];
         return pred_global(pr, q);
     };
-    // Metadata of an earlier row p < i.
+    // Metadata / adaptive record of an earlier row p < i.
     auto meta_of = [&](uint32_t p, uint32_t i) -> LMeta {
         if (i - p < kHist) return hmeta[p % kHist];
         const RowMeta g = meta_global(meta, p);
@@ -229,17 +278,18 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         }
         const int64_t h = ri.head, t = ri.tail;
 
-        // Adaptive state of row i: the reference's pushes from its computed
-        // predecessors (rows >= 1), replayed in rank order.
-        int64_t ofl = 0, ofr = 0, pkl = mm, pkr = 0, extra = 0;
         // Rows older than the LDS history are read back from global memory,
         // where this wave wrote them earlier.
         bool far = false;
-        #pragma clang loop vectorize(disable) interleave(disable)
+#pragma clang loop vectorize(disable) interleave(disable)
         for (uint32_t q = pb; q < pe; q++) far |= (i - pred_at(q).row) >= kHist;
         if (far) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+
+        // Adaptive state of row i: the reference's pushes from its computed
+        // predecessors (rows >= 1), replayed in rank order.
+        int64_t ofl = 0, ofr = 0, pkl = mm, pkr = 0, extra = 0;
         if (adaptive) {
-            #pragma clang loop vectorize(disable) interleave(disable)
+#pragma clang loop vectorize(disable) interleave(disable)
             for (uint32_t q = pb; q < pe; q++) {
                 const uint32_t p = pred_at(q).row;
                 if (p == 0 || meta_of(p, i).len == 0) continue;
@@ -277,12 +327,22 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         const uint32_t start = uint32_t(left) / 8 * 8;
         const uint32_t stop = (uint32_t(right) / 8 + 1) * 8;
         const uint32_t len = stop - start;
-        if (cursor + 3ull * len > J.arena_cap) {
-            status = kArenaOverflow;
-            break;
+        const uint64_t need = 3ull * len;
+        if (cur + need > rend) {
+            // Own region exhausted: take a chunk from the launch's pool.
+            uint64_t got = 0;
+            if (lane == 0) got = atomicAdd(P.pool_next, (unsigned long long)P.chunk);
+            got = uint64_t(__shfl(static_cast<long long>(got), 0, 64));
+            if (need > P.chunk || got + P.chunk > P.pool_end) {
+                status = kArenaOverflow;
+                break;
+            }
+            cur = got;
+            rend = got + P.chunk;
         }
-        const uint64_t off = cursor;
-        cursor += 3ull * len;
+        const uint64_t off = cur;
+        cur += need;
+        used += need;
         cells += len;
         T *Mr = arena + off;
         T *Dr = Mr + len;
@@ -307,7 +367,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         // Predecessors evicted from the ring (or never in it) also come back
         // from global memory.
         bool evicted = false;
-        #pragma clang loop vectorize(disable) interleave(disable)
+#pragma clang loop vectorize(disable) interleave(disable)
         for (uint32_t q = pb; q < pe; q++) {
             const uint32_t p = pred_at(q).row;
             if (i - p < kHist) {
@@ -351,12 +411,13 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             continue;
         }
 
-        const uint32_t sbase = (i == n - 1) ? J.unknown : daux[i % kBlk].base;
+        const uint32_t srow = ((i == n - 1) ? J.unknown : daux[i % kBlk].base) * alpha;
         T carry = NEG;
         T best = NI;
         uint32_t bestc = 0xFFFFFFFFu;
         for (uint32_t seg = start; seg < stop; seg += 64u * C) {
             const uint32_t c0 = seg + uint32_t(lane) * C;
+            const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
             T Mv[C], Dv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
@@ -365,7 +426,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             }
             // Predecessor transitions on the band overlap (src/align.rs:751-797);
             // the diagonal into the predecessor's first column reads NEG.
-            #pragma clang loop vectorize(disable) interleave(disable)
+#pragma clang loop vectorize(disable) interleave(disable)
             for (uint32_t q = pb; q < pe; q++) {
                 const uint32_t p = pred_at(q).row;
                 const LMeta pm = meta_of(p, i);
@@ -388,7 +449,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                         }
                     }
                 } else {
-                    const RowMeta gm = meta[p];
+                    const RowMeta gm = meta_global(meta, p);
                     const T *PM = arena + gm.off;
                     const T *PD = PM + gm.len;
 #pragma unroll
@@ -408,11 +469,9 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             T L = NI;
 #pragma unroll
             for (int k = 0; k < C; k++) {
-                const uint32_t col = c0 + k;
                 T v = NI;
-                if (col < stop) {
-                    const uint32_t sidx = sbase * alpha + seq[col];
-                    const T prof = T(mat_lds ? smat[sidx] : mat_global(gmat, sidx));
+                if (uint32_t(k) < valid) {
+                    const T prof = T(mat_at(srow + seq_at(c0 + k)));
                     v = tmax(Mv[k] + prof, Dv[k]);
                 }
                 md[k] = v;
@@ -423,24 +482,24 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             const T S = maxplus_scan<T>(L, a, d15, d31);
             const T Sp = dpp<kWaveShr1, 0xF>(NI, S);
             const T X = tmax(carry + T(lane) * a, Sp);
+            T Iv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
-                const uint32_t col = c0 + k;
-                if (col < stop) {
-                    const T Ik = tmax(X + T(k) * e, Lk[k]);
-                    const T Mk = tmax(md[k], Ik);
-                    const uint32_t rel = col - start;
-                    Mr[rel] = Mk;
-                    Dr[rel] = Dv[k];
-                    Ir[rel] = Ik;
-                    if (slot != kNoSlot) {
-                        ring[sm + rel] = Mk;
-                        ring[sd + rel] = Dv[k];
-                    }
-                    if (Mk > best) {
-                        best = Mk;
-                        bestc = col;
-                    }
+                Iv[k] = tmax(X + T(k) * e, Lk[k]);
+                Mv[k] = tmax(md[k], Iv[k]);
+                if (uint32_t(k) < valid && Mv[k] > best) {
+                    best = Mv[k];
+                    bestc = c0 + k;
+                }
+            }
+            if (valid > 0) {
+                const uint32_t rel = c0 - start;
+                store_run<T, C>(Mr + rel, Mv, valid);
+                store_run<T, C>(Dr + rel, Dv, valid);
+                store_run<T, C>(Ir + rel, Iv, valid);
+                if (slot != kNoSlot) {
+                    store_run<T, C>(ring + sm + rel, Mv, valid);
+                    store_run<T, C>(ring + sd + rel, Dv, valid);
                 }
             }
             carry = lane63(tmax(X + a, L));
@@ -455,8 +514,8 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             const int64_t maximum = int64_t(maxc);
             int64_t streak = 0, max_streak = 0;
             bool is_match = false;
-            if (daux[i % kBlk].base == seq[maxc]) {
-                #pragma clang loop vectorize(disable) interleave(disable)
+            if (daux[i % kBlk].base == seq_at(maxc)) {
+#pragma clang loop vectorize(disable) interleave(disable)
                 for (uint32_t q = pb; q < pe; q++) {
                     const uint32_t p = pred_at(q).row;
                     const RowRec r = rec_of(p, i);
@@ -487,7 +546,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         R.status = status;
         R.cells = cells;
         R.pred_cells = pcells;
-        R.arena_used = cursor;
+        R.arena_used = used;
         R.score = int64_t(NEG);
         if (status == kOk) {
             const RowMeta rm = meta[n - 1];
@@ -512,7 +571,7 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const int32_t *mat = P.matrix + J.matrix;
-    const T *arena = reinterpret_cast<const T *>(P.arena) + J.arena;
+    const T *arena = reinterpret_cast<const T *>(P.arena);
     const RowMeta *meta = P.meta + J.rowmeta;
     uint32_t *tb = P.tb + J.tb;
     const int64_t o = J.o, e = J.e;
@@ -630,15 +689,14 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
 }
 
 template <typename T, int C> static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
-    hipLaunchKernelGGL((poa_fill<T, C>), dim3(njobs), dim3(64), 0, s, P);
+    const uint32_t lds = fill_dynamic_lds(P.seq_lds, P.mat_lds);
+    hipLaunchKernelGGL((poa_fill<T, C>), dim3(njobs), dim3(64), lds, s, P);
 }
 
 template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njobs, int c, hipStream_t s) {
     switch (c) {
-        case 1: launch_fill_t<T, 1>(P, njobs, s); break;
         case 2: launch_fill_t<T, 2>(P, njobs, s); break;
         case 4: launch_fill_t<T, 4>(P, njobs, s); break;
-        case 6: launch_fill_t<T, 6>(P, njobs, s); break;
         case 8: launch_fill_t<T, 8>(P, njobs, s); break;
         case 12: launch_fill_t<T, 12>(P, njobs, s); break;
         default: launch_fill_t<T, 16>(P, njobs, s); break;
@@ -646,18 +704,19 @@ template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njob
 }
 
 int lanes_columns_for_width(uint32_t width) {
-    static const int cs[] = {1, 2, 4, 6, 8, 12, 16};
+    static const int cs[] = {2, 4, 8, 12, 16};
     for (int c : cs)
         if (uint32_t(64 * c) >= width) return c;
     return 16;
 }
 
-void launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s) {
-    if (njobs == 0) return;
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s) {
+    if (njobs == 0) return 0;
     if (i64)
         launch_fill_c<int64_t>(P, njobs, cols, s);
     else
         launch_fill_c<int32_t>(P, njobs, cols, s);
+    return 0;
 }
 
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s) {

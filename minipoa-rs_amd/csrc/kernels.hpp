@@ -22,11 +22,23 @@ struct DevPools {
     RowMeta *meta;
     RowRec *adapt;
     uint32_t *tb;
+    // Arena chunk pool shared by the launch: a job whose own region fills up
+    // takes further chunks from *pool_next (element offsets) up to pool_end.
+    unsigned long long *pool_next;
+    uint64_t pool_end;
+    uint32_t chunk;      // elements per chunk (>= 3 x widest padded row)
+    uint32_t seq_lds;    // bytes of LDS for the query codes (0 = global)
+    uint32_t mat_lds;    // bytes of LDS for the score matrix (0 = global)
 };
+
+// Dynamic LDS bytes a fill launch needs besides its static arrays.
+uint32_t fill_dynamic_lds(uint32_t seq_lds, uint32_t mat_lds);
+// Largest alphabet whose matrix is staged in LDS.
+constexpr uint32_t kMatLdsEntries = 640;
 
 // Columns per lane for a row width (64 lanes x C columns per segment).
 int lanes_columns_for_width(uint32_t width);
-void launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s);
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s);
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s);
 
 }  // namespace mpoa

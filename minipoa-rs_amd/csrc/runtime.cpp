@@ -226,14 +226,14 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     std::vector<Off> off(N);
     uint64_t rows = 0, aux = 0, preds = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
              tbn = 0;
-    uint32_t maxw = 0;
+    uint32_t maxw = 0, maxpad = 0;
     for (size_t k = 0; k < N; k++) {
         const JobPlan &J = *plans[idx[k]];
         Off &o = off[k];
         o.rows = rows;
         o.preds = preds;
         o.seq = seq;
-        o.cap = full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64);
+        o.cap = align_up(full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64), 64);
         o.arena = arena;
         o.meta = meta;
         o.adapt = adapt;
@@ -247,9 +247,17 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         if (J.flags & kAdaptive) adapt += J.n;
         tbn += J.m;
         maxw = std::max(maxw, J.max_width);
+        maxpad = std::max(maxpad, J.padded);
     }
+    // Arena chunk pool after the private regions (adaptive rows that outgrow
+    // their estimate take chunks of at least three full-width rows).
+    const uint64_t chunk = align_up(std::max<uint64_t>(65536, 3ull * maxpad), 64);
+    const uint64_t pool_beg = arena;
+    const uint64_t pool_len = std::max<uint64_t>(arena / 2, 4 * chunk);
+    arena += pool_len;
     const size_t alpha = p.alpha();
-    size_t b_jobs = 0;
+    size_t b_ctr = 0;  // the pool's next-chunk counter
+    size_t b_jobs = 256;
     size_t b_rows = align_up(b_jobs + N * sizeof(DevJob), 256);
     size_t b_aux = align_up(b_rows + rows * sizeof(RowIn), 256);
     size_t b_preds = align_up(b_aux + aux * sizeof(RowAux), 256);
@@ -291,6 +299,8 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         memcpy(h + b_seq + o.seq, J.seq.data(), J.seq.size());
     });
     memcpy(h + b_mat, p.matrix.data(), alpha * alpha * sizeof(int32_t));
+    const unsigned long long ctr0 = pool_beg;
+    memcpy(h + b_ctr, &ctr0, sizeof ctr0);
 
     HIP_TRY(ctx.d_in.ensure(b_end));
     HIP_TRY(ctx.d_res.ensure(N * sizeof(DevResult)));
@@ -314,6 +324,11 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     P.meta = static_cast<RowMeta *>(ctx.d_meta.p);
     P.adapt = static_cast<RowRec *>(ctx.d_adapt.p);
     P.tb = static_cast<uint32_t *>(ctx.d_tb.p);
+    P.pool_next = reinterpret_cast<unsigned long long *>(d + b_ctr);
+    P.pool_end = pool_beg + pool_len;
+    P.chunk = uint32_t(chunk);
+    P.seq_lds = maxpad <= 8192 ? maxpad : 0;
+    P.mat_lds = alpha * alpha <= kMatLdsEntries ? uint32_t(alpha * alpha * 4) : 0;
 
     const int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(maxw);
     HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
