@@ -134,6 +134,7 @@ struct LMeta {
     uint32_t start, len;
     uint32_t slot;   // byte offset in the ring, or kNoSlot
     uint32_t wpos;   // ring write position (monotonic bytes) of the slot
+    uint64_t off;    // arena element offset of the row
 };
 
 // Global-memory fallbacks of the LDS caches. Out of line so the compiler
@@ -183,8 +184,27 @@ __device__ __forceinline__ void store_run(T *dst, const T (&v)[C], uint32_t vali
     }
 }
 
+// Loads 8 consecutive values from a 16-byte aligned address (LDS or global,
+// resolved at the call site) with 16-byte vector loads.
+template <typename T> __device__ __forceinline__ void load8(const T *src, T (&v)[8]) {
+    if constexpr (sizeof(T) == 4) {
+        const int4 x = reinterpret_cast<const int4 *>(src)[0];
+        const int4 y = reinterpret_cast<const int4 *>(src)[1];
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+            const longlong2 x = reinterpret_cast<const longlong2 *>(src)[k / 2];
+            v[k] = x.x;
+            v[k + 1] = x.y;
+        }
+    }
+}
+
 template <typename T, int C>
 __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
+    static_assert(C % 8 == 0, "lane chunks follow the 8-column band grid");
     extern __shared__ __align__(16) char dyn_lds[];
     __shared__ LMeta hmeta[kHist];
     __shared__ RowRec hrec[kHist];
@@ -220,37 +240,12 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         for (uint32_t k = lane; k < alpha * alpha; k += 64) smat[k] = gmat[k];
     if (seq_lds)
         for (uint32_t k = lane; k < J.padded; k += 64) sseq[k] = seq[k];
-    auto seq_at = [&](uint32_t k) -> uint32_t {
-        if (seq_lds) return sseq[k];
-        return seq_global(seq, k);
-    };
-    auto mat_at = [&](uint32_t k) -> int32_t {
-        if (mat_lds) return smat[k];
-        return mat_global(gmat, k);
-    };
 
     uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region
     uint64_t used = 0, cells = 0, pcells = 0;
     uint32_t wpos = 0;      // ring bytes handed out so far
     uint32_t pb0 = 0, pcnt = 0;
     int32_t status = kOk;
-
-    // Predecessor entry q: cached block entries, else the global list.
-    auto pred_at = [&](uint32_t q) -> PredE {
-        const uint32_t k = q - pb0;
-        if (k < pcnt) return dpred[k];
-        return pred_global(pr, q);
-    };
-    // Metadata / adaptive record of an earlier row p < i.
-    auto meta_of = [&](uint32_t p, uint32_t i) -> LMeta {
-        if (i - p < kHist) return hmeta[p % kHist];
-        const RowMeta g = meta_global(meta, p);
-        return LMeta{g.start, g.len, kNoSlot, 0};
-    };
-    auto rec_of = [&](uint32_t p, uint32_t i) -> RowRec {
-        if (i - p < kHist) return hrec[p % kHist];
-        return rec_global(grec, p);
-    };
 
     for (uint32_t i = 0; i < n; i++) {
         if (i % kBlk == 0) {
@@ -268,7 +263,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         if (ri.head == kHeadNone || ri.tail == kHeadNone) {
             // Skipped row (src/align.rs:708-710): no band, never pushes state.
             if (lane == 0) {
-                hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0};
+                hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0, 0};
                 hrec[i % kHist] = RowRec{0, 0, 0, 0, 0};
                 meta[i] = RowMeta{0, 0, 0};
                 if (adaptive) grec[i] = RowRec{0, 0, 0, 0, 0};
@@ -277,32 +272,69 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             continue;
         }
         const int64_t h = ri.head, t = ri.tail;
+        const uint32_t np = pe - pb;
 
-        // Rows older than the LDS history are read back from global memory,
-        // where this wave wrote them earlier.
-        bool far = false;
-#pragma clang loop vectorize(disable) interleave(disable)
-        for (uint32_t q = pb; q < pe; q++) far |= (i - pred_at(q).row) >= kHist;
-        if (far) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        // Gather the predecessors in parallel, lane q <-> predecessor q (in
+        // groups of 64): row, geometry, ring slot and adaptive record.
+        // Rows older than the LDS history come from global memory, where this
+        // wave wrote them earlier.
+        uint32_t g_row = 0;
+        LMeta g_meta{0, 0, kNoSlot, 0, 0};
+        RowRec g_rec{0, 0, 0, 0, 0};
+        auto gather = [&](uint32_t g0) {
+            const uint32_t q = g0 + uint32_t(lane);
+            g_meta = LMeta{0, 0, kNoSlot, 0, 0};
+            g_rec = RowRec{0, 0, 0, 0, 0};
+            g_row = 0;
+            if (q < pe) {
+                const uint32_t k = q - pb0;
+                const PredE pq = k < pcnt ? dpred[k] : pred_global(pr, q);
+                g_row = pq.row;
+            }
+            const bool far = q < pe && i - g_row >= kHist;
+            if (__builtin_amdgcn_ballot_w64(far)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            if (q < pe) {
+                if (!far) {
+                    g_meta = hmeta[g_row % kHist];
+                    if (adaptive) g_rec = hrec[g_row % kHist];
+                } else {
+                    const RowMeta gm = meta_global(meta, g_row);
+                    g_meta = LMeta{gm.start, gm.len, kNoSlot, 0, gm.off};
+                    if (adaptive) g_rec = rec_global(grec, g_row);
+                }
+            }
+        };
+        auto rl32 = [](uint32_t v, uint32_t q) { return uint32_t(__builtin_amdgcn_readlane(int32_t(v), q)); };
+        auto rl64 = [](int64_t v, uint32_t q) {
+            const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int32_t(uint64_t(v)), q));
+            const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int32_t(uint64_t(v) >> 32), q));
+            return int64_t((uint64_t(hi) << 32) | lo);
+        };
+        if (np <= 64) gather(pb);
 
         // Adaptive state of row i: the reference's pushes from its computed
         // predecessors (rows >= 1), replayed in rank order.
         int64_t ofl = 0, ofr = 0, pkl = mm, pkr = 0, extra = 0;
         if (adaptive) {
-#pragma clang loop vectorize(disable) interleave(disable)
-            for (uint32_t q = pb; q < pe; q++) {
-                const uint32_t p = pred_at(q).row;
-                if (p == 0 || meta_of(p, i).len == 0) continue;
-                const RowRec r = rec_of(p, i);
-                if (r.streak >= 30) {
-                    ofl = r.ofl;
-                    ofr = r.ofr;
+            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                if (np > 64) gather(g0);
+                const uint32_t cnt = tmin(64u, pe - g0);
+                for (uint32_t q = 0; q < cnt; q++) {
+                    const uint32_t p = rl32(g_row, q);
+                    if (p == 0 || rl32(g_meta.len, q) == 0) continue;
+                    const int64_t r_streak = rl64(g_rec.streak, q), r_ofl = rl64(g_rec.ofl, q);
+                    const int64_t r_ofr = rl64(g_rec.ofr, q), r_peak = rl64(g_rec.peak, q);
+                    const int64_t r_extra = rl64(g_rec.extra, q);
+                    if (r_streak >= 30) {
+                        ofl = r_ofl;
+                        ofr = r_ofr;
+                    }
+                    ofl = tmin(ofl, r_ofl);
+                    ofr = tmax(ofr, r_ofr);
+                    pkl = tmin(pkl, r_peak);
+                    pkr = tmax(pkr, r_peak + 1);
+                    extra = r_extra != 0 ? tmax(extra, r_extra) : 0;
                 }
-                ofl = tmin(ofl, r.ofl);
-                ofr = tmax(ofr, r.ofr);
-                pkl = tmin(pkl, r.peak);
-                pkr = tmax(pkr, r.peak + 1);
-                extra = r.extra != 0 ? tmax(extra, r.extra) : 0;
             }
         }
 
@@ -361,21 +393,8 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             swpos = wpos;
             wpos += bytes;
         }
-        // Ring element offsets of this row's M and D.
         const uint32_t sm = (slot == kNoSlot ? 0u : slot) / uint32_t(sizeof(T));
         const uint32_t sd = sm + len;
-        // Predecessors evicted from the ring (or never in it) also come back
-        // from global memory.
-        bool evicted = false;
-#pragma clang loop vectorize(disable) interleave(disable)
-        for (uint32_t q = pb; q < pe; q++) {
-            const uint32_t p = pred_at(q).row;
-            if (i - p < kHist) {
-                const LMeta pm = hmeta[p % kHist];
-                evicted |= pm.len != 0 && (pm.slot == kNoSlot || wpos - pm.wpos > kRingBytes);
-            }
-        }
-        if (evicted) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         if (lane == 0) meta[i] = RowMeta{start, len, off};
 
         if (i == 0) {
@@ -403,7 +422,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                     iprev = ins;
                     mprev = mj;
                 }
-                hmeta[0] = LMeta{start, len, slot, swpos};
+                hmeta[0] = LMeta{start, len, slot, swpos, off};
                 hrec[0] = RowRec{0, 0, 0, 0, 0};
                 if (adaptive) grec[0] = RowRec{0, 0, 0, 0, 0};
             }
@@ -417,49 +436,53 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         uint32_t bestc = 0xFFFFFFFFu;
         for (uint32_t seg = start; seg < stop; seg += 64u * C) {
             const uint32_t c0 = seg + uint32_t(lane) * C;
-            const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
             T Mv[C], Dv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 Mv[k] = NEG;
                 Dv[k] = NEG;
             }
-            // Predecessor transitions on the band overlap (src/align.rs:751-797);
-            // the diagonal into the predecessor's first column reads NEG.
-#pragma clang loop vectorize(disable) interleave(disable)
-            for (uint32_t q = pb; q < pe; q++) {
-                const uint32_t p = pred_at(q).row;
-                const LMeta pm = meta_of(p, i);
-                const uint32_t ovs = tmax(start, pm.start);
-                const uint32_t ove = tmin(stop, pm.start + pm.len);
-                if (ovs >= ove) continue;
-                if (seg == start) pcells += ove - ovs;
-                const bool in_ring = pm.slot != kNoSlot && wpos - pm.wpos <= kRingBytes;
-                if (in_ring) {
-                    const uint32_t pmo = pm.slot / uint32_t(sizeof(T));
-                    const uint32_t pdo = pmo + pm.len;
+            // Predecessor transitions on the band overlap (src/align.rs:751-797).
+            // Bands and lane chunks sit on the same 8-column grid, so each
+            // 8-column piece is wholly inside or outside an overlap; the
+            // diagonal into the predecessor's first column reads NEG.
+            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                if (np > 64) gather(g0);
+                // Predecessors outside the ring come back from global memory.
+                const bool miss = g0 + uint32_t(lane) < pe && g_meta.len != 0 &&
+                                  (g_meta.slot == kNoSlot || wpos - g_meta.wpos > kRingBytes);
+                if (__builtin_amdgcn_ballot_w64(miss)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                const uint32_t cnt = tmin(64u, pe - g0);
+                for (uint32_t q = 0; q < cnt; q++) {
+                    const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
+                    const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
+                    if (ovs >= ove) continue;
+                    if (seg == start) pcells += ove - ovs;
+                    const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
+                    const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
 #pragma unroll
-                    for (int k = 0; k < C; k++) {
-                        const uint32_t col = c0 + k;
-                        if (col >= ovs && col < ove) {
-                            const uint32_t rel = col - pm.start;
-                            const T dg = rel > 0 ? ring[pmo + rel - 1] : NEG;
-                            Mv[k] = tmax(Mv[k], dg);
-                            Dv[k] = tmax(Dv[k], tmax(ring[pdo + rel] + e, ring[pmo + rel] + o));
+                    for (int s8 = 0; s8 < C; s8 += 8) {
+                        const uint32_t cs = c0 + s8;
+                        if (cs < ovs || cs >= ove) continue;
+                        const uint32_t rel = cs - ps;
+                        T pm8[8], pd8[8], dg0;
+                        if (in_ring) {
+                            const uint32_t pmo = pslot / uint32_t(sizeof(T));
+                            load8(ring + pmo + rel, pm8);
+                            load8(ring + pmo + pl + rel, pd8);
+                            dg0 = rel > 0 ? ring[pmo + rel - 1] : NEG;
+                        } else {
+                            const uint64_t poff = uint64_t(rl64(int64_t(g_meta.off), q));
+                            const T *PM = arena + poff;
+                            load8(PM + rel, pm8);
+                            load8(PM + pl + rel, pd8);
+                            dg0 = rel > 0 ? PM[rel - 1] : NEG;
                         }
-                    }
-                } else {
-                    const RowMeta gm = meta_global(meta, p);
-                    const T *PM = arena + gm.off;
-                    const T *PD = PM + gm.len;
 #pragma unroll
-                    for (int k = 0; k < C; k++) {
-                        const uint32_t col = c0 + k;
-                        if (col >= ovs && col < ove) {
-                            const uint32_t rel = col - gm.start;
-                            const T dg = rel > 0 ? PM[rel - 1] : NEG;
-                            Mv[k] = tmax(Mv[k], dg);
-                            Dv[k] = tmax(Dv[k], tmax(PD[rel] + e, PM[rel] + o));
+                        for (int k = 0; k < 8; k++) {
+                            const T dg = k == 0 ? dg0 : pm8[k - 1];
+                            Mv[s8 + k] = tmax(Mv[s8 + k], dg);
+                            Dv[s8 + k] = tmax(Dv[s8 + k], tmax(pd8[k] + e, pm8[k] + o));
                         }
                     }
                 }
@@ -468,20 +491,38 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             T md[C], Lk[C];
             T L = NI;
 #pragma unroll
-            for (int k = 0; k < C; k++) {
-                T v = NI;
-                if (uint32_t(k) < valid) {
-                    const T prof = T(mat_at(srow + seq_at(c0 + k)));
-                    v = tmax(Mv[k] + prof, Dv[k]);
+            for (int s8 = 0; s8 < C; s8 += 8) {
+                const uint32_t cs = c0 + s8;
+                const bool live = cs < stop;
+                uint32_t codes[8];
+                if (live) {
+                    if (seq_lds) {
+                        const uint2 w = *reinterpret_cast<const uint2 *>(sseq + cs);
+#pragma unroll
+                        for (int k = 0; k < 8; k++) codes[k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) codes[k] = seq_global(seq, cs + k);
+                    }
                 }
-                md[k] = v;
-                Lk[k] = L;
-                L = tmax(L + e, v + o);
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    T v = NI;
+                    if (live) {
+                        const uint32_t idx = srow + codes[k];
+                        const T prof = T(mat_lds ? smat[idx] : mat_global(gmat, idx));
+                        v = tmax(Mv[s8 + k] + prof, Dv[s8 + k]);
+                    }
+                    md[s8 + k] = v;
+                    Lk[s8 + k] = L;
+                    L = tmax(L + e, v + o);
+                }
             }
             // Carry into each lane: X_l = max(carry + l*a, S_{l-1}).
             const T S = maxplus_scan<T>(L, a, d15, d31);
             const T Sp = dpp<kWaveShr1, 0xF>(NI, S);
             const T X = tmax(carry + T(lane) * a, Sp);
+            const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
             T Iv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
@@ -514,15 +555,18 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             const int64_t maximum = int64_t(maxc);
             int64_t streak = 0, max_streak = 0;
             bool is_match = false;
-            if (daux[i % kBlk].base == seq_at(maxc)) {
-#pragma clang loop vectorize(disable) interleave(disable)
-                for (uint32_t q = pb; q < pe; q++) {
-                    const uint32_t p = pred_at(q).row;
-                    const RowRec r = rec_of(p, i);
-                    max_streak = tmax(max_streak, r.streak);
-                    if (r.peak + 1 == maximum) {
-                        streak = tmax(streak, r.streak + 1);
-                        is_match = true;
+            const uint32_t qbase = seq_lds ? uint32_t(sseq[maxc]) : seq_global(seq, maxc);
+            if (daux[i % kBlk].base == qbase) {
+                for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                    if (np > 64) gather(g0);
+                    const uint32_t cnt = tmin(64u, pe - g0);
+                    for (uint32_t q = 0; q < cnt; q++) {
+                        const int64_t r_streak = rl64(g_rec.streak, q), r_peak = rl64(g_rec.peak, q);
+                        max_streak = tmax(max_streak, r_streak);
+                        if (r_peak + 1 == maximum) {
+                            streak = tmax(streak, r_streak + 1);
+                            is_match = true;
+                        }
                     }
                 }
             }
@@ -535,7 +579,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             rec = RowRec{maximum, streak, ofl, ofr, extra};
         }
         if (lane == 0) {
-            hmeta[i % kHist] = LMeta{start, len, slot, swpos};
+            hmeta[i % kHist] = LMeta{start, len, slot, swpos, off};
             hrec[i % kHist] = rec;
             if (adaptive) grec[i] = rec;
         }
@@ -695,16 +739,13 @@ template <typename T, int C> static void launch_fill_t(const DevPools &P, uint32
 
 template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njobs, int c, hipStream_t s) {
     switch (c) {
-        case 2: launch_fill_t<T, 2>(P, njobs, s); break;
-        case 4: launch_fill_t<T, 4>(P, njobs, s); break;
         case 8: launch_fill_t<T, 8>(P, njobs, s); break;
-        case 12: launch_fill_t<T, 12>(P, njobs, s); break;
         default: launch_fill_t<T, 16>(P, njobs, s); break;
     }
 }
 
 int lanes_columns_for_width(uint32_t width) {
-    static const int cs[] = {2, 4, 8, 12, 16};
+    static const int cs[] = {8, 16};
     for (int c : cs)
         if (uint32_t(64 * c) >= width) return c;
     return 16;
