@@ -157,7 +157,7 @@ Ctx::~Ctx() {
         }                                                                             \
     } while (0)
 
-int ctx_create(int device, Ctx **out) {
+int ctx_create(int device, Ctx **out, unsigned threads) {
     *out = nullptr;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -186,7 +186,7 @@ int ctx_create(int device, Ctx **out) {
         delete c;
         return MPOA_ERR_DEVICE;
     }
-    c->pool = new ThreadPool(default_threads());
+    c->pool = new ThreadPool(threads ? threads : default_threads());
     *out = c;
     return MPOA_OK;
 }

@@ -76,7 +76,8 @@ struct Ctx {
     ~Ctx();
 };
 
-int ctx_create(int device, Ctx **out);
+// threads: host workers of the context's pool (0 = default_threads()).
+int ctx_create(int device, Ctx **out, unsigned threads = 0);
 unsigned default_threads();
 
 // Runs every plan with status MPOA_OK on the device (i32 and i64 groups in
