@@ -191,6 +191,7 @@ int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
                 uint64_t hint = prev_used[x] * J.n / prev_n[x];
                 hint += hint / 8 + 3ull * J.padded * 8;
                 J.arena_est = std::min(std::max(J.arena_est, hint), arena_full(J));
+                J.width_hint = uint32_t(prev_used[x] / (3 * prev_n[x]));
             }
         });
         dp.clear();

@@ -120,14 +120,13 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 
 // Per-wave LDS budget (static part; the ring, query codes and score matrix
 // live in dynamic LDS sized per launch).
-constexpr uint32_t kRingBytes = 16384;  // recent rows' M and D
 constexpr uint32_t kBlk = 64;           // rows per descriptor block
 constexpr uint32_t kPredCache = 256;    // predecessor entries cached per block
 constexpr uint32_t kHist = 64;          // rows of metadata/records kept in LDS
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
-uint32_t fill_dynamic_lds(uint32_t seq_lds, uint32_t mat_lds) {
-    return kRingBytes + ((seq_lds + 15u) & ~15u) + mat_lds;
+uint32_t fill_dynamic_lds(uint32_t ring_bytes, uint32_t seq_lds, uint32_t mat_lds) {
+    return ring_bytes + ((seq_lds + 15u) & ~15u) + mat_lds;
 }
 
 struct LMeta {
@@ -211,6 +210,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     __shared__ RowIn drow[kBlk + 1];
     __shared__ RowAux daux[kBlk];
     __shared__ PredE dpred[kPredCache];
+    const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
     T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
     uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
     int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
@@ -733,7 +733,7 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
 }
 
 template <typename T, int C> static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
-    const uint32_t lds = fill_dynamic_lds(P.seq_lds, P.mat_lds);
+    const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
     hipLaunchKernelGGL((poa_fill<T, C>), dim3(njobs), dim3(64), lds, s, P);
 }
 
@@ -742,6 +742,13 @@ template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njob
         case 8: launch_fill_t<T, 8>(P, njobs, s); break;
         default: launch_fill_t<T, 16>(P, njobs, s); break;
     }
+}
+
+uint32_t ring_bytes_for_width(uint32_t width, bool i64) {
+    uint32_t row = 2u * ((width + 7u) & ~7u) * (i64 ? 8u : 4u);
+    uint32_t want = 8u * row;
+    want = want < 16384u ? 16384u : (want > 49152u ? 49152u : want);
+    return (want + 63u) & ~63u;
 }
 
 int lanes_columns_for_width(uint32_t width) {

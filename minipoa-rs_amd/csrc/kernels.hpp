@@ -29,15 +29,18 @@ struct DevPools {
     uint32_t chunk;      // elements per chunk (>= 3 x widest padded row)
     uint32_t seq_lds;    // bytes of LDS for the query codes (0 = global)
     uint32_t mat_lds;    // bytes of LDS for the score matrix (0 = global)
+    uint32_t ring_bytes; // LDS ring of recent rows (multiple of 64 bytes)
 };
 
 // Dynamic LDS bytes a fill launch needs besides its static arrays.
-uint32_t fill_dynamic_lds(uint32_t seq_lds, uint32_t mat_lds);
+uint32_t fill_dynamic_lds(uint32_t ring_bytes, uint32_t seq_lds, uint32_t mat_lds);
 // Largest alphabet whose matrix is staged in LDS.
 constexpr uint32_t kMatLdsEntries = 640;
 
 // Columns per lane for a row width (64 lanes x C columns per segment).
 int lanes_columns_for_width(uint32_t width);
+// Default ring bytes for rows of a given width (about eight rows, 16-48 KB).
+uint32_t ring_bytes_for_width(uint32_t width, bool i64);
 int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s);
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s);
 

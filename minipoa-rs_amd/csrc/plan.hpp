@@ -63,6 +63,7 @@ struct JobPlan {
     uint64_t static_cells = 0;  // cells of the rows under the static/full band
     uint64_t arena_est = 0;     // elements requested for the arena
     uint32_t max_width = 0;     // widest static row (chooses columns per lane)
+    uint32_t width_hint = 0;    // mean adaptive row width seen on this job's last DP
     std::vector<RowIn> rows;    // n + 1
     std::vector<RowAux> aux;    // n
     std::vector<PredE> preds;

@@ -330,7 +330,30 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     P.seq_lds = maxpad <= 8192 ? maxpad : 0;
     P.mat_lds = alpha * alpha <= kMatLdsEntries ? uint32_t(alpha * alpha * 4) : 0;
 
-    const int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(maxw);
+    // Lane width and ring size follow the expected row width: the widest
+    // static row, or the mean adaptive width seen in earlier steps.
+    uint32_t wexp = maxw;
+    {
+        uint64_t num = 0, den = 0;
+        for (size_t k = 0; k < N; k++) {
+            const JobPlan &J = *plans[idx[k]];
+            if (J.width_hint) {
+                num += J.width_hint;
+                den += 1;
+            }
+        }
+        if (den) wexp = std::max<uint32_t>(wexp, uint32_t(num / den));
+    }
+    int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(wexp);
+    if (const char *e = getenv("MPOA_COLS")) cols = atoi(e);
+    P.ring_bytes = ring_bytes_for_width(wexp, i64);
+    if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
+    {
+        // Stay within 64 KB of LDS per workgroup with the fill's static arrays.
+        const uint32_t static_lds = 9216, dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
+        const uint32_t room = 65536u - static_lds - dyn_other;
+        if (P.ring_bytes > room) P.ring_bytes = room & ~63u;
+    }
     HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
     HIP_TRY(hipEventRecord(ctx.ev[0], ctx.stream));
     launch_fill(P, uint32_t(N), i64, cols, ctx.stream);
