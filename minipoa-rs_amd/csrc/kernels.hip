@@ -601,14 +601,41 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     }
 }
 
-// Weighted deterministic traceback (src/align.rs:1164-1313), one lane per job.
+// Weighted deterministic traceback (src/align.rs:1164-1313), one wave per job.
+// The walk is serial, so its cost is the latency of each step's reads. The
+// wave therefore stages a window of the matrix in LDS -- kTbRows rows ending
+// at the current row (geometry, predecessor lists and M/D/I over kTbCols
+// columns ending at the current column), loaded with all lanes and all loads
+// in flight at once -- and walks inside it, reloading when the path nears
+// the window's top-left edge. Predecessors older than the window, or rows
+// with more than kTbPred predecessors, read global memory directly.
+constexpr uint32_t kTbRows = 64, kTbPred = 8, kTbReach = 16;
+
+struct WRow {
+    uint32_t start, len;
+    uint64_t off;
+    uint32_t pb, np, thresh, base;
+};
+
+__device__ __attribute__((noinline)) RowIn rin_global(const RowIn *rin, uint32_t i) { return rin[i]; }
+
+template <typename T>
+__device__ __attribute__((noinline)) T cell_global(const T *arena, uint64_t k) {
+    return arena[k];
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) {
-    const uint32_t jid = blockIdx.x * 64u + threadIdx.x;
+    constexpr uint32_t kTbCols = sizeof(T) == 4 ? 40 : 20;
+    __shared__ WRow wrow[kTbRows];
+    __shared__ PredE wpred[kTbRows * kTbPred];
+    __shared__ T wm[kTbRows * kTbCols], wd[kTbRows * kTbCols], wi[kTbRows * kTbCols];
+    const uint32_t jid = blockIdx.x;
     if (jid >= njobs) return;
     DevResult *R = P.res + jid;
     if (R->status != kOk) return;
     const DevJob J = P.jobs[jid];
+    const int lane = threadIdx.x;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
     const RowAux *aux = P.aux + J.rows;
@@ -622,7 +649,57 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     const int64_t half = int64_t(Lim<T>::neg) / 2;
     const bool gap0 = (J.flags & kGapOpenZero) != 0;
 
-    auto inside = [](const RowMeta &r, uint32_t c) { return c >= r.start && c - r.start < r.len; };
+    uint32_t i0 = 0, j0 = 0, cw0 = 0;  // window: rows (i0-kTbRows, i0], columns [cw0, j0]
+    bool have = false;
+    auto load_window = [&](uint32_t ai, uint32_t aj) {
+        __syncthreads();
+        i0 = ai;
+        j0 = aj;
+        cw0 = aj + 1 > kTbCols ? aj + 1 - kTbCols : 0;
+        have = true;
+        const uint32_t r = uint32_t(lane);
+        if (r <= i0) {
+            const uint32_t ri = i0 - r;
+            const RowMeta rm = meta[ri];
+            const RowIn a = rin[ri], b = rin[ri + 1];
+            const RowAux x = aux[ri];
+            wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
+            const uint32_t np = tmin(b.pred_beg - a.pred_beg, kTbPred);
+            for (uint32_t q = 0; q < np; q++) wpred[r * kTbPred + q] = pr[a.pred_beg + q];
+            const uint32_t lo = tmax(cw0, rm.start), hi = tmin(j0 + 1, rm.start + rm.len);
+            for (uint32_t c = lo; c < hi; c++) {
+                const uint64_t k = rm.off + (c - rm.start);
+                wm[r * kTbCols + (c - cw0)] = arena[k];
+                wd[r * kTbCols + (c - cw0)] = arena[k + rm.len];
+                wi[r * kTbCols + (c - cw0)] = arena[k + 2ull * rm.len];
+            }
+        }
+        __syncthreads();
+    };
+    auto row_in_win = [&](uint32_t ri) { return have && ri <= i0 && i0 - ri < kTbRows; };
+    // Geometry of a row: window copy, else global.
+    auto geo = [&](uint32_t ri) -> WRow {
+        if (row_in_win(ri)) return wrow[i0 - ri];
+        const RowMeta rm = meta_global(meta, ri);
+        const RowIn a = rin_global(rin, ri), b = rin_global(rin, ri + 1);
+        return WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, 0, 0};
+    };
+    auto inside = [](const WRow &r, uint32_t c) { return c >= r.start && c - r.start < r.len; };
+    // State s (0 = M, 1 = D, 2 = I) of row ri at an in-band column c.
+    auto val = [&](uint32_t ri, const WRow &g, uint32_t c, int s) -> int64_t {
+        if (row_in_win(ri) && c >= cw0 && c <= j0) {
+            const uint32_t k = (i0 - ri) * kTbCols + (c - cw0);
+            if (s == 0) return int64_t(wm[k]);
+            if (s == 1) return int64_t(wd[k]);
+            return int64_t(wi[k]);
+        }
+        return int64_t(cell_global(arena, g.off + uint64_t(s) * g.len + (c - g.start)));
+    };
+    auto pred = [&](uint32_t ri, const WRow &g, uint32_t q) -> PredE {
+        if (row_in_win(ri) && g.np <= kTbPred) return wpred[(i0 - ri) * kTbPred + q];
+        return pred_global(pr, g.pb + q);
+    };
+
     uint32_t i = n - 1, j = m - 1, op = kAll;
     uint64_t steps = 0;
     int32_t status = kOk;
@@ -631,81 +708,78 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
             status = kTbNoAdvance;
             break;
         }
-        const RowMeta rm = meta[i];
-        if (!inside(rm, j)) {
+        if (!have || i0 - i + kTbReach > kTbRows || (j > 0 && j - 1 < cw0)) load_window(i, j);
+        const WRow rg = geo(i);
+        if (!inside(rg, j)) {
             status = kTbLeftBand;
             break;
         }
-        const T *Mr = arena + rm.off;
-        const T *Dr = Mr + rm.len;
-        const T *Ir = Dr + rm.len;
-        const int64_t Mij = int64_t(Mr[j - rm.start]);
+        const int64_t Mij = val(i, rg, j, 0);
         if (Mij <= half) {
             status = kTbLeftBand;
             break;
         }
-        const int64_t Dij = int64_t(Dr[j - rm.start]);
-        const uint32_t base = (i == n - 1) ? J.unknown : aux[i].base;
+        const int64_t Dij = val(i, rg, j, 1);
+        const uint32_t base = (i == n - 1) ? J.unknown : rg.base;
         const int64_t score = mat[size_t(base) * J.alpha + seq[j]];
-        const uint32_t pb = rin[i].pred_beg, pe = rin[i + 1].pred_beg;
         // `choose` (src/align.rs:1201-1225): among in-range predecessors whose
         // state plus penalty equals the target, the largest in_weight; ties go
-        // to the earliest in `incoming` order.
-        auto choose = [&](uint32_t state, int64_t penalty, uint32_t pj) -> int64_t {
+        // to the earliest in `incoming` order. Returns the predecessor row.
+        auto choose = [&](uint32_t state, int64_t penalty, uint32_t pj, uint32_t &w_out) -> int64_t {
             const int64_t target = state == kMatch ? Mij : Dij;
-            int64_t bestq = -1;
+            int64_t best = -1;
             uint32_t w = 0, bord = 0;
-            for (uint32_t q = pb; q < pe; q++) {
-                const PredE pq = pr[q];
-                const RowMeta pm = meta[pq.row];
-                if (!inside(pm, pj)) continue;
-                const T *PM = arena + pm.off;
-                const int64_t src = state == kDelete ? int64_t(PM[pm.len + pj - pm.start])
-                                                     : int64_t(PM[pj - pm.start]);
+            for (uint32_t q = 0; q < rg.np; q++) {
+                const PredE pq = pred(i, rg, q);
+                const WRow pg = geo(pq.row);
+                if (!inside(pg, pj)) continue;
+                const int64_t src = val(pq.row, pg, pj, state == kDelete ? 1 : 0);
                 if (target == src + penalty &&
-                    (bestq < 0 || pq.weight > w || (pq.weight == w && pq.ord < bord))) {
-                    bestq = q;
+                    (best < 0 || pq.weight > w || (pq.weight == w && pq.ord < bord))) {
+                    best = pq.row;
                     w = pq.weight;
                     bord = pq.ord;
                 }
             }
-            return bestq;
+            w_out = w;
+            return best;
         };
+        uint32_t w = 0;
         int64_t pref = -1;
-        if ((op & kMatch) && j > 0 && base == seq[j]) pref = choose(kMatch, score, j - 1);
-        if (pref >= 0 && pr[pref].weight < aux[i].thresh) pref = -1;
+        if ((op & kMatch) && j > 0 && base == seq[j]) pref = choose(kMatch, score, j - 1, w);
+        if (pref >= 0 && w < rg.thresh) pref = -1;
         if (pref >= 0) {
-            tb[j] = i;
-            i = pr[pref].row;
+            if (lane == 0) tb[j] = i;
+            i = uint32_t(pref);
             j -= 1;
             op = kAll;
             continue;
         }
         if ((op & kDelete) && (op == kDelete || Mij == Dij)) {
-            int64_t k = choose(kInsert, o, j);
+            int64_t k = choose(kInsert, o, j, w);
             if (k >= 0) {
-                i = pr[k].row;
+                i = uint32_t(k);
                 op = gap0 ? kAll : (kMatch | kInsert);
                 continue;
             }
-            k = choose(kDelete, e, j);
+            k = choose(kDelete, e, j, w);
             if (k >= 0) {
-                i = pr[k].row;
+                i = uint32_t(k);
                 op = kDelete;
                 continue;
             }
         }
-        if ((op & kInsert) && j > 0 && inside(rm, j - 1)) {
-            const int64_t Iij = int64_t(Ir[j - rm.start]);
+        if ((op & kInsert) && j > 0 && inside(rg, j - 1)) {
+            const int64_t Iij = val(i, rg, j, 2);
             if (op == kInsert || Mij == Iij) {
-                if (Iij == int64_t(Mr[j - 1 - rm.start]) + o) {
-                    tb[j] = kTbInsert;
+                if (Iij == val(i, rg, j - 1, 0) + o) {
+                    if (lane == 0) tb[j] = kTbInsert;
                     j -= 1;
                     op = gap0 ? kAll : (kMatch | kDelete);
                     continue;
                 }
-                if (Iij == int64_t(Ir[j - 1 - rm.start]) + e) {
-                    tb[j] = kTbInsert;
+                if (Iij == val(i, rg, j - 1, 2) + e) {
+                    if (lane == 0) tb[j] = kTbInsert;
                     j -= 1;
                     op = kInsert;
                     continue;
@@ -713,10 +787,10 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
             }
         }
         if ((op & kMatch) && j > 0) {
-            const int64_t k = choose(kMatch, score, j - 1);
+            const int64_t k = choose(kMatch, score, j - 1, w);
             if (k >= 0) {
-                tb[j] = i;
-                i = pr[k].row;
+                if (lane == 0) tb[j] = i;
+                i = uint32_t(k);
                 j -= 1;
                 op = kAll;
                 continue;
@@ -725,7 +799,7 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         status = kTbNoPredecessor;
         break;
     }
-    if (status != kOk) {
+    if (status != kOk && lane == 0) {
         R->status = status;
         R->err_i = i;
         R->err_j = j;
@@ -769,7 +843,7 @@ int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream
 
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s) {
     if (njobs == 0) return;
-    dim3 grid((njobs + 63) / 64);
+    dim3 grid(njobs);
     if (i64)
         hipLaunchKernelGGL((poa_traceback<int64_t>), grid, dim3(64), 0, s, P, njobs);
     else
