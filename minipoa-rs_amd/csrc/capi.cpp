@@ -158,7 +158,7 @@ int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
     Params full = p;
     full.f = 0;
     std::vector<uint64_t> active;
-    std::vector<JobPlan> plans;
+    std::vector<JobPlan> plans(ids.size());  // per job, reused across steps
     std::vector<JobPlan *> dp;
     std::vector<std::vector<AlignedBase>> paths(ids.size());
     std::vector<uint64_t> prev_used(ids.size(), 0), prev_n(ids.size(), 0);
@@ -167,7 +167,6 @@ int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
         active.clear();
         for (uint64_t x = 0; x < ids.size(); x++)
             if (s < B.nreads[ids[x]] && B.status[ids[x]] == MPOA_OK) active.push_back(x);
-        plans.assign(active.size(), JobPlan());
         std::vector<char> shortcut(active.size(), 0);
         c.pool->parallel_for(active.size(), [&](size_t k) {
             uint64_t x = active[k], j = ids[x];
@@ -183,7 +182,7 @@ int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
                 for (uint64_t t = 0; t < qlen; t++) v.push_back(AlignedBase{-1, int64_t(p.encode(q[t])), -1});
                 return;
             }
-            JobPlan &J = plans[k];
+            JobPlan &J = plans[x];
             plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, J);
             // Adaptive bands: start from what the previous step's DP used,
             // scaled to the grown graph.
@@ -198,7 +197,7 @@ int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
         std::vector<size_t> dp_k;
         for (size_t k = 0; k < active.size(); k++)
             if (!shortcut[k]) {
-                dp.push_back(&plans[k]);
+                dp.push_back(&plans[active[k]]);
                 dp_k.push_back(k);
             }
         host_ms += ms_since(t0);
@@ -454,7 +453,7 @@ uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, ui
         if (ids) ids[k] = v[k];
         if (coverage) {
             uint64_t s = 0;
-            for (uint32_t w : n.out_w) s += w;
+            for (const Edge &ed : n.out) s += ed.w;
             coverage[k] = std::max<uint64_t>(n.coverage, s);
         }
     }

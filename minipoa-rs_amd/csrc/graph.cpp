@@ -9,6 +9,39 @@
 
 namespace mpoa {
 
+Edges &Edges::operator=(const Edges &o) {
+    if (this == &o) return *this;
+    n_ = 0;
+    while (cap_ < o.n_) grow();
+    const Edge *src = o.data();
+    Edge *dst = data();
+    for (uint32_t k = 0; k < o.n_; k++) dst[k] = src[k];
+    n_ = o.n_;
+    return *this;
+}
+
+Edges &Edges::operator=(Edges &&o) noexcept {
+    if (this == &o) return *this;
+    if (cap_ > kInline) free(u_.heap);
+    n_ = o.n_;
+    cap_ = o.cap_;
+    u_ = o.u_;
+    o.n_ = 0;
+    o.cap_ = kInline;
+    return *this;
+}
+
+void Edges::grow() {
+    const uint32_t ncap = cap_ * 2;
+    Edge *h = static_cast<Edge *>(malloc(sizeof(Edge) * ncap));
+    if (!h) abort();
+    const Edge *d = data();
+    for (uint32_t k = 0; k < n_; k++) h[k] = d[k];
+    if (cap_ > kInline) free(u_.heap);
+    u_.heap = h;
+    cap_ = ncap;
+}
+
 Params Params::from_c(const mpoa_params &c) {
     Params p;
     p.match_score = c.match_score;
@@ -36,10 +69,14 @@ Params Params::from_c(const mpoa_params &c) {
     p.matrix_valid = c.matrix != nullptr || c.alphabet_len == 0;
     if (c.matrix && c.alphabet_len)
         p.matrix.assign(c.matrix, c.matrix + size_t(c.alphabet_len) * c.alphabet_len);
+    if (!p.alphabet.empty() && p.alphabet.size() <= 256) {
+        for (int b = 0; b < 256; b++) p.codes[b] = uint8_t(p.encode_slow(uint8_t(b)));
+        p.coded = true;
+    }
     return p;
 }
 
-uint32_t Params::encode(uint8_t c) const {
+uint32_t Params::encode_slow(uint8_t c) const {
     if (c >= 'a' && c <= 'z') c = uint8_t(c - 'a' + 'A');
     if (alphabet.size() <= 5 && c == 'U') c = 'T';
     for (size_t i = 0; i < alphabet.size(); i++)
@@ -82,6 +119,7 @@ uint32_t Graph::add_node(uint32_t base) {
     nd.parent = id;
     nd.base = base;
     nodes_.push_back(std::move(nd));
+    rank_flat_.push_back(0);
     aligned_.resize(aligned_.size() + alpha_, -1);
     if (alpha_) aligned_[size_t(id) * alpha_ + base] = id;
     return id;
@@ -91,21 +129,17 @@ uint32_t Graph::add_node(uint32_t base) {
 void Graph::add_edge(uint32_t from, uint32_t to) {
     if (from == to) return;
     Node &f = nodes_[from];
-    auto it = std::find(f.out.begin(), f.out.end(), to);
-    if (it != f.out.end()) {
-        f.out_w[size_t(it - f.out.begin())] += 1;
-    } else {
-        f.out.push_back(to);
-        f.out_w.push_back(1);
-    }
+    int k = f.out.find(to);
+    if (k >= 0)
+        f.out[size_t(k)].w += 1;
+    else
+        f.out.push(to, 1);
     Node &t = nodes_[to];
-    auto jt = std::find(t.in.begin(), t.in.end(), from);
-    if (jt != t.in.end()) {
-        t.in_w[size_t(jt - t.in.begin())] += 1;
-    } else {
-        t.in.push_back(from);
-        t.in_w.push_back(1);
-    }
+    k = t.in.find(from);
+    if (k >= 0)
+        t.in[size_t(k)].w += 1;
+    else
+        t.in.push(from, 1);
     t.coverage += 1;
 }
 
@@ -201,7 +235,9 @@ std::string Graph::toposort(bool grouped) {
             Node &mem = nodes_[mb[k]];
             mem.rank = rank;
             visited++;
-            for (uint32_t next : mem.out) {
+            rank_flat_[mb[k]] = rank;
+            for (const Edge &ed : mem.out) {
+                uint32_t next = ed.node;
                 if (grouped) next = nodes_[next].parent;
                 if (degree[next] == 0) return "invalid graph column dependencies";
                 if (--degree[next] == 0) stack.push_back(next);
@@ -229,17 +265,17 @@ void Graph::consensus(std::vector<uint32_t> &ids) const {
         if (id != 0 && !node.in.empty()) {
             size_t best = 0;
             for (size_t k = 1; k < node.in.size(); k++) {
-                uint32_t wb = node.in_w[best], wk = node.in_w[k];
-                uint64_t sb = score[node.in[best]], sk = score[node.in[k]];
+                uint32_t wb = node.in[best].w, wk = node.in[k].w;
+                uint64_t sb = score[node.in[best].node], sk = score[node.in[k].node];
                 bool keep = wb != wk ? wb > wk : (sb != sk ? sb > sk : true);
                 if (!keep) best = k;
             }
-            previous[id] = node.in[best];
-            score[id] = score[node.in[best]] + node.in_w[best];
+            previous[id] = node.in[best].node;
+            score[id] = score[node.in[best].node] + node.in[best].w;
         }
         if (id == 1) break;
-        for (uint32_t next : node.out)
-            if (--degree[next] == 0) queue.push_back(next);
+        for (const Edge &ed : node.out)
+            if (--degree[ed.node] == 0) queue.push_back(ed.node);
     }
     ids.clear();
     for (int64_t cur = previous[1]; cur > 0; cur = previous[size_t(cur)]) ids.push_back(uint32_t(cur));
@@ -296,7 +332,7 @@ std::string Graph::msa(const Params &p, const std::vector<std::string> &names,
         }
     }
     rank_ = std::move(saved_order);
-    for (size_t i = 0; i < nodes_.size(); i++) nodes_[i].rank = saved_rank[i];
+    for (size_t i = 0; i < nodes_.size(); i++) nodes_[i].rank = rank_flat_[i] = saved_rank[i];
     if (!err.empty()) out.clear();
     return err;
 }
@@ -322,14 +358,15 @@ std::string Graph::gfa(const Params &p, const std::vector<std::string> &names,
         if (id >= 2) {
             int k = snprintf(buf, sizeof buf, "S\t%u\t%c\n", id - 1, char(p.decode(node.base)));
             out.append(buf, size_t(k));
-            for (uint32_t prev : node.in) {
+            for (const Edge &ed : node.in) {
+                const uint32_t prev = ed.node;
                 if (prev == 0) continue;
                 k = snprintf(buf, sizeof buf, "L\t%u\t+\t%u\t+\t0M\n", prev - 1, id - 1);
                 out.append(buf, size_t(k));
             }
         }
-        for (uint32_t next : node.out)
-            if (--degree[next] == 0) queue.push_back(next);
+        for (const Edge &ed : node.out)
+            if (--degree[ed.node] == 0) queue.push_back(ed.node);
     }
     for (const auto &pa : paths_) {
         if (pa.first >= names.size()) {
@@ -366,10 +403,14 @@ mpoa_graph_view Graph::view() {
         v_rank_[i] = n.rank;
         v_parent_[i] = n.parent;
         v_base_[i] = n.base;
-        v_in_.insert(v_in_.end(), n.in.begin(), n.in.end());
-        v_in_w_.insert(v_in_w_.end(), n.in_w.begin(), n.in_w.end());
-        v_out_.insert(v_out_.end(), n.out.begin(), n.out.end());
-        v_out_w_.insert(v_out_w_.end(), n.out_w.begin(), n.out_w.end());
+        for (const Edge &ed : n.in) {
+            v_in_.push_back(ed.node);
+            v_in_w_.push_back(ed.w);
+        }
+        for (const Edge &ed : n.out) {
+            v_out_.push_back(ed.node);
+            v_out_w_.push_back(ed.w);
+        }
         v_in_off_[i + 1] = v_in_.size();
         v_out_off_[i + 1] = v_out_.size();
     }

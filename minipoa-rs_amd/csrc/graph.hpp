@@ -4,6 +4,7 @@
 // and every tie, so they are kept exactly.
 #pragma once
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -24,11 +25,14 @@ struct Params {
     std::vector<uint8_t> alphabet;
     std::vector<int32_t> matrix;
     bool matrix_valid = true;
+    uint8_t codes[256] = {};  // encode() of every byte, filled by from_c
+    bool coded = false;
 
     static Params from_c(const mpoa_params &p);
     size_t alpha() const { return alphabet.size(); }
     // src/lib.rs:110-126
-    uint32_t encode(uint8_t c) const;
+    uint32_t encode(uint8_t c) const { return coded ? codes[c] : encode_slow(c); }
+    uint32_t encode_slow(uint8_t c) const;
     // src/lib.rs:127-134
     uint8_t decode(uint32_t b) const;
     // src/lib.rs:135-137
@@ -40,6 +44,55 @@ struct Params {
 // src/lib.rs:140-145 (node/aligned -1 = None).
 using AlignedBase = mpoa_aligned_base;
 
+// One adjacency entry: neighbour id and edge weight.
+struct Edge {
+    uint32_t node, w;
+};
+
+// Adjacency list in insertion order with three entries inline: most POA
+// nodes have one to three neighbours, so fusion allocates nothing per node.
+class Edges {
+public:
+    Edges() : n_(0), cap_(kInline) {}
+    Edges(const Edges &o) : n_(0), cap_(kInline) { *this = o; }
+    Edges(Edges &&o) noexcept : n_(o.n_), cap_(o.cap_), u_(o.u_) {
+        o.n_ = 0;
+        o.cap_ = kInline;
+    }
+    Edges &operator=(const Edges &o);
+    Edges &operator=(Edges &&o) noexcept;
+    ~Edges() {
+        if (cap_ > kInline) free(u_.heap);
+    }
+    uint32_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    Edge *data() { return cap_ > kInline ? u_.heap : u_.inl; }
+    const Edge *data() const { return cap_ > kInline ? u_.heap : u_.inl; }
+    Edge &operator[](size_t k) { return data()[k]; }
+    const Edge &operator[](size_t k) const { return data()[k]; }
+    const Edge *begin() const { return data(); }
+    const Edge *end() const { return data() + n_; }
+    int find(uint32_t node) const {
+        const Edge *d = data();
+        for (uint32_t k = 0; k < n_; k++)
+            if (d[k].node == node) return int(k);
+        return -1;
+    }
+    void push(uint32_t node, uint32_t w) {
+        if (n_ == cap_) grow();
+        data()[n_++] = Edge{node, w};
+    }
+
+private:
+    static constexpr uint32_t kInline = 3;
+    void grow();
+    uint32_t n_, cap_;
+    union U {
+        Edge inl[kInline];
+        Edge *heap;
+    } u_;
+};
+
 // src/graph.rs:6-27. Node ids are dense indices; adjacency keeps insertion
 // order. `aligned` lives in one flat table of n_nodes x alphabet entries.
 class Graph {
@@ -49,11 +102,13 @@ public:
     struct Node {
         uint32_t rank = 0, parent = 0, base = 0;
         uint64_t coverage = 0;
-        std::vector<uint32_t> in, in_w, out, out_w;
+        Edges in, out;  // (neighbour, weight) in insertion order
     };
 
     size_t size() const { return nodes_.size(); }
     const Node &node(size_t id) const { return nodes_[id]; }
+    // nodes[id].rank from a flat array (cache-friendly for planning).
+    uint32_t rank_of(size_t id) const { return rank_flat_[id]; }
     int64_t aligned(size_t id, size_t b) const { return aligned_[id * alpha_ + b]; }
     const std::vector<uint32_t> &order() const { return rank_; }
     const std::vector<std::pair<uint64_t, std::vector<uint32_t>>> &paths() const { return paths_; }
@@ -80,6 +135,7 @@ private:
     std::vector<Node> nodes_;
     std::vector<int64_t> aligned_;
     std::vector<uint32_t> rank_;
+    std::vector<uint32_t> rank_flat_;
     std::vector<std::pair<uint64_t, std::vector<uint32_t>>> paths_;
     // view buffers
     std::vector<uint64_t> v_rank_, v_parent_, v_base_, v_in_off_, v_in_, v_in_w_, v_out_off_,

@@ -818,19 +818,17 @@ template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njob
     }
 }
 
+// Tuned on the 4,096-job DNA workload (mean adaptive width ~544): a 16 KB
+// ring with more resident waves beats deeper rings, and two 512-column
+// segments beat one 1,024-column segment.
 uint32_t ring_bytes_for_width(uint32_t width, bool i64) {
     uint32_t row = 2u * ((width + 7u) & ~7u) * (i64 ? 8u : 4u);
-    uint32_t want = 8u * row;
+    uint32_t want = 3u * row;
     want = want < 16384u ? 16384u : (want > 49152u ? 49152u : want);
     return (want + 63u) & ~63u;
 }
 
-int lanes_columns_for_width(uint32_t width) {
-    static const int cs[] = {8, 16};
-    for (int c : cs)
-        if (uint32_t(64 * c) >= width) return c;
-    return 16;
-}
+int lanes_columns_for_width(uint32_t width) { return width > 1024u ? 16 : 8; }
 
 int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s) {
     if (njobs == 0) return 0;

@@ -19,19 +19,17 @@ namespace mpoa {
 struct GraphAcc {
     const Graph &g;
     uint64_t size() const { return g.size(); }
-    uint64_t rank(uint64_t id) const { return g.node(id).rank; }
+    uint64_t rank(uint64_t id) const { return g.rank_of(id); }
     uint64_t order_size() const { return g.order().size(); }
     uint64_t order(uint64_t r) const { return g.order()[r]; }
     uint32_t base(uint64_t id) const { return g.node(id).base; }
     uint64_t parent(uint64_t id) const { return g.node(id).parent; }
     int64_t aligned(uint64_t id, uint64_t b) const { return g.aligned(id, b); }
     template <class F> void for_in(uint64_t id, F &&f) const {
-        const auto &n = g.node(id);
-        for (size_t k = 0; k < n.in.size(); k++) f(uint64_t(n.in[k]), uint64_t(n.in_w[k]));
+        for (const Edge &ed : g.node(id).in) f(uint64_t(ed.node), uint64_t(ed.w));
     }
     template <class F> void for_out(uint64_t id, F &&f) const {
-        const auto &n = g.node(id);
-        for (size_t k = 0; k < n.out.size(); k++) f(uint64_t(n.out[k]), uint64_t(n.out_w[k]));
+        for (const Edge &ed : g.node(id).out) f(uint64_t(ed.node), uint64_t(ed.w));
     }
 };
 
@@ -69,6 +67,29 @@ struct JobPlan {
     std::vector<PredE> preds;
     std::vector<uint8_t> seq;   // encoded query, padded with N
     std::vector<uint32_t> row_node;
+
+    // Back to the default state, keeping the pools' capacity.
+    void reset() {
+        status = MPOA_OK;
+        err.clear();
+        n = m = padded = beg = 0;
+        use_i64 = false;
+        flags = unknown = 0;
+        band = 0;
+        o = e = score_nn = 0;
+        static_cells = arena_est = 0;
+        max_width = width_hint = 0;
+        rows.clear();
+        aux.clear();
+        preds.clear();
+        seq.clear();
+        row_node.clear();
+    }
+};
+
+// Most-supported-path scratch of one row (src/align.rs:623-678).
+struct PathDist {
+    int64_t head, tail, support;
 };
 
 constexpr uint64_t kWidth = 8;
@@ -116,7 +137,7 @@ inline bool static_band(const JobPlan &J, uint32_t i, uint64_t &start, uint64_t 
 template <class G>
 void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, const uint8_t *q,
               uint64_t qlen, bool adaptive, uint64_t band_b, int force_bits, JobPlan &J) {
-    J = JobPlan();
+    J.reset();
     if (beg_id >= g.size() || end_id >= g.size()) {
         J.status = MPOA_ERR_ANCHOR_NOT_IN_GRAPH;
         J.err = "alignment anchor is not in graph";
@@ -153,70 +174,24 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
     J.flags = (p.f == 0 ? kFull : 0) | ((p.f != 0 && adapt) ? kAdaptive : 0) |
               (p.gap_open == 0 ? kGapOpenZero : 0);
 
-    // Most-supported-path distances (src/align.rs:623-678): heaviest edge,
-    // ties to larger support; `support` is shared by both passes.
-    std::vector<int64_t> head(n, INT64_MIN), tail(n, INT64_MIN), support(n, 0);
+    // One pass over `incoming` per row in rank order builds the head
+    // distances (src/align.rs:623-650: heaviest edge, ties to larger
+    // support) and the row's predecessor list. Both keep only in-range
+    // predecessors with head != NEG, and a predecessor's head is final
+    // before its successors are visited. Each entry keeps its position in
+    // `incoming` for traceback ties; lists are sorted by row.
+    thread_local std::vector<PathDist> dist;
+    dist.assign(n, PathDist{INT64_MIN, INT64_MIN, 0});
     J.row_node.resize(n);
     for (uint64_t i = 0; i < n; i++) J.row_node[i] = uint32_t(g.order(beg + i));
-    head[0] = 0;
-    for (uint64_t i = 1; i < n; i++) {
-        bool have = false;
-        uint64_t bp = 0, bwt = 0;
-        g.for_in(J.row_node[i], [&](uint64_t pred, uint64_t w) {
-            uint64_t r = g.rank(pred);
-            if (r < beg || r >= beg + i) return;
-            uint64_t pr = r - beg;
-            if (head[pr] == INT64_MIN) return;
-            if (!have || w > bwt || (w == bwt && support[pr] > support[bp])) {
-                have = true;
-                bp = pr;
-                bwt = w;
-            }
-        });
-        if (have) {
-            head[i] = head[bp] + 1;
-            support[i] = support[bp] + int64_t(bwt);
-        }
-    }
-    tail[n - 1] = 1;
-    for (uint64_t ii = n - 1; ii-- > 0;) {
-        bool have = false;
-        uint64_t bs = 0, bwt = 0;
-        g.for_out(J.row_node[ii], [&](uint64_t succ, uint64_t w) {
-            uint64_t r = g.rank(succ);
-            if (r <= beg + ii || r > end) return;
-            uint64_t sc = r - beg;
-            if (tail[sc] == INT64_MIN) return;
-            if (!have || w > bwt || (w == bwt && support[sc] > support[bs])) {
-                have = true;
-                bs = sc;
-                bwt = w;
-            }
-        });
-        if (have) {
-            tail[ii] = tail[bs] + 1;
-            support[ii] = support[bs] + int64_t(bwt);
-        }
-    }
-    if (head[n - 1] == INT64_MIN) {
-        J.status = MPOA_ERR_NO_PATH;
-        J.err = "no graph path connects alignment anchors";
-        return;
-    }
-
-    // Row pools: in-range predecessors with head != NEG (every consumer of a
-    // predecessor applies that filter), sorted by row, each keeping its
-    // position in `incoming` for traceback ties.
     J.rows.resize(n + 1);
     J.aux.resize(n);
-    J.preds.clear();
+    dist[0].head = 0;
     for (uint64_t i = 0; i < n; i++) {
-        RowIn &r = J.rows[i];
-        r.pred_beg = uint32_t(J.preds.size());
-        r.head = head[i] == INT64_MIN ? kHeadNone : int32_t(head[i]);
-        r.tail = tail[i] == INT64_MIN ? kHeadNone : int32_t(tail[i]);
-        r.pad_ = 0;
-        uint64_t id = J.row_node[i], tot = 0;
+        const uint64_t id = J.row_node[i];
+        const uint32_t pbeg = uint32_t(J.preds.size());
+        bool have = false;
+        uint64_t bp = 0, bwt = 0, tot = 0;
         uint32_t ord = 0;
         g.for_in(id, [&](uint64_t pred, uint64_t w) {
             tot += w;
@@ -224,13 +199,60 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
             uint64_t rk = g.rank(pred);
             if (rk < beg || rk >= beg + i) return;
             uint64_t pr = rk - beg;
-            if (head[pr] == INT64_MIN) return;
+            if (dist[pr].head == INT64_MIN) return;
             J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX)), k});
+            if (!have || w > bwt || (w == bwt && dist[pr].support > dist[bp].support)) {
+                have = true;
+                bp = pr;
+                bwt = w;
+            }
         });
-        std::sort(J.preds.begin() + r.pred_beg, J.preds.end(),
-                  [](const PredE &x, const PredE &y) { return x.row < y.row; });
+        if (i > 0 && have) {
+            dist[i].head = dist[bp].head + 1;
+            dist[i].support = dist[bp].support + int64_t(bwt);
+        }
+        PredE *pe = J.preds.data();
+        for (uint32_t a = pbeg + 1; a < J.preds.size(); a++) {
+            PredE v = pe[a];
+            uint32_t b = a;
+            for (; b > pbeg && pe[b - 1].row > v.row; b--) pe[b] = pe[b - 1];
+            pe[b] = v;
+        }
+        J.rows[i].pred_beg = pbeg;
+        J.rows[i].pad_ = 0;
         J.aux[i].thresh = uint32_t(std::min<uint64_t>(tot / 10, UINT32_MAX));
         J.aux[i].base = g.base(id);
+    }
+    // Tail distances (src/align.rs:651-678), sharing `support` with the
+    // head pass as the reference does.
+    dist[n - 1].tail = 1;
+    for (uint64_t ii = n - 1; ii-- > 0;) {
+        bool have = false;
+        uint64_t bs = 0, bwt = 0;
+        g.for_out(J.row_node[ii], [&](uint64_t succ, uint64_t w) {
+            uint64_t r = g.rank(succ);
+            if (r <= beg + ii || r > end) return;
+            uint64_t sc = r - beg;
+            if (dist[sc].tail == INT64_MIN) return;
+            if (!have || w > bwt || (w == bwt && dist[sc].support > dist[bs].support)) {
+                have = true;
+                bs = sc;
+                bwt = w;
+            }
+        });
+        if (have) {
+            dist[ii].tail = dist[bs].tail + 1;
+            dist[ii].support = dist[bs].support + int64_t(bwt);
+        }
+    }
+    if (dist[n - 1].head == INT64_MIN) {
+        J.status = MPOA_ERR_NO_PATH;
+        J.err = "no graph path connects alignment anchors";
+        return;
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        J.rows[i].head = dist[i].head == INT64_MIN ? kHeadNone : int32_t(dist[i].head);
+        J.rows[i].tail = dist[i].tail == INT64_MIN ? kHeadNone : int32_t(dist[i].tail);
     }
     J.rows[n].pred_beg = uint32_t(J.preds.size());
     J.rows[n].head = J.rows[n].tail = kHeadNone;
