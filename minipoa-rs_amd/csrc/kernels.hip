@@ -50,6 +50,10 @@ template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b
 
 constexpr uint32_t kMatch = 1, kDelete = 2, kInsert = 4, kAll = 7;
 
+// s_waitcnt immediate (GFX9 layout: vmcnt [3:0]+[15:14], expcnt [6:4],
+// lgkmcnt [11:8]) that waits for vmcnt == 0 only.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 // DPP controls (GFX9 encoding).
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 constexpr int kBcast15 = 0x142, kBcast31 = 0x143, kWaveShr1 = 0x138;
@@ -183,17 +187,22 @@ __device__ __forceinline__ void store_run(T *dst, const T (&v)[C], uint32_t vali
     }
 }
 
-// Loads 8 consecutive values from a 16-byte aligned address (LDS or global,
-// resolved at the call site) with 16-byte vector loads.
-template <typename T> __device__ __forceinline__ void load8(const T *src, T (&v)[8]) {
+// Loads G (4 or 8) consecutive values from a 16-byte aligned address (LDS or
+// global, resolved at the call site) with 16-byte vector loads.
+template <typename T, int G> __device__ __forceinline__ void loadg(const T *src, T (&v)[G]) {
+    static_assert(G == 4 || G == 8, "pieces are half or whole 8-column blocks");
     if constexpr (sizeof(T) == 4) {
-        const int4 x = reinterpret_cast<const int4 *>(src)[0];
-        const int4 y = reinterpret_cast<const int4 *>(src)[1];
-        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+#pragma unroll
+        for (int k = 0; k < G; k += 4) {
+            const int4 x = reinterpret_cast<const int4 *>(src)[k / 4];
+            v[k] = x.x;
+            v[k + 1] = x.y;
+            v[k + 2] = x.z;
+            v[k + 3] = x.w;
+        }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; k += 2) {
+        for (int k = 0; k < G; k += 2) {
             const longlong2 x = reinterpret_cast<const longlong2 *>(src)[k / 2];
             v[k] = x.x;
             v[k + 1] = x.y;
@@ -201,22 +210,55 @@ template <typename T> __device__ __forceinline__ void load8(const T *src, T (&v)
     }
 }
 
-template <typename T, int C>
-__global__ __launch_bounds__(64) void poa_fill(DevPools P) {
-    static_assert(C % 8 == 0, "lane chunks follow the 8-column band grid");
+// Workgroup barrier over LDS only: this wave's LDS writes are complete and
+// the compiler keeps LDS accesses on their side of the barrier. Global
+// stores stay in flight (no vmcnt wait); the rare global reads of data
+// another wave wrote use sync_global() first. With one wave the fences
+// remain: lanes exchange data through LDS, and without them the compiler
+// may move a lane's LDS read above another lane's write.
+template <int W> __device__ __forceinline__ void sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    if constexpr (W > 1) __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// Full workgroup barrier: every wave's global stores are complete and
+// visible to the workgroup afterwards.
+template <int W> __device__ __forceinline__ void sync_global() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if constexpr (W > 1) __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Fill: a workgroup of W waves per job. The row is covered in stripes of
+// 64*C*W columns; in each stripe wave w owns the 64*C columns from
+// w*64*C and lane l of it the C columns from l*C. Every wave computes the
+// row's band, arena and ring placement redundantly from the same LDS
+// state; the waves meet twice per row: to chain the insertion carry
+// across waves (per stripe) and to reduce the row's first argmax.
+template <typename T, int C, int W>
+__global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
+    static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
+    constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
     extern __shared__ __align__(16) char dyn_lds[];
     __shared__ LMeta hmeta[kHist];
     __shared__ RowRec hrec[kHist];
     __shared__ RowIn drow[kBlk + 1];
     __shared__ RowAux daux[kBlk];
     __shared__ PredE dpred[kPredCache];
+    __shared__ T xcar[2][W];          // per-wave scan totals, by stripe parity
+    __shared__ T xbest[W];            // per-wave row maximum of M
+    __shared__ uint32_t xbestc[W];    // its first column
+    __shared__ uint64_t xpool;        // arena chunk taken by thread 0
     const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
     T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
     uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
     int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
 
     const DevJob J = P.jobs[blockIdx.x];
-    const int lane = threadIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const int lane = int(tid & 63u);
+    const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));
+    const bool leader = tid == 0;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
     const RowAux *aux = P.aux + J.rows;
@@ -229,7 +271,8 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     const bool adaptive = (J.flags & kAdaptive) != 0;
     const bool full = (J.flags & kFull) != 0;
     const T e = T(J.e), o = T(J.o);
-    const T a = T(C) * e;
+    const T a = T(C) * e;      // decay over one lane run
+    const T a64 = T(64) * a;   // decay over one wave's columns
     const T d15 = T((lane & 15) + 1) * a, d31 = T((lane & 31) + 1) * a;
     constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
     const int64_t mm = int64_t(m);
@@ -237,38 +280,46 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
     const bool mat_lds = P.mat_lds != 0;
     const bool seq_lds = P.seq_lds >= J.padded;
     if (mat_lds)
-        for (uint32_t k = lane; k < alpha * alpha; k += 64) smat[k] = gmat[k];
+        for (uint32_t k = tid; k < alpha * alpha; k += 64u * W) smat[k] = gmat[k];
     if (seq_lds)
-        for (uint32_t k = lane; k < J.padded; k += 64) sseq[k] = seq[k];
+        for (uint32_t k = tid; k < J.padded; k += 64u * W) sseq[k] = seq[k];
 
     uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region
     uint64_t used = 0, cells = 0, pcells = 0;
     uint32_t wpos = 0;      // ring bytes handed out so far
+    uint32_t ring_at = 0;   // wpos modulo the ring size
     uint32_t pb0 = 0, pcnt = 0;
     int32_t status = kOk;
+    // Record of the previous row, kept in registers: the leader writes it
+    // to LDS after the row's last barrier, so the next row reads it here.
+    RowRec last_rec{0, 0, 0, 0, 0};
 
     for (uint32_t i = 0; i < n; i++) {
         if (i % kBlk == 0) {
-            __syncthreads();
+            // Descriptor blocks. The global loads also retire this wave's
+            // earlier stores, so after the barrier every row before i is in
+            // global memory for all waves (the far-row fallbacks rely on it).
+            sync_global<W>();
             const uint32_t cnt = tmin(kBlk, n - i);
-            for (uint32_t k = lane; k <= cnt; k += 64) drow[k] = rin[i + k];
-            for (uint32_t k = lane; k < cnt; k += 64) daux[k] = aux[i + k];
+            for (uint32_t k = tid; k <= cnt; k += 64u * W) drow[k] = rin[i + k];
+            for (uint32_t k = tid; k < cnt; k += 64u * W) daux[k] = aux[i + k];
             pb0 = rin[i].pred_beg;
             pcnt = tmin(rin[i + cnt].pred_beg - pb0, kPredCache);
-            for (uint32_t k = lane; k < pcnt; k += 64) dpred[k] = pr[pb0 + k];
-            __syncthreads();
+            for (uint32_t k = tid; k < pcnt; k += 64u * W) dpred[k] = pr[pb0 + k];
+            sync_global<W>();
         }
         const RowIn ri = drow[i % kBlk];
         const uint32_t pb = ri.pred_beg, pe = drow[i % kBlk + 1].pred_beg;
         if (ri.head == kHeadNone || ri.tail == kHeadNone) {
             // Skipped row (src/align.rs:708-710): no band, never pushes state.
-            if (lane == 0) {
+            if (leader) {
                 hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0, 0};
                 hrec[i % kHist] = RowRec{0, 0, 0, 0, 0};
                 meta[i] = RowMeta{0, 0, 0};
                 if (adaptive) grec[i] = RowRec{0, 0, 0, 0, 0};
             }
-            __syncthreads();
+            last_rec = RowRec{0, 0, 0, 0, 0};
+            sync_lds<W>();
             continue;
         }
         const int64_t h = ri.head, t = ri.tail;
@@ -276,8 +327,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
 
         // Gather the predecessors in parallel, lane q <-> predecessor q (in
         // groups of 64): row, geometry, ring slot and adaptive record.
-        // Rows older than the LDS history come from global memory, where this
-        // wave wrote them earlier.
+        // Rows older than the LDS history come from global memory.
         uint32_t g_row = 0;
         LMeta g_meta{0, 0, kNoSlot, 0, 0};
         RowRec g_rec{0, 0, 0, 0, 0};
@@ -292,11 +342,10 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                 g_row = pq.row;
             }
             const bool far = q < pe && i - g_row >= kHist;
-            if (__builtin_amdgcn_ballot_w64(far)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             if (q < pe) {
                 if (!far) {
                     g_meta = hmeta[g_row % kHist];
-                    if (adaptive) g_rec = hrec[g_row % kHist];
+                    if (adaptive) g_rec = g_row + 1 == i ? last_rec : hrec[g_row % kHist];
                 } else {
                     const RowMeta gm = meta_global(meta, g_row);
                     g_meta = LMeta{gm.start, gm.len, kNoSlot, 0, gm.off};
@@ -361,10 +410,19 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         const uint32_t len = stop - start;
         const uint64_t need = 3ull * len;
         if (cur + need > rend) {
-            // Own region exhausted: take a chunk from the launch's pool.
+            // Own region exhausted: take a chunk from the launch's pool. The
+            // leader's result reaches its wave by readlane and other waves
+            // through LDS behind a barrier.
             uint64_t got = 0;
-            if (lane == 0) got = atomicAdd(P.pool_next, (unsigned long long)P.chunk);
-            got = uint64_t(__shfl(static_cast<long long>(got), 0, 64));
+            if (leader) got = atomicAdd(P.pool_next, (unsigned long long)P.chunk);
+            if constexpr (W > 1) {
+                if (leader) xpool = got;
+                sync_lds<W>();
+                got = xpool;
+                sync_lds<W>();
+            } else {
+                got = uint64_t(rl64(int64_t(got), 0));
+            }
             if (need > P.chunk || got + P.chunk > P.pool_end) {
                 status = kArenaOverflow;
                 break;
@@ -384,18 +442,22 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
         const uint32_t bytes = 2u * len * uint32_t(sizeof(T));
         uint32_t slot = kNoSlot, swpos = 0;
         if (bytes <= kRingBytes / 2) {
-            uint32_t at = wpos % kRingBytes;
-            if (at + bytes > kRingBytes) {
-                wpos += kRingBytes - at;
-                at = 0;
+            if (ring_at + bytes > kRingBytes) {
+                wpos += kRingBytes - ring_at;
+                ring_at = 0;
             }
-            slot = at;
+            slot = ring_at;
             swpos = wpos;
             wpos += bytes;
+            ring_at += bytes;
         }
         const uint32_t sm = (slot == kNoSlot ? 0u : slot) / uint32_t(sizeof(T));
         const uint32_t sd = sm + len;
-        if (lane == 0) meta[i] = RowMeta{start, len, off};
+        if (leader) {
+            meta[i] = RowMeta{start, len, off};
+            // Read by later rows only after this row's first barrier.
+            hmeta[i % kHist] = LMeta{start, len, slot, swpos, off};
+        }
 
         if (i == 0) {
             // Source row (src/align.rs:737-750): M[0] = score(N,N), then
@@ -404,7 +466,7 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                 status = kBandExcludesSource;
                 break;
             }
-            if (lane == 0) {
+            if (leader) {
                 T mprev = T(J.score_nn), iprev = NEG;
                 for (uint32_t j = 0; j < len; j++) {
                     T mj = mprev, ins = iprev;
@@ -422,20 +484,34 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                     iprev = ins;
                     mprev = mj;
                 }
-                hmeta[0] = LMeta{start, len, slot, swpos, off};
                 hrec[0] = RowRec{0, 0, 0, 0, 0};
                 if (adaptive) grec[0] = RowRec{0, 0, 0, 0, 0};
             }
-            __syncthreads();
+            last_rec = RowRec{0, 0, 0, 0, 0};
+            sync_global<W>();  // the source row's arena values too
             continue;
         }
 
+        // Predecessors outside the ring are read back from global memory,
+        // where any wave may have written them: retire and publish every
+        // wave's stores first. Uniform across the workgroup.
+        {
+            bool anymiss = np > 64;
+            if (!anymiss) {
+                const bool miss = pb + uint32_t(lane) < pe && g_meta.len != 0 &&
+                                  (g_meta.slot == kNoSlot || wpos - g_meta.wpos > kRingBytes);
+                anymiss = __builtin_amdgcn_ballot_w64(miss) != 0;
+            }
+            if (anymiss) sync_global<W>();
+        }
+
         const uint32_t srow = ((i == n - 1) ? J.unknown : daux[i % kBlk].base) * alpha;
-        T carry = NEG;
+        T carry = NEG;  // insertion carry into the stripe's first column
         T best = NI;
         uint32_t bestc = 0xFFFFFFFFu;
-        for (uint32_t seg = start; seg < stop; seg += 64u * C) {
-            const uint32_t c0 = seg + uint32_t(lane) * C;
+        uint32_t parity = 0;
+        for (uint32_t seg = start; seg < stop; seg += 64u * C * W, parity ^= 1u) {
+            const uint32_t c0 = seg + wave * 64u * C + uint32_t(lane) * C;
             T Mv[C], Dv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
@@ -443,15 +519,11 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                 Dv[k] = NEG;
             }
             // Predecessor transitions on the band overlap (src/align.rs:751-797).
-            // Bands and lane chunks sit on the same 8-column grid, so each
-            // 8-column piece is wholly inside or outside an overlap; the
+            // Bands and lane runs sit on the same 8-column grid, so each
+            // piece of G columns is wholly inside or outside an overlap; the
             // diagonal into the predecessor's first column reads NEG.
             for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
                 if (np > 64) gather(g0);
-                // Predecessors outside the ring come back from global memory.
-                const bool miss = g0 + uint32_t(lane) < pe && g_meta.len != 0 &&
-                                  (g_meta.slot == kNoSlot || wpos - g_meta.wpos > kRingBytes);
-                if (__builtin_amdgcn_ballot_w64(miss)) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 const uint32_t cnt = tmin(64u, pe - g0);
                 for (uint32_t q = 0; q < cnt; q++) {
                     const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
@@ -460,29 +532,34 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                     if (seg == start) pcells += ove - ovs;
                     const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
                     const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
+                    const uint64_t poff = in_ring ? 0 : uint64_t(rl64(int64_t(g_meta.off), q));
 #pragma unroll
-                    for (int s8 = 0; s8 < C; s8 += 8) {
-                        const uint32_t cs = c0 + s8;
+                    for (int sg = 0; sg < C; sg += G) {
+                        const uint32_t cs = c0 + sg;
                         if (cs < ovs || cs >= ove) continue;
                         const uint32_t rel = cs - ps;
-                        T pm8[8], pd8[8], dg0;
+                        T pm[G], pd[G], dg0;
                         if (in_ring) {
                             const uint32_t pmo = pslot / uint32_t(sizeof(T));
-                            load8(ring + pmo + rel, pm8);
-                            load8(ring + pmo + pl + rel, pd8);
+                            loadg<T, G>(ring + pmo + rel, pm);
+                            loadg<T, G>(ring + pmo + pl + rel, pd);
                             dg0 = rel > 0 ? ring[pmo + rel - 1] : NEG;
                         } else {
-                            const uint64_t poff = uint64_t(rl64(int64_t(g_meta.off), q));
                             const T *PM = arena + poff;
-                            load8(PM + rel, pm8);
-                            load8(PM + pl + rel, pd8);
+                            loadg<T, G>(PM + rel, pm);
+                            loadg<T, G>(PM + pl + rel, pd);
                             dg0 = rel > 0 ? PM[rel - 1] : NEG;
+                            // Retire these loads inside the branch. Otherwise
+                            // the registers reach the join with loads in
+                            // flight and the ring branch gets vmcnt waits,
+                            // which also wait for every store of the wave.
+                            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                         }
 #pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            const T dg = k == 0 ? dg0 : pm8[k - 1];
-                            Mv[s8 + k] = tmax(Mv[s8 + k], dg);
-                            Dv[s8 + k] = tmax(Dv[s8 + k], tmax(pd8[k] + e, pm8[k] + o));
+                        for (int k = 0; k < G; k++) {
+                            const T dg = k == 0 ? dg0 : pm[k - 1];
+                            Mv[sg + k] = tmax(Mv[sg + k], dg);
+                            Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
                         }
                     }
                 }
@@ -491,37 +568,61 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
             T md[C], Lk[C];
             T L = NI;
 #pragma unroll
-            for (int s8 = 0; s8 < C; s8 += 8) {
-                const uint32_t cs = c0 + s8;
+            for (int sg = 0; sg < C; sg += G) {
+                const uint32_t cs = c0 + sg;
                 const bool live = cs < stop;
-                uint32_t codes[8];
+                uint32_t codes[G];
                 if (live) {
                     if (seq_lds) {
-                        const uint2 w = *reinterpret_cast<const uint2 *>(sseq + cs);
+                        if constexpr (G == 8) {
+                            const uint2 w2 = *reinterpret_cast<const uint2 *>(sseq + cs);
 #pragma unroll
-                        for (int k = 0; k < 8; k++) codes[k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu;
+                            for (int k = 0; k < 8; k++)
+                                codes[k] = ((k < 4 ? w2.x : w2.y) >> (8 * (k & 3))) & 0xFFu;
+                        } else {
+                            const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cs);
+#pragma unroll
+                            for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
+                        }
                     } else {
 #pragma unroll
-                        for (int k = 0; k < 8; k++) codes[k] = seq_global(seq, cs + k);
+                        for (int k = 0; k < G; k++) codes[k] = seq_global(seq, cs + k);
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
+                for (int k = 0; k < G; k++) {
                     T v = NI;
                     if (live) {
                         const uint32_t idx = srow + codes[k];
                         const T prof = T(mat_lds ? smat[idx] : mat_global(gmat, idx));
-                        v = tmax(Mv[s8 + k] + prof, Dv[s8 + k]);
+                        v = tmax(Mv[sg + k] + prof, Dv[sg + k]);
                     }
-                    md[s8 + k] = v;
-                    Lk[s8 + k] = L;
+                    md[sg + k] = v;
+                    Lk[sg + k] = L;
                     L = tmax(L + e, v + o);
                 }
             }
-            // Carry into each lane: X_l = max(carry + l*a, S_{l-1}).
+            // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
+            // enters wave w: c_0 = carry, c_{w+1} = max(c_w + 64a, U_w) with
+            // U_w the wave's inclusive scan total (integer max-plus is
+            // associative, so this is the serial recurrence exactly).
             const T S = maxplus_scan<T>(L, a, d15, d31);
             const T Sp = dpp<kWaveShr1, 0xF>(NI, S);
-            const T X = tmax(carry + T(lane) * a, Sp);
+            T cw = carry;
+            if constexpr (W > 1) {
+                if (lane == 63) xcar[parity][wave] = S;
+                sync_lds<W>();
+#pragma unroll
+                for (uint32_t w = 0; w + 1 < W; w++) {
+                    const T u = xcar[parity][w];
+                    if (w < wave) cw = tmax(cw + a64, u);
+                    carry = tmax(carry + a64, u);
+                }
+                carry = tmax(carry + a64, xcar[parity][W - 1]);
+            } else {
+                carry = tmax(carry + a64, lane63(S));
+            }
+            const T X = tmax(cw + T(lane) * a, Sp);
             const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
             T Iv[C];
 #pragma unroll
@@ -543,15 +644,35 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                     store_run<T, C>(ring + sd + rel, Dv, valid);
                 }
             }
-            carry = lane63(tmax(X + a, L));
         }
 
+        // One wave: order this row's LDS writes (ring, geometry) before the
+        // next row's reads. Several waves meet at the barrier below instead.
+        if constexpr (W == 1) sync_lds<W>();
         RowRec rec{0, 0, ofl, ofr, extra};
         if (adaptive) {
             // First argmax of M over the row (src/align.rs:830-844), then the
             // streak / extra / offset update of src/align.rs:845-865.
-            const T gmax = wave_max<T>(best, NI);
-            const uint32_t maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+            T gmax = wave_max<T>(best, NI);
+            uint32_t maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+            if constexpr (W > 1) {
+                if (lane == 0) {
+                    xbest[wave] = gmax;
+                    xbestc[wave] = maxc;
+                }
+                sync_lds<W>();  // also publishes the ring writes
+                gmax = xbest[0];
+                maxc = xbestc[0];
+#pragma unroll
+                for (uint32_t w = 1; w < W; w++) {
+                    const T b = xbest[w];
+                    const uint32_t c = xbestc[w];
+                    if (b > gmax || (b == gmax && c < maxc)) {
+                        gmax = b;
+                        maxc = c;
+                    }
+                }
+            }
             const int64_t maximum = int64_t(maxc);
             int64_t streak = 0, max_streak = 0;
             bool is_match = false;
@@ -577,15 +698,18 @@ __global__ __launch_bounds__(64) void poa_fill(DevPools P) {
                 extra = 0;
             }
             rec = RowRec{maximum, streak, ofl, ofr, extra};
+        } else if (W > 1) {
+            sync_lds<W>();  // publishes the ring writes
         }
-        if (lane == 0) {
-            hmeta[i % kHist] = LMeta{start, len, slot, swpos, off};
+        if (leader) {
+            // Read by rows after the next one, past its first barrier.
             hrec[i % kHist] = rec;
             if (adaptive) grec[i] = rec;
         }
-        __syncthreads();
+        last_rec = rec;
     }
-    if (lane == 0) {
+    sync_global<W>();  // every wave's stores, for the score read below
+    if (leader) {
         DevResult R{};
         R.status = status;
         R.cells = cells;
@@ -806,21 +930,33 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     }
 }
 
-template <typename T, int C> static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
+template <typename T, int C, int W>
+static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
     const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
-    hipLaunchKernelGGL((poa_fill<T, C>), dim3(njobs), dim3(64), lds, s, P);
+    hipLaunchKernelGGL((poa_fill<T, C, W>), dim3(njobs), dim3(64 * W), lds, s, P);
 }
 
-template <typename T> static void launch_fill_c(const DevPools &P, uint32_t njobs, int c, hipStream_t s) {
-    switch (c) {
-        case 8: launch_fill_t<T, 8>(P, njobs, s); break;
-        default: launch_fill_t<T, 16>(P, njobs, s); break;
+template <typename T>
+static int launch_fill_s(const DevPools &P, uint32_t njobs, FillShape f, hipStream_t s) {
+    switch (f.cols * 8 + f.waves) {
+        case 8 * 8 + 1: launch_fill_t<T, 8, 1>(P, njobs, s); return 0;
+        case 16 * 8 + 1: launch_fill_t<T, 16, 1>(P, njobs, s); return 0;
+        case 4 * 8 + 2: launch_fill_t<T, 4, 2>(P, njobs, s); return 0;
+        case 8 * 8 + 2: launch_fill_t<T, 8, 2>(P, njobs, s); return 0;
+        case 4 * 8 + 4: launch_fill_t<T, 4, 4>(P, njobs, s); return 0;
+        case 8 * 8 + 4: launch_fill_t<T, 8, 4>(P, njobs, s); return 0;
+        default: return -1;
     }
 }
 
+bool fill_shape_ok(FillShape f) {
+    const int k = f.cols * 8 + f.waves;
+    return k == 8 * 8 + 1 || k == 16 * 8 + 1 || k == 4 * 8 + 2 || k == 8 * 8 + 2 ||
+           k == 4 * 8 + 4 || k == 8 * 8 + 4;
+}
+
 // Tuned on the 4,096-job DNA workload (mean adaptive width ~544): a 16 KB
-// ring with more resident waves beats deeper rings, and two 512-column
-// segments beat one 1,024-column segment.
+// ring with more resident workgroups beats deeper rings.
 uint32_t ring_bytes_for_width(uint32_t width, bool i64) {
     uint32_t row = 2u * ((width + 7u) & ~7u) * (i64 ? 8u : 4u);
     uint32_t want = 3u * row;
@@ -828,15 +964,18 @@ uint32_t ring_bytes_for_width(uint32_t width, bool i64) {
     return (want + 63u) & ~63u;
 }
 
-int lanes_columns_for_width(uint32_t width) { return width > 1024u ? 16 : 8; }
+// More waves shorten a row's serial path (each wave pays the row's fixed
+// work in parallel and owns fewer columns). Measured on the 4,096-job DNA
+// workload (mean adaptive width ~544, rows up to ~1,500): 8x4 beat 8x2,
+// 4x4, 4x2, 8x1 and 16x1 by 8-30% in fill time. Narrow rows keep one wave.
+FillShape fill_shape_for_width(uint32_t width) {
+    if (width <= 384u) return FillShape{8, 1};
+    return FillShape{8, 4};
+}
 
-int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s) {
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStream_t s) {
     if (njobs == 0) return 0;
-    if (i64)
-        launch_fill_c<int64_t>(P, njobs, cols, s);
-    else
-        launch_fill_c<int32_t>(P, njobs, cols, s);
-    return 0;
+    return i64 ? launch_fill_s<int64_t>(P, njobs, f, s) : launch_fill_s<int32_t>(P, njobs, f, s);
 }
 
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s) {

@@ -37,11 +37,19 @@ uint32_t fill_dynamic_lds(uint32_t ring_bytes, uint32_t seq_lds, uint32_t mat_ld
 // Largest alphabet whose matrix is staged in LDS.
 constexpr uint32_t kMatLdsEntries = 640;
 
-// Columns per lane for a row width (64 lanes x C columns per segment).
-int lanes_columns_for_width(uint32_t width);
-// Default ring bytes for rows of a given width (about eight rows, 16-48 KB).
+// Fill launch shape: `waves` waves per job, `cols` columns per lane; one
+// stripe of a row spans 64 * cols * waves columns.
+struct FillShape {
+    int cols, waves;
+};
+// Shapes with a compiled kernel: (8,1) (16,1) (4,2) (8,2) (4,4) (8,4).
+bool fill_shape_ok(FillShape f);
+// Shape for an expected row width.
+FillShape fill_shape_for_width(uint32_t width);
+// Default ring bytes for rows of a given width (about three rows, 16-48 KB).
 uint32_t ring_bytes_for_width(uint32_t width, bool i64);
-int launch_fill(const DevPools &P, uint32_t njobs, bool i64, int cols, hipStream_t s);
+// Returns -1 for a shape without a compiled kernel.
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStream_t s);
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s);
 
 }  // namespace mpoa

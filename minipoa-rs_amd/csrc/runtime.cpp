@@ -344,8 +344,11 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         }
         if (den) wexp = std::max<uint32_t>(wexp, uint32_t(num / den));
     }
-    int cols = ctx.force_cols ? ctx.force_cols : lanes_columns_for_width(wexp);
-    if (const char *e = getenv("MPOA_COLS")) cols = atoi(e);
+    FillShape shape = ctx.force_shape.cols ? ctx.force_shape : fill_shape_for_width(wexp);
+    if (const char *e = getenv("MPOA_FILL_SHAPE")) {
+        FillShape f{0, 0};
+        if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
+    }
     P.ring_bytes = ring_bytes_for_width(wexp, i64);
     if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
     {
@@ -356,7 +359,10 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     }
     HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
     HIP_TRY(hipEventRecord(ctx.ev[0], ctx.stream));
-    launch_fill(P, uint32_t(N), i64, cols, ctx.stream);
+    if (launch_fill(P, uint32_t(N), i64, shape, ctx.stream) != 0) {
+        set_error("no fill kernel for the requested launch shape");
+        return MPOA_ERR_DEVICE;
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx.ev[1], ctx.stream));
     launch_traceback(P, uint32_t(N), i64, ctx.stream);

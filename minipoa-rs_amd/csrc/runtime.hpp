@@ -72,7 +72,7 @@ struct Ctx {
     DevBuf d_in, d_res, d_meta, d_adapt, d_tb, d_arena;
     PinBuf h_in, h_res, h_tb;
     ThreadPool *pool = nullptr;
-    int force_cols = 0;  // 0: choose from the widest static row
+    FillShape force_shape{0, 0};  // cols 0: choose from the expected row width
     ~Ctx();
 };
 
