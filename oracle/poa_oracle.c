@@ -8,7 +8,7 @@
  * Pinning (what fixes each part independently of this file):
  *   insertion prefix      serial recurrence I<-max(I+e, md+o) (tests)
  *   DP optimum            textbook Gotoh DP + path rescoring on the
- *                         reference's LCG seeds (tests/test_oracle_align.py)
+ *                         reference's LCG seeds (tests/test_oracle.py)
  *   branched optimum      brute force over all DAG paths (tiny graphs)
  *   i32 kernel == i64     cross-implementation equality incl. errors
  *   graph/consensus       reference KATs (src/graph.rs:344-392)
