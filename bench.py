@@ -52,6 +52,29 @@ def fnv1a_many(texts):
     return h
 
 
+def copy_calibration(mib):
+    """A device-to-device copy of `mib` MiB: a plain copy kernel of known size
+    whose rocprofv3 FETCH_SIZE / WRITE_SIZE scale the fill kernel's counters
+    to bytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(a, 1, n)
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed for the copy calibration")
+    log(f"copy calibration: {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +85,8 @@ def main():
                     help="jobs of the workload timed on host cores (0 = skip)")
     ap.add_argument("--plain-jobs", type=int, default=4,
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a device-to-device copy of this size first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -79,6 +104,8 @@ def main():
         f"({time.time() - t0:.1f}s to generate)")
     params = Params(adaptive=True)
     ctx = mh.Context(0)
+    if args.calib_copy_mib > 0:
+        copy_calibration(args.calib_copy_mib)
 
     for w in range(args.warmup):
         t = time.time()
@@ -131,6 +158,7 @@ def main():
                                "8-32 reads/job, 5-15% error 60/20/20, consensus+MSA",
                    "jobs": args.jobs, "reads": int(job_off[-1]), "cells": cells},
         "jobs_per_s": round(jobs_per_s, 1), "failed_jobs": failed,
+        "dp_calls": st["dp_calls"], "retries": st["retries"], "i64_calls": st["i64_calls"],
         "fill_gcups": round(cells / fill_s / 1e9, 2),
         "host_ms": round(st["host_ms"], 1), "device_ms": round(st["device_ms"], 1),
         "roofline": roofline,

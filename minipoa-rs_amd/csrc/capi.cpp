@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "engine.hpp"
 #include "graph.hpp"
 #include "minipoa_hip.h"
 #include "plan.hpp"
@@ -20,7 +21,8 @@ using namespace mpoa;
 struct mpoa_ctx {
     Ctx *c = nullptr;
     std::mutex mu;
-    std::vector<std::unique_ptr<Ctx>> lanes;  // batched-engine pipeline lanes
+    std::vector<std::unique_ptr<Ctx>> lanes;  // batched-engine launch slots
+    std::unique_ptr<TaskQueue> tasks;         // batched-engine host workers
 };
 struct mpoa_graph {
     Graph g;
@@ -147,146 +149,17 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// One pipeline lane of the batched engine: progressive insertion for the
-// jobs `ids`, one fill and one traceback launch per step (src/lib.rs:184-204
-// per job; `poa`'s full-DP retry, src/align.rs:202-207, as a second launch).
-int run_batch_lane(Ctx &c, const Params &p, const std::vector<uint64_t> &ids,
-                   const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
-                   mpoa_batch &B, RunStats &st, double &host_ms) {
-    uint32_t max_reads = 0;
-    for (uint64_t j : ids) max_reads = std::max(max_reads, B.nreads[j]);
-    Params full = p;
-    full.f = 0;
-    std::vector<uint64_t> active;
-    std::vector<JobPlan> plans(ids.size());  // per job, reused across steps
-    std::vector<JobPlan *> dp;
-    std::vector<std::vector<AlignedBase>> paths(ids.size());
-    std::vector<uint64_t> prev_used(ids.size(), 0), prev_n(ids.size(), 0);
-    for (uint32_t s = 0; s < max_reads; s++) {
-        auto t0 = std::chrono::steady_clock::now();
-        active.clear();
-        for (uint64_t x = 0; x < ids.size(); x++)
-            if (s < B.nreads[ids[x]] && B.status[ids[x]] == MPOA_OK) active.push_back(x);
-        std::vector<char> shortcut(active.size(), 0);
-        c.pool->parallel_for(active.size(), [&](size_t k) {
-            uint64_t x = active[k], j = ids[x];
-            uint64_t r = job_off[j] + s;
-            const uint8_t *q = bases + read_off[r];
-            uint64_t qlen = read_off[r + 1] - read_off[r];
-            Graph &g = *B.graphs[j];
-            if (g.size() <= 2) {
-                shortcut[k] = 1;
-                auto &v = paths[x];
-                v.clear();
-                v.push_back(AlignedBase{0, int64_t(p.encode('N')), -1});
-                for (uint64_t t = 0; t < qlen; t++) v.push_back(AlignedBase{-1, int64_t(p.encode(q[t])), -1});
-                return;
-            }
-            JobPlan &J = plans[x];
-            plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, J);
-            // Adaptive bands: start from what the previous step's DP used,
-            // scaled to the grown graph.
-            if (J.status == MPOA_OK && (J.flags & kAdaptive) && prev_n[x] > 0) {
-                uint64_t hint = prev_used[x] * J.n / prev_n[x];
-                hint += hint / 8 + 3ull * J.padded * 8;
-                J.arena_est = std::min(std::max(J.arena_est, hint), arena_full(J));
-                J.width_hint = uint32_t(prev_used[x] / (3 * prev_n[x]));
-            }
-        });
-        dp.clear();
-        std::vector<size_t> dp_k;
-        for (size_t k = 0; k < active.size(); k++)
-            if (!shortcut[k]) {
-                dp.push_back(&plans[active[k]]);
-                dp_k.push_back(k);
-            }
-        host_ms += ms_since(t0);
-        std::vector<DevResult> res;
-        std::vector<uint32_t> tb;
-        std::vector<uint64_t> tb_off;
-        int rc = run_plans(c, p, dp, res, tb, tb_off, &st);
-        if (rc != MPOA_OK) return rc;
-        std::vector<size_t> retry;
-        for (size_t t = 0; t < dp.size(); t++) {
-            bool failed = dp[t]->status != MPOA_OK || res[t].status != kOk;
-            if (failed && p.f != 0) retry.push_back(t);
-        }
-        std::vector<JobPlan> rplans(retry.size());
-        std::vector<DevResult> rres;
-        std::vector<uint32_t> rtb;
-        std::vector<uint64_t> rtb_off;
-        if (!retry.empty()) {
-            auto t1 = std::chrono::steady_clock::now();
-            c.pool->parallel_for(retry.size(), [&](size_t u) {
-                size_t k = dp_k[retry[u]];
-                uint64_t j = ids[active[k]];
-                uint64_t r = job_off[j] + s;
-                plan_job(full, GraphAcc{*B.graphs[j]}, 0, 1, bases + read_off[r],
-                         read_off[r + 1] - read_off[r], false, p.b, 0, rplans[u]);
-            });
-            host_ms += ms_since(t1);
-            std::vector<JobPlan *> rp;
-            for (auto &x : rplans) rp.push_back(&x);
-            rc = run_plans(c, full, rp, rres, rtb, rtb_off, &st);
-            if (rc != MPOA_OK) return rc;
-            st.retries += retry.size();
-        }
-        std::vector<int> retry_of(dp.size(), -1);
-        for (size_t u = 0; u < retry.size(); u++) retry_of[retry[u]] = int(u);
-        auto t2 = std::chrono::steady_clock::now();
-        c.pool->parallel_for(active.size(), [&](size_t k) {
-            uint64_t x = active[k], j = ids[x];
-            Graph &g = *B.graphs[j];
-            if (!shortcut[k]) {
-                size_t t = size_t(std::lower_bound(dp_k.begin(), dp_k.end(), k) - dp_k.begin());
-                const JobPlan *J = dp[t];
-                const DevResult *R = &res[t];
-                const uint32_t *T = J->status == MPOA_OK ? tb.data() + tb_off[t] : nullptr;
-                if (retry_of[t] >= 0) {
-                    int u = retry_of[t];
-                    J = &rplans[size_t(u)];
-                    R = &rres[size_t(u)];
-                    T = J->status == MPOA_OK ? rtb.data() + rtb_off[size_t(u)] : nullptr;
-                } else {
-                    prev_used[x] = R->arena_used;
-                    prev_n[x] = J->n;
-                }
-                if (J->status != MPOA_OK) {
-                    B.status[j] = J->status;
-                    B.err[j] = J->err;
-                    return;
-                }
-                std::string msg;
-                int sst = device_status(*R, msg);
-                if (sst != MPOA_OK) {
-                    B.status[j] = sst;
-                    B.err[j] = msg;
-                    return;
-                }
-                assemble(GraphAcc{g}, *J, T, paths[x]);
-            }
-            std::string e = g.add_path(s, paths[x].data(), paths[x].size());
-            if (e.empty()) e = g.toposort(false);
-            if (!e.empty()) {
-                B.status[j] = MPOA_ERR_GRAPH;
-                B.err[j] = e;
-            }
-        });
-        host_ms += ms_since(t2);
-    }
-    return MPOA_OK;
-}
-
-// Pipeline lanes are created on first use and kept with the context.
-int ensure_pipeline(mpoa_ctx *ctx, int lanes) {
+// Launch slots (a stream with its own buffers each) and the host workers of
+// the batched engine are created on first use and kept with the context.
+int ensure_engine(mpoa_ctx *ctx, int slots) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    unsigned per = std::max(1u, default_threads() / unsigned(lanes));
-    while (int(ctx->lanes.size()) < lanes) {
+    while (int(ctx->lanes.size()) < slots) {
         Ctx *c = nullptr;
-        int rc = ctx_create(ctx->c->device, &c, per);
+        int rc = ctx_create(ctx->c->device, &c, 2);  // 2 threads pack each launch
         if (rc != MPOA_OK) return rc;
         ctx->lanes.emplace_back(c);
     }
+    if (!ctx->tasks) ctx->tasks = std::make_unique<TaskQueue>(std::max(1u, default_threads() - 1));
     return MPOA_OK;
 }
 
@@ -549,63 +422,33 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
         B->graphs[j] = std::make_unique<Graph>(p);
         B->nreads[j] = uint32_t(job_off[j + 1] - job_off[j]);
     }
-    // Two pipeline lanes, each with its own stream, device buffers, staging
-    // and host workers, run disjoint halves of the jobs: one lane's planning
-    // and graph fusion overlap the other lane's kernels. Jobs are dealt to
-    // the lanes by decreasing size so the halves carry similar work.
-    int lanes = 2;
-    if (const char *s = getenv("MPOA_PIPELINE")) lanes = std::max(1, std::min(2, atoi(s)));
-    rc = ensure_pipeline(ctx, lanes);
+    // Jobs advance independently; launch slots take whichever jobs are
+    // ready (engine.hpp).
+    // Three launch slots: with the context's own stream that makes four
+    // streams, one per hardware queue HIP opens by default (GPU_MAX_HW_QUEUES
+    // = 4); a fifth stream shares a queue and its copies wait behind another
+    // slot's work.
+    int nslots = 3;
+    if (const char *s = getenv("MPOA_SLOTS")) nslots = std::max(1, std::min(8, atoi(s)));
+    rc = ensure_engine(ctx, nslots);
     if (rc != MPOA_OK) return rc;
-    std::vector<uint64_t> order(njobs);
-    std::vector<uint64_t> work(njobs, 0);
-    for (uint64_t j = 0; j < njobs; j++) {
-        order[j] = j;
-        uint64_t r0 = job_off[j], r1 = job_off[j + 1];
-        uint64_t L = r1 > r0 ? read_off[r0 + 1] - read_off[r0] : 0;
-        work[j] = L * (r1 - r0) * (r1 - r0);
-    }
-    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return work[a] > work[b]; });
-    std::vector<std::vector<uint64_t>> part(lanes);
-    for (uint64_t k = 0; k < njobs; k++) part[k % lanes].push_back(order[k]);
-    std::vector<RunStats> st(lanes);
-    std::vector<double> host_ms(lanes, 0.0);
-    std::vector<int> lrc(lanes, MPOA_OK);
-    std::vector<std::thread> th;
-    for (int l = 0; l < lanes; l++)
-        th.emplace_back([&, l] {
-            lrc[l] = run_batch_lane(*ctx->lanes[l], p, part[l], job_off, read_off, bases, *B, st[l],
-                                    host_ms[l]);
-        });
-    for (auto &t : th) t.join();
-    for (int l = 0; l < lanes; l++)
-        if (lrc[l] != MPOA_OK) return lrc[l];
-    auto t3 = std::chrono::steady_clock::now();
-    ctx->c->pool->parallel_for(njobs, [&](size_t j) {
-        if (B->status[j] != MPOA_OK) return;
-        Graph &g = *B->graphs[j];
-        B->out0[j] = ">consensus sequence\n" + g.consensus_bytes(p) + "\n";
-        std::vector<std::string> names = read_names(B->nreads[j]), comments(B->nreads[j]);
-        std::string e = g.msa(p, names, comments, B->out1[j]);
-        if (!e.empty()) {
-            B->status[j] = MPOA_ERR_GRAPH;
-            B->err[j] = e;
-        }
-    });
-    double render_ms = ms_since(t3);
+    std::vector<Ctx *> slots;
+    for (int k = 0; k < nslots; k++) slots.push_back(ctx->lanes[size_t(k)].get());
+    EngineInput in{njobs, job_off, read_off, bases, true};
+    EngineOutput eo{&B->status, &B->err, &B->out0, &B->out1, &B->graphs};
+    EngineStats es;
+    rc = run_engine(slots, *ctx->tasks, p, in, eo, es);
+    if (rc != MPOA_OK) return rc;
     mpoa_batch_stats &S = B->stats;
-    for (int l = 0; l < lanes; l++) {
-        S.cells += st[l].cells;
-        S.dp_calls += st[l].dp_calls;
-        S.retries += st[l].retries;
-        S.i64_calls += st[l].i64_calls;
-        S.pred_cells += st[l].pred_cells;
-        S.fill_launches += st[l].fill_launches;
-        S.device_ms += st[l].device_ms;
-        S.fill_ms += st[l].fill_ms;
-        S.host_ms = std::max(S.host_ms, host_ms[l]);
-    }
-    S.host_ms += render_ms;
+    S.cells = es.run.cells;
+    S.dp_calls = es.run.dp_calls;
+    S.retries = es.run.retries;
+    S.i64_calls = es.run.i64_calls;
+    S.pred_cells = es.run.pred_cells;
+    S.fill_launches = es.run.fill_launches;
+    S.device_ms = es.run.device_ms;
+    S.fill_ms = es.run.fill_ms;
+    S.host_ms = es.host_ms;
     S.wall_ms = ms_since(t_all);
     *out = B.release();
     return MPOA_OK;

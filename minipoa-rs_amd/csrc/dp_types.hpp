@@ -45,9 +45,16 @@ struct PredE {
 // Final adaptive band state of a computed row (src/align.rs:829-897): its
 // M argmax column, match streak, band offsets and extra width. Successors
 // fold these in rank order, which reproduces the reference's pushes.
+// All fields are exact as int32 because the device path takes jobs with
+// n + padded <= 2^29 (kMaxSpan): columns, ranks and offsets stay below 2^30,
+// and `extra` saturates at 2^30 (kBandCap), past which every row is full
+// width, so saturation changes no band.
 struct RowRec {
-    int64_t peak, streak, ofl, ofr, extra;
+    int32_t peak, streak, ofl, ofr, extra;
 };
+
+constexpr uint64_t kMaxSpan = uint64_t(1) << 29;  // n + padded of a device DP
+constexpr int32_t kBandCap = int32_t(1) << 30;
 
 // Row geometry written by the fill kernel, read by traceback.
 struct RowMeta {
@@ -72,6 +79,8 @@ struct DevJob {
     uint32_t unknown;   // encode('N')
     int32_t score_nn;   // score(N, N)
     uint32_t flags;
+    int32_t band32;     // min(band, kBandCap)
+    int32_t cap5;       // min(5 * band, kBandCap): the bound of `extra`
     uint32_t pad_;
 };
 

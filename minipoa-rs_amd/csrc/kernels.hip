@@ -45,6 +45,24 @@ template <> struct Lim<int64_t> {
     static constexpr int64_t ninf = -(int64_t(1) << 61);
 };
 
+// Global data is read through address_space(1) pointers, so the compiler
+// never merges an LDS load and a global load into one generic (flat) load:
+// gfx950's backend rejects a select between an LDS and a global generic
+// pointer. ldg() copies a record of 4-byte words.
+#define AS_GLOBAL __attribute__((address_space(1)))
+template <typename T> using gptr = T AS_GLOBAL *;
+template <typename T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>)p; }
+template <typename T> __device__ __forceinline__ T ldg(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole 4-byte words");
+    const gptr<const uint32_t> w = glb(reinterpret_cast<const uint32_t *>(p));
+    uint32_t t[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; k++) t[k] = w[k];
+    T v;
+    __builtin_memcpy(&v, t, sizeof(T));
+    return v;
+}
+
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 
@@ -140,25 +158,6 @@ struct LMeta {
     uint64_t off;    // arena element offset of the row
 };
 
-// Global-memory fallbacks of the LDS caches. Out of line so the compiler
-// cannot fold an LDS load and a global load into one load through a
-// selected generic pointer (the gfx950 backend rejects that form). None of
-// them runs in the steady state: on gfx9 a global load waits for every
-// earlier global store of the wave, which would serialise rows on stores.
-__device__ __attribute__((noinline)) PredE pred_global(const PredE *pr, uint32_t q) { return pr[q]; }
-__device__ __attribute__((noinline)) RowMeta meta_global(const RowMeta *meta, uint32_t p) {
-    return meta[p];
-}
-__device__ __attribute__((noinline)) int32_t mat_global(const int32_t *mat, uint32_t k) {
-    return mat[k];
-}
-__device__ __attribute__((noinline)) uint32_t seq_global(const uint8_t *seq, uint32_t k) {
-    return seq[k];
-}
-__device__ __attribute__((noinline)) RowRec rec_global(const RowRec *rec, uint32_t p) {
-    return rec[p];
-}
-
 // Stores a lane's C contiguous values; whole runs use the widest aligned
 // vector store (row starts are 8-element aligned, lane runs C elements).
 template <typename T, int C>
@@ -210,6 +209,30 @@ template <typename T, int G> __device__ __forceinline__ void loadg(const T *src,
     }
 }
 
+// The same from global memory (clang vector types: HIP's vector classes
+// cannot be copied out of an address_space(1) object).
+typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
+typedef int64_t v2i64 __attribute__((ext_vector_type(2)));
+template <typename T, int G> __device__ __forceinline__ void loadg(gptr<const T> src, T (&v)[G]) {
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < G; k += 4) {
+            const v4i32 x = reinterpret_cast<gptr<const v4i32>>(src)[k / 4];
+            v[k] = x.x;
+            v[k + 1] = x.y;
+            v[k + 2] = x.z;
+            v[k + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; k += 2) {
+            const v2i64 x = reinterpret_cast<gptr<const v2i64>>(src)[k / 2];
+            v[k] = x.x;
+            v[k + 1] = x.y;
+        }
+    }
+}
+
 // Workgroup barrier over LDS only: this wave's LDS writes are complete and
 // the compiler keeps LDS accesses on their side of the barrier. Global
 // stores stay in flight (no vmcnt wait); the rare global reads of data
@@ -235,7 +258,9 @@ template <int W> __device__ __forceinline__ void sync_global() {
 // row's band, arena and ring placement redundantly from the same LDS
 // state; the waves meet twice per row: to chain the insertion carry
 // across waves (per stripe) and to reduce the row's first argmax.
-template <typename T, int C, int W>
+// LT: the score matrix and the query codes are staged in LDS (every launch
+// whose alphabet and queries fit); otherwise they are read from global memory.
+template <typename T, int C, int W, bool LT>
 __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
     static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
     constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
@@ -249,12 +274,13 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
     __shared__ T xbest[W];            // per-wave row maximum of M
     __shared__ uint32_t xbestc[W];    // its first column
     __shared__ uint64_t xpool;        // arena chunk taken by thread 0
+    __shared__ __align__(16) T negrow[16];  // NEG: reads outside a predecessor's band
     const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
     T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
     uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
     int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
 
-    const DevJob J = P.jobs[blockIdx.x];
+    const DevJob J = ldg(P.jobs + blockIdx.x);
     const uint32_t tid = threadIdx.x;
     const int lane = int(tid & 63u);
     const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));
@@ -275,14 +301,13 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
     const T a64 = T(64) * a;   // decay over one wave's columns
     const T d15 = T((lane & 15) + 1) * a, d31 = T((lane & 31) + 1) * a;
     constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
-    const int64_t mm = int64_t(m);
     const uint32_t alpha = J.alpha;
-    const bool mat_lds = P.mat_lds != 0;
-    const bool seq_lds = P.seq_lds >= J.padded;
+    constexpr bool mat_lds = LT, seq_lds = LT;
+    if (tid < 16) negrow[tid] = NEG;
     if (mat_lds)
-        for (uint32_t k = tid; k < alpha * alpha; k += 64u * W) smat[k] = gmat[k];
+        for (uint32_t k = tid; k < alpha * alpha; k += 64u * W) smat[k] = glb(gmat)[k];
     if (seq_lds)
-        for (uint32_t k = tid; k < J.padded; k += 64u * W) sseq[k] = seq[k];
+        for (uint32_t k = tid; k < J.padded; k += 64u * W) sseq[k] = glb(seq)[k];
 
     uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region
     uint64_t used = 0, cells = 0, pcells = 0;
@@ -301,11 +326,11 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             // global memory for all waves (the far-row fallbacks rely on it).
             sync_global<W>();
             const uint32_t cnt = tmin(kBlk, n - i);
-            for (uint32_t k = tid; k <= cnt; k += 64u * W) drow[k] = rin[i + k];
-            for (uint32_t k = tid; k < cnt; k += 64u * W) daux[k] = aux[i + k];
-            pb0 = rin[i].pred_beg;
-            pcnt = tmin(rin[i + cnt].pred_beg - pb0, kPredCache);
-            for (uint32_t k = tid; k < pcnt; k += 64u * W) dpred[k] = pr[pb0 + k];
+            for (uint32_t k = tid; k <= cnt; k += 64u * W) drow[k] = ldg(rin + i + k);
+            for (uint32_t k = tid; k < cnt; k += 64u * W) daux[k] = ldg(aux + i + k);
+            pb0 = ldg(rin + i).pred_beg;
+            pcnt = tmin(ldg(rin + i + cnt).pred_beg - pb0, kPredCache);
+            for (uint32_t k = tid; k < pcnt; k += 64u * W) dpred[k] = ldg(pr + pb0 + k);
             sync_global<W>();
         }
         const RowIn ri = drow[i % kBlk];
@@ -322,7 +347,6 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             sync_lds<W>();
             continue;
         }
-        const int64_t h = ri.head, t = ri.tail;
         const uint32_t np = pe - pb;
 
         // Gather the predecessors in parallel, lane q <-> predecessor q (in
@@ -336,9 +360,19 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             g_meta = LMeta{0, 0, kNoSlot, 0, 0};
             g_rec = RowRec{0, 0, 0, 0, 0};
             g_row = 0;
+            // The global fallbacks below wait for their loads where they are
+            // issued: a load still counted at a control-flow join makes the
+            // compiler put vmcnt waits on the hot path, and on gfx9 vmcnt
+            // also counts the wave's outstanding stores.
             if (q < pe) {
                 const uint32_t k = q - pb0;
-                const PredE pq = k < pcnt ? dpred[k] : pred_global(pr, q);
+                PredE pq;
+                if (k < pcnt) {
+                    pq = dpred[k];
+                } else {
+                    pq = ldg(pr + q);
+                    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                }
                 g_row = pq.row;
             }
             const bool far = q < pe && i - g_row >= kHist;
@@ -347,13 +381,15 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
                     g_meta = hmeta[g_row % kHist];
                     if (adaptive) g_rec = g_row + 1 == i ? last_rec : hrec[g_row % kHist];
                 } else {
-                    const RowMeta gm = meta_global(meta, g_row);
+                    const RowMeta gm = ldg(meta + g_row);
                     g_meta = LMeta{gm.start, gm.len, kNoSlot, 0, gm.off};
-                    if (adaptive) g_rec = rec_global(grec, g_row);
+                    if (adaptive) g_rec = ldg(grec + g_row);
+                    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 }
             }
         };
         auto rl32 = [](uint32_t v, uint32_t q) { return uint32_t(__builtin_amdgcn_readlane(int32_t(v), q)); };
+        auto rli = [](int32_t v, uint32_t q) { return __builtin_amdgcn_readlane(v, q); };
         auto rl64 = [](int64_t v, uint32_t q) {
             const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int32_t(uint64_t(v)), q));
             const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int32_t(uint64_t(v) >> 32), q));
@@ -363,7 +399,7 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
 
         // Adaptive state of row i: the reference's pushes from its computed
         // predecessors (rows >= 1), replayed in rank order.
-        int64_t ofl = 0, ofr = 0, pkl = mm, pkr = 0, extra = 0;
+        int32_t ofl = 0, ofr = 0, pkl = int32_t(m), pkr = 0, extra = 0;
         if (adaptive) {
             for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
                 if (np > 64) gather(g0);
@@ -371,9 +407,9 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
                 for (uint32_t q = 0; q < cnt; q++) {
                     const uint32_t p = rl32(g_row, q);
                     if (p == 0 || rl32(g_meta.len, q) == 0) continue;
-                    const int64_t r_streak = rl64(g_rec.streak, q), r_ofl = rl64(g_rec.ofl, q);
-                    const int64_t r_ofr = rl64(g_rec.ofr, q), r_peak = rl64(g_rec.peak, q);
-                    const int64_t r_extra = rl64(g_rec.extra, q);
+                    const int32_t r_streak = rli(g_rec.streak, q), r_ofl = rli(g_rec.ofl, q);
+                    const int32_t r_ofr = rli(g_rec.ofr, q), r_peak = rli(g_rec.peak, q);
+                    const int32_t r_extra = rli(g_rec.extra, q);
                     if (r_streak >= 30) {
                         ofl = r_ofl;
                         ofr = r_ofr;
@@ -387,19 +423,22 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             }
         }
 
-        // Row band (src/align.rs:711-734).
-        int64_t left, right;
+        // Row band (src/align.rs:711-734), in 32 bits: n + padded <= 2^29,
+        // and band + extra saturates at 2^30, past which the row is full.
+        const int32_t mi = int32_t(m), hh = ri.head, tt = ri.tail;
+        int32_t left, right;
         if (full) {
             left = 0;
-            right = mm;
+            right = mi;
         } else if (adaptive) {
-            const int64_t lo = tmin(tmin(pkl, h + ofl), mm - t);
-            const int64_t hi = tmax(tmax(pkr, h + ofr), mm - t);
-            left = tmax<int64_t>(lo - J.band - extra, 0);
-            right = tmin<int64_t>(hi + J.band + extra, mm);
+            const int32_t lo = tmin(tmin(pkl, hh + ofl), mi - tt);
+            const int32_t hi = tmax(tmax(pkr, hh + ofr), mi - tt);
+            const int32_t be = extra > kBandCap - J.band32 ? kBandCap : J.band32 + extra;
+            left = tmax(lo - be, 0);
+            right = tmin(hi + be, mi);
         } else {
-            left = tmax<int64_t>(tmin(h, mm - t) - J.band, 0);
-            right = tmin<int64_t>(tmax(h, mm - t) + J.band, mm);
+            left = tmax(tmin(hh, mi - tt) - J.band32, 0);
+            right = tmin(tmax(hh, mi - tt) + J.band32, mi);
         }
         if (left > right || right < 0) {
             status = kBandNoIntersect;
@@ -495,15 +534,13 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
         // Predecessors outside the ring are read back from global memory,
         // where any wave may have written them: retire and publish every
         // wave's stores first. Uniform across the workgroup.
-        {
-            bool anymiss = np > 64;
-            if (!anymiss) {
-                const bool miss = pb + uint32_t(lane) < pe && g_meta.len != 0 &&
-                                  (g_meta.slot == kNoSlot || wpos - g_meta.wpos > kRingBytes);
-                anymiss = __builtin_amdgcn_ballot_w64(miss) != 0;
-            }
-            if (anymiss) sync_global<W>();
+        bool anymiss = np > 64;
+        if (!anymiss) {
+            const bool miss = pb + uint32_t(lane) < pe && g_meta.len != 0 &&
+                              (g_meta.slot == kNoSlot || wpos - g_meta.wpos > kRingBytes);
+            anymiss = __builtin_amdgcn_ballot_w64(miss) != 0;
         }
+        if (anymiss) sync_global<W>();
 
         const uint32_t srow = ((i == n - 1) ? J.unknown : daux[i % kBlk].base) * alpha;
         T carry = NEG;  // insertion carry into the stripe's first column
@@ -512,102 +549,166 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
         uint32_t parity = 0;
         for (uint32_t seg = start; seg < stop; seg += 64u * C * W, parity ^= 1u) {
             const uint32_t c0 = seg + wave * 64u * C + uint32_t(lane) * C;
-            T Mv[C], Dv[C];
+            // A wave whose columns all lie past the row's end only joins the
+            // carry exchange: its issue slots go to other workgroups.
+            const bool live = seg + wave * 64u * C < stop;
+            T Mv[C], Dv[C], md[C], Lk[C];
+            T S = NI, Sp = NI;
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < C; k++) {
-                Mv[k] = NEG;
-                Dv[k] = NEG;
-            }
-            // Predecessor transitions on the band overlap (src/align.rs:751-797).
-            // Bands and lane runs sit on the same 8-column grid, so each
-            // piece of G columns is wholly inside or outside an overlap; the
-            // diagonal into the predecessor's first column reads NEG.
-            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
-                if (np > 64) gather(g0);
-                const uint32_t cnt = tmin(64u, pe - g0);
-                for (uint32_t q = 0; q < cnt; q++) {
-                    const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
-                    const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
-                    if (ovs >= ove) continue;
-                    if (seg == start) pcells += ove - ovs;
-                    const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
-                    const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
-                    const uint64_t poff = in_ring ? 0 : uint64_t(rl64(int64_t(g_meta.off), q));
+                for (int k = 0; k < C; k++) {
+                    Mv[k] = NEG;
+                    Dv[k] = NEG;
+                }
+                // Predecessor transitions on the band overlap (src/align.rs:751-797).
+                // Bands and lane runs sit on the same 8-column grid, so each
+                // piece of G columns is wholly inside or outside an overlap; the
+                // diagonal into the predecessor's first column reads NEG.
+                if (!anymiss) {
+                    // Every predecessor row is in the LDS ring: this loop has no
+                    // global loads, so nothing in it waits on vmcnt (which on
+                    // gfx9 also counts the wave's outstanding stores).
+                    for (uint32_t q = 0; q < np; q++) {
+                        const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
+                        const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
+                        if (ovs >= ove) continue;
+                        if (seg == start) pcells += ove - ovs;
+                        const uint32_t pmo = rl32(g_meta.slot, q) / uint32_t(sizeof(T));
 #pragma unroll
-                    for (int sg = 0; sg < C; sg += G) {
-                        const uint32_t cs = c0 + sg;
-                        if (cs < ovs || cs >= ove) continue;
-                        const uint32_t rel = cs - ps;
-                        T pm[G], pd[G], dg0;
+                        for (int sg = 0; sg < C; sg += G) {
+                            // Pieces outside the overlap read the NEG row: max-plus
+                            // with NEG leaves M and D as they are.
+                            const uint32_t cs = c0 + sg;
+                            const bool in = cs >= ovs && cs < ove;
+                            const uint32_t rel = cs - ps;
+                            const T *mp = in ? ring + pmo + rel : negrow;
+                            const T *dp = in ? ring + pmo + pl + rel : negrow;
+                            const T *gp0 = in && rel > 0 ? ring + pmo + rel - 1 : negrow;
+                            T pm[G], pd[G];
+                            loadg<T, G>(mp, pm);
+                            loadg<T, G>(dp, pd);
+                            const T dg0 = *gp0;
+#pragma unroll
+                            for (int k = 0; k < G; k++) {
+                                const T dg = k == 0 ? dg0 : pm[k - 1];
+                                Mv[sg + k] = tmax(Mv[sg + k], dg);
+                                Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
+                            }
+                        }
+                    }
+                } else {
+                for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                    if (np > 64) gather(g0);
+                    const uint32_t cnt = tmin(64u, pe - g0);
+                    for (uint32_t q = 0; q < cnt; q++) {
+                        const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
+                        const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
+                        if (ovs >= ove) continue;
+                        if (seg == start) pcells += ove - ovs;
+                        const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
+                        const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
+                        const uint64_t poff = in_ring ? 0 : uint64_t(rl64(int64_t(g_meta.off), q));
                         if (in_ring) {
+                            // Pieces outside the overlap read the NEG row: max-plus
+                            // with NEG leaves M and D as they are, so the ring path
+                            // has no per-piece branch.
                             const uint32_t pmo = pslot / uint32_t(sizeof(T));
-                            loadg<T, G>(ring + pmo + rel, pm);
-                            loadg<T, G>(ring + pmo + pl + rel, pd);
-                            dg0 = rel > 0 ? ring[pmo + rel - 1] : NEG;
-                        } else {
-                            const T *PM = arena + poff;
+#pragma unroll
+                            for (int sg = 0; sg < C; sg += G) {
+                                const uint32_t cs = c0 + sg;
+                                const bool in = cs >= ovs && cs < ove;
+                                const uint32_t rel = cs - ps;
+                                const T *mp = in ? ring + pmo + rel : negrow;
+                                const T *dp = in ? ring + pmo + pl + rel : negrow;
+                                const T *gp0 = in && rel > 0 ? ring + pmo + rel - 1 : negrow;
+                                T pm[G], pd[G];
+                                loadg<T, G>(mp, pm);
+                                loadg<T, G>(dp, pd);
+                                const T dg0 = *gp0;
+#pragma unroll
+                                for (int k = 0; k < G; k++) {
+                                    const T dg = k == 0 ? dg0 : pm[k - 1];
+                                    Mv[sg + k] = tmax(Mv[sg + k], dg);
+                                    Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
+                                }
+                            }
+                            continue;
+                        }
+#pragma unroll
+                        for (int sg = 0; sg < C; sg += G) {
+                            const uint32_t cs = c0 + sg;
+                            if (cs < ovs || cs >= ove) continue;
+                            const uint32_t rel = cs - ps;
+                            T pm[G], pd[G], dg0;
+                            const gptr<const T> PM = glb(static_cast<const T *>(arena + poff));
                             loadg<T, G>(PM + rel, pm);
                             loadg<T, G>(PM + pl + rel, pd);
                             dg0 = rel > 0 ? PM[rel - 1] : NEG;
-                            // Retire these loads inside the branch. Otherwise
-                            // the registers reach the join with loads in
-                            // flight and the ring branch gets vmcnt waits,
-                            // which also wait for every store of the wave.
+                            // Retire these loads inside the branch. Otherwise the
+                            // registers reach the join with loads in flight and
+                            // the ring path gets vmcnt waits, which also wait for
+                            // every store of the wave.
                             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-                        }
 #pragma unroll
-                        for (int k = 0; k < G; k++) {
-                            const T dg = k == 0 ? dg0 : pm[k - 1];
-                            Mv[sg + k] = tmax(Mv[sg + k], dg);
-                            Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
+                            for (int k = 0; k < G; k++) {
+                                const T dg = k == 0 ? dg0 : pm[k - 1];
+                                Mv[sg + k] = tmax(Mv[sg + k], dg);
+                                Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
+                            }
                         }
                     }
                 }
-            }
-            // Profile + in-lane insertion prefix without carry.
-            T md[C], Lk[C];
-            T L = NI;
+                }
+                // Profile + in-lane insertion prefix without carry.
+                T L = NI;
 #pragma unroll
-            for (int sg = 0; sg < C; sg += G) {
-                const uint32_t cs = c0 + sg;
-                const bool live = cs < stop;
-                uint32_t codes[G];
-                if (live) {
-                    if (seq_lds) {
+                for (int sg = 0; sg < C; sg += G) {
+                    const uint32_t cs = c0 + sg;
+                    const bool piece_live = cs < stop;
+                    uint32_t codes[G];
+                    if constexpr (LT) {
+                        // Dead pieces read column 0's codes and are discarded.
+                        const uint32_t cr = piece_live ? cs : 0u;
                         if constexpr (G == 8) {
-                            const uint2 w2 = *reinterpret_cast<const uint2 *>(sseq + cs);
+                            const uint2 w2 = *reinterpret_cast<const uint2 *>(sseq + cr);
 #pragma unroll
                             for (int k = 0; k < 8; k++)
                                 codes[k] = ((k < 4 ? w2.x : w2.y) >> (8 * (k & 3))) & 0xFFu;
                         } else {
-                            const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cs);
+                            const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cr);
 #pragma unroll
                             for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
                         }
+#pragma unroll
+                        for (int k = 0; k < G; k++) {
+                            const T prof = T(smat[srow + codes[k]]);
+                            const T v = piece_live ? tmax(Mv[sg + k] + prof, Dv[sg + k]) : NI;
+                            md[sg + k] = v;
+                            Lk[sg + k] = L;
+                            L = tmax(L + e, v + o);
+                        }
                     } else {
+                        if (piece_live) {
 #pragma unroll
-                        for (int k = 0; k < G; k++) codes[k] = seq_global(seq, cs + k);
+                            for (int k = 0; k < G; k++) codes[k] = glb(seq)[cs + k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < G; k++) {
+                            T v = NI;
+                            if (piece_live) v = tmax(Mv[sg + k] + T(glb(gmat)[srow + codes[k]]), Dv[sg + k]);
+                            md[sg + k] = v;
+                            Lk[sg + k] = L;
+                            L = tmax(L + e, v + o);
+                        }
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < G; k++) {
-                    T v = NI;
-                    if (live) {
-                        const uint32_t idx = srow + codes[k];
-                        const T prof = T(mat_lds ? smat[idx] : mat_global(gmat, idx));
-                        v = tmax(Mv[sg + k] + prof, Dv[sg + k]);
-                    }
-                    md[sg + k] = v;
-                    Lk[sg + k] = L;
-                    L = tmax(L + e, v + o);
-                }
+                // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
+                // enters wave w: c_0 = carry, c_{w+1} = max(c_w + 64a, U_w) with
+                // U_w the wave's inclusive scan total (integer max-plus is
+                // associative, so this is the serial recurrence exactly).
+                S = maxplus_scan<T>(L, a, d15, d31);
+                Sp = dpp<kWaveShr1, 0xF>(NI, S);
             }
-            // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
-            // enters wave w: c_0 = carry, c_{w+1} = max(c_w + 64a, U_w) with
-            // U_w the wave's inclusive scan total (integer max-plus is
-            // associative, so this is the serial recurrence exactly).
-            const T S = maxplus_scan<T>(L, a, d15, d31);
-            const T Sp = dpp<kWaveShr1, 0xF>(NI, S);
             T cw = carry;
             if constexpr (W > 1) {
                 if (lane == 63) xcar[parity][wave] = S;
@@ -622,6 +723,7 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             } else {
                 carry = tmax(carry + a64, lane63(S));
             }
+            if (!live) continue;
             const T X = tmax(cw + T(lane) * a, Sp);
             const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
             T Iv[C];
@@ -654,7 +756,15 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
             // First argmax of M over the row (src/align.rs:830-844), then the
             // streak / extra / offset update of src/align.rs:845-865.
             T gmax = wave_max<T>(best, NI);
-            uint32_t maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+            uint32_t maxc;
+            if (len <= 64u * C * W) {
+                // One stripe: lanes own increasing columns, so the lowest
+                // lane holding the maximum has its first column.
+                const uint64_t hit = __builtin_amdgcn_ballot_w64(best == gmax);
+                maxc = rl32(bestc, uint32_t(__builtin_ctzll(hit)));
+            } else {
+                maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+            }
             if constexpr (W > 1) {
                 if (lane == 0) {
                     xbest[wave] = gmax;
@@ -673,16 +783,20 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
                     }
                 }
             }
-            const int64_t maximum = int64_t(maxc);
-            int64_t streak = 0, max_streak = 0;
+            const int32_t maximum = int32_t(maxc);
+            int32_t streak = 0, max_streak = 0;
             bool is_match = false;
-            const uint32_t qbase = seq_lds ? uint32_t(sseq[maxc]) : seq_global(seq, maxc);
+            uint32_t qbase;
+            if constexpr (LT)
+                qbase = sseq[maxc];
+            else
+                qbase = glb(seq)[maxc];
             if (daux[i % kBlk].base == qbase) {
                 for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
                     if (np > 64) gather(g0);
                     const uint32_t cnt = tmin(64u, pe - g0);
                     for (uint32_t q = 0; q < cnt; q++) {
-                        const int64_t r_streak = rl64(g_rec.streak, q), r_peak = rl64(g_rec.peak, q);
+                        const int32_t r_streak = rli(g_rec.streak, q), r_peak = rli(g_rec.peak, q);
                         max_streak = tmax(max_streak, r_streak);
                         if (r_peak + 1 == maximum) {
                             streak = tmax(streak, r_streak + 1);
@@ -691,10 +805,10 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
                     }
                 }
             }
-            if (!is_match && max_streak < 30) extra = tmin(extra + max_streak + 1, 5 * J.band);
+            if (!is_match && max_streak < 30) extra = tmin(extra + max_streak + 1, J.cap5);
             if (streak >= 30) {
-                ofl = maximum - h;
-                ofr = maximum - h;
+                ofl = maximum - hh;
+                ofr = maximum - hh;
                 extra = 0;
             }
             rec = RowRec{maximum, streak, ofl, ofr, extra};
@@ -717,9 +831,9 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
         R.arena_used = used;
         R.score = int64_t(NEG);
         if (status == kOk) {
-            const RowMeta rm = meta[n - 1];
+            const RowMeta rm = ldg(meta + (n - 1));
             if (m - 1 >= rm.start && m - 1 - rm.start < rm.len)
-                R.score = int64_t(arena[rm.off + (m - 1 - rm.start)]);
+                R.score = int64_t(glb(arena)[rm.off + (m - 1 - rm.start)]);
         }
         P.res[blockIdx.x] = R;
     }
@@ -740,13 +854,6 @@ struct WRow {
     uint64_t off;
     uint32_t pb, np, thresh, base;
 };
-
-__device__ __attribute__((noinline)) RowIn rin_global(const RowIn *rin, uint32_t i) { return rin[i]; }
-
-template <typename T>
-__device__ __attribute__((noinline)) T cell_global(const T *arena, uint64_t k) {
-    return arena[k];
-}
 
 template <typename T>
 __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) {
@@ -784,18 +891,18 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         const uint32_t r = uint32_t(lane);
         if (r <= i0) {
             const uint32_t ri = i0 - r;
-            const RowMeta rm = meta[ri];
-            const RowIn a = rin[ri], b = rin[ri + 1];
-            const RowAux x = aux[ri];
+            const RowMeta rm = ldg(meta + ri);
+            const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
+            const RowAux x = ldg(aux + ri);
             wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
             const uint32_t np = tmin(b.pred_beg - a.pred_beg, kTbPred);
-            for (uint32_t q = 0; q < np; q++) wpred[r * kTbPred + q] = pr[a.pred_beg + q];
+            for (uint32_t q = 0; q < np; q++) wpred[r * kTbPred + q] = ldg(pr + a.pred_beg + q);
             const uint32_t lo = tmax(cw0, rm.start), hi = tmin(j0 + 1, rm.start + rm.len);
             for (uint32_t c = lo; c < hi; c++) {
                 const uint64_t k = rm.off + (c - rm.start);
-                wm[r * kTbCols + (c - cw0)] = arena[k];
-                wd[r * kTbCols + (c - cw0)] = arena[k + rm.len];
-                wi[r * kTbCols + (c - cw0)] = arena[k + 2ull * rm.len];
+                wm[r * kTbCols + (c - cw0)] = glb(arena)[k];
+                wd[r * kTbCols + (c - cw0)] = glb(arena)[k + rm.len];
+                wi[r * kTbCols + (c - cw0)] = glb(arena)[k + 2ull * rm.len];
             }
         }
         __syncthreads();
@@ -804,8 +911,8 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     // Geometry of a row: window copy, else global.
     auto geo = [&](uint32_t ri) -> WRow {
         if (row_in_win(ri)) return wrow[i0 - ri];
-        const RowMeta rm = meta_global(meta, ri);
-        const RowIn a = rin_global(rin, ri), b = rin_global(rin, ri + 1);
+        const RowMeta rm = ldg(meta + ri);
+        const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
         return WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, 0, 0};
     };
     auto inside = [](const WRow &r, uint32_t c) { return c >= r.start && c - r.start < r.len; };
@@ -817,11 +924,11 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
             if (s == 1) return int64_t(wd[k]);
             return int64_t(wi[k]);
         }
-        return int64_t(cell_global(arena, g.off + uint64_t(s) * g.len + (c - g.start)));
+        return int64_t(glb(arena)[g.off + uint64_t(s) * g.len + (c - g.start)]);
     };
     auto pred = [&](uint32_t ri, const WRow &g, uint32_t q) -> PredE {
         if (row_in_win(ri) && g.np <= kTbPred) return wpred[(i0 - ri) * kTbPred + q];
-        return pred_global(pr, g.pb + q);
+        return ldg(pr + g.pb + q);
     };
 
     uint32_t i = n - 1, j = m - 1, op = kAll;
@@ -930,21 +1037,27 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     }
 }
 
-template <typename T, int C, int W>
+template <typename T, int C, int W, bool LT>
 static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
     const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
-    hipLaunchKernelGGL((poa_fill<T, C, W>), dim3(njobs), dim3(64 * W), lds, s, P);
+    hipLaunchKernelGGL((poa_fill<T, C, W, LT>), dim3(njobs), dim3(64 * W), lds, s, P);
 }
 
 template <typename T>
 static int launch_fill_s(const DevPools &P, uint32_t njobs, FillShape f, hipStream_t s) {
+    if (P.seq_lds == 0 || P.mat_lds == 0) {
+        // Tables in global memory: one shape only.
+        if (f.cols != 8 || f.waves != 1) return -1;
+        launch_fill_t<T, 8, 1, false>(P, njobs, s);
+        return 0;
+    }
     switch (f.cols * 8 + f.waves) {
-        case 8 * 8 + 1: launch_fill_t<T, 8, 1>(P, njobs, s); return 0;
-        case 16 * 8 + 1: launch_fill_t<T, 16, 1>(P, njobs, s); return 0;
-        case 4 * 8 + 2: launch_fill_t<T, 4, 2>(P, njobs, s); return 0;
-        case 8 * 8 + 2: launch_fill_t<T, 8, 2>(P, njobs, s); return 0;
-        case 4 * 8 + 4: launch_fill_t<T, 4, 4>(P, njobs, s); return 0;
-        case 8 * 8 + 4: launch_fill_t<T, 8, 4>(P, njobs, s); return 0;
+        case 8 * 8 + 1: launch_fill_t<T, 8, 1, true>(P, njobs, s); return 0;
+        case 16 * 8 + 1: launch_fill_t<T, 16, 1, true>(P, njobs, s); return 0;
+        case 4 * 8 + 2: launch_fill_t<T, 4, 2, true>(P, njobs, s); return 0;
+        case 8 * 8 + 2: launch_fill_t<T, 8, 2, true>(P, njobs, s); return 0;
+        case 4 * 8 + 4: launch_fill_t<T, 4, 4, true>(P, njobs, s); return 0;
+        case 8 * 8 + 4: launch_fill_t<T, 8, 4, true>(P, njobs, s); return 0;
         default: return -1;
     }
 }

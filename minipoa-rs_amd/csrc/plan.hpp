@@ -62,6 +62,7 @@ struct JobPlan {
     uint64_t arena_est = 0;     // elements requested for the arena
     uint32_t max_width = 0;     // widest static row (chooses columns per lane)
     uint32_t width_hint = 0;    // mean adaptive row width seen on this job's last DP
+    bool full_arena = false;    // size the arena for full-width rows (after an overflow)
     std::vector<RowIn> rows;    // n + 1
     std::vector<RowAux> aux;    // n
     std::vector<PredE> preds;
@@ -79,6 +80,7 @@ struct JobPlan {
         o = e = score_nn = 0;
         static_cells = arena_est = 0;
         max_width = width_hint = 0;
+        full_arena = false;
         rows.clear();
         aux.clear();
         preds.clear();
@@ -150,7 +152,8 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
         return;
     }
     uint64_t n = end - beg + 1, m = qlen + 2;
-    if (n > uint64_t(INT32_MAX) || m > uint64_t(INT32_MAX)) {
+    // The device keeps rows, columns and band offsets in 32 bits (RowRec).
+    if (n > uint64_t(INT32_MAX) || m > uint64_t(INT32_MAX) || n + (m / kWidth + 1) * kWidth > kMaxSpan) {
         J.status = MPOA_ERR_COORDINATE_LIMITS;
         J.err = "alignment exceeds coordinate limits";
         return;

@@ -82,8 +82,12 @@ unsigned default_threads() {
 
 hipError_t DevBuf::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
+    // Grow with 50% headroom: hipFree synchronises the whole device, stalling
+    // the other streams, so reallocations should be rare. If the headroom does
+    // not fit, take the exact size.
+    const size_t old = cap;
     release();
-    size_t want = std::max(bytes, cap + cap / 4);
+    size_t want = std::max(bytes, old) / 2 * 3;
     want = (want + 4095) & ~size_t(4095);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
@@ -111,14 +115,21 @@ void DevBuf::release() {
 
 hipError_t PinBuf::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
+    const size_t old = cap;
     release();
-    size_t want = std::max(bytes, cap + cap / 4);
+    size_t want = std::max(bytes, old) / 2 * 3;
     want = (want + 4095) & ~size_t(4095);
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         p = nullptr;
-        return e;
+        want = bytes;
+        e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return e;
+        }
     }
     cap = want;
     return hipSuccess;
@@ -213,13 +224,13 @@ int device_status(const DevResult &r, std::string &msg) {
 
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// One launch over plans[idx[*]] that share the score width.
-static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
-                     const std::vector<size_t> &idx, bool i64, bool full_arena,
-                     std::vector<DevResult> &res, std::vector<uint32_t> &tb,
-                     const std::vector<uint64_t> &tb_off, RunStats *st) {
-    const size_t N = idx.size();
+int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
+                 bool full_arena, LaunchInfo &li) {
+    const size_t N = plans.size();
     const size_t tsz = i64 ? 8 : 4;
+    li.n = N;
+    li.i64 = i64;
+    li.tb_at.assign(N, 0);
     struct Off {
         uint64_t rows, preds, seq, arena, cap, meta, adapt, tb;
     };
@@ -228,12 +239,14 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
              tbn = 0;
     uint32_t maxw = 0, maxpad = 0;
     for (size_t k = 0; k < N; k++) {
-        const JobPlan &J = *plans[idx[k]];
+        const JobPlan &J = *plans[k];
         Off &o = off[k];
         o.rows = rows;
         o.preds = preds;
         o.seq = seq;
-        o.cap = align_up(full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64), 64);
+        o.cap = align_up((full_arena || J.full_arena) ? arena_full(J)
+                                                      : std::max<uint64_t>(J.arena_est, 64),
+                         64);
         o.arena = arena;
         o.meta = meta;
         o.adapt = adapt;
@@ -266,10 +279,20 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     size_t b_end = align_up(b_mat + alpha * alpha * sizeof(int32_t), 256);
 
     HIP_TRY(hipSetDevice(ctx.device));
-    HIP_TRY(ctx.h_in.ensure(b_end));
+    li.grown = 0;
+    li.arena_bytes = 0;
+    int nbuf = 0;
+    auto grow = [&li, &nbuf](auto &buf, size_t bytes) {
+        const size_t cap = buf.cap;
+        const hipError_t e = buf.ensure(bytes);
+        if (buf.cap != cap) li.grown |= 1u << nbuf;
+        nbuf++;
+        return e;
+    };
+    HIP_TRY(grow(ctx.h_in, b_end));
     char *h = static_cast<char *>(ctx.h_in.p);
     ctx.pool->parallel_for(N, [&](size_t k) {
-        const JobPlan &J = *plans[idx[k]];
+        const JobPlan &J = *plans[k];
         const Off &o = off[k];
         DevJob d{};
         d.rows = o.rows;
@@ -291,6 +314,8 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         d.unknown = J.unknown;
         d.score_nn = J.score_nn;
         d.flags = J.flags;
+        d.band32 = int32_t(std::min<int64_t>(J.band, kBandCap));
+        d.cap5 = int32_t(std::min<int64_t>(5 * J.band, kBandCap));
         memcpy(h + b_jobs + k * sizeof(DevJob), &d, sizeof d);
         memcpy(h + b_rows + o.rows * sizeof(RowIn), J.rows.data(), J.rows.size() * sizeof(RowIn));
         memcpy(h + b_aux + o.rows * sizeof(RowAux), J.aux.data(), J.aux.size() * sizeof(RowAux));
@@ -302,14 +327,15 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     const unsigned long long ctr0 = pool_beg;
     memcpy(h + b_ctr, &ctr0, sizeof ctr0);
 
-    HIP_TRY(ctx.d_in.ensure(b_end));
-    HIP_TRY(ctx.d_res.ensure(N * sizeof(DevResult)));
-    HIP_TRY(ctx.d_meta.ensure(std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
-    HIP_TRY(ctx.d_adapt.ensure(std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
-    HIP_TRY(ctx.d_tb.ensure(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
-    HIP_TRY(ctx.d_arena.ensure(std::max<uint64_t>(arena, 1) * tsz));
-    HIP_TRY(ctx.h_res.ensure(N * sizeof(DevResult)));
-    HIP_TRY(ctx.h_tb.ensure(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+    HIP_TRY(grow(ctx.d_in, b_end));
+    HIP_TRY(grow(ctx.d_res, N * sizeof(DevResult)));
+    HIP_TRY(grow(ctx.d_meta, std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
+    HIP_TRY(grow(ctx.d_adapt, std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
+    HIP_TRY(grow(ctx.d_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+    HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 1) * tsz));
+    li.arena_bytes = std::max<uint64_t>(arena, 1) * tsz;
+    HIP_TRY(grow(ctx.h_res, N * sizeof(DevResult)));
+    HIP_TRY(grow(ctx.h_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
 
     char *d = static_cast<char *>(ctx.d_in.p);
     DevPools P;
@@ -329,6 +355,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     P.chunk = uint32_t(chunk);
     P.seq_lds = maxpad <= 8192 ? maxpad : 0;
     P.mat_lds = alpha * alpha <= kMatLdsEntries ? uint32_t(alpha * alpha * 4) : 0;
+    if (P.seq_lds == 0 || P.mat_lds == 0) P.seq_lds = P.mat_lds = 0;  // both tables in global memory
 
     // Lane width and ring size follow the expected row width: the widest
     // static row, or the mean adaptive width seen in earlier steps.
@@ -336,7 +363,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
     {
         uint64_t num = 0, den = 0;
         for (size_t k = 0; k < N; k++) {
-            const JobPlan &J = *plans[idx[k]];
+            const JobPlan &J = *plans[k];
             if (J.width_hint) {
                 num += J.width_hint;
                 den += 1;
@@ -349,6 +376,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         FillShape f{0, 0};
         if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
     }
+    if (P.seq_lds == 0) shape = FillShape{8, 1};  // the global-table kernel has one shape
     P.ring_bytes = ring_bytes_for_width(wexp, i64);
     if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
     {
@@ -357,8 +385,12 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
         const uint32_t room = 65536u - static_lds - dyn_other;
         if (P.ring_bytes > room) P.ring_bytes = room & ~63u;
     }
+    HIP_TRY(hipEventRecord(ctx.ev[4], ctx.stream));
     HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
     HIP_TRY(hipEventRecord(ctx.ev[0], ctx.stream));
+    li.ring_bytes = P.ring_bytes;
+    li.shape = shape;
+    li.width = wexp;
     if (launch_fill(P, uint32_t(N), i64, shape, ctx.stream) != 0) {
         set_error("no fill kernel for the requested launch shape");
         return MPOA_ERR_DEVICE;
@@ -372,33 +404,84 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
                            ctx.stream));
     HIP_TRY(hipMemcpyAsync(ctx.h_tb.p, P.tb, tbn * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx.stream));
-    HIP_TRY(hipStreamSynchronize(ctx.stream));
+    HIP_TRY(hipEventRecord(ctx.ev[3], ctx.stream));
+    for (size_t k = 0; k < N; k++) li.tb_at[k] = off[k].tb;
+    return MPOA_OK;
+}
+
+bool launch_done(Ctx &ctx) {
+    const hipError_t e = hipEventQuery(ctx.ev[3]);
+    if (e == hipErrorNotReady) return false;
+    if (e != hipSuccess) (void)hipGetLastError();  // launch_finish reports it
+    return true;
+}
+
+uint64_t job_arena_bytes(const JobPlan &J, bool i64) {
+    const uint64_t cap = align_up(J.full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64), 64);
+    return (cap + cap / 2) * (i64 ? 8 : 4);
+}
+
+uint64_t device_free_bytes(int device) {
+    size_t free = 0, total = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&free, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return free;
+}
+
+int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
+    HIP_TRY(hipSetDevice(ctx.device));
+    HIP_TRY(hipEventSynchronize(ctx.ev[3]));
     float fill_ms = 0, all_ms = 0;
     HIP_TRY(hipEventElapsedTime(&fill_ms, ctx.ev[0], ctx.ev[1]));
     HIP_TRY(hipEventElapsedTime(&all_ms, ctx.ev[0], ctx.ev[2]));
-
-    const DevResult *hr = static_cast<const DevResult *>(ctx.h_res.p);
-    const uint32_t *ht = static_cast<const uint32_t *>(ctx.h_tb.p);
-    for (size_t k = 0; k < N; k++) res[idx[k]] = hr[k];
-    ctx.pool->parallel_for(N, [&](size_t k) {
-        const JobPlan &J = *plans[idx[k]];
-        if (hr[k].status == kOk)
-            memcpy(tb.data() + tb_off[idx[k]], ht + off[k].tb, size_t(J.m) * sizeof(uint32_t));
-    });
+    HIP_TRY(hipEventElapsedTime(&li.h2d_ms, ctx.ev[4], ctx.ev[0]));
+    HIP_TRY(hipEventElapsedTime(&li.d2h_ms, ctx.ev[2], ctx.ev[3]));
+    li.fill_ms = fill_ms;
+    li.all_ms = all_ms;
+    const DevResult *hr = launch_results(ctx);
     if (st) {
         // Overflowed attempts are re-run with the full arena; count the cells
         // of the run that completes.
         st->fill_ms += fill_ms;
         st->device_ms += all_ms;
         st->fill_launches += 1;
-        for (size_t k = 0; k < N; k++) {
+        for (size_t k = 0; k < li.n; k++) {
             if (hr[k].status == kArenaOverflow) continue;
             st->cells += hr[k].cells;
             st->pred_cells += hr[k].pred_cells;
             st->dp_calls += 1;
-            if (i64) st->i64_calls += 1;
+            if (li.i64) st->i64_calls += 1;
         }
     }
+    return MPOA_OK;
+}
+
+const DevResult *launch_results(const Ctx &ctx) { return static_cast<const DevResult *>(ctx.h_res.p); }
+const uint32_t *launch_tb(const Ctx &ctx, const LaunchInfo &li, size_t k) {
+    return static_cast<const uint32_t *>(ctx.h_tb.p) + li.tb_at[k];
+}
+
+// One synchronous launch over plans[idx[*]] that share the score width.
+static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
+                     const std::vector<size_t> &idx, bool i64, bool full_arena,
+                     std::vector<DevResult> &res, std::vector<uint32_t> &tb,
+                     const std::vector<uint64_t> &tb_off, RunStats *st) {
+    std::vector<JobPlan *> group(idx.size());
+    for (size_t k = 0; k < idx.size(); k++) group[k] = plans[idx[k]];
+    LaunchInfo li;
+    int rc = launch_begin(ctx, p, group, i64, full_arena, li);
+    if (rc != MPOA_OK) return rc;
+    rc = launch_finish(ctx, li, st);
+    if (rc != MPOA_OK) return rc;
+    const DevResult *hr = launch_results(ctx);
+    for (size_t k = 0; k < idx.size(); k++) res[idx[k]] = hr[k];
+    ctx.pool->parallel_for(idx.size(), [&](size_t k) {
+        if (hr[k].status == kOk)
+            memcpy(tb.data() + tb_off[idx[k]], launch_tb(ctx, li, k),
+                   size_t(group[k]->m) * sizeof(uint32_t));
+    });
     return MPOA_OK;
 }
 

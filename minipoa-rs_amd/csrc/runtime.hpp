@@ -68,7 +68,8 @@ struct RunStats {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // fill start, fill end, traceback end, results on the host, inputs copy start
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf d_in, d_res, d_meta, d_adapt, d_tb, d_arena;
     PinBuf h_in, h_res, h_tb;
     ThreadPool *pool = nullptr;
@@ -86,6 +87,36 @@ unsigned default_threads();
 int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
               std::vector<DevResult> &res, std::vector<uint32_t> &tb,
               std::vector<uint64_t> &tb_off, RunStats *stats);
+
+// An asynchronous launch: one fill and one traceback over plans that share
+// the score width, with staging in and results out on ctx's stream. The plans
+// must stay alive until launch_finish; results stay in ctx's pinned buffers
+// until its next launch.
+struct LaunchInfo {
+    size_t n = 0;
+    bool i64 = false;
+    std::vector<uint64_t> tb_at;  // per job: offset of its traceback record
+    uint32_t ring_bytes = 0;      // the fill's LDS ring
+    FillShape shape{0, 0};        // the fill's launch shape
+    uint32_t width = 0;           // expected row width the shape was chosen for
+    uint32_t grown = 0;           // bit mask of the buffers this launch reallocated
+    uint64_t arena_bytes = 0;     // score arena the launch asked for
+    float fill_ms = 0, all_ms = 0;  // device time of the fill, and of the fill and traceback
+    float h2d_ms = 0, d2h_ms = 0;   // device time of the input and result copies
+};
+int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
+                 bool full_arena, LaunchInfo &li);
+// Score-arena bytes job J adds to a launch, its share of the overflow pool
+// included.
+uint64_t job_arena_bytes(const JobPlan &J, bool i64);
+// Free device memory (0 if it cannot be queried).
+uint64_t device_free_bytes(int device);
+// True once the launch's results are on the host (or it failed).
+bool launch_done(Ctx &ctx);
+// Waits for the launch and adds its timings and cells to *st.
+int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st);
+const DevResult *launch_results(const Ctx &ctx);
+const uint32_t *launch_tb(const Ctx &ctx, const LaunchInfo &li, size_t k);
 
 // Maps a failed DevResult to (status, message) as the reference reports it.
 int device_status(const DevResult &r, std::string &msg);

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import minipoa_hip as mh
-from minipoa_hip import Params, synth
+from minipoa_hip import Params, records, synth
 from test_gpu_batch import fnv1a
 from test_gpu_parity import gv, linear
 from test_oracle import BAND_SETTINGS, SETTINGS_1080, _branched_cases, cases_1080
@@ -112,3 +112,40 @@ def test_shape_batched_workload_sample(ctx, workload_sample, shape):
                         for j in range(len(jo) - 1)], dtype=np.uint64)
         assert (got == want).all(), (shape, name, np.nonzero(got != want))
         assert res.stats()["cells"] == cells, (shape, name)
+
+
+def test_global_table_kernel_large_alphabet(ctx, oracle):
+    """A 30-letter alphabet: its 900-entry score matrix does not fit the LDS
+    table, so the fill reads the matrix and query codes from global memory
+    (the one-shape fallback kernel). Consensus, MSA and GFA bytes equal the
+    oracle's for full, static and adaptive bands."""
+    import random
+
+    rng = random.Random(3030)
+    alphabet = bytes(range(ord("A"), ord("A") + 26)) + b"0123"
+    n = len(alphabet)
+    matrix = [0] * (n * n)
+    for a in range(n):
+        for b in range(a, n):
+            v = rng.randint(4, 11) if a == b else rng.randint(-4, 3)
+            matrix[a * n + b] = matrix[b * n + a] = v
+    root = bytes(rng.choice(alphabet) for _ in range(160))
+    reads = []
+    for _ in range(6):
+        r = bytearray()
+        for c in root:
+            x = rng.random()
+            if x < 0.05:
+                continue
+            r.append(rng.choice(alphabet) if x < 0.15 else c)
+            if rng.random() < 0.05:
+                r.append(rng.choice(alphabet))
+        reads.append(bytes(r))
+    seqs = records(reads)
+    for f, adaptive in ((0, False), (40, False), (40, True)):
+        params = Params(alphabet=alphabet, matrix=matrix, f=f, adaptive=adaptive)
+        dg = mh.align_sequences(ctx, params, seqs)
+        og = oracle.align_sequences(params, seqs)
+        assert dg.consensus(params) == og.consensus(params), (f, adaptive)
+        assert dg.msa(params, seqs) == og.msa(params, seqs), (f, adaptive)
+        assert dg.gfa(params, seqs) == og.gfa(params, seqs), (f, adaptive)
