@@ -131,16 +131,26 @@ def main():
     gcups = cells * len(times) / total / 1e9
     jobs_per_s = args.jobs * len(times) / total
 
-    # Dominant kernel: the fill. Algorithmic bytes: M, D, I (4 B each) written
-    # per cell, predecessor M and D (8 B) read per predecessor-overlap cell.
+    # Dominant kernel: the fill. Algorithmic bytes: the M, D and I scores
+    # (4 B each) written per cell for the traceback. Predecessor rows are
+    # re-read from the LDS ring, so they add no HBM bytes by design.
     fill_s = st["fill_ms"] / 1e3
     launches = st["fill_launches"]
-    alg_bytes = 12 * cells + 8 * st["pred_cells"]
+    alg_bytes = 12 * cells
     int_ops = 6 * cells + 5 * st["pred_cells"] + 2 * cells
     achieved_gbs = alg_bytes / fill_s / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "poa_fill_traffic.json"
+    if tfile.exists():
+        # HBM bytes per cell measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
+        # passes over this workload (copy-kernel scaled), times this run's
+        # cells per launch.
+        t = json.loads(tfile.read_text())
+        traffic = int((t["fetch_bytes_per_cell"] + t["write_bytes_per_cell"]) * cells
+                      / max(1, launches))
     roofline = {
         "bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
         "kernel": "poa_fill", "launches": launches,
         "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3),
         "bytes_per_launch": alg_bytes // max(1, launches),
