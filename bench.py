@@ -11,6 +11,16 @@ fusion and rendering.
 
 Prints one JSON line. value = aggregate GCUPS (DP cells of the reference's
 exact band geometry / wall second).
+
+Besides the headline workload, `legs` reports the other workloads once each,
+every one checked byte for byte against the oracle:
+  c1  plain unseeded consensus: one 2,000 bp template, 10 reads at 10%,
+      one DP per launch (the drop-in path), beside the oracle on one core;
+  c4  protein POA: 2,000 families over a 20-letter alphabet with a drawn
+      symmetric matrix, 16 members at 30% divergence, full DP (-f0);
+  c5  serving: Poisson arrivals of DNA jobs (template log-uniform in
+      [300, 20000] bp, 5-64 reads at 5-15% error; 10% extend a 50-read graph
+      by 20 reads), admitted by device footprint; p50/p99 job latency.
 """
 import argparse
 import json
@@ -87,6 +97,11 @@ def main():
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (counter scaling)")
+    ap.add_argument("--legs", default="c1,c4,c5",
+                    help="other workloads to report once each (c1, c4, c5; empty = none)")
+    ap.add_argument("--serve-jobs", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=0, help="serving launch slots (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -213,8 +228,149 @@ def main():
                             "what": "FNV-1a of consensus FASTA + MSA per job, device vs oracle"}
         out["vs_cpu"] = round(gcups / (ccells / dt / 1e9), 1)
         log(f"cpu baseline: {n} jobs {dt:.1f}s, bit-exact {exact}/{n}")
+    legs = [x for x in args.legs.split(",") if x]
+    if legs:
+        import oracle
+
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        out["legs"] = {}
+        if "c1" in legs:
+            out["legs"]["c1"] = leg_c1(ctx, oracle)
+        if "c4" in legs:
+            out["legs"]["c4"] = leg_c4(ctx, oracle, cores)
+        if "c5" in legs:
+            out["legs"]["c5"] = leg_c5(ctx, oracle, cores, args.serve_jobs, args.serve_rate,
+                                       args.serve_slots)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def hashes_of(get_text, n):
+    import numpy as np
+
+    return np.concatenate([fnv1a_many([get_text(j).encode() for j in range(c, min(n, c + 512))])
+                           for c in range(0, n, 512)]) if n else np.zeros(0, dtype=np.uint64)
+
+
+def leg_c1(ctx, oracle):
+    """Plain unseeded consensus (-r0): one job, one DP per launch."""
+    import minipoa_hip as mh
+    from minipoa_hip import Params, records, synth
+
+    reads = synth.c1()
+    seqs = records(reads)
+    p = Params()
+    mh.align_sequences(ctx, p, seqs)  # warm
+    t = time.perf_counter()
+    g = mh.align_sequences(ctx, p, seqs)
+    dt = time.perf_counter() - t
+    oracle.reset_cells()
+    t = time.perf_counter()
+    og = oracle.align_sequences(p, seqs)
+    ct = time.perf_counter() - t
+    cells = oracle.cells()
+    same = g.consensus(p) == og.consensus(p) and g.msa(p, seqs) == og.msa(p, seqs)
+    log(f"c1: gpu {dt*1e3:.1f} ms, oracle {ct*1e3:.1f} ms, {cells/1e6:.1f} M cells, identical {same}")
+    return {"workload": "plain unseeded consensus: 2000 bp template, 10 reads at 10% error, "
+                        "one DP per launch",
+            "cells": cells, "gpu_ms": round(dt * 1e3, 2), "gpu_gcups": round(cells / dt / 1e9, 4),
+            "cpu_ms": round(ct * 1e3, 2), "cpu_gcups": round(cells / ct / 1e9, 4), "cpu_cores": 1,
+            "cpu_kind": "port", "identical": bool(same)}
+
+
+def leg_c4(ctx, oracle, cores):
+    """Protein families, full DP, custom matrix, batched."""
+    import minipoa_hip as mh
+    from minipoa_hip import synth
+
+    p, jo, ro, b = synth.protein_families()
+    n = len(jo) - 1
+    r = mh.batch_run(ctx, p, jo, ro, b)  # warm
+    del r
+    t = time.perf_counter()
+    r = mh.batch_run(ctx, p, jo, ro, b)
+    dt = time.perf_counter() - t
+    st = r.stats()
+    t = time.perf_counter()
+    ccells, want = oracle.batch_run(p, jo, ro, b, cores)
+    ct = time.perf_counter() - t
+    got = hashes_of(lambda j: r.output(j, 0) + r.output(j, 1), n)
+    exact = int((got == want).sum())
+    log(f"c4: {n} families {dt*1e3:.0f} ms {st['cells']/dt/1e9:.2f} GCUPS; oracle {ct:.1f}s on {cores} cores; "
+        f"bit-exact {exact}/{n}")
+    return {"workload": "protein POA, full DP (-f0): 2000 families, root U[100,800], 16 members at 30% "
+                        "divergence, 20-letter alphabet, drawn symmetric matrix",
+            "cells": st["cells"], "ms": round(dt * 1e3, 1), "gcups": round(st["cells"] / dt / 1e9, 3),
+            "jobs_per_s": round(n / dt, 1), "fill_launches": st["fill_launches"],
+            "cpu_gcups": round(ccells / ct / 1e9, 3), "cpu_jobs_per_s": round(n / ct, 1),
+            "cpu_cores": cores, "cpu_kind": "port", "bit_exact": {"jobs": n, "identical": exact}}
+
+
+def leg_c5(ctx, oracle, cores, njobs, rate, slots=0):
+    """Serving: jobs submitted at their Poisson arrival times."""
+    import numpy as np
+
+    import minipoa_hip as mh
+    from minipoa_hip import Params, records, synth
+
+    p = Params(adaptive=True)
+    stream = synth.serving_stream(njobs=njobs, rate_per_s=rate)
+    # Graphs of the incremental jobs' first 50 reads exist before the stream
+    # (as a -i GFA input would): built here, outside the measured stream.
+    bases_of = {}
+    with mh.Server(ctx, p, render=False) as pre:
+        tk = {k: pre.submit(j["base"]) for k, j in enumerate(stream) if j["base"]}
+        for k, t in tk.items():
+            pre.wait(t)
+            bases_of[k] = pre.graph(t)
+    srv = mh.Server(ctx, p, slots=slots)
+    tickets = []
+    t0 = time.perf_counter()
+    for k, j in enumerate(stream):
+        delay = j["t"] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        g = bases_of.get(k)
+        tickets.append(srv.submit(j["reads"], graph=g, existing=len(j["base"])))
+    for t in tickets:
+        srv.wait(t)
+    wall = time.perf_counter() - t0
+    times = [srv.times(t) for t in tickets]
+    lat = np.array([x["done_ms"] - x["submit_ms"] for x in times])
+    qwait = np.array([x["admit_ms"] - x["submit_ms"] for x in times])
+    first = min(x["submit_ms"] for x in times)
+    span = (max(x["done_ms"] for x in times) - first) / 1e3
+    cells = int(sum(x["cells"] for x in times))
+    failed = sum(srv.status(t) != 0 for t in tickets)
+    got = hashes_of(lambda k: srv.output(tickets[k], 0) + srv.output(tickets[k], 1), len(tickets))
+    st = srv.stats()
+    srv.close()
+    # Oracle: each job from scratch over all its reads (an extension of the
+    # 50-read graph equals aligning all 70 reads in order).
+    full = [j["base"] + j["reads"] for j in stream]
+    jo = np.zeros(len(full) + 1, dtype=np.uint64)
+    jo[1:] = np.cumsum([len(r) for r in full])
+    lens = [len(x) for r in full for x in r]
+    ro = np.zeros(len(lens) + 1, dtype=np.uint64)
+    ro[1:] = np.cumsum(lens)
+    bb = np.frombuffer(b"".join(x for r in full for x in r), dtype=np.uint8)
+    t = time.perf_counter()
+    _, want = oracle.batch_run(p, jo, ro, bb, cores)
+    ct = time.perf_counter() - t
+    exact = int((got == want).sum())
+    pct = lambda a, q: round(float(np.percentile(a, q)), 1)
+    log(f"c5: {len(stream)} jobs over {stream[-1]['t']:.1f}s of arrivals, wall {wall:.1f}s, "
+        f"p50 {pct(lat, 50)} ms p99 {pct(lat, 99)} ms, {cells/span/1e9:.2f} GCUPS, bit-exact {exact}/{len(stream)} "
+        f"(oracle {ct:.1f}s)")
+    return {"workload": f"serving: Poisson arrivals at {rate}/s, {len(stream)} DNA jobs, template log-uniform "
+                        "[300,20000] bp, 5-64 reads at 5-15% error, 10% extend a 50-read graph by 20 reads, "
+                        "adaptive band",
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": round(float(lat.max()), 1),
+            "admission_wait_p99_ms": pct(qwait, 99), "jobs_per_s": round(len(stream) / span, 2),
+            "gcups": round(cells / span / 1e9, 3), "cells": cells, "span_s": round(span, 2),
+            "fill_launches": st["fill_launches"], "failed_jobs": int(failed),
+            "bit_exact": {"jobs": len(stream), "identical": exact,
+                          "what": "FNV-1a of consensus FASTA + MSA per job, served vs oracle from scratch"}}
 
 
 if __name__ == "__main__":

@@ -40,7 +40,8 @@ typedef enum {
     MPOA_ERR_NO_DEVICE = 14,              /* no gfx950 device / library not loadable */
     MPOA_ERR_UNSUPPORTED = 15,            /* a reference feature outside this backend */
     MPOA_ERR_BUFFER_TOO_SMALL = 16,       /* *out_len holds the size needed */
-    MPOA_ERR_INVALID_ARGUMENT = 17
+    MPOA_ERR_INVALID_ARGUMENT = 17,
+    MPOA_ERR_PENDING = 18                 /* a served job has not finished yet */
 } mpoa_status;
 
 /* src/lib.rs:47-71 Params. `alphabet` and `matrix` are borrowed for the
@@ -208,6 +209,54 @@ int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job);
  * would print them. */
 int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out, uint64_t cap,
                           uint64_t *out_len);
+
+/* ---- serving (additive; no reference counterpart) ----
+ * A server admits POA jobs as they arrive and runs them in the same
+ * continuous engine as a batch: every job advances one progressive step per
+ * DP (src/lib.rs:184-204), and the DPs of all admitted jobs that are ready
+ * share launches. A job is admitted when the device footprint of the
+ * admitted jobs (their next DP's score arena, or an estimate before the
+ * first) leaves room for it; others wait in arrival order. A job's results
+ * do not depend on what else is running: each DP is the same computation
+ * alone or in a launch with others. While a server exists, its context's
+ * launch slots are busy: mpoa_batch_run and a second server fail. */
+typedef struct mpoa_server mpoa_server;
+typedef struct {
+    uint64_t admit_bytes; /* footprint bound for admission (0 = the launch slots' arena budget) */
+    uint32_t slots;       /* launch slots, streams of their own (0 = 4: two take DPs of
+                             6,000 rows or more first, two take only shorter DPs) */
+    int32_t render;       /* nonzero: render the -r0 and -r1 text of each finished job */
+} mpoa_server_opts;
+typedef struct {
+    double submit_ms, admit_ms, done_ms; /* on the server's clock (ms since creation) */
+    uint64_t cells;                      /* DP cells of the job's alignments */
+} mpoa_job_times;
+
+int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_server_opts *opts,
+                       mpoa_server **out);
+/* One job: nreads reads, read k = bases[read_off[k] .. read_off[k+1]) (copied).
+ * extend: NULL for a new graph, or a graph holding `existing` sequences that
+ * the job extends as src/lib.rs:184-204 `extend_graph` does for the -i mode
+ * (the job works on a copy; `extend` is not modified). *ticket: the job id. */
+int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *read_off, uint64_t nreads,
+                       const mpoa_graph *extend, uint64_t existing, uint64_t *ticket);
+/* 1 when the job finished, 0 while it runs or waits for admission. */
+int mpoa_server_done(mpoa_server *s, uint64_t ticket);
+/* Blocks until the job finished (MPOA_OK) or the server failed. */
+int mpoa_server_wait(mpoa_server *s, uint64_t ticket);
+/* The finished job's status (MPOA_ERR_PENDING before it finished). */
+int mpoa_server_job_status(mpoa_server *s, uint64_t ticket);
+/* As mpoa_batch_job_output. */
+int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out, uint64_t cap,
+                           uint64_t *out_len);
+int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out);
+/* A copy of the finished job's graph (to extend later, or to render). */
+int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out);
+int mpoa_server_stats_get(mpoa_server *s, mpoa_batch_stats *out);
+/* The server's clock (ms since creation). */
+double mpoa_server_now_ms(mpoa_server *s);
+/* Finishes every submitted job, then releases the context's launch slots. */
+void mpoa_server_free(mpoa_server *s);
 
 #ifdef __cplusplus
 }

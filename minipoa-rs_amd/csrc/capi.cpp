@@ -23,10 +23,16 @@ struct mpoa_ctx {
     std::mutex mu;
     std::vector<std::unique_ptr<Ctx>> lanes;  // batched-engine launch slots
     std::unique_ptr<TaskQueue> tasks;         // batched-engine host workers
+    bool engine_busy = false;                 // a batch or a server holds the slots (under mu)
 };
 struct mpoa_graph {
     Graph g;
     explicit mpoa_graph(const Params &p) : g(p) {}
+};
+struct mpoa_server {
+    mpoa_ctx *ctx = nullptr;
+    Params params;
+    std::unique_ptr<Engine> engine;
 };
 struct mpoa_batch {
     std::vector<int> status;
@@ -153,6 +159,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 // the batched engine are created on first use and kept with the context.
 int ensure_engine(mpoa_ctx *ctx, int slots) {
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->engine_busy) return fail(MPOA_ERR_INVALID_ARGUMENT, "the context's launch slots are in use by a batch or a server");
     while (int(ctx->lanes.size()) < slots) {
         Ctx *c = nullptr;
         int rc = ctx_create(ctx->c->device, &c, 2);  // 2 threads pack each launch
@@ -160,7 +167,35 @@ int ensure_engine(mpoa_ctx *ctx, int slots) {
         ctx->lanes.emplace_back(c);
     }
     if (!ctx->tasks) ctx->tasks = std::make_unique<TaskQueue>(std::max(1u, default_threads() - 1));
+    ctx->engine_busy = true;
     return MPOA_OK;
+}
+
+void release_engine(mpoa_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->engine_busy = false;
+}
+
+// Launch slots of the batched engine: three, so that with the context's own
+// stream there are four streams, one per hardware queue HIP opens by default
+// (GPU_MAX_HW_QUEUES = 4); a fifth stream shares a queue and its copies wait
+// behind another slot's work.
+int engine_slots(int want) {
+    int n = want > 0 ? want : 3;
+    if (const char *s = getenv("MPOA_SLOTS")) n = atoi(s);
+    return std::max(1, std::min(8, n));
+}
+
+void fill_stats(const EngineStats &es, mpoa_batch_stats &S) {
+    S.cells = es.run.cells;
+    S.dp_calls = es.run.dp_calls;
+    S.retries = es.run.retries;
+    S.i64_calls = es.run.i64_calls;
+    S.pred_cells = es.run.pred_cells;
+    S.fill_launches = es.run.fill_launches;
+    S.device_ms = es.run.device_ms;
+    S.fill_ms = es.run.fill_ms;
+    S.host_ms = es.host_ms;
 }
 
 }  // namespace
@@ -424,31 +459,19 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     }
     // Jobs advance independently; launch slots take whichever jobs are
     // ready (engine.hpp).
-    // Three launch slots: with the context's own stream that makes four
-    // streams, one per hardware queue HIP opens by default (GPU_MAX_HW_QUEUES
-    // = 4); a fifth stream shares a queue and its copies wait behind another
-    // slot's work.
-    int nslots = 3;
-    if (const char *s = getenv("MPOA_SLOTS")) nslots = std::max(1, std::min(8, atoi(s)));
+    const int nslots = engine_slots(0);
     rc = ensure_engine(ctx, nslots);
     if (rc != MPOA_OK) return rc;
     std::vector<Ctx *> slots;
     for (int k = 0; k < nslots; k++) slots.push_back(ctx->lanes[size_t(k)].get());
-    EngineInput in{njobs, job_off, read_off, bases, true};
-    EngineOutput eo{&B->status, &B->err, &B->out0, &B->out1, &B->graphs};
+    BatchInput in{njobs, job_off, read_off, bases, true};
+    BatchOutput eo{&B->status, &B->err, &B->out0, &B->out1, &B->graphs};
     EngineStats es;
     rc = run_engine(slots, *ctx->tasks, p, in, eo, es);
+    release_engine(ctx);
     if (rc != MPOA_OK) return rc;
     mpoa_batch_stats &S = B->stats;
-    S.cells = es.run.cells;
-    S.dp_calls = es.run.dp_calls;
-    S.retries = es.run.retries;
-    S.i64_calls = es.run.i64_calls;
-    S.pred_cells = es.run.pred_cells;
-    S.fill_launches = es.run.fill_launches;
-    S.device_ms = es.run.device_ms;
-    S.fill_ms = es.run.fill_ms;
-    S.host_ms = es.host_ms;
+    fill_stats(es, S);
     S.wall_ms = ms_since(t_all);
     *out = B.release();
     return MPOA_OK;
@@ -481,6 +504,137 @@ int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out
         return emit_text(text, out, cap, out_len);
     }
     return fail(MPOA_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+}
+
+// ---- serving: jobs admitted as they arrive ----
+
+int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_server_opts *opts,
+                       mpoa_server **out) {
+    if (!ctx || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    if (p.seeding || p.progressive)
+        return fail(MPOA_ERR_UNSUPPORTED, "seeded or guide-ordered jobs are not implemented");
+    // Four slots by default: two for long DPs, two for short ones.
+    const int nslots = engine_slots(opts && opts->slots ? int(opts->slots) : 4);
+    rc = ensure_engine(ctx, nslots);
+    if (rc != MPOA_OK) return rc;
+    std::vector<Ctx *> slots;
+    for (int k = 0; k < nslots; k++) slots.push_back(ctx->lanes[size_t(k)].get());
+    EngineOpts o;
+    o.render = opts ? opts->render != 0 : true;
+    o.crit_lane = false;
+    o.admit_bytes = opts ? opts->admit_bytes : 0;
+    // Served jobs range from hundreds to tens of thousands of rows per DP:
+    // DPs of 6,000 rows or more (about 25 ms of fill) share half the slots.
+    o.long_rows = 6000;
+    o.long_slots = uint32_t(nslots / 2);
+    auto S = std::make_unique<mpoa_server>();
+    S->ctx = ctx;
+    S->params = p;
+    S->engine = std::make_unique<Engine>(slots, *ctx->tasks, p, o);
+    *out = S.release();
+    return MPOA_OK;
+}
+
+int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *read_off, uint64_t nreads,
+                       const mpoa_graph *extend, uint64_t existing, uint64_t *ticket) {
+    if (!s || !ticket || (nreads && (!bases || !read_off)))
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nreads > UINT32_MAX || existing > UINT32_MAX) return fail(MPOA_ERR_INVALID_ARGUMENT, "too many reads");
+    if (extend && extend->g.alpha() != s->params.alpha())
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "graph alphabet differs from params");
+    if (!extend && existing != 0) return fail(MPOA_ERR_INVALID_ARGUMENT, "existing reads need a graph to extend");
+    JobIn ji;
+    ji.nreads = uint32_t(nreads);
+    ji.first = uint32_t(existing);
+    const uint64_t b0 = nreads ? read_off[0] : 0;
+    ji.own_off.resize(nreads + 1);
+    for (uint64_t k = 0; k <= nreads; k++) ji.own_off[k] = nreads ? read_off[k] - b0 : 0;
+    if (nreads) ji.own_bases.assign(bases + b0, bases + read_off[nreads]);
+    if (extend) ji.graph = std::make_unique<Graph>(extend->g);
+    const uint64_t L = nreads ? ji.own_off[1] : 0;
+    ji.work = L * nreads * nreads;
+    *ticket = s->engine->submit(std::move(ji));
+    return MPOA_OK;
+}
+
+int mpoa_server_done(mpoa_server *s, uint64_t ticket) {
+    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    return s->engine->done(ticket) ? 1 : 0;
+}
+
+int mpoa_server_wait(mpoa_server *s, uint64_t ticket) {
+    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->engine->wait(ticket)) return fail(s->engine->error(), "the engine stopped: " + std::string(last_error()));
+    return MPOA_OK;
+}
+
+int mpoa_server_job_status(mpoa_server *s, uint64_t ticket) {
+    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
+    const JobOut &o = s->engine->out(ticket);
+    if (o.status != MPOA_OK) set_error(o.err);
+    return o.status;
+}
+
+int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out, uint64_t cap,
+                           uint64_t *out_len) {
+    const int st = mpoa_server_job_status(s, ticket);
+    if (st != MPOA_OK) return st;
+    JobOut &o = s->engine->out(ticket);
+    if (mode == 0) return emit_text(o.out0, out, cap, out_len);
+    if (mode == 1) return emit_text(o.out1, out, cap, out_len);
+    if (mode == 2) {
+        std::string text;
+        std::string e = o.graph->gfa(s->params, read_names(o.nseq), text);
+        if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+        return emit_text(text, out, cap, out_len);
+    }
+    return fail(MPOA_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+}
+
+int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out) {
+    if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
+    const JobOut &o = s->engine->out(ticket);
+    out->submit_ms = o.t_submit;
+    out->admit_ms = o.t_admit;
+    out->done_ms = o.t_done;
+    out->cells = o.cells;
+    return MPOA_OK;
+}
+
+int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) {
+    if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    const int st = mpoa_server_job_status(s, ticket);
+    if (st != MPOA_OK) return st;
+    auto *g = new mpoa_graph(s->params);
+    g->g = *s->engine->out(ticket).graph;
+    *out = g;
+    return MPOA_OK;
+}
+
+int mpoa_server_stats_get(mpoa_server *s, mpoa_batch_stats *out) {
+    if (!s || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = mpoa_batch_stats{};
+    fill_stats(s->engine->stats(), *out);
+    out->wall_ms = s->engine->now_ms();
+    return MPOA_OK;
+}
+
+double mpoa_server_now_ms(mpoa_server *s) { return s ? s->engine->now_ms() : 0.0; }
+
+void mpoa_server_free(mpoa_server *s) {
+    if (!s) return;
+    mpoa_ctx *ctx = s->ctx;
+    s->engine.reset();  // finishes every submitted job first
+    delete s;
+    release_engine(ctx);
 }
 
 }  // extern "C"

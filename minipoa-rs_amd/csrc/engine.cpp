@@ -2,13 +2,10 @@
 #include "engine.hpp"
 
 #include <algorithm>
-#include <array>
-#include <thread>
-#include <atomic>
-#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 
+#include "minipoa_hip.h"
 #include "plan.hpp"
 
 namespace mpoa {
@@ -52,25 +49,26 @@ void TaskQueue::loop() {
     }
 }
 
-namespace {
-
 // One job's progressive state.
-struct EJob {
-    uint64_t j = 0;
-    uint32_t step = 0, nreads = 0;
+struct Engine::Job {
+    uint64_t id = 0;
+    JobIn in;
+    JobOut out;
+    uint32_t step = 0;       // reads of `in` aligned so far
     bool retry = false;      // the planned DP is poa's full-matrix retry
     JobPlan plan;
     uint64_t prev_used = 0, prev_n = 0;
     uint64_t cost = 0;       // expected cells of the planned DP
-    bool crit = false;       // one of the largest jobs: runs in the priority slot
     uint64_t rem = ~0ull;    // cells left when last queued (0 once finished; under the engine lock)
+    uint64_t footprint = 0;  // bytes counted in active_bytes_ (under the engine lock)
+    bool admitted = false;
     std::vector<AlignedBase> path;
     DevResult res{};
     std::vector<uint32_t> tb;
 };
 
 // One line per launch for MPOA_TRACE=<file>: scheduling diagnostics.
-struct TraceRec {
+struct Engine::TraceRec {
     int slot;
     double t_begin, t_launched, t_done;  // host clock: before packing, after enqueueing, results on host
     float fill_ms, all_ms, h2d_ms, d2h_ms;  // device clock
@@ -78,10 +76,13 @@ struct TraceRec {
     uint64_t arena_bytes, arena_cap;
     size_t njobs;
     uint64_t max_cost, sum_cost;
-    int64_t watched_step;  // step of the largest job if it is in this launch, else -1
+    int64_t watched_step;  // step of the job with the most work if it is in this launch, else -1
     uint32_t ring_bytes, width;
     FillShape shape;
+    uint64_t rows;
 };
+
+namespace {
 
 uint64_t env_u64(const char *name, uint64_t dflt, uint64_t lo, uint64_t hi) {
     const char *s = getenv(name);
@@ -94,345 +95,116 @@ uint64_t env_u64(const char *name, uint64_t dflt, uint64_t lo, uint64_t hi) {
 
 }  // namespace
 
-int run_engine(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, const EngineInput &in,
-               EngineOutput &out, EngineStats &st) {
-    const uint64_t N = in.njobs;
-    Params full = p;
-    full.f = 0;
-    std::vector<EJob> jobs(N);
-    std::mutex mu;
-    std::condition_variable cv;
-    std::vector<EJob *> ready, ready_crit, crit_jobs;
-    uint64_t finished = 0, active_crit = 0;
-    bool stop = false;  // every job finished, or a launch failed
-    const auto t_start = std::chrono::steady_clock::now();
-    auto now_ms = [&] {
-        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    };
-    double crit_since = 0;  // when the first of the waiting critical jobs became ready
-    uint64_t outstanding = 0;  // queued or running host tasks
-    std::atomic<uint64_t> retries{0};
-    std::atomic<int64_t> task_ns{0};
-
-    auto finish = [&](EJob &J) {
-        std::lock_guard<std::mutex> lk(mu);
-        finished++;
-        J.rem = 0;
-        if (J.crit) active_crit--;
-        if (finished == N) stop = true;
-        cv.notify_all();
-    };
-    auto requeue = [&](EJob &J) {
-        std::lock_guard<std::mutex> lk(mu);
-        J.rem = J.cost * uint64_t(J.nreads - J.step);
-        if (J.crit) {
-            if (ready_crit.empty()) crit_since = now_ms();
-            ready_crit.push_back(&J);
-        } else {
-            ready.push_back(&J);
-        }
-        cv.notify_all();
-    };
-    auto fail_job = [&](EJob &J, int status, const std::string &e) {
-        (*out.status)[J.j] = status;
-        (*out.err)[J.j] = e;
-        finish(J);
-    };
-    auto query = [&](const EJob &J, const uint8_t *&q, uint64_t &qlen) {
-        const uint64_t r = in.job_off[J.j] + J.step;
-        q = in.bases + in.read_off[r];
-        qlen = in.read_off[r + 1] - in.read_off[r];
-    };
-    // graph.rs:82-178 per read; false when the job ended with an error.
-    auto fuse = [&](EJob &J) {
-        Graph &g = *(*out.graphs)[J.j];
-        std::string e = g.add_path(J.step, J.path.data(), J.path.size());
-        if (e.empty()) e = g.toposort(false);
-        if (!e.empty()) {
-            fail_job(J, MPOA_ERR_GRAPH, e);
-            return false;
-        }
-        J.step++;
-        return true;
-    };
-    auto render = [&](EJob &J) {
-        Graph &g = *(*out.graphs)[J.j];
-        (*out.out0)[J.j] = ">consensus sequence\n" + g.consensus_bytes(p) + "\n";
-        std::vector<std::string> names(J.nreads), comments(J.nreads);
-        for (uint32_t k = 0; k < J.nreads; k++) names[k] = "read" + std::to_string(k);
-        std::string e = g.msa(p, names, comments, (*out.out1)[J.j]);
-        if (!e.empty()) {
-            (*out.status)[J.j] = MPOA_ERR_GRAPH;
-            (*out.err)[J.j] = e;
-        }
-    };
-    // Consumes the job's DP result (have), fuses it and plans the next DP.
-    // src/lib.rs:195-202 (one read per step), src/align.rs:81-98 (empty
-    // graph), src/align.rs:202-207 (full-matrix retry on any error).
-    auto advance = [&](EJob &J, bool have) {
-        Graph &g = *(*out.graphs)[J.j];
-        for (;;) {
-            if (have) {
-                have = false;
-                if (J.plan.status == MPOA_OK && J.res.status == kArenaOverflow) {
-                    J.plan.full_arena = true;  // same DP, arena for full rows
-                    requeue(J);
-                    return;
-                }
-                const bool failed = J.plan.status != MPOA_OK || J.res.status != kOk;
-                if (failed && p.f != 0 && !J.retry) {
-                    J.retry = true;
-                    retries++;
-                    const uint8_t *q;
-                    uint64_t qlen;
-                    query(J, q, qlen);
-                    plan_job(full, GraphAcc{g}, 0, 1, q, qlen, false, p.b, 0, J.plan);
-                    if (J.plan.status != MPOA_OK) {
-                        have = true;
-                        continue;
-                    }
-                    J.cost = J.plan.static_cells;
-                    requeue(J);
-                    return;
-                }
-                if (J.plan.status != MPOA_OK) {
-                    fail_job(J, J.plan.status, J.plan.err);
-                    return;
-                }
-                std::string msg;
-                const int sst = device_status(J.res, msg);
-                if (sst != MPOA_OK) {
-                    fail_job(J, sst, msg);
-                    return;
-                }
-                if (!J.retry) {
-                    J.prev_used = J.res.arena_used;
-                    J.prev_n = J.plan.n;
-                }
-                J.retry = false;
-                assemble(GraphAcc{g}, J.plan, J.tb.data(), J.path);
-                if (!fuse(J)) return;
-            }
-            if (J.step >= J.nreads) {
-                if (in.render) render(J);
-                finish(J);
-                return;
-            }
-            const uint8_t *q;
-            uint64_t qlen;
-            query(J, q, qlen);
-            if (g.size() <= 2) {
-                J.path.clear();
-                J.path.push_back(AlignedBase{0, int64_t(p.encode('N')), -1});
-                for (uint64_t t = 0; t < qlen; t++)
-                    J.path.push_back(AlignedBase{-1, int64_t(p.encode(q[t])), -1});
-                if (!fuse(J)) return;
-                continue;
-            }
-            JobPlan &P = J.plan;
-            plan_job(p, GraphAcc{g}, 0, 1, q, qlen, p.adaptive, p.b, 0, P);
-            if (P.status != MPOA_OK) {
-                have = true;
-                continue;
-            }
-            J.cost = P.static_cells;
-            // Adaptive bands: start from what the previous step's DP used,
-            // scaled to the grown graph.
-            if ((P.flags & kAdaptive) && J.prev_n > 0) {
-                uint64_t hint = J.prev_used * P.n / J.prev_n;
-                hint += hint / 8 + 3ull * P.padded * 8;
-                P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
-                P.width_hint = uint32_t(J.prev_used / (3 * J.prev_n));
-                J.cost = uint64_t(P.width_hint) * P.n;
-            }
-            requeue(J);
-            return;
-        }
-    };
-    // MPOA_TRACE also records the host side of the largest job's steps:
-    // result on the host, task start, task end (host clock, ms).
-    const EJob *watched = nullptr;
-    std::vector<std::array<double, 3>> wtrace;
-    auto submit = [&](EJob &J, bool have) {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            outstanding++;
-        }
-        tasks.push([&, jp = &J, have, ts = now_ms()] {
-            const auto t0 = std::chrono::steady_clock::now();
-            const double tb = jp == watched ? now_ms() : 0;
-            advance(*jp, have);
-            if (jp == watched) wtrace.push_back({ts, tb, now_ms()});
-            task_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
-                           std::chrono::steady_clock::now() - t0)
-                           .count();
-            std::lock_guard<std::mutex> lk(mu);
-            outstanding--;
-            cv.notify_all();
-        }, J.crit);
-    };
-
-    // Largest jobs first: they bound the makespan.
-    std::vector<uint64_t> order(N), work(N);
-    for (uint64_t j = 0; j < N; j++) {
-        EJob &J = jobs[j];
-        J.j = j;
-        J.nreads = uint32_t(in.job_off[j + 1] - in.job_off[j]);
-        const uint64_t r0 = in.job_off[j];
-        const uint64_t L = J.nreads ? in.read_off[r0 + 1] - in.read_off[r0] : 0;
-        work[j] = L * J.nreads * J.nreads;
-        order[j] = j;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return work[a] > work[b]; });
-    // The largest jobs bound the makespan through their chain of steps. With
-    // two or more slots they get slot 0 to themselves and advance in
-    // lockstep: relaunched as soon as all of them are ready (or after a short
-    // wait), instead of queueing behind other launches.
-    const bool crit_lane = slots.size() >= 2 && N > 1;
-    if (crit_lane) {
-        const double frac = double(env_u64("MPOA_CRIT_PCT", 50, 0, 100)) / 100.0;
-        const uint64_t cap = env_u64("MPOA_CRIT_MAX", 256, 0, N);
-        for (uint64_t k = 0; k < N && k < cap; k++) {
-            if (double(work[order[k]]) < frac * double(work[order[0]])) break;
-            jobs[order[k]].crit = true;
-            crit_jobs.push_back(&jobs[order[k]]);
-            active_crit++;
-        }
-    }
-    const double crit_wait_ms = double(env_u64("MPOA_CRIT_WAIT_US", 50000, 0, 10000000)) / 1000.0;
-    const char *trace_path = getenv("MPOA_TRACE");
-    if (trace_path && N) watched = &jobs[order[0]];
-    for (uint64_t j : order) submit(jobs[j], false);
-
-    const uint64_t max_launch = env_u64("MPOA_MAX_LAUNCH", 1024, 1, 1u << 20);
-    const size_t nslots = slots.size();
-    std::vector<RunStats> sst(nslots);
-    std::vector<std::vector<TraceRec>> strace(nslots);
+Engine::Engine(std::vector<Ctx *> slots, TaskQueue &tasks, const Params &p, const EngineOpts &o)
+    : slots_(std::move(slots)), tasks_(tasks), p_(p), full_(p), o_(o), t0_(std::chrono::steady_clock::now()) {
+    full_.f = 0;
+    const size_t nslots = slots_.size();
+    o_.crit_lane = o_.crit_lane && nslots >= 2;
+    // The priority slot's waves issue ahead of the other slots' waves on
+    // shared SIMDs: its jobs are the makespan's critical chain.
+    const bool crit_prio = o_.crit_lane && env_u64("MPOA_CRIT_PRIO", 1, 0, 1) != 0;
+    for (size_t si = 0; si < nslots; si++) slots_[si]->prio = crit_prio && si == 0;
+    crit_wait_ms_ = double(env_u64("MPOA_CRIT_WAIT_US", 50000, 0, 10000000)) / 1000.0;
+    // Once the largest job is ready, the priority slot waits this long for
+    // the other critical jobs' host steps before it launches.
+    crit_gather_ms_ = double(env_u64("MPOA_CRIT_GATHER_US", 3000, 0, 10000000)) / 1000.0;
+    max_launch_ = env_u64("MPOA_MAX_LAUNCH", 1024, 1, 1u << 20);
     // Each slot's score arena is bounded so that every slot's buffers (which
     // grow with 50% headroom) fit the device together; a launch takes jobs
     // until its arena reaches the bound (at least one job).
-    uint64_t arena_budget = env_u64("MPOA_SLOT_ARENA_MB", 0, 0, 1ull << 30) << 20;
-    if (!arena_budget && nslots) {
-        const uint64_t free = device_free_bytes(slots[0]->device);
-        arena_budget = std::max<uint64_t>(free / 10 * 6 / nslots / 3 * 2, 256ull << 20);
+    arena_budget_ = env_u64("MPOA_SLOT_ARENA_MB", 0, 0, 1ull << 30) << 20;
+    if (!arena_budget_ && nslots) {
+        const uint64_t free = device_free_bytes(slots_[0]->device);
+        arena_budget_ = std::max<uint64_t>(free / 10 * 6 / nslots / 3 * 2, 256ull << 20);
     }
-    int rc = MPOA_OK;  // under mu
-    size_t rest_idle = crit_lane ? nslots - 1 : nslots;  // under mu
+    admit_budget_ = o_.admit_bytes ? o_.admit_bytes : arena_budget_ * std::max<size_t>(nslots, 1);
+    const size_t nord = o_.crit_lane ? nslots - 1 : nslots;
+    if (const char *e = getenv("MPOA_LONG_ROWS")) o_.long_rows = uint32_t(atoll(e));
+    // At least one slot for each queue; with a single ordinary slot there
+    // is one queue.
+    if (nord < 2) o_.long_rows = 0;
+    if (o_.long_rows) o_.long_slots = uint32_t(std::min<size_t>(std::max<uint32_t>(o_.long_slots, 1), nord - 1));
+    else o_.long_slots = 0;
+    idle_[1] = o_.long_slots;
+    idle_[0] = nord - o_.long_slots;
+    sst_.resize(nslots);
+    strace_.resize(nslots);
+    trace_path_ = getenv("MPOA_TRACE");
+    for (size_t si = 0; si < nslots; si++) threads_.emplace_back([this, si] { slot_main(si); });
+}
+
+Engine::~Engine() {
+    wait_all();
     {
-        std::lock_guard<std::mutex> lk(mu);
-        if (finished == N) stop = true;
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
     }
-    // One thread per slot: take a batch, launch it, wait for it on the slot's
-    // own stream and hand its jobs back to the host workers. A slot's
-    // packing and waiting never hold up another slot.
-    auto slot_main = [&](size_t si) {
-        Ctx &c = *slots[si];
-        const bool crit_slot = crit_lane && si == 0;
-        std::vector<EJob *> batch;
-        std::vector<JobPlan *> plans;
-        LaunchInfo li;
-        for (;;) {
-            batch.clear();
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                for (;;) {
-                    if (stop) return;
-                    if (crit_slot && !ready_crit.empty()) {
-                        // Launch once the critical job with the most cells
-                        // left is ready, taking whichever others are ready
-                        // too: a job that misses a launch has fewer cells
-                        // left than the one it would hold up.
-                        uint64_t top = 0, top_ready = 0;
-                        for (const EJob *J : crit_jobs) top = std::max(top, J->rem);
-                        for (const EJob *J : ready_crit) top_ready = std::max(top_ready, J->rem);
-                        if (top_ready >= top || ready_crit.size() >= active_crit ||
-                            now_ms() - crit_since >= crit_wait_ms)
-                            break;
-                        cv.wait_for(lk, std::chrono::microseconds(200));
-                    } else if (crit_slot || ready.empty()) {
-                        cv.wait(lk);
-                    } else {
-                        break;
-                    }
-                }
-                std::vector<EJob *> &q = crit_slot ? ready_crit : ready;
-                if (!crit_slot) {
-                    // Longest remaining chain first (remaining reads times the
-                    // current step's cells); neighbours in that order have
-                    // similar step sizes, and a launch lasts as long as its
-                    // longest job.
-                    auto rem = [](const EJob *J) { return J->cost * uint64_t(J->nreads - J->step); };
-                    std::sort(q.begin(), q.end(), [&](const EJob *x, const EJob *y) { return rem(x) > rem(y); });
-                }
-                const uint64_t want =
-                    crit_slot ? q.size() : std::min<uint64_t>(max_launch, (q.size() + rest_idle - 1) / rest_idle);
-                const bool i64 = q[0]->plan.use_i64;  // one score width per launch
-                size_t w = 0;
-                uint64_t arena = 0;
-                for (size_t k = 0; k < q.size(); k++) {
-                    const uint64_t a = job_arena_bytes(q[k]->plan, i64);
-                    if (batch.size() < want && q[k]->plan.use_i64 == i64 &&
-                        (batch.empty() || arena + a <= arena_budget)) {
-                        batch.push_back(q[k]);
-                        arena += a;
-                    } else {
-                        q[w++] = q[k];
-                    }
-                }
-                q.resize(w);
-                if (crit_slot && w) crit_since = now_ms();
-                if (!crit_slot) rest_idle--;
-            }
-            plans.resize(batch.size());
-            for (size_t k = 0; k < batch.size(); k++) plans[k] = &batch[k]->plan;
-            const double t0 = now_ms();
-            int r = launch_begin(c, p, plans, batch[0]->plan.use_i64, false, li);
-            const double t1 = now_ms();
-            if (r == MPOA_OK) r = launch_finish(c, li, &sst[si]);
-            if (r != MPOA_OK) {
-                std::lock_guard<std::mutex> lk(mu);
-                rc = r;
-                stop = true;
-                cv.notify_all();
-                return;
-            }
-            if (trace_path) {
-                TraceRec t{int(si), t0, t1, now_ms(), li.fill_ms, li.all_ms, li.h2d_ms, li.d2h_ms, li.grown, li.arena_bytes, c.d_arena.cap, batch.size(), 0, 0, -1,
-                           li.ring_bytes, li.width, li.shape};
-                for (const EJob *J : batch) {
-                    t.max_cost = std::max(t.max_cost, J->cost);
-                    t.sum_cost += J->cost;
-                    if (J == watched) t.watched_step = J->step;
-                }
-                strace[si].push_back(t);
-            }
-            const DevResult *hr = launch_results(c);
-            for (size_t k = 0; k < batch.size(); k++) {
-                EJob &J = *batch[k];
-                J.res = hr[k];
-                if (hr[k].status == kOk) {
-                    const uint32_t *t = launch_tb(c, li, k);
-                    J.tb.assign(t, t + J.plan.m);
-                }
-                submit(J, true);
-            }
-            if (!crit_slot) {
-                std::lock_guard<std::mutex> lk(mu);
-                rest_idle++;
-            }
-        }
-    };
-    std::vector<std::thread> threads;
-    for (size_t si = 0; si < nslots; si++) threads.emplace_back(slot_main, si);
-    for (auto &t : threads) t.join();
+    cv_.notify_all();
+    for (auto &t : threads_) t.join();
     // Let every queued task finish before the state they reference goes away.
     {
-        std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return outstanding == 0; });
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return outstanding_ == 0; });
     }
-    std::vector<TraceRec> trace;
-    for (size_t si = 0; si < nslots; si++) {
-        const RunStats &x = sst[si];
+    if (trace_path_) {
+        std::vector<TraceRec> trace;
+        for (const auto &v : strace_) trace.insert(trace.end(), v.begin(), v.end());
+        std::sort(trace.begin(), trace.end(), [](const TraceRec &x, const TraceRec &y) { return x.t_begin < y.t_begin; });
+        if (FILE *f = fopen(trace_path_, "w")) {
+            fprintf(f, "slot,t_begin_ms,t_launched_ms,t_done_ms,fill_ms,all_ms,h2d_ms,d2h_ms,grown,arena_bytes,arena_cap,njobs,max_cost,sum_cost,"
+                       "watched_step,ring_bytes,width,cols,waves,rows\n");
+            for (const TraceRec &t : trace)
+                fprintf(f, "%d,%.3f,%.3f,%.3f,%.3f,%.3f,%.3f,%.3f,%u,%llu,%llu,%zu,%llu,%llu,%lld,%u,%u,%d,%d,%llu\n", t.slot, t.t_begin,
+                        t.t_launched, t.t_done, t.fill_ms, t.all_ms, t.h2d_ms, t.d2h_ms, t.grown, (unsigned long long)t.arena_bytes,
+                        (unsigned long long)t.arena_cap, t.njobs, (unsigned long long)t.max_cost, (unsigned long long)t.sum_cost,
+                        (long long)t.watched_step, t.ring_bytes, t.width, t.shape.cols, t.shape.waves, (unsigned long long)t.rows);
+            fclose(f);
+        }
+    }
+}
+
+double Engine::now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+uint64_t Engine::submitted() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return jobs_.size();
+}
+
+int Engine::error() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return rc_;
+}
+
+bool Engine::done(uint64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    return id < jobs_.size() && jobs_[id]->out.done;
+}
+
+bool Engine::wait(uint64_t id) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (id >= jobs_.size()) return false;
+    Job *J = jobs_[id].get();
+    cv_.wait(lk, [&] { return J->out.done || rc_ != 0; });
+    return J->out.done;
+}
+
+int Engine::wait_all() {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] { return finished_ == jobs_.size() || rc_ != 0; });
+    return rc_;
+}
+
+JobOut &Engine::out(uint64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    return jobs_[id]->out;
+}
+
+EngineStats Engine::stats() {
+    EngineStats st;
+    std::lock_guard<std::mutex> lk(mu_);
+    for (const RunStats &x : sst_) {
         st.run.cells += x.cells;
         st.run.dp_calls += x.dp_calls;
         st.run.i64_calls += x.i64_calls;
@@ -440,30 +212,437 @@ int run_engine(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, con
         st.run.fill_launches += x.fill_launches;
         st.run.device_ms += x.device_ms;
         st.run.fill_ms += x.fill_ms;
-        trace.insert(trace.end(), strace[si].begin(), strace[si].end());
     }
-    std::sort(trace.begin(), trace.end(), [](const TraceRec &x, const TraceRec &y) { return x.t_begin < y.t_begin; });
-    if (trace_path) {
-        if (FILE *f = fopen(trace_path, "w")) {
-            fprintf(f, "slot,t_begin_ms,t_launched_ms,t_done_ms,fill_ms,all_ms,h2d_ms,d2h_ms,grown,arena_bytes,arena_cap,njobs,max_cost,sum_cost,"
-                       "watched_step,ring_bytes,width,cols,waves\n");
-            for (const TraceRec &t : trace)
-                fprintf(f, "%d,%.3f,%.3f,%.3f,%.3f,%.3f,%.3f,%.3f,%u,%llu,%llu,%zu,%llu,%llu,%lld,%u,%u,%d,%d\n", t.slot, t.t_begin,
-                        t.t_launched, t.t_done, t.fill_ms, t.all_ms, t.h2d_ms, t.d2h_ms, t.grown, (unsigned long long)t.arena_bytes,
-                        (unsigned long long)t.arena_cap,
-                        t.njobs, (unsigned long long)t.max_cost, (unsigned long long)t.sum_cost,
-                        (long long)t.watched_step, t.ring_bytes, t.width, t.shape.cols, t.shape.waves);
-            fclose(f);
+    st.run.retries = retries_.load();
+    st.host_ms = double(task_ns_.load()) / 1e6 / double(std::max(1u, tasks_.size()));
+    return st;
+}
+
+// Device bytes a job is expected to need before its first DP is planned:
+// score rows (M, D, I) for a graph of about twice the first read's length,
+// each row as wide as the band allows (the whole query when f == 0).
+uint64_t Engine::footprint_estimate(const Job &J) const {
+    if (J.in.nreads == 0) return 0;
+    const uint64_t L = J.in.read_off[1] - J.in.read_off[0];
+    uint64_t width = L + 16;
+    if (p_.f != 0) width = std::min<uint64_t>(width, 2 * (p_.b + L / p_.f) + 16);
+    uint64_t rows = 2 * L + 2;
+    if (J.out.graph) rows = std::max<uint64_t>(rows, J.out.graph->size() * 3 / 2);
+    return 12 * rows * width;
+}
+
+void Engine::admit_locked(Job &J) {
+    J.admitted = true;
+    J.out.t_admit = now_ms();
+    J.footprint = footprint_estimate(J);
+    active_bytes_ += J.footprint;
+}
+
+uint64_t Engine::submit(JobIn in) {
+    auto up = std::make_unique<Job>();
+    Job &J = *up;
+    J.in = std::move(in);
+    if (!J.in.own_bases.empty() || !J.in.own_off.empty()) {
+        J.in.bases = J.in.own_bases.data();
+        J.in.read_off = J.in.own_off.data();
+    }
+    J.out.graph = J.in.graph ? std::move(J.in.graph) : std::make_unique<Graph>(p_);
+    J.out.nseq = J.in.first + J.in.nreads;
+    bool start = false;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        J.id = jobs_.size();
+        J.out.t_submit = now_ms();
+        if (!o_.crit_lane) J.in.crit = false;
+        if (J.in.crit) {
+            crit_jobs_.push_back(&J);
+            std::stable_sort(crit_jobs_.begin(), crit_jobs_.end(),
+                             [](const Job *x, const Job *y) { return x->in.work > y->in.work; });
+            active_crit_++;
+            if (!top_job_ || J.in.work > top_job_->in.work) top_job_ = &J;
         }
-        const std::string wpath = std::string(trace_path) + ".watch";
-        if (FILE *f = fopen(wpath.c_str(), "w")) {
-            fprintf(f, "t_result_ms,t_task_begin_ms,t_task_end_ms\n");
-            for (const auto &w : wtrace) fprintf(f, "%.3f,%.3f,%.3f\n", w[0], w[1], w[2]);
-            fclose(f);
+        if (!watched_ || J.in.work > watched_->in.work) watched_ = &J;
+        jobs_.push_back(std::move(up));
+        // Admission by device footprint: first come, first served; a job
+        // that does not fit waits for finished jobs to release theirs.
+        const uint64_t est = footprint_estimate(J);
+        if (waiting_.empty() && (active_bytes_ == 0 || active_bytes_ + est <= admit_budget_)) {
+            admit_locked(J);
+            start = true;
+        } else {
+            waiting_.push_back(&J);
         }
     }
-    st.run.retries += retries.load();
-    st.host_ms += double(task_ns.load()) / 1e6 / double(std::max(1u, tasks.size()));
+    if (start) task(J, false);
+    return J.id;
+}
+
+void Engine::finish(Job &J) {
+    std::vector<Job *> admitted;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        finished_++;
+        J.rem = 0;
+        J.out.t_done = now_ms();
+        J.out.done = true;
+        active_bytes_ -= std::min(active_bytes_, J.footprint);
+        J.footprint = 0;
+        if (J.in.crit) {
+            active_crit_--;
+            if (&J == top_job_) {
+                top_job_ = nullptr;  // crit_jobs_ is in decreasing order of work
+                for (const Job *K : crit_jobs_)
+                    if (!K->out.done) {
+                        top_job_ = K;
+                        break;
+                    }
+            }
+        }
+        while (!waiting_.empty()) {
+            Job *W = waiting_.front();
+            if (active_bytes_ != 0 && active_bytes_ + footprint_estimate(*W) > admit_budget_) break;
+            waiting_.pop_front();
+            admit_locked(*W);
+            admitted.push_back(W);
+        }
+        cv_.notify_all();
+    }
+    for (Job *W : admitted) task(*W, false);
+}
+
+void Engine::requeue(Job &J) {
+    std::lock_guard<std::mutex> lk(mu_);
+    J.rem = J.cost * uint64_t(J.in.nreads - J.step);
+    const uint64_t fp = std::max(job_arena_bytes(J.plan, J.plan.use_i64), footprint_estimate(J));
+    active_bytes_ = active_bytes_ - std::min(active_bytes_, J.footprint) + fp;
+    J.footprint = fp;
+    if (J.in.crit && (!crit_busy_ || &J == top_job_)) {
+        if (ready_crit_.empty()) crit_since_ = now_ms();
+        if (&J == top_job_) top_since_ = now_ms();
+        ready_crit_.push_back(&J);
+    } else {
+        // A critical job that missed the priority slot's launch takes an
+        // ordinary slot for this step rather than holding the priority slot
+        // when the largest job comes back.
+        (o_.long_rows && J.plan.n >= o_.long_rows ? ready_long_ : ready_).push_back(&J);
+    }
+    cv_.notify_all();
+}
+
+void Engine::fail_job(Job &J, int status, const std::string &e) {
+    J.out.status = status;
+    J.out.err = e;
+    finish(J);
+}
+
+// graph.rs:82-178 per read; false when the job ended with an error.
+bool Engine::fuse(Job &J) {
+    Graph &g = *J.out.graph;
+    std::string e = g.add_path(J.in.first + J.step, J.path.data(), J.path.size());
+    if (e.empty()) e = g.toposort(false);
+    if (!e.empty()) {
+        fail_job(J, MPOA_ERR_GRAPH, e);
+        return false;
+    }
+    J.step++;
+    return true;
+}
+
+void Engine::render(Job &J) {
+    Graph &g = *J.out.graph;
+    J.out.out0 = ">consensus sequence\n" + g.consensus_bytes(p_) + "\n";
+    const uint32_t ns = J.out.nseq;
+    std::vector<std::string> names(ns), comments(ns);
+    for (uint32_t k = 0; k < ns; k++) names[k] = "read" + std::to_string(k);
+    std::string e = g.msa(p_, names, comments, J.out.out1);
+    if (!e.empty()) {
+        J.out.status = MPOA_ERR_GRAPH;
+        J.out.err = e;
+    }
+}
+
+// Consumes the job's DP result (have), fuses it and plans the next DP.
+// src/lib.rs:195-202 (one read per step), src/align.rs:81-98 (empty
+// graph), src/align.rs:202-207 (full-matrix retry on any error).
+void Engine::advance(Job &J, bool have) {
+    Graph &g = *J.out.graph;
+    auto query = [&](const uint8_t *&q, uint64_t &qlen) {
+        q = J.in.bases + J.in.read_off[J.step];
+        qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+    };
+    for (;;) {
+        if (have) {
+            have = false;
+            if (J.plan.status == MPOA_OK && J.res.status == kArenaOverflow) {
+                J.plan.full_arena = true;  // same DP, arena for full rows
+                requeue(J);
+                return;
+            }
+            const bool failed = J.plan.status != MPOA_OK || J.res.status != kOk;
+            if (failed && p_.f != 0 && !J.retry) {
+                J.retry = true;
+                retries_++;
+                const uint8_t *q;
+                uint64_t qlen;
+                query(q, qlen);
+                plan_job(full_, GraphAcc{g}, 0, 1, q, qlen, false, p_.b, 0, J.plan);
+                if (J.plan.status != MPOA_OK) {
+                    have = true;
+                    continue;
+                }
+                J.cost = J.plan.static_cells;
+                requeue(J);
+                return;
+            }
+            if (J.plan.status != MPOA_OK) {
+                fail_job(J, J.plan.status, J.plan.err);
+                return;
+            }
+            std::string msg;
+            const int sst = device_status(J.res, msg);
+            if (sst != MPOA_OK) {
+                fail_job(J, sst, msg);
+                return;
+            }
+            if (!J.retry) {
+                J.prev_used = J.res.arena_used;
+                J.prev_n = J.plan.n;
+            }
+            J.retry = false;
+            J.out.cells += J.res.cells;
+            assemble(GraphAcc{g}, J.plan, J.tb.data(), J.path);
+            if (!fuse(J)) return;
+        }
+        if (J.step >= J.in.nreads) {
+            if (o_.render) render(J);
+            finish(J);
+            return;
+        }
+        const uint8_t *q;
+        uint64_t qlen;
+        query(q, qlen);
+        if (g.size() <= 2) {
+            J.path.clear();
+            J.path.push_back(AlignedBase{0, int64_t(p_.encode('N')), -1});
+            for (uint64_t t = 0; t < qlen; t++) J.path.push_back(AlignedBase{-1, int64_t(p_.encode(q[t])), -1});
+            if (!fuse(J)) return;
+            continue;
+        }
+        JobPlan &P = J.plan;
+        plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, P);
+        if (P.status != MPOA_OK) {
+            have = true;
+            continue;
+        }
+        J.cost = P.static_cells;
+        // Adaptive bands: start from what the previous step's DP used,
+        // scaled to the grown graph.
+        if ((P.flags & kAdaptive) && J.prev_n > 0) {
+            uint64_t hint = J.prev_used * P.n / J.prev_n;
+            hint += hint / 8 + 3ull * P.padded * 8;
+            P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
+            P.width_hint = uint32_t(J.prev_used / (3 * J.prev_n));
+            J.cost = uint64_t(P.width_hint) * P.n;
+        }
+        requeue(J);
+        return;
+    }
+}
+
+void Engine::task(Job &J, bool have) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        outstanding_++;
+    }
+    tasks_.push([this, jp = &J, have] {
+        const auto t0 = std::chrono::steady_clock::now();
+        advance(*jp, have);
+        task_ns_ += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+        std::lock_guard<std::mutex> lk(mu_);
+        outstanding_--;
+        cv_.notify_all();
+    }, J.in.crit);
+}
+
+// One thread per slot: take a batch, launch it, wait for it on the slot's own
+// stream and hand its jobs back to the host workers. A slot's packing and
+// waiting never hold up another slot.
+void Engine::slot_main(size_t si) {
+    Ctx &c = *slots_[si];
+    const bool crit_slot = o_.crit_lane && si == 0;
+    // Ordinary slots: the first long_slots take long DPs first, then short
+    // ones; the others take short DPs only.
+    const size_t ord = o_.crit_lane ? si - 1 : si;
+    const int role = !crit_slot && ord < o_.long_slots ? 1 : 0;
+    std::vector<Job *> batch;
+    std::vector<JobPlan *> plans;
+    LaunchInfo li;
+    for (;;) {
+        batch.clear();
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            for (;;) {
+                if (stop_ || rc_ != 0) return;
+                if (crit_slot && !ready_crit_.empty()) {
+                    // Launch once the critical job with the most work is
+                    // ready and the others have had a short while to join; a
+                    // job that misses the launch takes an ordinary slot for
+                    // that step (requeue).
+                    const bool top_ready =
+                        std::find(ready_crit_.begin(), ready_crit_.end(), top_job_) != ready_crit_.end();
+                    const double t = now_ms();
+                    if ((top_ready && (ready_crit_.size() >= active_crit_ || t - top_since_ >= crit_gather_ms_)) ||
+                        t - crit_since_ >= crit_wait_ms_)
+                        break;
+                    cv_.wait_for(lk, std::chrono::microseconds(200));
+                } else if (crit_slot || (ready_.empty() && (role == 0 || ready_long_.empty()))) {
+                    cv_.wait(lk);
+                } else {
+                    break;
+                }
+            }
+            std::vector<Job *> &q = crit_slot ? ready_crit_ : (role == 1 && !ready_long_.empty() ? ready_long_ : ready_);
+            const size_t idle = std::max<size_t>(1, &q == &ready_long_ ? idle_[1] : idle_[0] + (role == 1 ? 1 : 0));
+            if (!crit_slot) {
+                // Longest remaining chain first (remaining reads times the
+                // current step's cells); neighbours in that order have
+                // similar step sizes, and a launch lasts as long as its
+                // longest job.
+                std::sort(q.begin(), q.end(), [](const Job *x, const Job *y) { return x->rem > y->rem; });
+            }
+            const uint64_t want = crit_slot ? q.size() : std::min<uint64_t>(max_launch_, (q.size() + idle - 1) / idle);
+            const bool i64 = q[0]->plan.use_i64;  // one score width per launch
+            size_t w = 0;
+            uint64_t arena = 0;
+            for (size_t k = 0; k < q.size(); k++) {
+                const uint64_t a = job_arena_bytes(q[k]->plan, i64);
+                if (batch.size() < want && q[k]->plan.use_i64 == i64 && (batch.empty() || arena + a <= arena_budget_)) {
+                    batch.push_back(q[k]);
+                    arena += a;
+                } else {
+                    q[w++] = q[k];
+                }
+            }
+            q.resize(w);
+            if (crit_slot) {
+                crit_busy_ = true;
+                // Jobs left behind (another score width) stay queued.
+                if (w) crit_since_ = now_ms();
+            } else {
+                idle_[role]--;
+            }
+        }
+        plans.resize(batch.size());
+        for (size_t k = 0; k < batch.size(); k++) plans[k] = &batch[k]->plan;
+        const double t0 = now_ms();
+        RunStats rs;
+        int r = launch_begin(c, p_, plans, batch[0]->plan.use_i64, false, li);
+        const double t1 = now_ms();
+        if (r == MPOA_OK) r = launch_finish(c, li, &rs);
+        if (r != MPOA_OK) {
+            std::lock_guard<std::mutex> lk(mu_);
+            rc_ = r;
+            stop_ = true;
+            cv_.notify_all();
+            return;
+        }
+        const DevResult *hr = launch_results(c);
+        for (size_t k = 0; k < batch.size(); k++) {
+            Job &J = *batch[k];
+            J.res = hr[k];
+            if (hr[k].status == kOk) {
+                const uint32_t *t = launch_tb(c, li, k);
+                J.tb.assign(t, t + J.plan.m);
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            RunStats &x = sst_[si];
+            x.cells += rs.cells;
+            x.dp_calls += rs.dp_calls;
+            x.i64_calls += rs.i64_calls;
+            x.pred_cells += rs.pred_cells;
+            x.fill_launches += rs.fill_launches;
+            x.device_ms += rs.device_ms;
+            x.fill_ms += rs.fill_ms;
+            if (trace_path_) {
+                TraceRec t{int(si), t0, t1, now_ms(), li.fill_ms, li.all_ms, li.h2d_ms, li.d2h_ms, li.grown, li.arena_bytes,
+                           c.d_arena.cap, batch.size(), 0, 0, -1, li.ring_bytes, li.width, li.shape, li.rows};
+                for (const Job *J : batch) {
+                    t.max_cost = std::max(t.max_cost, J->cost);
+                    t.sum_cost += J->cost;
+                    if (J == watched_) t.watched_step = J->step;
+                }
+                strace_[si].push_back(t);
+            }
+            if (crit_slot) {
+                // The top job's host step first, then the rest by cells left.
+                std::stable_sort(batch.begin(), batch.end(), [&](const Job *x, const Job *y) {
+                    if ((x == top_job_) != (y == top_job_)) return x == top_job_;
+                    return x->rem > y->rem;
+                });
+                crit_busy_ = false;
+            } else {
+                idle_[role]++;
+            }
+        }
+        for (Job *J : batch) task(*J, true);
+    }
+}
+
+int run_engine(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, const BatchInput &in,
+               BatchOutput &out, EngineStats &st) {
+    const uint64_t N = in.njobs;
+    // Largest jobs first: they bound the makespan through their chain of
+    // steps. With two or more slots they get slot 0 to themselves.
+    std::vector<uint64_t> order(N), work(N);
+    for (uint64_t j = 0; j < N; j++) {
+        const uint64_t R = in.job_off[j + 1] - in.job_off[j];
+        const uint64_t r0 = in.job_off[j];
+        const uint64_t L = R ? in.read_off[r0 + 1] - in.read_off[r0] : 0;
+        work[j] = L * R * R;
+        order[j] = j;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return work[a] > work[b]; });
+    std::vector<uint8_t> crit(N, 0);
+    if (slots.size() >= 2 && N > 1) {
+        const double frac = double(env_u64("MPOA_CRIT_PCT", 50, 0, 100)) / 100.0;
+        const uint64_t cap = env_u64("MPOA_CRIT_MAX", 256, 0, N);
+        for (uint64_t k = 0; k < N && k < cap; k++) {
+            if (double(work[order[k]]) < frac * double(work[order[0]])) break;
+            crit[order[k]] = 1;
+        }
+    }
+    EngineOpts o;
+    o.render = in.render;
+    o.crit_lane = true;
+    o.admit_bytes = ~0ull;  // a batch admits every job at once
+    int rc;
+    std::vector<uint64_t> id(N);
+    {
+        Engine E(slots, tasks, p, o);
+        for (uint64_t j : order) {
+            JobIn ji;
+            ji.bases = in.bases;
+            ji.read_off = in.read_off + in.job_off[j];
+            ji.nreads = uint32_t(in.job_off[j + 1] - in.job_off[j]);
+            ji.crit = crit[j] != 0;
+            ji.work = work[j];
+            id[j] = E.submit(std::move(ji));
+        }
+        rc = E.wait_all();
+        if (rc == MPOA_OK) {
+            for (uint64_t j = 0; j < N; j++) {
+                JobOut &jo = E.out(id[j]);
+                (*out.status)[j] = jo.status;
+                (*out.err)[j] = std::move(jo.err);
+                (*out.out0)[j] = std::move(jo.out0);
+                (*out.out1)[j] = std::move(jo.out1);
+                (*out.graphs)[j] = std::move(jo.graph);
+            }
+        }
+        st = E.stats();
+    }
     return rc;
 }
 

@@ -280,6 +280,7 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
     uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
     int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
 
+    if (P.prio) __builtin_amdgcn_s_setprio(3);
     const DevJob J = ldg(P.jobs + blockIdx.x);
     const uint32_t tid = threadIdx.x;
     const int lane = int(tid & 63u);
@@ -847,6 +848,11 @@ __global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
 // in flight at once -- and walks inside it, reloading when the path nears
 // the window's top-left edge. Predecessors older than the window, or rows
 // with more than kTbPred predecessors, read global memory directly.
+// The query codes and the score matrix are staged in LDS once (LT), and the
+// per-column records collect in LDS and go out as one coalesced store per 64
+// columns: on gfx9 a global load's vmcnt wait also waits for the wave's
+// stores, so a step that stored to and loaded from global memory would pay
+// a full memory round trip.
 constexpr uint32_t kTbRows = 64, kTbPred = 8, kTbReach = 16;
 
 struct WRow {
@@ -855,14 +861,17 @@ struct WRow {
     uint32_t pb, np, thresh, base;
 };
 
-template <typename T>
+template <typename T, bool LT>
 __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) {
     constexpr uint32_t kTbCols = sizeof(T) == 4 ? 40 : 20;
     __shared__ WRow wrow[kTbRows];
     __shared__ PredE wpred[kTbRows * kTbPred];
     __shared__ T wm[kTbRows * kTbCols], wd[kTbRows * kTbCols], wi[kTbRows * kTbCols];
+    __shared__ uint32_t tbw[64];  // records of the columns of the current 64-column chunk
+    extern __shared__ __align__(16) char tb_lds[];  // LT: query codes, then the matrix
     const uint32_t jid = blockIdx.x;
     if (jid >= njobs) return;
+    if (P.prio) __builtin_amdgcn_s_setprio(3);
     DevResult *R = P.res + jid;
     if (R->status != kOk) return;
     const DevJob J = P.jobs[jid];
@@ -879,33 +888,97 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     const int64_t o = J.o, e = J.e;
     const int64_t half = int64_t(Lim<T>::neg) / 2;
     const bool gap0 = (J.flags & kGapOpenZero) != 0;
+    uint8_t *sseq = reinterpret_cast<uint8_t *>(tb_lds);
+    int32_t *smat = reinterpret_cast<int32_t *>(tb_lds + ((P.seq_lds + 15u) & ~15u));
+    if constexpr (LT) {
+        for (uint32_t k = uint32_t(lane); k < J.alpha * J.alpha; k += 64u) smat[k] = glb(mat)[k];
+        for (uint32_t k = uint32_t(lane); k < m; k += 64u) sseq[k] = glb(seq)[k];
+    }
+    auto qcode = [&](uint32_t c) -> uint32_t {
+        if constexpr (LT) return sseq[c];
+        else return glb(seq)[c];
+    };
+    auto score_of = [&](uint32_t a, uint32_t b) -> int64_t {
+        if constexpr (LT) return smat[a * J.alpha + b];
+        else return glb(mat)[size_t(a) * J.alpha + b];
+    };
+    // Columns are consumed in decreasing order, each exactly once, so a
+    // chunk is complete when the walk moves below it.
+    uint32_t chunk = (m - 1) / 64u;
+    auto flush = [&]() {
+        sync_lds<1>();
+        const uint32_t c = chunk * 64u + uint32_t(lane);
+        if (c >= 1 && c < m) tb[c] = tbw[lane];
+        sync_lds<1>();
+    };
+    auto put = [&](uint32_t col, uint32_t v) {
+        if (col / 64u != chunk) {
+            flush();
+            chunk = col / 64u;
+        }
+        if (lane == 0) tbw[col & 63u] = v;
+    };
 
     uint32_t i0 = 0, j0 = 0, cw0 = 0;  // window: rows (i0-kTbRows, i0], columns [cw0, j0]
     bool have = false;
     auto load_window = [&](uint32_t ai, uint32_t aj) {
-        __syncthreads();
+        sync_lds<1>();
         i0 = ai;
         j0 = aj;
         cw0 = aj + 1 > kTbCols ? aj + 1 - kTbCols : 0;
         have = true;
-        const uint32_t r = uint32_t(lane);
-        if (r <= i0) {
-            const uint32_t ri = i0 - r;
+        // Geometry and predecessor lists: lane r holds row i0 - r. Lanes past
+        // row 0 load row 0 again and store nothing, so no load sits behind a
+        // branch (a load in flight at a join costs a wait on every path).
+        {
+            const uint32_t r = uint32_t(lane);
+            const uint32_t ri = r <= i0 ? i0 - r : 0u;
             const RowMeta rm = ldg(meta + ri);
             const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
             const RowAux x = ldg(aux + ri);
-            wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
             const uint32_t np = tmin(b.pred_beg - a.pred_beg, kTbPred);
-            for (uint32_t q = 0; q < np; q++) wpred[r * kTbPred + q] = ldg(pr + a.pred_beg + q);
-            const uint32_t lo = tmax(cw0, rm.start), hi = tmin(j0 + 1, rm.start + rm.len);
-            for (uint32_t c = lo; c < hi; c++) {
-                const uint64_t k = rm.off + (c - rm.start);
-                wm[r * kTbCols + (c - cw0)] = glb(arena)[k];
-                wd[r * kTbCols + (c - cw0)] = glb(arena)[k + rm.len];
-                wi[r * kTbCols + (c - cw0)] = glb(arena)[k + 2ull * rm.len];
+            PredE pe[kTbPred];
+#pragma unroll
+            for (uint32_t q = 0; q < kTbPred; q++) pe[q] = ldg(pr + a.pred_beg + (q < np ? q : 0u));
+            if (r <= i0) {
+                wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
+#pragma unroll
+                for (uint32_t q = 0; q < kTbPred; q++)
+                    if (q < np) wpred[r * kTbPred + q] = pe[q];
             }
         }
-        __syncthreads();
+        sync_lds<1>();
+        // M, D and I: lane k holds column cw0 + k; 16 rows per batch, every
+        // load of a batch in flight before the first is used.
+        const uint32_t c = cw0 + uint32_t(lane);
+        const bool col_ok = uint32_t(lane) < kTbCols && c <= j0;
+        const gptr<const T> A = glb(static_cast<const T *>(arena));
+        constexpr uint32_t kB = 16;
+        for (uint32_t r0 = 0; r0 < kTbRows && r0 <= i0; r0 += kB) {
+            T vm[kB], vd[kB], vi[kB];
+            bool ok[kB];
+#pragma unroll
+            for (uint32_t u = 0; u < kB; u++) {
+                const uint32_t r = r0 + u;
+                const WRow g = wrow[r <= i0 ? r : 0u];
+                ok[u] = col_ok && r <= i0 && c >= g.start && c - g.start < g.len;
+                const uint64_t k = ok[u] ? g.off + (c - g.start) : 0ull;
+                const uint64_t l = ok[u] ? uint64_t(g.len) : 0ull;
+                vm[u] = A[k];
+                vd[u] = A[k + l];
+                vi[u] = A[k + 2ull * l];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kB; u++) {
+                if (ok[u]) {
+                    const uint32_t w = (r0 + u) * kTbCols + uint32_t(lane);
+                    wm[w] = vm[u];
+                    wd[w] = vd[u];
+                    wi[w] = vi[u];
+                }
+            }
+        }
+        sync_lds<1>();
     };
     auto row_in_win = [&](uint32_t ri) { return have && ri <= i0 && i0 - ri < kTbRows; };
     // Geometry of a row: window copy, else global.
@@ -952,7 +1025,8 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         }
         const int64_t Dij = val(i, rg, j, 1);
         const uint32_t base = (i == n - 1) ? J.unknown : rg.base;
-        const int64_t score = mat[size_t(base) * J.alpha + seq[j]];
+        const uint32_t qj = qcode(j);
+        const int64_t score = score_of(base, qj);
         // `choose` (src/align.rs:1201-1225): among in-range predecessors whose
         // state plus penalty equals the target, the largest in_weight; ties go
         // to the earliest in `incoming` order. Returns the predecessor row.
@@ -977,10 +1051,10 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         };
         uint32_t w = 0;
         int64_t pref = -1;
-        if ((op & kMatch) && j > 0 && base == seq[j]) pref = choose(kMatch, score, j - 1, w);
+        if ((op & kMatch) && j > 0 && base == qj) pref = choose(kMatch, score, j - 1, w);
         if (pref >= 0 && w < rg.thresh) pref = -1;
         if (pref >= 0) {
-            if (lane == 0) tb[j] = i;
+            put(j, i);
             i = uint32_t(pref);
             j -= 1;
             op = kAll;
@@ -1004,13 +1078,13 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
             const int64_t Iij = val(i, rg, j, 2);
             if (op == kInsert || Mij == Iij) {
                 if (Iij == val(i, rg, j - 1, 0) + o) {
-                    if (lane == 0) tb[j] = kTbInsert;
+                    put(j, kTbInsert);
                     j -= 1;
                     op = gap0 ? kAll : (kMatch | kDelete);
                     continue;
                 }
                 if (Iij == val(i, rg, j - 1, 2) + e) {
-                    if (lane == 0) tb[j] = kTbInsert;
+                    put(j, kTbInsert);
                     j -= 1;
                     op = kInsert;
                     continue;
@@ -1020,7 +1094,7 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         if ((op & kMatch) && j > 0) {
             const int64_t k = choose(kMatch, score, j - 1, w);
             if (k >= 0) {
-                if (lane == 0) tb[j] = i;
+                put(j, i);
                 i = uint32_t(k);
                 j -= 1;
                 op = kAll;
@@ -1030,7 +1104,9 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
         status = kTbNoPredecessor;
         break;
     }
-    if (status != kOk && lane == 0) {
+    if (status == kOk) {
+        flush();
+    } else if (lane == 0) {
         R->status = status;
         R->err_i = i;
         R->err_j = j;
@@ -1091,13 +1167,28 @@ int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStr
     return i64 ? launch_fill_s<int64_t>(P, njobs, f, s) : launch_fill_s<int32_t>(P, njobs, f, s);
 }
 
+// Static LDS of the traceback kernel (window rows, predecessor lists, M/D/I
+// window, column records); the query codes and matrix come on top when LT.
+template <typename T> constexpr uint32_t tb_static_lds() {
+    constexpr uint32_t cols = sizeof(T) == 4 ? 40 : 20;
+    return uint32_t(sizeof(WRow) * kTbRows + sizeof(PredE) * kTbRows * kTbPred +
+                    3 * kTbRows * cols * sizeof(T) + 64 * sizeof(uint32_t));
+}
+
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s) {
     if (njobs == 0) return;
     dim3 grid(njobs);
-    if (i64)
-        hipLaunchKernelGGL((poa_traceback<int64_t>), grid, dim3(64), 0, s, P, njobs);
-    else
-        hipLaunchKernelGGL((poa_traceback<int32_t>), grid, dim3(64), 0, s, P, njobs);
+    const uint32_t dyn = fill_dynamic_lds(0, P.seq_lds, P.mat_lds);
+    const uint32_t stat = i64 ? tb_static_lds<int64_t>() : tb_static_lds<int32_t>();
+    const bool lt = P.seq_lds != 0 && P.mat_lds != 0 && stat + dyn <= 65536u;
+    const uint32_t lds = lt ? dyn : 0;
+    if (i64) {
+        if (lt) hipLaunchKernelGGL((poa_traceback<int64_t, true>), grid, dim3(64), lds, s, P, njobs);
+        else hipLaunchKernelGGL((poa_traceback<int64_t, false>), grid, dim3(64), 0, s, P, njobs);
+    } else {
+        if (lt) hipLaunchKernelGGL((poa_traceback<int32_t, true>), grid, dim3(64), lds, s, P, njobs);
+        else hipLaunchKernelGGL((poa_traceback<int32_t, false>), grid, dim3(64), 0, s, P, njobs);
+    }
 }
 
 }  // namespace mpoa

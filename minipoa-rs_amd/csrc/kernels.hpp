@@ -30,6 +30,7 @@ struct DevPools {
     uint32_t seq_lds;    // bytes of LDS for the query codes (0 = global)
     uint32_t mat_lds;    // bytes of LDS for the score matrix (0 = global)
     uint32_t ring_bytes; // LDS ring of recent rows (multiple of 64 bytes)
+    uint32_t prio;       // nonzero: the launch's waves issue at raised priority
 };
 
 // Dynamic LDS bytes a fill launch needs besides its static arrays.

@@ -334,6 +334,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     HIP_TRY(grow(ctx.d_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
     HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 1) * tsz));
     li.arena_bytes = std::max<uint64_t>(arena, 1) * tsz;
+    li.rows = meta;
     HIP_TRY(grow(ctx.h_res, N * sizeof(DevResult)));
     HIP_TRY(grow(ctx.h_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
 
@@ -353,7 +354,9 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     P.pool_next = reinterpret_cast<unsigned long long *>(d + b_ctr);
     P.pool_end = pool_beg + pool_len;
     P.chunk = uint32_t(chunk);
-    P.seq_lds = maxpad <= 8192 ? maxpad : 0;
+    // Query codes up to 32 KB stage in LDS (the ring shrinks to fit below);
+    // longer queries read the global-table kernel.
+    P.seq_lds = maxpad <= 32768 ? maxpad : 0;
     P.mat_lds = alpha * alpha <= kMatLdsEntries ? uint32_t(alpha * alpha * 4) : 0;
     if (P.seq_lds == 0 || P.mat_lds == 0) P.seq_lds = P.mat_lds = 0;  // both tables in global memory
 
@@ -388,6 +391,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     HIP_TRY(hipEventRecord(ctx.ev[4], ctx.stream));
     HIP_TRY(hipMemcpyAsync(d, h, b_end, hipMemcpyHostToDevice, ctx.stream));
     HIP_TRY(hipEventRecord(ctx.ev[0], ctx.stream));
+    P.prio = ctx.prio ? 1u : 0u;
     li.ring_bytes = P.ring_bytes;
     li.shape = shape;
     li.width = wexp;

@@ -74,6 +74,9 @@ struct Ctx {
     PinBuf h_in, h_res, h_tb;
     ThreadPool *pool = nullptr;
     FillShape force_shape{0, 0};  // cols 0: choose from the expected row width
+    // Launches of this context run their waves at raised issue priority
+    // (s_setprio): the engine's priority slot, whose jobs bound the makespan.
+    bool prio = false;
     ~Ctx();
 };
 
@@ -99,6 +102,7 @@ struct LaunchInfo {
     uint32_t ring_bytes = 0;      // the fill's LDS ring
     FillShape shape{0, 0};        // the fill's launch shape
     uint32_t width = 0;           // expected row width the shape was chosen for
+    uint64_t rows = 0;            // DP rows over the launch's jobs
     uint32_t grown = 0;           // bit mask of the buffers this launch reallocated
     uint64_t arena_bytes = 0;     // score arena the launch asked for
     float fill_ms = 0, all_ms = 0;  // device time of the fill, and of the fill and traceback

@@ -23,7 +23,7 @@ STATUS = {
     8: "MPOA_ERR_TRACEBACK_NO_ADVANCE", 9: "MPOA_ERR_TRACEBACK_NO_PREDECESSOR",
     10: "MPOA_ERR_ALLOC", 11: "MPOA_ERR_INVALID_PARAMS", 12: "MPOA_ERR_GRAPH",
     13: "MPOA_ERR_DEVICE", 14: "MPOA_ERR_NO_DEVICE", 15: "MPOA_ERR_UNSUPPORTED",
-    16: "MPOA_ERR_BUFFER_TOO_SMALL", 17: "MPOA_ERR_INVALID_ARGUMENT",
+    16: "MPOA_ERR_BUFFER_TOO_SMALL", 17: "MPOA_ERR_INVALID_ARGUMENT", 18: "MPOA_ERR_PENDING",
 }
 BUFFER_TOO_SMALL = 16
 
@@ -80,6 +80,15 @@ class _Stats(C.Structure):
     ]
 
 
+class _ServerOpts(C.Structure):
+    _fields_ = [("admit_bytes", C.c_uint64), ("slots", C.c_uint32), ("render", C.c_int32)]
+
+
+class _JobTimes(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+                ("cells", C.c_uint64)]
+
+
 # Exported symbols of include/minipoa_hip.h with their ctypes signatures.
 _P = C.POINTER(_Params)
 _V = C.POINTER(_View)
@@ -121,6 +130,17 @@ SIGNATURES = {
     "mpoa_batch_stats_get": (C.c_int, [_vp, C.POINTER(_Stats)]),
     "mpoa_batch_job_status": (C.c_int, [_vp, _u64]),
     "mpoa_batch_job_output": (C.c_int, [_vp, _u64, C.c_int, C.c_char_p, _u64, U64P]),
+    "mpoa_server_create": (C.c_int, [_vp, _P, C.POINTER(_ServerOpts), C.POINTER(_vp)]),
+    "mpoa_server_submit": (C.c_int, [_vp, _vp, U64P, _u64, _vp, _u64, U64P]),
+    "mpoa_server_done": (C.c_int, [_vp, _u64]),
+    "mpoa_server_wait": (C.c_int, [_vp, _u64]),
+    "mpoa_server_job_status": (C.c_int, [_vp, _u64]),
+    "mpoa_server_job_output": (C.c_int, [_vp, _u64, C.c_int, C.c_char_p, _u64, U64P]),
+    "mpoa_server_job_times": (C.c_int, [_vp, _u64, C.POINTER(_JobTimes)]),
+    "mpoa_server_job_graph": (C.c_int, [_vp, _u64, C.POINTER(_vp)]),
+    "mpoa_server_stats_get": (C.c_int, [_vp, C.POINTER(_Stats)]),
+    "mpoa_server_now_ms": (C.c_double, [_vp]),
+    "mpoa_server_free": (None, [_vp]),
 }
 
 _lib = None
@@ -470,6 +490,98 @@ class BatchResult:
             return buf.raw[:n.value].decode()
 
 
+def _text_out(fn, *args):
+    cap = 1 << 16
+    while True:
+        buf = C.create_string_buffer(cap)
+        n = C.c_uint64(0)
+        rc = fn(*args, buf, cap, C.byref(n))
+        if rc == BUFFER_TOO_SMALL:
+            cap = n.value + 1
+            continue
+        _check(rc)
+        return buf.raw[:n.value].decode()
+
+
+class Server:
+    """Serving: POA jobs admitted as they arrive (include/minipoa_hip.h).
+
+    submit() copies a job's reads and returns a ticket at once; the job runs
+    in the continuous engine alongside whatever else is admitted. A job may
+    extend a graph that already holds `existing` sequences (the -i mode of
+    src/lib.rs:212-226, which continues with `extend_graph`)."""
+
+    def __init__(self, ctx, params, admit_bytes=0, slots=0, render=True):
+        self._cp = CParams(params)
+        self.params = params
+        o = _ServerOpts(admit_bytes, slots, 1 if render else 0)
+        h = C.c_void_p()
+        _check(lib().mpoa_server_create(ctx._h, self._cp.ref, C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mpoa_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, reads, graph=None, existing=0):
+        """reads: list of bytes. graph: a Graph holding `existing` sequences to extend."""
+        import numpy as np
+
+        lens = [len(r) for r in reads]
+        off = np.zeros(len(reads) + 1, dtype=np.uint64)
+        off[1:] = np.cumsum(lens, dtype=np.uint64)
+        buf = np.frombuffer(b"".join(bytes(r) for r in reads) or b"\0", dtype=np.uint8)
+        t = C.c_uint64(0)
+        _check(lib().mpoa_server_submit(self._h, buf.ctypes.data_as(C.c_void_p),
+                                        off.ctypes.data_as(U64P), len(reads),
+                                        graph._h if graph is not None else None, existing,
+                                        C.byref(t)))
+        return t.value
+
+    def done(self, ticket):
+        rc = lib().mpoa_server_done(self._h, ticket)
+        if rc < 0 or rc > 1:
+            _check(rc)
+        return rc == 1
+
+    def wait(self, ticket):
+        _check(lib().mpoa_server_wait(self._h, ticket))
+
+    def status(self, ticket):
+        return lib().mpoa_server_job_status(self._h, ticket)
+
+    def output(self, ticket, mode):
+        return _text_out(lib().mpoa_server_job_output, self._h, ticket, mode)
+
+    def times(self, ticket):
+        t = _JobTimes()
+        _check(lib().mpoa_server_job_times(self._h, ticket, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _JobTimes._fields_}
+
+    def graph(self, ticket):
+        h = C.c_void_p()
+        _check(lib().mpoa_server_job_graph(self._h, ticket, C.byref(h)))
+        return Graph(self.params, _handle=h)
+
+    def stats(self):
+        s = _Stats()
+        _check(lib().mpoa_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _Stats._fields_}
+
+    def now_ms(self):
+        return lib().mpoa_server_now_ms(self._h)
+
+
 def batch_run(ctx, params, job_off, read_off, bases):
     """Batched progressive POA over numpy job/read offsets and a base buffer."""
     import numpy as np
@@ -489,5 +601,5 @@ __all__ = [
     "AlignedBase", "Params", "Sequence", "records", "default_matrix", "HipPoaError", "Context",
     "Graph", "GraphView", "view_from_graph", "poa", "poa_inner", "poa_kernel", "align",
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
-    "validate", "result_or_error", "lib", "last_error", "SIGNATURES",
+    "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server",
 ]
