@@ -260,8 +260,11 @@ template <int W> __device__ __forceinline__ void sync_global() {
 // across waves (per stripe) and to reduce the row's first argmax.
 // LT: the score matrix and the query codes are staged in LDS (every launch
 // whose alphabet and queries fit); otherwise they are read from global memory.
+// amdgpu_waves_per_eu(4): at most 128 VGPRs, so four waves share each SIMD
+// (three at the 147 VGPRs the compiler picks by itself). Rows are latency
+// chains, so throughput comes from the number of resident jobs.
 template <typename T, int C, int W, bool LT>
-__global__ __launch_bounds__(64 * W) void poa_fill(DevPools P) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void poa_fill(DevPools P) {
     static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
     constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
     extern __shared__ __align__(16) char dyn_lds[];
@@ -1154,11 +1157,14 @@ uint32_t ring_bytes_for_width(uint32_t width, bool i64) {
 }
 
 // More waves shorten a row's serial path (each wave pays the row's fixed
-// work in parallel and owns fewer columns). Measured on the 4,096-job DNA
-// workload (mean adaptive width ~544, rows up to ~1,500): 8x4 beat 8x2,
-// 4x4, 4x2, 8x1 and 16x1 by 8-30% in fill time. Narrow rows keep one wave.
+// work in parallel and owns fewer columns); fewer registers per wave let
+// more jobs stay resident. Measured on the 4,096-job DNA workload (mean
+// adaptive width ~544, rows up to ~1,500), wall time per run with four
+// waves per SIMD: 4x4 1451 ms, 8x4 1565, 8x2 1568, 4x2 1649. Narrow rows
+// keep one wave; rows wider than one 4x4 stripe (1,024 columns) take 8x4.
 FillShape fill_shape_for_width(uint32_t width) {
     if (width <= 384u) return FillShape{8, 1};
+    if (width <= 1024u) return FillShape{4, 4};
     return FillShape{8, 4};
 }
 
