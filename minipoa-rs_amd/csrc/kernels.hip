@@ -866,7 +866,7 @@ struct WRow {
 
 template <typename T, bool LT>
 __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) {
-    constexpr uint32_t kTbCols = sizeof(T) == 4 ? 40 : 20;
+    constexpr uint32_t kTbCols = sizeof(T) == 4 ? 32 : 16;
     __shared__ WRow wrow[kTbRows];
     __shared__ PredE wpred[kTbRows * kTbPred];
     __shared__ T wm[kTbRows * kTbCols], wd[kTbRows * kTbCols], wi[kTbRows * kTbCols];
@@ -1176,7 +1176,7 @@ int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStr
 // Static LDS of the traceback kernel (window rows, predecessor lists, M/D/I
 // window, column records); the query codes and matrix come on top when LT.
 template <typename T> constexpr uint32_t tb_static_lds() {
-    constexpr uint32_t cols = sizeof(T) == 4 ? 40 : 20;
+    constexpr uint32_t cols = sizeof(T) == 4 ? 32 : 16;
     return uint32_t(sizeof(WRow) * kTbRows + sizeof(PredE) * kTbRows * kTbPred +
                     3 * kTbRows * cols * sizeof(T) + 64 * sizeof(uint32_t));
 }
