@@ -260,11 +260,17 @@ template <int W> __device__ __forceinline__ void sync_global() {
 // across waves (per stripe) and to reduce the row's first argmax.
 // LT: the score matrix and the query codes are staged in LDS (every launch
 // whose alphabet and queries fit); otherwise they are read from global memory.
-// amdgpu_waves_per_eu(4): at most 128 VGPRs, so four waves share each SIMD
-// (three at the 147 VGPRs the compiler picks by itself). Rows are latency
-// chains, so throughput comes from the number of resident jobs.
+// amdgpu_waves_per_eu(4) for the i32 shapes that fit 128 VGPRs without
+// spilling (8x1, 4x2, 4x4): four waves share each SIMD (three at the 147
+// VGPRs the compiler picks for 4x4 by itself). Rows are latency chains, so
+// throughput comes from the number of resident jobs. The other kernels
+// would spill in the row loop under that limit, and on gfx9 a scratch
+// reload waits for the wave's outstanding stores; they keep the default.
+template <typename T, int C, int W, bool LT> constexpr int fill_waves_per_eu() {
+    return sizeof(T) == 4 && LT && (C == 4 || (C == 8 && W == 1)) ? 4 : 1;
+}
 template <typename T, int C, int W, bool LT>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void poa_fill(DevPools P) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT>()))) void poa_fill(DevPools P) {
     static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
     constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
     extern __shared__ __align__(16) char dyn_lds[];
