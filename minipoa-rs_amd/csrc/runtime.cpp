@@ -224,6 +224,52 @@ int device_status(const DevResult &r, std::string &msg) {
 
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+namespace {
+
+// Launch-shape table measured on the device by minipoa_hip/tune.py and named
+// by MPOA_TUNING: one line per row-width bucket, "max_width cols waves
+// ring_kb", in increasing max_width. The first bucket that covers a launch's
+// expected row width decides its i32 fill shape and LDS ring; wider rows, i64
+// launches and the global-table kernel keep the built-in choice. The file is
+// read again when the variable names another file.
+struct TuneEntry {
+    uint32_t max_width;
+    FillShape shape;
+    uint32_t ring_bytes;
+};
+std::mutex g_tune_mu;
+std::string g_tune_path;
+std::vector<TuneEntry> g_tune;
+
+bool tuned_for(uint32_t width, TuneEntry &out) {
+    const char *e = getenv("MPOA_TUNING");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = e ? e : "";
+    if (path != g_tune_path) {
+        g_tune_path = path;
+        g_tune.clear();
+        if (FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r")) {
+            unsigned w, c, v, r;
+            char line[256];
+            while (fgets(line, sizeof line, f)) {
+                if (sscanf(line, "%u %u %u %u", &w, &c, &v, &r) != 4) continue;
+                const FillShape sh{int(c), int(v)};
+                if (!fill_shape_ok(sh) || r < 1 || r > 48) continue;
+                g_tune.push_back(TuneEntry{w, sh, r * 1024u});
+            }
+            fclose(f);
+        }
+    }
+    for (const TuneEntry &t : g_tune)
+        if (width <= t.max_width) {
+            out = t;
+            return true;
+        }
+    return false;
+}
+
+}  // namespace
+
 int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
                  bool full_arena, LaunchInfo &li) {
     const size_t N = plans.size();
@@ -375,12 +421,17 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         if (den) wexp = std::max<uint32_t>(wexp, uint32_t(num / den));
     }
     FillShape shape = ctx.force_shape.cols ? ctx.force_shape : fill_shape_for_width(wexp);
+    P.ring_bytes = ring_bytes_for_width(wexp, i64);
+    TuneEntry tuned{};
+    if (!ctx.force_shape.cols && !i64 && P.seq_lds != 0 && tuned_for(wexp, tuned)) {
+        shape = tuned.shape;
+        P.ring_bytes = tuned.ring_bytes;
+    }
     if (const char *e = getenv("MPOA_FILL_SHAPE")) {
         FillShape f{0, 0};
         if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
     }
     if (P.seq_lds == 0) shape = FillShape{8, 1};  // the global-table kernel has one shape
-    P.ring_bytes = ring_bytes_for_width(wexp, i64);
     if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
     {
         // Stay within 64 KB of LDS per workgroup with the fill's static arrays.
