@@ -44,6 +44,23 @@ def test_context_creation_fails_cleanly_without_device():
         ctx.close()
 
 
+def test_server_entry_points_reject_null_arguments():
+    """The serving entry points fail cleanly, with no device and no server."""
+    import ctypes as C
+
+    L = mh.lib()
+    h = C.c_void_p()
+    assert L.mpoa_server_create(None, None, None, C.byref(h)) == 17
+    t = C.c_uint64(0)
+    assert L.mpoa_server_submit(None, None, None, 0, None, 0, C.byref(t)) == 17
+    assert L.mpoa_server_wait(None, 0) == 17
+    assert L.mpoa_server_done(None, 0) == 17
+    assert L.mpoa_server_job_status(None, 0) == 17
+    assert L.mpoa_server_job_times(None, 0, None) == 17
+    L.mpoa_server_free(None)
+    assert L.mpoa_server_now_ms(None) == 0.0
+
+
 def test_params_validation_messages():
     """src/lib.rs:149-173"""
     mh.validate(Params())

@@ -132,6 +132,25 @@ def test_long_queries_stage_codes_in_lds(ctx, oracle):
         assert outputs(srv, t) == oracle_outputs(oracle, params, reads)
 
 
+def test_degenerate_jobs_match_oracle(ctx, oracle):
+    """A job with no reads (empty graph), one read (no DP: the empty-graph
+    shortcut of src/align.rs:82-95) and an empty read inside a job
+    (tests/io_cli.rs:612-630) give the oracle's bytes; they run in the same
+    stream as ordinary jobs."""
+    params = Params()
+    cases = [[], [b"ACGTACGTTGCA"], [b"ACGT", b""], [b"", b"ACGT", b"ACGA"],
+             [b"ACGTTGCAACGT", b"ACGTTGCACGT", b"ACGTAGCAACGT"]]
+    with mh.Server(ctx, params) as srv:
+        tickets = [srv.submit(r) for r in cases]
+        for t, reads in zip(tickets, cases):
+            srv.wait(t)
+            assert srv.status(t) == 0, mh.last_error()
+            if not reads:
+                assert srv.output(t, 0) == ">consensus sequence\n\n"
+                continue
+            assert outputs(srv, t) == oracle_outputs(oracle, params, reads), reads
+
+
 def fnv1a(data, h=14695981039346656037):
     for x in data:
         h ^= x
