@@ -279,11 +279,18 @@ def leg_c1(ctx, oracle):
 
 
 def leg_c4(ctx, oracle, cores):
-    """Protein families, full DP, custom matrix, batched."""
+    """Protein families, full DP, custom matrix (-m file), batched."""
     import minipoa_hip as mh
-    from minipoa_hip import synth
+    from minipoa_hip import Params, synth
 
     p, jo, ro, b = synth.protein_families()
+    # The drawn matrix goes through the -m file format and io::load_matrix.
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        f = Path(d) / "protein.mtx"
+        f.write_text(synth.matrix_text(p.alphabet, p.matrix))
+        p = mh.load_matrix(Params(f=0), f)
     n = len(jo) - 1
     r = mh.batch_run(ctx, p, jo, ro, b)  # warm
     del r

@@ -168,6 +168,16 @@ int mpoa_graph_gfa(const mpoa_graph *g, const mpoa_params *params, const char *c
 
 /* src/lib.rs:149-173 */
 int mpoa_params_validate(const mpoa_params *params);
+/* `io::load_matrix` for the -m option (src/lib.rs:209-211; src/io.rs is not
+ * in the reference's tree, so the format follows tests/io_cli.rs:326-372 and
+ * test-data/HOXD70.mtx): '#' lines and blank lines are skipped; the first
+ * other line lists one single-character label per column; then one row per
+ * label, in any order: the label and one integer per column. Writes the
+ * alphabet in header order and the alphabet_len^2 matrix row-major in that
+ * order (cap: capacity of `alphabet`; `matrix` holds cap * cap). Malformed
+ * text returns MPOA_ERR_INVALID_PARAMS and writes nothing. */
+int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t *matrix, uint32_t cap,
+                     uint32_t *alphabet_len);
 /* src/lib.rs:177-204 `align_sequences` (input order) on the device. */
 int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
                          const uint64_t *lens, uint64_t nseq, mpoa_graph **out);

@@ -1,6 +1,9 @@
 // C ABI of the gfx950 POA backend (include/minipoa_hip.h): the drop-in DP
 // entry points, the library-owned graph and the batched progressive engine.
 #include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <climits>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -233,6 +236,77 @@ uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) {
     add(*ctx->c);
     for (const auto &l : ctx->lanes) add(*l);
     return total;
+}
+
+int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t *matrix, uint32_t cap,
+                     uint32_t *alphabet_len) {
+    if (!text || !alphabet_len || (cap && (!alphabet || !matrix)))
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *alphabet_len = 0;
+    // Lines: '#' comments and blank lines skipped; the first other line is
+    // the header (one single-character label per column); then one row per
+    // label, in any order: the label and one integer per column.
+    std::vector<std::vector<std::string>> lines;
+    {
+        size_t at = 0;
+        while (at < len) {
+            size_t end = at;
+            while (end < len && text[end] != '\n') end++;
+            std::string line(text + at, end - at);
+            at = end + 1;
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            size_t k = 0;
+            while (k < line.size() && isspace((unsigned char)line[k])) k++;
+            if (k == line.size() || line[k] == '#') continue;
+            std::vector<std::string> tok;
+            size_t i = 0;
+            while (i < line.size()) {
+                while (i < line.size() && isspace((unsigned char)line[i])) i++;
+                size_t j = i;
+                while (j < line.size() && !isspace((unsigned char)line[j])) j++;
+                if (j > i) tok.emplace_back(line.substr(i, j - i));
+                i = j;
+            }
+            lines.push_back(std::move(tok));
+        }
+    }
+    if (lines.empty()) return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix has no header");
+    const std::vector<std::string> &head = lines[0];
+    const size_t n = head.size();
+    int pos[256];
+    for (int &x : pos) x = -1;
+    for (size_t c = 0; c < n; c++) {
+        if (head[c].size() != 1) return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix labels must be single characters");
+        const unsigned char ch = (unsigned char)head[c][0];
+        if (pos[ch] >= 0) return fail(MPOA_ERR_INVALID_PARAMS, "duplicate scoring matrix label " + head[c]);
+        pos[ch] = int(c);
+    }
+    if (lines.size() - 1 != n) return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix needs one row per label");
+    std::vector<int32_t> m(n * n, 0);
+    std::vector<uint8_t> seen(n, 0);
+    for (size_t r = 1; r < lines.size(); r++) {
+        const std::vector<std::string> &row = lines[r];
+        if (row[0].size() != 1 || pos[(unsigned char)row[0][0]] < 0)
+            return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix row label " + row[0] + " is not in the header");
+        const size_t a = size_t(pos[(unsigned char)row[0][0]]);
+        if (seen[a]) return fail(MPOA_ERR_INVALID_PARAMS, "duplicate scoring matrix row " + row[0]);
+        seen[a] = 1;
+        if (row.size() != n + 1) return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix row " + row[0] + " has the wrong length");
+        for (size_t c = 0; c < n; c++) {
+            const std::string &v = row[c + 1];
+            char *endp = nullptr;
+            errno = 0;
+            const long x = strtol(v.c_str(), &endp, 10);
+            if (v.empty() || *endp != '\0' || errno != 0 || x < INT32_MIN || x > INT32_MAX)
+                return fail(MPOA_ERR_INVALID_PARAMS, "scoring matrix entry " + v + " is not an integer");
+            m[a * n + c] = int32_t(x);
+        }
+    }
+    *alphabet_len = uint32_t(n);
+    if (n > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "alphabet buffer is too small");
+    for (size_t c = 0; c < n; c++) alphabet[c] = (uint8_t)head[c][0];
+    memcpy(matrix, m.data(), m.size() * sizeof(int32_t));
+    return MPOA_OK;
 }
 
 int mpoa_params_validate(const mpoa_params *params) {

@@ -122,6 +122,8 @@ SIGNATURES = {
                                  C.c_char_p, _u64, U64P]),
     "mpoa_graph_gfa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), _u64, C.c_char_p, _u64, U64P]),
     "mpoa_params_validate": (C.c_int, [_P]),
+    "mpoa_load_matrix": (C.c_int, [C.c_char_p, _u64, C.POINTER(C.c_uint8), C.POINTER(C.c_int32),
+                                   C.c_uint32, C.POINTER(C.c_uint32)]),
     "mpoa_align_sequences": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), U64P, _u64,
                                        C.POINTER(_vp)]),
     "mpoa_extend_graph": (C.c_int, [_vp, _P, _vp, C.POINTER(C.c_char_p), U64P, _u64, _u64]),
@@ -436,6 +438,22 @@ def result_or_error(fn, *args, **kw):
         return ("err", str(e))
 
 
+def load_matrix(params, path):
+    """`io::load_matrix` (the -m option, src/lib.rs:209-211): reads the
+    matrix file at `path` into params.alphabet and params.matrix (header
+    order). Raises HipPoaError on malformed text, leaving params unchanged."""
+    text = Path(path).read_bytes()
+    cap = 256
+    alpha = (C.c_uint8 * cap)()
+    mat = (C.c_int32 * (cap * cap))()
+    n = C.c_uint32(0)
+    _check(lib().mpoa_load_matrix(text, len(text), alpha, mat, cap, C.byref(n)))
+    k = n.value
+    params.alphabet = bytes(alpha[:k])
+    params.matrix = list(mat[:k * k])
+    return params
+
+
 def validate(params):
     _check(lib().mpoa_params_validate(CParams(params).ref))
 
@@ -601,5 +619,5 @@ __all__ = [
     "AlignedBase", "Params", "Sequence", "records", "default_matrix", "HipPoaError", "Context",
     "Graph", "GraphView", "view_from_graph", "poa", "poa_inner", "poa_kernel", "align",
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
-    "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server",
+    "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server", "load_matrix",
 ]

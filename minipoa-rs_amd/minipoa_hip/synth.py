@@ -109,6 +109,16 @@ def _pack(job_reads_list):
 PROTEIN = b"ARNDCQEGHILKMFPSTWYV"
 
 
+def matrix_text(alphabet, matrix):
+    """A scoring matrix in the -m file format (header of labels, then one
+    labelled row per letter)."""
+    n = len(alphabet)
+    lines = ["# drawn by minipoa_hip.synth", " ".join(chr(c) for c in alphabet)]
+    for i in range(n):
+        lines.append(chr(alphabet[i]) + " " + " ".join(str(matrix[i * n + j]) for j in range(n)))
+    return "\n".join(lines) + "\n"
+
+
 def protein_matrix(rng, alphabet=PROTEIN, dlo=4, dhi=11, olo=-4, ohi=3):
     """Symmetric integer matrix: diagonal U[dlo,dhi], off-diagonal U[olo,ohi]."""
     n = len(alphabet)

@@ -53,3 +53,13 @@ def test_tune_calibration_rows_match_their_bucket():
         band = p.b + m // p.f
         assert abs(2 * band + 16 - width) <= width // 4 + 32, (width, band)
         assert p.adaptive and len(jo) >= 17
+
+
+def test_matrix_text_round_trips_through_load_matrix(tmp_path):
+    import minipoa_hip as mh
+
+    p, *_ = synth.protein_families(nfam=2, members=2)
+    f = tmp_path / "p.mtx"
+    f.write_text(synth.matrix_text(p.alphabet, p.matrix))
+    q = mh.load_matrix(Params(f=0), f)
+    assert bytes(q.alphabet) == bytes(p.alphabet) and q.matrix == p.matrix
