@@ -41,7 +41,8 @@ typedef enum {
     MPOA_ERR_UNSUPPORTED = 15,            /* a reference feature outside this backend */
     MPOA_ERR_BUFFER_TOO_SMALL = 16,       /* *out_len holds the size needed */
     MPOA_ERR_INVALID_ARGUMENT = 17,
-    MPOA_ERR_PENDING = 18                 /* a served job has not finished yet */
+    MPOA_ERR_PENDING = 18,                /* a served job has not finished yet */
+    MPOA_ERR_SEED_SPAN = 19               /* align.rs:116-130 invalid minimizer span */
 } mpoa_status;
 
 /* src/lib.rs:47-71 Params. `alphabet` and `matrix` are borrowed for the
@@ -136,11 +137,46 @@ int mpoa_poa_kernel(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_v
 /* src/align.rs:493-529 */
 int mpoa_i32_scores_safe(const mpoa_params *params, const mpoa_graph_view *graph,
                          uint64_t beg_id, uint64_t end_id, uint64_t query_len);
-/* src/align.rs:81-98 `align` (empty-graph shortcut and unseeded DP). Seeded
- * alignment (params.seeding) returns MPOA_ERR_UNSUPPORTED. */
+/* src/align.rs:81-188 `align`: empty-graph shortcut, unseeded DP, or with
+ * params.seeding the windows between chained minimizer anchors of the query
+ * and the graph's consensus, all windows of the query in one launch (each
+ * with poa's retry), concatenated with the anchors' own nodes. */
 int mpoa_align(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                const uint8_t *query, uint64_t qlen, mpoa_aligned_base *out, uint64_t out_cap,
                uint64_t *out_len);
+
+/* ---- minimizer seeding (src/minimizer.rs), host side ----
+ * Each writes up to cap entries and sets *out_len to the full count
+ * (MPOA_ERR_BUFFER_TOO_SMALL when it exceeds cap). k-mers are ranked by the
+ * upper 16 bits of a 32-bit forward ntHash, leftmost minimum per window of w
+ * k-mers: the ordering of the simd-minimizers crate the reference uses. */
+
+/* src/minimizer.rs:20-26 Anchor (adaptive_before 0 or 1). */
+typedef struct {
+    uint64_t target_end, query_end, span;
+    int64_t adaptive_before;
+} mpoa_anchor;
+
+/* src/minimizer.rs:32-76 `sketch`: value = exact 2-bit k-mer (A 0, C 1,
+ * T/U 2, G 3; first base lowest), end = inclusive position of its last base. */
+int mpoa_sketch(const uint8_t *seq, uint64_t len, uint64_t k, uint64_t w, uint64_t *value,
+                uint64_t *end, uint64_t cap, uint64_t *out_len);
+/* src/minimizer.rs:128-176 `collect_anchors` (query vs target). */
+int mpoa_collect_anchors(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
+                         const uint8_t *target, uint64_t tlen, mpoa_anchor *out, uint64_t cap,
+                         uint64_t *out_len);
+/* src/minimizer.rs:199-377 `chain` (anchors sorted by target, then query end). */
+int mpoa_chain(const mpoa_anchor *anchors, uint64_t n, uint64_t k, mpoa_anchor *out, uint64_t cap,
+               uint64_t *out_len);
+/* src/minimizer.rs:379-467 `partition_windows`; *tail_adaptive: the last
+ * window's adaptive flag. */
+int mpoa_partition_windows(const mpoa_anchor *chain, uint64_t n, uint64_t tlen, uint64_t qlen,
+                           uint64_t max_drift, uint64_t max_win, mpoa_anchor *out, uint64_t cap,
+                           uint64_t *out_len, int *tail_adaptive);
+/* src/minimizer.rs:475-488 `anchor_windows`. */
+int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
+                        const uint8_t *consensus, uint64_t clen, mpoa_anchor *out, uint64_t cap,
+                        uint64_t *out_len, int *tail_adaptive);
 
 /* ---- library-owned graph (src/graph.rs), for callers without one ---- */
 

@@ -18,6 +18,7 @@
 #include "minipoa_hip.h"
 #include "plan.hpp"
 #include "runtime.hpp"
+#include "seed.hpp"
 
 using namespace mpoa;
 
@@ -92,7 +93,63 @@ int poa_retry(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end,
     return rc;
 }
 
-// src/align.rs:81-98
+// src/align.rs:162-187: every window of one query in one launch; windows
+// whose DP fails retry with the full matrix (poa's retry) in a second
+// launch; the first failing window in query order decides the error.
+template <class G>
+int run_windows(Ctx &ctx, const Params &p, const G &g, const uint8_t *q,
+                const std::vector<SeedWindow> &win, std::vector<AlignedBase> &out) {
+    const size_t W = win.size();
+    Params full = p;
+    full.f = 0;
+    std::vector<JobPlan> plan(W);
+    std::vector<int> st(W, MPOA_OK);
+    std::vector<std::string> msg(W);
+    std::vector<std::vector<AlignedBase>> path(W);
+    std::vector<size_t> todo(W);
+    for (size_t k = 0; k < W; k++) todo[k] = k;
+    for (int attempt = 0; attempt < 2 && !todo.empty(); attempt++) {
+        const Params &pp = attempt ? full : p;
+        std::vector<JobPlan *> launch;
+        std::vector<size_t> who;
+        for (size_t k : todo) {
+            const SeedWindow &w = win[k];
+            plan_job(pp, g, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, attempt ? false : w.adaptive, w.band, 0,
+                     plan[k]);
+            if (plan[k].status != MPOA_OK) {
+                st[k] = plan[k].status;
+                msg[k] = plan[k].err;
+                continue;
+            }
+            launch.push_back(&plan[k]);
+            who.push_back(k);
+        }
+        if (!launch.empty()) {
+            std::vector<DevResult> res;
+            std::vector<uint32_t> tb;
+            std::vector<uint64_t> off;
+            const int rc = run_plans(ctx, pp, launch, res, tb, off, nullptr);
+            if (rc != MPOA_OK) return rc;
+            for (size_t i = 0; i < who.size(); i++) {
+                const size_t k = who[i];
+                st[k] = device_status(res[i], msg[k]);
+                if (st[k] == MPOA_OK) assemble(g, plan[k], tb.data() + off[i], path[k]);
+            }
+        }
+        std::vector<size_t> again;
+        if (attempt == 0 && p.f != 0)
+            for (size_t k : todo)
+                if (st[k] != MPOA_OK && is_dp_error(st[k])) again.push_back(k);
+        todo.swap(again);
+    }
+    for (size_t k = 0; k < W; k++)
+        if (st[k] != MPOA_OK) return fail(st[k], msg[k]);
+    out.clear();
+    for (size_t k = 0; k < W; k++) append_window(g, win[k], path[k], out);
+    return MPOA_OK;
+}
+
+// src/align.rs:81-188
 template <class G>
 int align_one(Ctx &ctx, const Params &p, const G &g, const uint8_t *q, uint64_t qlen,
               std::vector<AlignedBase> &out) {
@@ -102,9 +159,12 @@ int align_one(Ctx &ctx, const Params &p, const G &g, const uint8_t *q, uint64_t 
         for (uint64_t k = 0; k < qlen; k++) out.push_back(AlignedBase{-1, int64_t(p.encode(q[k])), -1});
         return MPOA_OK;
     }
-    if (p.seeding)
-        return fail(MPOA_ERR_UNSUPPORTED, "minimizer-seeded windows are not implemented by this backend");
-    return poa_retry(ctx, p, g, 0, 1, q, qlen, p.adaptive, p.b, out, nullptr);
+    if (!p.seeding) return poa_retry(ctx, p, g, 0, 1, q, qlen, p.adaptive, p.b, out, nullptr);
+    std::vector<SeedWindow> win;
+    std::string err;
+    const int rc = seed_windows(p, g, q, qlen, win, err);
+    if (rc != MPOA_OK) return fail(rc, err);
+    return run_windows(ctx, p, g, q, win, out);
 }
 
 int emit(const std::vector<AlignedBase> &v, mpoa_aligned_base *out, uint64_t cap, uint64_t *len) {
@@ -525,8 +585,8 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     Params p;
     int rc = MPOA_OK;
     if (!check_params(params, p, rc)) return rc;
-    if (p.seeding || p.progressive)
-        return fail(MPOA_ERR_UNSUPPORTED, "seeded or guide-ordered batches are not implemented");
+    if (p.progressive)
+        return fail(MPOA_ERR_UNSUPPORTED, "guide-ordered batches (-p) are not implemented");
     auto t_all = std::chrono::steady_clock::now();
     auto B = std::make_unique<mpoa_batch>();
     B->params = p;
@@ -598,8 +658,8 @@ int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_serv
     Params p;
     int rc = MPOA_OK;
     if (!check_params(params, p, rc)) return rc;
-    if (p.seeding || p.progressive)
-        return fail(MPOA_ERR_UNSUPPORTED, "seeded or guide-ordered jobs are not implemented");
+    if (p.progressive)
+        return fail(MPOA_ERR_UNSUPPORTED, "guide-ordered jobs (-p) are not implemented");
     // Four slots by default: two for long DPs, two for short ones.
     const int nslots = engine_slots(opts && opts->slots ? int(opts->slots) : 4);
     rc = ensure_engine(ctx, nslots);
@@ -718,6 +778,92 @@ void mpoa_server_free(mpoa_server *s) {
     s->engine.reset();  // finishes every submitted job first
     delete s;
     release_engine(ctx);
+}
+
+}  // extern "C"
+
+// ---- minimizer seeding (host) ----
+
+namespace {
+
+template <class T, class F>
+int emit_list(const std::vector<T> &v, uint64_t cap, uint64_t *len, F &&put) {
+    if (len) *len = v.size();
+    if (v.size() > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "output buffer is too small");
+    for (size_t i = 0; i < v.size(); i++) put(i, v[i]);
+    return MPOA_OK;
+}
+
+void anchors_in(const mpoa_anchor *a, uint64_t n, std::vector<Anchor> &v) {
+    v.resize(n);
+    for (uint64_t i = 0; i < n; i++) v[i] = Anchor{a[i].target_end, a[i].query_end, a[i].span, a[i].adaptive_before != 0};
+}
+
+int anchors_out(const std::vector<Anchor> &v, mpoa_anchor *out, uint64_t cap, uint64_t *len) {
+    return emit_list(v, cap, len, [&](size_t i, const Anchor &a) {
+        out[i] = mpoa_anchor{a.target_end, a.query_end, a.span, a.adaptive_before ? 1 : 0};
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpoa_sketch(const uint8_t *seq, uint64_t len, uint64_t k, uint64_t w, uint64_t *value,
+                uint64_t *end, uint64_t cap, uint64_t *out_len) {
+    if ((!seq && len) || (cap && (!value || !end))) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::vector<Minimizer> mm;
+    sketch(seq, len, k, w, mm);
+    return emit_list(mm, cap, out_len, [&](size_t i, const Minimizer &m) {
+        value[i] = m.value;
+        end[i] = m.end;
+    });
+}
+
+int mpoa_collect_anchors(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
+                         const uint8_t *target, uint64_t tlen, mpoa_anchor *out, uint64_t cap,
+                         uint64_t *out_len) {
+    if ((!query && qlen) || (!target && tlen) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    std::vector<Anchor> a;
+    collect_anchors(p, query, qlen, target, tlen, a);
+    return anchors_out(a, out, cap, out_len);
+}
+
+int mpoa_chain(const mpoa_anchor *anchors, uint64_t n, uint64_t k, mpoa_anchor *out, uint64_t cap,
+               uint64_t *out_len) {
+    if ((!anchors && n) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::vector<Anchor> in, c;
+    anchors_in(anchors, n, in);
+    chain(in, k, c);
+    return anchors_out(c, out, cap, out_len);
+}
+
+int mpoa_partition_windows(const mpoa_anchor *chain_in, uint64_t n, uint64_t tlen, uint64_t qlen,
+                           uint64_t max_drift, uint64_t max_win, mpoa_anchor *out, uint64_t cap,
+                           uint64_t *out_len, int *tail_adaptive) {
+    if ((!chain_in && n) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::vector<Anchor> in, w;
+    anchors_in(chain_in, n, in);
+    const bool tail = partition_windows(in, tlen, qlen, max_drift, max_win, w);
+    if (tail_adaptive) *tail_adaptive = tail ? 1 : 0;
+    return anchors_out(w, out, cap, out_len);
+}
+
+int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
+                        const uint8_t *consensus, uint64_t clen, mpoa_anchor *out, uint64_t cap,
+                        uint64_t *out_len, int *tail_adaptive) {
+    if ((!query && qlen) || (!consensus && clen) || (cap && !out))
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    std::vector<Anchor> w;
+    const bool tail = anchor_windows(p, query, qlen, consensus, clen, w);
+    if (tail_adaptive) *tail_adaptive = tail ? 1 : 0;
+    return anchors_out(w, out, cap, out_len);
 }
 
 }  // extern "C"

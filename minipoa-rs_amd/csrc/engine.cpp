@@ -7,6 +7,7 @@
 
 #include "minipoa_hip.h"
 #include "plan.hpp"
+#include "seed.hpp"
 
 namespace mpoa {
 
@@ -49,22 +50,38 @@ void TaskQueue::loop() {
     }
 }
 
+// One DP of a job's step: the whole-graph DP, or one seeded window.
+struct Engine::Dp {
+    Job *job = nullptr;
+    uint32_t idx = 0;        // window index within the step
+    JobPlan plan;
+    DevResult res{};
+    std::vector<uint32_t> tb;
+    bool retry = false;      // the planned DP is poa's full-matrix retry
+    bool fresh = false;      // res (or a planning error) awaits consumption
+    int status = MPOA_OK;    // final outcome once consumed
+    std::string err;
+    uint64_t cost = 0;       // expected cells of the planned DP
+    uint64_t rem = ~0ull;    // the job's cells left when last queued
+    std::vector<AlignedBase> path;
+};
+
 // One job's progressive state.
 struct Engine::Job {
     uint64_t id = 0;
     JobIn in;
     JobOut out;
     uint32_t step = 0;       // reads of `in` aligned so far
-    bool retry = false;      // the planned DP is poa's full-matrix retry
-    JobPlan plan;
+    std::vector<std::unique_ptr<Dp>> dp;  // the step's DPs are dp[0 .. ndp)
+    uint32_t ndp = 0;
+    std::vector<SeedWindow> win;          // the step's seeded windows
+    uint32_t pending = 0;    // DPs of the step on the device (under the engine lock)
     uint64_t prev_used = 0, prev_n = 0;
-    uint64_t cost = 0;       // expected cells of the planned DP
+    uint64_t cost = 0;       // expected cells of the step
     uint64_t rem = ~0ull;    // cells left when last queued (0 once finished; under the engine lock)
     uint64_t footprint = 0;  // bytes counted in active_bytes_ (under the engine lock)
     bool admitted = false;
     std::vector<AlignedBase> path;
-    DevResult res{};
-    std::vector<uint32_t> tb;
 };
 
 // One line per launch for MPOA_TRACE=<file>: scheduling diagnostics.
@@ -310,21 +327,28 @@ void Engine::finish(Job &J) {
     for (Job *W : admitted) task(*W, false);
 }
 
-void Engine::requeue(Job &J) {
+void Engine::requeue(Job &J, const std::vector<Dp *> &dps) {
     std::lock_guard<std::mutex> lk(mu_);
+    J.pending = uint32_t(dps.size());
     J.rem = J.cost * uint64_t(J.in.nreads - J.step);
-    const uint64_t fp = std::max(job_arena_bytes(J.plan, J.plan.use_i64), footprint_estimate(J));
+    uint64_t bytes = 0;
+    for (const Dp *D : dps) bytes += job_arena_bytes(D->plan, D->plan.use_i64);
+    const uint64_t fp = std::max(bytes, footprint_estimate(J));
     active_bytes_ = active_bytes_ - std::min(active_bytes_, J.footprint) + fp;
     J.footprint = fp;
-    if (J.in.crit && (!crit_busy_ || &J == top_job_)) {
-        if (ready_crit_.empty()) crit_since_ = now_ms();
-        if (&J == top_job_) top_since_ = now_ms();
-        ready_crit_.push_back(&J);
-    } else {
-        // A critical job that missed the priority slot's launch takes an
-        // ordinary slot for this step rather than holding the priority slot
-        // when the largest job comes back.
-        (o_.long_rows && J.plan.n >= o_.long_rows ? ready_long_ : ready_).push_back(&J);
+    const bool crit = J.in.crit && (!crit_busy_ || &J == top_job_);
+    if (crit && ready_crit_.empty()) crit_since_ = now_ms();
+    if (crit && &J == top_job_) top_since_ = now_ms();
+    for (Dp *D : dps) {
+        D->rem = J.rem;
+        if (crit) {
+            ready_crit_.push_back(D);
+        } else {
+            // A critical job that missed the priority slot's launch takes an
+            // ordinary slot for this step rather than holding the priority
+            // slot when the largest job comes back.
+            (o_.long_rows && D->plan.n >= o_.long_rows ? ready_long_ : ready_).push_back(D);
+        }
     }
     cv_.notify_all();
 }
@@ -361,56 +385,146 @@ void Engine::render(Job &J) {
     }
 }
 
-// Consumes the job's DP result (have), fuses it and plans the next DP.
-// src/lib.rs:195-202 (one read per step), src/align.rs:81-98 (empty
-// graph), src/align.rs:202-207 (full-matrix retry on any error).
+// Consumes the step's fresh DP results. A DP that overflowed its arena, or
+// failed with f != 0 on its first attempt (poa's full-matrix retry,
+// src/align.rs:202-207), goes into `again`; every other DP gets its final
+// status and path. False when the job ended with an error.
+bool Engine::consume(Job &J, std::vector<Dp *> &again) {
+    Graph &g = *J.out.graph;
+    const uint8_t *q = J.in.bases + J.in.read_off[J.step];
+    const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+    again.clear();
+    for (uint32_t k = 0; k < J.ndp; k++) {
+        Dp &D = *J.dp[k];
+        if (!D.fresh) continue;
+        D.fresh = false;
+        if (D.plan.status == MPOA_OK && D.res.status == kArenaOverflow) {
+            D.plan.full_arena = true;  // same DP, arena for full rows
+            again.push_back(&D);
+            continue;
+        }
+        const bool failed = D.plan.status != MPOA_OK || D.res.status != kOk;
+        if (failed && p_.f != 0 && !D.retry) {
+            D.retry = true;
+            retries_++;
+            if (J.win.empty()) {
+                plan_job(full_, GraphAcc{g}, 0, 1, q, qlen, false, p_.b, 0, D.plan);
+            } else {
+                const SeedWindow &w = J.win[k];
+                plan_job(full_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, false, w.band, 0, D.plan);
+            }
+            if (D.plan.status == MPOA_OK) {
+                D.cost = D.plan.static_cells;
+                again.push_back(&D);
+            } else {
+                D.status = D.plan.status;
+                D.err = D.plan.err;
+            }
+            continue;
+        }
+        if (D.plan.status != MPOA_OK) {
+            D.status = D.plan.status;
+            D.err = D.plan.err;
+            continue;
+        }
+        D.status = device_status(D.res, D.err);
+        if (D.status != MPOA_OK) continue;
+        if (!D.retry && J.win.empty()) {
+            J.prev_used = D.res.arena_used;
+            J.prev_n = D.plan.n;
+        }
+        J.out.cells += D.res.cells;
+        assemble(GraphAcc{g}, D.plan, D.tb.data(), D.path);
+    }
+    if (!again.empty()) return true;
+    // Every DP of the step is final: the first failing window (in query
+    // order) fails the job, as map_ordered returns the first error.
+    for (uint32_t k = 0; k < J.ndp; k++) {
+        const Dp &D = *J.dp[k];
+        if (D.status != MPOA_OK) {
+            fail_job(J, D.status, D.err);
+            return false;
+        }
+    }
+    if (J.win.empty()) {
+        J.path.swap(J.dp[0]->path);
+    } else {
+        J.path.clear();
+        for (uint32_t k = 0; k < J.ndp; k++) append_window(GraphAcc{g}, J.win[k], J.dp[k]->path, J.path);
+    }
+    return true;
+}
+
+// Plans the DPs of the job's next step (src/align.rs:96-188). DPs whose
+// planning failed are marked fresh for consume(); the others go to `launch`.
+// False when the job ended with an error.
+bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
+    Graph &g = *J.out.graph;
+    const uint8_t *q = J.in.bases + J.in.read_off[J.step];
+    const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+    J.win.clear();
+    if (p_.seeding) {
+        std::string err;
+        const int rc = seed_windows(p_, GraphAcc{g}, q, qlen, J.win, err);
+        if (rc != MPOA_OK) {
+            fail_job(J, rc, err);
+            return false;
+        }
+    }
+    J.ndp = uint32_t(std::max<size_t>(1, J.win.size()));
+    while (J.dp.size() < J.ndp) {
+        J.dp.push_back(std::make_unique<Dp>());
+        J.dp.back()->job = &J;
+        J.dp.back()->idx = uint32_t(J.dp.size() - 1);
+    }
+    launch.clear();
+    J.cost = 0;
+    for (uint32_t k = 0; k < J.ndp; k++) {
+        Dp &D = *J.dp[k];
+        D.retry = false;
+        D.fresh = false;
+        D.status = MPOA_OK;
+        D.err.clear();
+        JobPlan &P = D.plan;
+        if (J.win.empty()) {
+            plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, P);
+        } else {
+            const SeedWindow &w = J.win[k];
+            plan_job(p_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, w.adaptive, w.band, 0, P);
+        }
+        if (P.status != MPOA_OK) {
+            D.fresh = true;
+            continue;
+        }
+        D.cost = P.static_cells;
+        // Adaptive bands: start from what the previous step's DP used,
+        // scaled to the grown graph.
+        if (J.win.empty() && (P.flags & kAdaptive) && J.prev_n > 0) {
+            uint64_t hint = J.prev_used * P.n / J.prev_n;
+            hint += hint / 8 + 3ull * P.padded * 8;
+            P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
+            P.width_hint = uint32_t(J.prev_used / (3 * J.prev_n));
+            D.cost = uint64_t(P.width_hint) * P.n;
+        }
+        J.cost += D.cost;
+        launch.push_back(&D);
+    }
+    return true;
+}
+
+// Consumes the job's DP results (have), fuses the step's path and plans the
+// next step. src/lib.rs:195-202 (one read per step), src/align.rs:81-188.
 void Engine::advance(Job &J, bool have) {
     Graph &g = *J.out.graph;
-    auto query = [&](const uint8_t *&q, uint64_t &qlen) {
-        q = J.in.bases + J.in.read_off[J.step];
-        qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
-    };
+    std::vector<Dp *> again;
     for (;;) {
         if (have) {
             have = false;
-            if (J.plan.status == MPOA_OK && J.res.status == kArenaOverflow) {
-                J.plan.full_arena = true;  // same DP, arena for full rows
-                requeue(J);
+            if (!consume(J, again)) return;
+            if (!again.empty()) {
+                requeue(J, again);
                 return;
             }
-            const bool failed = J.plan.status != MPOA_OK || J.res.status != kOk;
-            if (failed && p_.f != 0 && !J.retry) {
-                J.retry = true;
-                retries_++;
-                const uint8_t *q;
-                uint64_t qlen;
-                query(q, qlen);
-                plan_job(full_, GraphAcc{g}, 0, 1, q, qlen, false, p_.b, 0, J.plan);
-                if (J.plan.status != MPOA_OK) {
-                    have = true;
-                    continue;
-                }
-                J.cost = J.plan.static_cells;
-                requeue(J);
-                return;
-            }
-            if (J.plan.status != MPOA_OK) {
-                fail_job(J, J.plan.status, J.plan.err);
-                return;
-            }
-            std::string msg;
-            const int sst = device_status(J.res, msg);
-            if (sst != MPOA_OK) {
-                fail_job(J, sst, msg);
-                return;
-            }
-            if (!J.retry) {
-                J.prev_used = J.res.arena_used;
-                J.prev_n = J.plan.n;
-            }
-            J.retry = false;
-            J.out.cells += J.res.cells;
-            assemble(GraphAcc{g}, J.plan, J.tb.data(), J.path);
             if (!fuse(J)) return;
         }
         if (J.step >= J.in.nreads) {
@@ -418,9 +532,8 @@ void Engine::advance(Job &J, bool have) {
             finish(J);
             return;
         }
-        const uint8_t *q;
-        uint64_t qlen;
-        query(q, qlen);
+        const uint8_t *q = J.in.bases + J.in.read_off[J.step];
+        const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
         if (g.size() <= 2) {
             J.path.clear();
             J.path.push_back(AlignedBase{0, int64_t(p_.encode('N')), -1});
@@ -428,23 +541,12 @@ void Engine::advance(Job &J, bool have) {
             if (!fuse(J)) return;
             continue;
         }
-        JobPlan &P = J.plan;
-        plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, P);
-        if (P.status != MPOA_OK) {
+        if (!plan_step(J, again)) return;
+        if (again.empty()) {  // every DP failed at planning
             have = true;
             continue;
         }
-        J.cost = P.static_cells;
-        // Adaptive bands: start from what the previous step's DP used,
-        // scaled to the grown graph.
-        if ((P.flags & kAdaptive) && J.prev_n > 0) {
-            uint64_t hint = J.prev_used * P.n / J.prev_n;
-            hint += hint / 8 + 3ull * P.padded * 8;
-            P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
-            P.width_hint = uint32_t(J.prev_used / (3 * J.prev_n));
-            J.cost = uint64_t(P.width_hint) * P.n;
-        }
-        requeue(J);
+        requeue(J, again);
         return;
     }
 }
@@ -474,7 +576,8 @@ void Engine::slot_main(size_t si) {
     // ones; the others take short DPs only.
     const size_t ord = o_.crit_lane ? si - 1 : si;
     const int role = !crit_slot && ord < o_.long_slots ? 1 : 0;
-    std::vector<Job *> batch;
+    std::vector<Dp *> batch;
+    std::vector<Job *> back;
     std::vector<JobPlan *> plans;
     LaunchInfo li;
     for (;;) {
@@ -488,8 +591,8 @@ void Engine::slot_main(size_t si) {
                     // ready and the others have had a short while to join; a
                     // job that misses the launch takes an ordinary slot for
                     // that step (requeue).
-                    const bool top_ready =
-                        std::find(ready_crit_.begin(), ready_crit_.end(), top_job_) != ready_crit_.end();
+                    const bool top_ready = std::any_of(ready_crit_.begin(), ready_crit_.end(),
+                                                       [&](const Dp *D) { return D->job == top_job_; });
                     const double t = now_ms();
                     if ((top_ready && (ready_crit_.size() >= active_crit_ || t - top_since_ >= crit_gather_ms_)) ||
                         t - crit_since_ >= crit_wait_ms_)
@@ -501,14 +604,14 @@ void Engine::slot_main(size_t si) {
                     break;
                 }
             }
-            std::vector<Job *> &q = crit_slot ? ready_crit_ : (role == 1 && !ready_long_.empty() ? ready_long_ : ready_);
+            std::vector<Dp *> &q = crit_slot ? ready_crit_ : (role == 1 && !ready_long_.empty() ? ready_long_ : ready_);
             const size_t idle = std::max<size_t>(1, &q == &ready_long_ ? idle_[1] : idle_[0] + (role == 1 ? 1 : 0));
             if (!crit_slot) {
                 // Longest remaining chain first (remaining reads times the
                 // current step's cells); neighbours in that order have
                 // similar step sizes, and a launch lasts as long as its
                 // longest job.
-                std::sort(q.begin(), q.end(), [](const Job *x, const Job *y) { return x->rem > y->rem; });
+                std::sort(q.begin(), q.end(), [](const Dp *x, const Dp *y) { return x->rem > y->rem; });
             }
             const uint64_t want = crit_slot ? q.size() : std::min<uint64_t>(max_launch_, (q.size() + idle - 1) / idle);
             const bool i64 = q[0]->plan.use_i64;  // one score width per launch
@@ -548,11 +651,12 @@ void Engine::slot_main(size_t si) {
         }
         const DevResult *hr = launch_results(c);
         for (size_t k = 0; k < batch.size(); k++) {
-            Job &J = *batch[k];
-            J.res = hr[k];
+            Dp &D = *batch[k];
+            D.res = hr[k];
+            D.fresh = true;
             if (hr[k].status == kOk) {
                 const uint32_t *t = launch_tb(c, li, k);
-                J.tb.assign(t, t + J.plan.m);
+                D.tb.assign(t, t + D.plan.m);
             }
         }
         {
@@ -568,16 +672,20 @@ void Engine::slot_main(size_t si) {
             if (trace_path_) {
                 TraceRec t{int(si), t0, t1, now_ms(), li.fill_ms, li.all_ms, li.h2d_ms, li.d2h_ms, li.grown, li.arena_bytes,
                            c.d_arena.cap, batch.size(), 0, 0, -1, li.ring_bytes, li.width, li.shape, li.rows};
-                for (const Job *J : batch) {
-                    t.max_cost = std::max(t.max_cost, J->cost);
-                    t.sum_cost += J->cost;
-                    if (J == watched_) t.watched_step = J->step;
+                for (const Dp *D : batch) {
+                    t.max_cost = std::max(t.max_cost, D->cost);
+                    t.sum_cost += D->cost;
+                    if (D->job == watched_) t.watched_step = D->job->step;
                 }
                 strace_[si].push_back(t);
             }
+            // A job goes back to the host once every DP of its step is back.
+            back.clear();
+            for (Dp *D : batch)
+                if (--D->job->pending == 0) back.push_back(D->job);
             if (crit_slot) {
                 // The top job's host step first, then the rest by cells left.
-                std::stable_sort(batch.begin(), batch.end(), [&](const Job *x, const Job *y) {
+                std::stable_sort(back.begin(), back.end(), [&](const Job *x, const Job *y) {
                     if ((x == top_job_) != (y == top_job_)) return x == top_job_;
                     return x->rem > y->rem;
                 });
@@ -586,7 +694,7 @@ void Engine::slot_main(size_t si) {
                 idle_[role]++;
             }
         }
-        for (Job *J : batch) task(*J, true);
+        for (Job *J : back) task(*J, true);
     }
 }
 

@@ -1,7 +1,9 @@
 // Continuous batched POA engine. Every job advances through its progressive
-// steps on its own (src/lib.rs:184-204 per job): when a job's DP comes back
+// steps on its own (src/lib.rs:184-204 per job): when a job's DPs come back
 // from the device, a host worker fuses the path into its graph, plans the
-// job's next DP and hands it back to the dispatcher. The dispatcher keeps
+// job's next step and hands its DPs back to the dispatcher. A step is one
+// whole-graph DP, or with minimizer seeding one DP per anchored window of
+// the read (src/align.rs:99-188), all launched side by side. The dispatcher keeps
 // several launch slots (streams with their own buffers) busy with whatever
 // jobs are ready, grouping jobs of similar size, so no job waits for a
 // longer one to finish its step.
@@ -112,13 +114,16 @@ public:
     int error();
 
     struct Job;
+    struct Dp;
 
 private:
     void finish(Job &J);
-    void requeue(Job &J);
+    void requeue(Job &J, const std::vector<Dp *> &dps);
     void fail_job(Job &J, int status, const std::string &e);
     bool fuse(Job &J);
     void render(Job &J);
+    bool consume(Job &J, std::vector<Dp *> &again);
+    bool plan_step(Job &J, std::vector<Dp *> &launch);
     void advance(Job &J, bool have);
     void task(Job &J, bool have);
     void admit_locked(Job &J);
@@ -134,7 +139,9 @@ private:
     std::mutex mu_;
     std::condition_variable cv_;
     std::deque<std::unique_ptr<Job>> jobs_;
-    std::vector<Job *> ready_, ready_long_, ready_crit_, crit_jobs_;
+    // DPs ready to launch (one per job step, or one per seeded window).
+    std::vector<Dp *> ready_, ready_long_, ready_crit_;
+    std::vector<Job *> crit_jobs_;
     std::deque<Job *> waiting_;  // submitted, not yet admitted (FIFO)
     uint64_t finished_ = 0, active_crit_ = 0, outstanding_ = 0;
     uint64_t active_bytes_ = 0, admit_budget_ = 0;

@@ -9,6 +9,7 @@ library or device raises.
 from __future__ import annotations
 
 import ctypes as C
+from collections import namedtuple
 from pathlib import Path
 
 from .params import AlignedBase, Params, Sequence, default_matrix, records
@@ -24,6 +25,7 @@ STATUS = {
     10: "MPOA_ERR_ALLOC", 11: "MPOA_ERR_INVALID_PARAMS", 12: "MPOA_ERR_GRAPH",
     13: "MPOA_ERR_DEVICE", 14: "MPOA_ERR_NO_DEVICE", 15: "MPOA_ERR_UNSUPPORTED",
     16: "MPOA_ERR_BUFFER_TOO_SMALL", 17: "MPOA_ERR_INVALID_ARGUMENT", 18: "MPOA_ERR_PENDING",
+    19: "MPOA_ERR_SEED_SPAN",
 }
 BUFFER_TOO_SMALL = 16
 
@@ -89,6 +91,16 @@ class _JobTimes(C.Structure):
                 ("cells", C.c_uint64)]
 
 
+class _Anchor(C.Structure):
+    _fields_ = [("target_end", C.c_uint64), ("query_end", C.c_uint64), ("span", C.c_uint64),
+                ("adaptive_before", C.c_int64)]
+
+
+# src/minimizer.rs:11-26
+Minimizer = namedtuple("Minimizer", "value end span")
+Anchor = namedtuple("Anchor", "target_end query_end span adaptive_before")
+
+
 # Exported symbols of include/minipoa_hip.h with their ctypes signatures.
 _P = C.POINTER(_Params)
 _V = C.POINTER(_View)
@@ -122,6 +134,14 @@ SIGNATURES = {
                                  C.c_char_p, _u64, U64P]),
     "mpoa_graph_gfa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), _u64, C.c_char_p, _u64, U64P]),
     "mpoa_params_validate": (C.c_int, [_P]),
+    "mpoa_sketch": (C.c_int, [C.c_char_p, _u64, _u64, _u64, U64P, U64P, _u64, U64P]),
+    "mpoa_collect_anchors": (C.c_int, [_P, C.c_char_p, _u64, C.c_char_p, _u64, C.POINTER(_Anchor),
+                                       _u64, U64P]),
+    "mpoa_chain": (C.c_int, [C.POINTER(_Anchor), _u64, _u64, C.POINTER(_Anchor), _u64, U64P]),
+    "mpoa_partition_windows": (C.c_int, [C.POINTER(_Anchor), _u64, _u64, _u64, _u64, _u64,
+                                         C.POINTER(_Anchor), _u64, U64P, C.POINTER(C.c_int)]),
+    "mpoa_anchor_windows": (C.c_int, [_P, C.c_char_p, _u64, C.c_char_p, _u64, C.POINTER(_Anchor),
+                                      _u64, U64P, C.POINTER(C.c_int)]),
     "mpoa_load_matrix": (C.c_int, [C.c_char_p, _u64, C.POINTER(C.c_uint8), C.POINTER(C.c_int32),
                                    C.c_uint32, C.POINTER(C.c_uint32)]),
     "mpoa_align_sequences": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), U64P, _u64,
@@ -458,6 +478,83 @@ def validate(params):
     _check(lib().mpoa_params_validate(CParams(params).ref))
 
 
+# ---- minimizer seeding (src/minimizer.rs), host side ----
+
+def _sized(call, cap):
+    """call(cap) -> (rc, n, result(n)); grows the buffer once if needed."""
+    while True:
+        rc, n, res = call(cap)
+        if rc == BUFFER_TOO_SMALL and n > cap:
+            cap = n
+            continue
+        _check(rc)
+        return res(n)
+
+
+def sketch(seq, k, w):
+    """src/minimizer.rs:32-76 -> [Minimizer(value, end, span)]."""
+    s = bytes(seq)
+
+    def call(cap):
+        v = (C.c_uint64 * max(1, cap))()
+        e = (C.c_uint64 * max(1, cap))()
+        n = C.c_uint64(0)
+        rc = lib().mpoa_sketch(s, len(s), k, w, v, e, cap, C.byref(n))
+        return rc, n.value, lambda n: [Minimizer(v[i], e[i], k) for i in range(n)]
+    return _sized(call, len(s) + 1)
+
+
+def _anchor_arr(anchors):
+    arr = (_Anchor * max(1, len(anchors)))()
+    for i, a in enumerate(anchors):
+        arr[i] = _Anchor(a.target_end, a.query_end, a.span, int(bool(a.adaptive_before)))
+    return arr
+
+
+def _anchor_list(arr, n):
+    return [Anchor(arr[i].target_end, arr[i].query_end, arr[i].span, bool(arr[i].adaptive_before))
+            for i in range(n)]
+
+
+def _anchor_call(fn, args, cap, tail=None):
+    def call(cap):
+        out = (_Anchor * max(1, cap))()
+        n = C.c_uint64(0)
+        extra = (C.byref(tail),) if tail is not None else ()
+        rc = fn(*args, out, cap, C.byref(n), *extra)
+        return rc, n.value, lambda n: _anchor_list(out, n)
+    return _sized(call, cap)
+
+
+def collect_anchors(params, query, target):
+    """src/minimizer.rs:128-176"""
+    cp, q, t = CParams(params), bytes(query), bytes(target)
+    return _anchor_call(lib().mpoa_collect_anchors, (cp.ref, q, len(q), t, len(t)), 256)
+
+
+def chain(anchors, k):
+    """src/minimizer.rs:199-377"""
+    arr = _anchor_arr(anchors)
+    return _anchor_call(lib().mpoa_chain, (arr, len(anchors), k), len(anchors) + 1)
+
+
+def partition_windows(chain_, tlen, qlen, max_drift, max_win):
+    """src/minimizer.rs:379-467 -> (boundaries, tail_adaptive)."""
+    arr = _anchor_arr(chain_)
+    tail = C.c_int(0)
+    res = _anchor_call(lib().mpoa_partition_windows, (arr, len(chain_), tlen, qlen, max_drift, max_win),
+                       len(chain_) + 1, tail)
+    return res, bool(tail.value)
+
+
+def anchor_windows(params, query, consensus):
+    """src/minimizer.rs:475-488 -> (boundaries, tail_adaptive)."""
+    cp, q, t = CParams(params), bytes(query), bytes(consensus)
+    tail = C.c_int(0)
+    res = _anchor_call(lib().mpoa_anchor_windows, (cp.ref, q, len(q), t, len(t)), 256, tail)
+    return res, bool(tail.value)
+
+
 def align_sequences(ctx, params, seqs):
     """src/lib.rs:177-182 with every DP on the device."""
     cp = CParams(params)
@@ -620,4 +717,6 @@ __all__ = [
     "Graph", "GraphView", "view_from_graph", "poa", "poa_inner", "poa_kernel", "align",
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
     "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server", "load_matrix",
+    "Minimizer", "Anchor", "sketch", "collect_anchors", "chain", "partition_windows",
+    "anchor_windows",
 ]

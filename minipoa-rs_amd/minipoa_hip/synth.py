@@ -45,6 +45,24 @@ def c1(seed=20261, length=2000, nreads=10, rate=0.10):
     return [mutate(rng, tmpl, rate) for _ in range(nreads)]
 
 
+def seeded(seed=20263, njobs=1, length=16500, nreads=100, rate=0.10, trim=500):
+    """Seeded long-read consensus: per job one template of `length` bases and
+    `nreads` reads at `rate`, each trimmed by U[0, trim] bases at both ends.
+    Returns (job_off, read_off, bases)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    per_job = []
+    for _ in range(njobs):
+        tmpl = rng.integers(0, 4, size=length).astype(np.uint8)
+        reads = []
+        for _ in range(nreads):
+            r = mutate(rng, tmpl, rate)
+            a = int(rng.integers(0, trim + 1))
+            b = int(rng.integers(0, trim + 1))
+            reads.append(r[a:max(a, len(r) - b)])
+        per_job.append(reads)
+    return _pack(per_job)
+
+
 def jobs(seed=20262, njobs=4096, lmin=200, lmax=2000, rmin=8, rmax=32, emin=0.05, emax=0.15):
     """Batched DNA POA: per job a template length U[lmin,lmax], U[rmin,rmax]
     reads, per-read error rate U[emin,emax]. Returns (job_off, read_off,

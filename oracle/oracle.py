@@ -51,6 +51,16 @@ class _Base(C.Structure):
     _fields_ = [("node", C.c_int64), ("base", C.c_int64), ("aligned", C.c_int64)]
 
 
+class _Anchor(C.Structure):
+    _fields_ = [("target_end", C.c_uint64), ("query_end", C.c_uint64), ("span", C.c_uint64),
+                ("adaptive_before", C.c_int64)]
+
+
+# src/minimizer.rs:11-26
+Minimizer = namedtuple("Minimizer", "value end span")
+Anchor = namedtuple("Anchor", "target_end query_end span adaptive_before")
+
+
 def build():
     """Compile the oracle with its Makefile (gcc, no GPU needed)."""
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
@@ -113,6 +123,21 @@ def lib():
         L.o_batch_run.argtypes = [P, sz, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]
         L.o_batch_run.restype = C.c_uint64
+        u64p = C.POINTER(C.c_uint64)
+        A = C.POINTER(_Anchor)
+        L.o_sketch.argtypes = [C.c_char_p, sz, sz, sz, u64p, u64p, sz]
+        L.o_sketch.restype = sz
+        L.o_repeat_intervals.argtypes = [u64p, u64p, u64p, sz, sz, sz, u64p, u64p, sz]
+        L.o_repeat_intervals.restype = sz
+        L.o_collect_anchors.argtypes = [P, C.c_char_p, sz, C.c_char_p, sz, A, sz]
+        L.o_collect_anchors.restype = sz
+        L.o_chain.argtypes = [A, sz, sz, A, sz]
+        L.o_chain.restype = sz
+        L.o_partition_windows.argtypes = [A, sz, sz, sz, sz, sz, A, sz, C.POINTER(C.c_int)]
+        L.o_partition_windows.restype = sz
+        L.o_anchor_windows.argtypes = [P, C.c_char_p, sz, C.c_char_p, sz, A, sz,
+                                       C.POINTER(C.c_int)]
+        L.o_anchor_windows.restype = sz
         _lib = L
     return _lib
 
@@ -350,3 +375,104 @@ def batch_run(params, job_off, read_off, bases, nthreads):
         _P(params).ref, njobs, job_off.ctypes.data_as(u64p), read_off.ctypes.data_as(u64p),
         bases.ctypes.data_as(C.c_void_p), int(nthreads), out.ctypes.data_as(u64p))
     return int(cells_), out
+
+
+# ---- minimizer seeding (src/minimizer.rs:32-488) ----
+
+def _grow(call, cap=64):
+    """Calls call(cap) -> (count, result) until the buffer held everything."""
+    while True:
+        n, res = call(cap)
+        if n <= cap:
+            return res(n)
+        cap = n + 16
+
+
+def sketch(seq, k, w):
+    """src/minimizer.rs:32-76 (positions parity unpinned)."""
+    s = bytes(seq)
+
+    def call(cap):
+        v = (C.c_uint64 * cap)()
+        e = (C.c_uint64 * cap)()
+        n = lib().o_sketch(s, len(s), k, w, v, e, cap)
+        return n, lambda n: [Minimizer(v[i], e[i], k) for i in range(n)]
+    return _grow(call, len(s) + 1)
+
+
+def repeat_intervals(mm, length, max_dist):
+    """src/minimizer.rs:78-110"""
+    n = len(mm)
+    v = (C.c_uint64 * max(1, n))(*[m.value for m in mm])
+    e = (C.c_uint64 * max(1, n))(*[m.end for m in mm])
+    sp = (C.c_uint64 * max(1, n))(*[m.span for m in mm])
+
+    def call(cap):
+        a = (C.c_uint64 * cap)()
+        b = (C.c_uint64 * cap)()
+        r = lib().o_repeat_intervals(v, e, sp, n, length, max_dist, a, b, cap)
+        return r, lambda r: [(a[i], b[i]) for i in range(r)]
+    return _grow(call, n + 1)
+
+
+def _anchors_in(anchors):
+    arr = (_Anchor * max(1, len(anchors)))()
+    for i, a in enumerate(anchors):
+        arr[i] = _Anchor(a.target_end, a.query_end, a.span, int(bool(a.adaptive_before)))
+    return arr
+
+
+def _anchors_out(arr, n):
+    return [Anchor(arr[i].target_end, arr[i].query_end, arr[i].span, bool(arr[i].adaptive_before))
+            for i in range(n)]
+
+
+def collect_anchors(params, query, target):
+    """src/minimizer.rs:128-176"""
+    P = _P(params)
+    q, t = bytes(query), bytes(target)
+
+    def call(cap):
+        out = (_Anchor * cap)()
+        n = lib().o_collect_anchors(P.ref, q, len(q), t, len(t), out, cap)
+        return n, lambda n: _anchors_out(out, n)
+    return _grow(call)
+
+
+def chain(anchors, k):
+    """src/minimizer.rs:199-377"""
+    arr = _anchors_in(anchors)
+
+    def call(cap):
+        out = (_Anchor * cap)()
+        n = lib().o_chain(arr, len(anchors), k, out, cap)
+        return n, lambda n: _anchors_out(out, n)
+    return _grow(call, len(anchors) + 1)
+
+
+def partition_windows(chain_, tlen, qlen, max_drift, max_win):
+    """src/minimizer.rs:379-467 -> (boundaries, tail_adaptive)"""
+    arr = _anchors_in(chain_)
+    tail = C.c_int(0)
+
+    def call(cap):
+        out = (_Anchor * cap)()
+        n = lib().o_partition_windows(arr, len(chain_), tlen, qlen, max_drift, max_win, out, cap,
+                                      C.byref(tail))
+        return n, lambda n: _anchors_out(out, n)
+    res = _grow(call, len(chain_) + 1)
+    return res, bool(tail.value)
+
+
+def anchor_windows(params, query, consensus):
+    """src/minimizer.rs:475-488 -> (boundaries, tail_adaptive)"""
+    P = _P(params)
+    q, t = bytes(query), bytes(consensus)
+    tail = C.c_int(0)
+
+    def call(cap):
+        out = (_Anchor * cap)()
+        n = lib().o_anchor_windows(P.ref, q, len(q), t, len(t), out, cap, C.byref(tail))
+        return n, lambda n: _anchors_out(out, n)
+    res = _grow(call)
+    return res, bool(tail.value)

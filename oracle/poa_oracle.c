@@ -14,13 +14,21 @@
  *   graph/consensus       reference KATs (src/graph.rs:344-392)
  *   pipeline bytes        reference KATs (tests/io_cli.rs:421-451, regressions)
  *   MSA/GFA structure     invariants of tests/avx512.rs:27-91
- * Parity unpinned: exact traceback tie-breaks, adaptive band evolution,
+ *   seeding               the reference's src/minimizer.rs tests (k-mer
+ *                         values, run splitting, repeat filter, interval
+ *                         merge, chain monotonicity, drift split, minimum
+ *                         chain score) and its seeded-alignment invariants
+ *                         (src/align.rs:1525-1559, tests/avx512.rs:145-180)
+ * Parity unpinned: minimizer positions (see the seeding section),
+ * exact traceback tie-breaks, adaptive band evolution,
  * head/tail support ties and toposort order beyond what the KATs reach are
  * pinned only by this restatement being line-faithful.
  */
 #include "poa_oracle.h"
 
 #include <pthread.h>
+#include <ctype.h>
+#include <math.h>
 #include <stdarg.h>
 #include <stdatomic.h>
 #include <stdio.h>
@@ -1304,7 +1312,701 @@ int o_poa(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
     return rc;
 }
 
-/* src/align.rs:81-98 (empty graph and unseeded branches). */
+/* ---------------- Minimizer seeding (src/minimizer.rs) ------------------
+ *
+ * Parity unpinned for minimizer POSITIONS: the reference takes them from the
+ * simd-minimizers 3.0.0 crate (not vendored), which ranks k-mers by a
+ * forward ntHash truncated inside its SIMD scan and keeps the leftmost of
+ * equal keys (src/minimizer.rs:1-6, 39-76). This restatement ranks each
+ * k-mer by the upper 16 bits of a 32-bit forward ntHash whose per-base seeds
+ * are the published 64-bit ntHash seeds truncated to their low 32 bits, and
+ * keeps the leftmost minimum of each window of w k-mers. The k-mer VALUES,
+ * the run splitting, the window clamp and everything downstream (repeat
+ * filter, anchors, chaining, windows) follow the reference exactly.
+ */
+
+typedef struct {
+    uint64_t value;
+    size_t end, span;
+} o_mm;
+
+typedef struct {
+    o_mm *v;
+    size_t n, cap;
+} mmvec;
+
+static void mm_push(mmvec *a, o_mm x) {
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 64;
+        a->v = xrealloc(a->v, a->cap * sizeof(o_mm));
+    }
+    a->v[a->n++] = x;
+}
+
+typedef struct {
+    o_anchor *v;
+    size_t n, cap;
+} anvec;
+
+static void an_push(anvec *a, o_anchor x) {
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 64;
+        a->v = xrealloc(a->v, a->cap * sizeof(o_anchor));
+    }
+    a->v[a->n++] = x;
+}
+
+/* src/minimizer.rs:28-30 */
+static int is_dna(uint8_t c) {
+    c = (uint8_t)toupper(c);
+    return c == 'A' || c == 'C' || c == 'G' || c == 'T' || c == 'U';
+}
+
+/* 2-bit code of packed-seq: (base >> 1) & 3 (A 0, C 1, T/U 2, G 3). */
+static inline uint32_t pack2(uint8_t c) { return (uint32_t)(c >> 1) & 3u; }
+
+static inline uint32_t rotl32(uint32_t x, unsigned r) {
+    r &= 31u;
+    return r ? (x << r) | (x >> (32u - r)) : x;
+}
+
+static const uint32_t kNtSeed[4] = {0x95c60474u, 0x62a02b4cu, 0x4be24456u, 0x82572324u};
+
+/* Minimizers of one run of DNA (len >= k) at offset `base_pos`. */
+static void sketch_run(const uint8_t *s, size_t len, size_t k, size_t w, size_t base_pos,
+                       mmvec *out) {
+    size_t nk = len - k + 1;
+    uint32_t *key = xmalloc(nk * sizeof(uint32_t));
+    uint32_t h = 0;
+    for (size_t i = 0; i < k; i++) h ^= rotl32(kNtSeed[pack2(s[i])], (unsigned)(k - 1 - i));
+    key[0] = h >> 16;
+    for (size_t t = 1; t < nk; t++) {
+        h = rotl32(h, 1) ^ rotl32(kNtSeed[pack2(s[t - 1])], (unsigned)k) ^ kNtSeed[pack2(s[t + k - 1])];
+        key[t] = h >> 16;
+    }
+    /* Sliding leftmost minimum over windows [st, st + w), monotone deque. */
+    size_t *dq = xmalloc(nk * sizeof(size_t));
+    size_t head = 0, tail = 0;
+    size_t last = (size_t)-1;
+    for (size_t t = 0; t < nk; t++) {
+        while (tail > head && key[dq[tail - 1]] > key[t]) tail--;
+        dq[tail++] = t;
+        if (t + 1 < w) continue;
+        size_t st = t + 1 - w;
+        while (dq[head] < st) head++;
+        size_t pos = dq[head];
+        if (pos == last) continue;
+        last = pos;
+        uint64_t v = 0;
+        for (size_t i = 0; i < k; i++) v |= (uint64_t)pack2(s[pos + i]) << (2 * i);
+        o_mm m = {v, base_pos + pos + k - 1, k};
+        mm_push(out, m);
+    }
+    free(dq);
+    free(key);
+}
+
+/* src/minimizer.rs:32-76 */
+static void sketch_into(const uint8_t *seq, size_t len, size_t k, size_t w, mmvec *out) {
+    out->n = 0;
+    if (k < 1 || k > 31 || w < 1 || w >= 32768 || len >= (size_t)UINT32_MAX) return;
+    size_t start = 0;
+    while (start < len) {
+        while (start < len && !is_dna(seq[start])) start++;
+        size_t end = start;
+        while (end < len && is_dna(seq[end])) end++;
+        if (end - start >= k) {
+            size_t run = end - start;
+            size_t win = w < run - k + 1 ? w : run - k + 1;
+            sketch_run(seq + start, run, k, win, start, out);
+        }
+        start = end + 1;
+    }
+}
+
+size_t o_sketch(const uint8_t *seq, size_t len, size_t k, size_t w, uint64_t *value,
+                uint64_t *end, size_t cap) {
+    mmvec mm = {0};
+    sketch_into(seq, len, k, w, &mm);
+    for (size_t i = 0; i < mm.n && i < cap; i++) {
+        value[i] = mm.v[i].value;
+        end[i] = mm.v[i].end;
+    }
+    size_t n = mm.n;
+    free(mm.v);
+    return n;
+}
+
+typedef struct {
+    size_t start, end;
+} o_iv;
+
+static int iv_cmp(const void *a, const void *b) {
+    const o_iv *x = a, *y = b;
+    if (x->start != y->start) return x->start < y->start ? -1 : 1;
+    if (x->end != y->end) return x->end < y->end ? -1 : 1;
+    return 0;
+}
+
+/* Sort order of minimizer indices by (value, index). */
+static const o_mm *g_sort_mm;
+static int mmidx_cmp(const void *a, const void *b) {
+    size_t i = *(const size_t *)a, j = *(const size_t *)b;
+    uint64_t vi = g_sort_mm[i].value, vj = g_sort_mm[j].value;
+    if (vi != vj) return vi < vj ? -1 : 1;
+    return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+static size_t *mm_order(const o_mm *mm, size_t n) {
+    size_t *ord = xmalloc((n ? n : 1) * sizeof(size_t));
+    for (size_t i = 0; i < n; i++) ord[i] = i;
+    g_sort_mm = mm;
+    qsort(ord, n, sizeof(size_t), mmidx_cmp);
+    return ord;
+}
+
+static inline size_t sat_sub(size_t a, size_t b) { return a > b ? a - b : 0; }
+
+/* src/minimizer.rs:78-110; returns the merged intervals (malloc'd). */
+static o_iv *repeat_intervals(const o_mm *mm, size_t n, size_t len, size_t max_dist,
+                              size_t *out_n) {
+    *out_n = 0;
+    if (max_dist == 0) return NULL;
+    /* prev[i]: the latest earlier minimizer with the same value (the
+     * HashMap `last` of the reference, read before its insert). */
+    size_t *ord = mm_order(mm, n);
+    size_t *prev = xmalloc((n ? n : 1) * sizeof(size_t));
+    for (size_t r = 0; r < n; r++) {
+        size_t i = ord[r];
+        prev[i] = (r > 0 && mm[ord[r - 1]].value == mm[i].value) ? ord[r - 1] : (size_t)-1;
+    }
+    o_iv *iv = xmalloc((n ? n : 1) * sizeof(o_iv));
+    size_t ni = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (prev[i] == (size_t)-1) continue;
+        size_t previous = mm[prev[i]].end;
+        size_t period = sat_sub(mm[i].end, previous);
+        if (period > 0 && period <= max_dist) {
+            size_t start = sat_sub(sat_sub(previous + 1, mm[i].span), period);
+            size_t end = mm[i].end + 1 + period;
+            if (end > len) end = len;
+            iv[ni].start = start;
+            iv[ni].end = end;
+            ni++;
+        }
+    }
+    qsort(iv, ni, sizeof(o_iv), iv_cmp);
+    size_t nm = 0;
+    for (size_t k = 0; k < ni; k++) {
+        if (nm > 0 && iv[k].start <= iv[nm - 1].end + 50) {
+            if (iv[k].end > iv[nm - 1].end) iv[nm - 1].end = iv[k].end;
+            continue;
+        }
+        iv[nm++] = iv[k];
+    }
+    free(prev);
+    free(ord);
+    *out_n = nm;
+    return iv;
+}
+
+size_t o_repeat_intervals(const uint64_t *value, const uint64_t *end, const uint64_t *span,
+                          size_t n, size_t len, size_t max_dist, uint64_t *out_start,
+                          uint64_t *out_end, size_t cap) {
+    o_mm *mm = xmalloc((n ? n : 1) * sizeof(o_mm));
+    for (size_t i = 0; i < n; i++) {
+        mm[i].value = value[i];
+        mm[i].end = end[i];
+        mm[i].span = span[i];
+    }
+    size_t ni = 0;
+    o_iv *iv = repeat_intervals(mm, n, len, max_dist, &ni);
+    for (size_t k = 0; k < ni && k < cap; k++) {
+        out_start[k] = iv[k].start;
+        out_end[k] = iv[k].end;
+    }
+    free(iv);
+    free(mm);
+    return ni;
+}
+
+/* src/minimizer.rs:112-126: prefix counts of repeat-covered positions. */
+static size_t *repeat_coverage(const o_mm *mm, size_t n, size_t len, size_t max_dist) {
+    size_t *prefix = xcalloc(len + 1, sizeof(size_t));
+    size_t ni = 0;
+    o_iv *iv = repeat_intervals(mm, n, len, max_dist, &ni);
+    for (size_t k = 0; k < ni; k++)
+        for (size_t x = iv[k].start + 1; x <= iv[k].end; x++) prefix[x] = 1;
+    for (size_t i = 0; i < len; i++) prefix[i + 1] += prefix[i];
+    free(iv);
+    return prefix;
+}
+
+static int overlaps_repeat(const o_mm *m, const size_t *prefix) {
+    return prefix[m->end + 1] > prefix[m->end + 1 - m->span];
+}
+
+static int anchor_cmp(const void *a, const void *b) {
+    const o_anchor *x = a, *y = b;
+    if (x->target_end != y->target_end) return x->target_end < y->target_end ? -1 : 1;
+    if (x->query_end != y->query_end) return x->query_end < y->query_end ? -1 : 1;
+    return 0;
+}
+
+/* src/minimizer.rs:128-176 */
+static void collect_anchors(const op_params *p, const uint8_t *q, size_t qlen,
+                            const uint8_t *t, size_t tlen, anvec *out) {
+    out->n = 0;
+    if (p->alphabet_len > 5) return;
+    mmvec qm = {0}, tm = {0};
+    sketch_into(q, qlen, p->k, p->w, &qm);
+    sketch_into(t, tlen, p->k, p->w, &tm);
+    size_t *qcov = repeat_coverage(qm.v, qm.n, qlen, p->tr_dist);
+    size_t *tcov = repeat_coverage(tm.v, tm.n, tlen, p->tr_dist);
+    size_t *qo = mm_order(qm.v, qm.n), *to = mm_order(tm.v, tm.n);
+    size_t a = 0, b = 0;
+    while (a < tm.n && b < qm.n) {
+        uint64_t tv = tm.v[to[a]].value, qv = qm.v[qo[b]].value;
+        if (tv < qv) {
+            a++;
+            continue;
+        }
+        if (qv < tv) {
+            b++;
+            continue;
+        }
+        size_t a1 = a, b1 = b;
+        while (a1 < tm.n && tm.v[to[a1]].value == tv) a1++;
+        while (b1 < qm.n && qm.v[qo[b1]].value == tv) b1++;
+        int keep = !(p->max_occ > 0 && (a1 - a > p->max_occ || b1 - b > p->max_occ));
+        for (size_t x = a; keep && x < a1; x++) {
+            const o_mm *tt = &tm.v[to[x]];
+            if (overlaps_repeat(tt, tcov)) continue;
+            for (size_t y = b; y < b1; y++) {
+                const o_mm *qq = &qm.v[qo[y]];
+                if (overlaps_repeat(qq, qcov)) continue;
+                o_anchor an = {tt->end, qq->end, p->k, 0};
+                an_push(out, an);
+            }
+        }
+        a = a1;
+        b = b1;
+    }
+    qsort(out->v, out->n, sizeof(o_anchor), anchor_cmp);
+    size_t w = 0;
+    for (size_t i = 0; i < out->n; i++)
+        if (w == 0 || anchor_cmp(&out->v[w - 1], &out->v[i]) != 0) out->v[w++] = out->v[i];
+    out->n = w;
+    free(qo);
+    free(to);
+    free(qcov);
+    free(tcov);
+    free(qm.v);
+    free(tm.v);
+}
+
+/* src/minimizer.rs:178-197; returns 0 for None. */
+static int chain_score(const o_anchor *a, const o_anchor *b, size_t k, int32_t *score) {
+    if (a->query_end < b->query_end || a->target_end < b->target_end) return 0;
+    size_t dq = a->query_end - b->query_end, dr = a->target_end - b->target_end;
+    if (dq == 0 || dq > 1000) return 0;
+    size_t delta = dr > dq ? dr - dq : dq - dr;
+    size_t distance = dr < dq ? dr : dq;
+    int32_t s = (int32_t)(a->span < distance ? a->span : distance);
+    if (delta > 0 || distance > a->span) {
+        if (dr == 0) {
+            s += 1;
+        } else {
+            double linear = 0.01 * (double)k * (double)delta;
+            double logarithmic = log2((double)(delta + 1));
+            s -= (int32_t)(linear < logarithmic ? linear : logarithmic);
+        }
+    }
+    *score = s;
+    return 1;
+}
+
+typedef struct {
+    size_t *v;
+    size_t n;
+} o_local;
+
+static const o_anchor *g_sort_an;
+static int local_cmp(const void *a, const void *b) {
+    const o_local *x = a, *y = b;
+    return anchor_cmp(&g_sort_an[x->v[x->n - 1]], &g_sort_an[y->v[y->n - 1]]);
+}
+
+typedef struct {
+    int32_t score;
+    size_t i;
+} o_term;
+static int term_cmp(const void *a, const void *b) {
+    const o_term *x = a, *y = b;
+    if (x->score != y->score) return x->score < y->score ? -1 : 1;
+    return x->i < y->i ? -1 : (x->i > y->i ? 1 : 0);
+}
+
+/* src/minimizer.rs:199-377 */
+static void chain(const o_anchor *an, size_t n, size_t k, anvec *out) {
+    out->n = 0;
+    if (n == 0) return;
+    const size_t NONE = (size_t)-1;
+    int32_t *scores = xcalloc(n, sizeof(int32_t));
+    size_t *parent = xmalloc(n * sizeof(size_t));
+    size_t *touched = xmalloc(n * sizeof(size_t));
+    for (size_t i = 0; i < n; i++) parent[i] = touched[i] = NONE;
+    size_t start = 0;
+    for (size_t i = 0; i < n; i++) {
+        while (start < i && sat_sub(an[i].target_end, an[start].target_end) > 1000) start++;
+        if (start < sat_sub(i, 5000)) start = sat_sub(i, 5000);
+        int32_t best = (int32_t)an[i].span;
+        size_t skip = 0, iterations = 0;
+        for (size_t j = i; j-- > start;) {
+            int32_t step;
+            if (!chain_score(&an[i], &an[j], k, &step)) continue;
+            int32_t cand = step + scores[j];
+            if (cand > best) {
+                best = cand;
+                parent[i] = j;
+                skip = sat_sub(skip, 1);
+            } else {
+                if (cand != best) {
+                    iterations++;
+                    if (iterations > 50) break;
+                }
+                if (touched[j] == i) {
+                    skip++;
+                    if (skip > 25) break;
+                }
+            }
+            if (parent[j] != NONE) touched[parent[j]] = i;
+        }
+        scores[i] = best;
+    }
+    uint8_t *has_child = xcalloc(n, 1);
+    for (size_t i = 0; i < n; i++)
+        if (parent[i] != NONE) has_child[parent[i]] = 1;
+    o_term *terms = xmalloc(n * sizeof(o_term));
+    size_t nt = 0;
+    for (size_t i = 0; i < n; i++)
+        if (!has_child[i] && scores[i] >= 100) {
+            terms[nt].score = scores[i];
+            terms[nt].i = i;
+            nt++;
+        }
+    qsort(terms, nt, sizeof(o_term), term_cmp);
+    uint8_t *used = xcalloc(n, 1);
+    o_local *locals = xmalloc((nt ? nt : 1) * sizeof(o_local));
+    size_t nl = 0;
+    size_t *idx = xmalloc(n * sizeof(size_t));
+    for (size_t r = nt; r-- > 0;) {
+        size_t end = terms[r].i;
+        if (used[end]) continue;
+        size_t cur = end, ni = 0, best_cut = NONE;
+        int32_t best_gain = 0;
+        while (cur != NONE) {
+            size_t i = cur;
+            if (used[i]) break;
+            idx[ni++] = i;
+            cur = parent[i];
+            int32_t gain = scores[end] - (cur == NONE ? 0 : scores[cur]);
+            if (gain > best_gain) {
+                best_gain = gain;
+                best_cut = cur;
+            } else if (best_gain - gain > 1000) {
+                cur = best_cut;
+                break;
+            }
+        }
+        if (cur != NONE)
+            while (ni > 0 && idx[ni - 1] <= cur) ni--;
+        for (size_t x = 0; x < ni; x++) used[idx[x]] = 1;
+        int32_t local_score = scores[end] - (cur == NONE ? 0 : scores[cur]);
+        if (ni >= 3 && local_score >= 100) {
+            o_local L;
+            L.n = ni;
+            L.v = xmalloc(ni * sizeof(size_t));
+            for (size_t x = 0; x < ni; x++) L.v[x] = idx[ni - 1 - x];
+            locals[nl++] = L;
+        }
+    }
+    if (nl > 0) {
+        g_sort_an = an;
+        qsort(locals, nl, sizeof(o_local), local_cmp);
+        int32_t *totals = xcalloc(nl, sizeof(int32_t));
+        size_t *previous = xmalloc(nl * sizeof(size_t));
+        size_t *trim = xcalloc(nl, sizeof(size_t));
+        for (size_t i = 0; i < nl; i++) {
+            previous[i] = NONE;
+            const o_local *L = &locals[i];
+            int32_t *prefix = xmalloc(L->n * sizeof(int32_t));
+            prefix[0] = (int32_t)an[L->v[0]].span;
+            for (size_t x = 1; x < L->n; x++) {
+                int32_t s = 0;
+                if (!chain_score(&an[L->v[x]], &an[L->v[x - 1]], k, &s)) s = 0;
+                prefix[x] = prefix[x - 1] + s;
+            }
+            totals[i] = prefix[L->n - 1];
+            const o_anchor *first = &an[L->v[0]];
+            for (size_t j = i; j-- > 0;) {
+                if (first->target_end < an[locals[j].v[0]].target_end) continue;
+                const o_anchor *lastj = &an[locals[j].v[locals[j].n - 1]];
+                size_t off = NONE;
+                for (size_t x = 0; x < L->n; x++)
+                    if (an[L->v[x]].target_end > lastj->target_end &&
+                        an[L->v[x]].query_end > lastj->query_end) {
+                        off = x;
+                        break;
+                    }
+                if (off == NONE) continue;
+                int32_t remaining = prefix[L->n - 1] - prefix[off] + (int32_t)an[L->v[off]].span;
+                int32_t cand = totals[j] + remaining;
+                if (cand > totals[i]) {
+                    totals[i] = cand;
+                    previous[i] = j;
+                    trim[i] = off;
+                }
+            }
+            free(prefix);
+        }
+        size_t best = 0;
+        for (size_t i = 1; i < nl; i++)
+            if (totals[i] > totals[best]) best = i;
+        size_t *ids = xmalloc(nl * sizeof(size_t));
+        size_t nc = 0;
+        for (size_t i = best; i != NONE; i = previous[i]) ids[nc++] = i;
+        for (size_t c = nc; c-- > 0;) {
+            const o_local *L = &locals[ids[c]];
+            for (size_t x = trim[ids[c]]; x < L->n; x++) {
+                const o_anchor *a = &an[L->v[x]];
+                if (out->n == 0 || (a->target_end > out->v[out->n - 1].target_end &&
+                                    a->query_end > out->v[out->n - 1].query_end))
+                    an_push(out, *a);
+            }
+        }
+        free(ids);
+        free(totals);
+        free(previous);
+        free(trim);
+    }
+    for (size_t i = 0; i < nl; i++) free(locals[i].v);
+    free(locals);
+    free(idx);
+    free(used);
+    free(terms);
+    free(has_child);
+    free(touched);
+    free(parent);
+    free(scores);
+}
+
+/* src/minimizer.rs:379-467 */
+static void partition_windows(const o_anchor *ch, size_t n, size_t tlen, size_t qlen,
+                              size_t max_drift, size_t max_win, anvec *out, int *tail) {
+    out->n = 0;
+    *tail = 0;
+#define TE(i) ((i) < 0 ? (int64_t)-1 : (int64_t)ch[(i)].target_end)
+#define QE(i) ((i) < 0 ? (int64_t)-1 : (int64_t)ch[(i)].query_end)
+#define TS(i) ((size_t)(i) == n ? (int64_t)tlen : (int64_t)ch[(i)].target_end - (int64_t)ch[(i)].span + 1)
+#define QS(i) ((size_t)(i) == n ? (int64_t)qlen : (int64_t)ch[(i)].query_end - (int64_t)ch[(i)].span + 1)
+#define DIAG(i) ((i) < 0 ? (int64_t)0 : ((size_t)(i) == n ? (int64_t)tlen - (int64_t)qlen : TE(i) - QE(i)))
+    int64_t s = -1;
+    while (s < (int64_t)n) {
+        int64_t boundary = -1;
+        for (int64_t i = s + 1; i <= (int64_t)n; i++) {
+            int64_t gt = TS(i) - TE(s) - 1;
+            int64_t gq = QS(i) - QE(s) - 1;
+            int64_t dd = DIAG(i) - DIAG(s);
+            uint64_t drift = (uint64_t)(dd < 0 ? -dd : dd);
+            int64_t g1 = TS(i) - TE(i - 1) - 1, g2 = QS(i) - QE(i - 1) - 1;
+            int64_t gap = g1 > g2 ? g1 : g2;
+            int64_t gm = gt > gq ? gt : gq;
+            if (drift > (uint64_t)max_drift || gap > (int64_t)max_drift || gm > (int64_t)max_win)
+                break;
+            if (gt >= 0 && gq >= 0) boundary = i;
+        }
+        int adaptive;
+        int64_t e;
+        if (boundary >= 0) {
+            adaptive = 0;
+            e = boundary;
+        } else {
+            e = s + 1;
+            while (e < (int64_t)n && (TS(e) - TE(s) - 1 < 0 || QS(e) - QE(s) - 1 < 0)) e++;
+            int64_t dd = DIAG(e) - DIAG(s);
+            uint64_t drift = (uint64_t)(dd < 0 ? -dd : dd);
+            int64_t g1 = TS(e) - TE(s) - 1, g2 = QS(e) - QE(s) - 1;
+            int64_t gap = g1 > g2 ? g1 : g2;
+            adaptive = drift > (uint64_t)max_drift || gap > (int64_t)max_drift;
+        }
+        if (e < (int64_t)n) {
+            o_anchor a = ch[e];
+            a.adaptive_before = adaptive;
+            an_push(out, a);
+        } else {
+            *tail = adaptive;
+        }
+        s = e;
+    }
+#undef TE
+#undef QE
+#undef TS
+#undef QS
+#undef DIAG
+}
+
+/* src/minimizer.rs:475-488 */
+static void anchor_windows(const op_params *p, const uint8_t *q, size_t qlen,
+                           const uint8_t *cons, size_t clen, anvec *out, int *tail) {
+    anvec m = {0}, c = {0};
+    collect_anchors(p, q, qlen, cons, clen, &m);
+    chain(m.v, m.n, p->k, &c);
+    size_t mw = qlen / 25 > 500 ? qlen / 25 : 500;
+    if (p->win_max < mw) mw = p->win_max;
+    partition_windows(c.v, c.n, clen, qlen, p->win_band, mw, out, tail);
+    free(m.v);
+    free(c.v);
+}
+
+static size_t copy_anchors(const anvec *a, o_anchor *out, size_t cap) {
+    for (size_t i = 0; i < a->n && i < cap; i++) out[i] = a->v[i];
+    return a->n;
+}
+
+size_t o_collect_anchors(const op_params *p, const uint8_t *q, size_t qlen, const uint8_t *t,
+                         size_t tlen, o_anchor *out, size_t cap) {
+    anvec a = {0};
+    collect_anchors(p, q, qlen, t, tlen, &a);
+    size_t n = copy_anchors(&a, out, cap);
+    free(a.v);
+    return n;
+}
+
+size_t o_chain(const o_anchor *anchors, size_t n, size_t k, o_anchor *out, size_t cap) {
+    anvec a = {0};
+    chain(anchors, n, k, &a);
+    size_t r = copy_anchors(&a, out, cap);
+    free(a.v);
+    return r;
+}
+
+size_t o_partition_windows(const o_anchor *chain_in, size_t n, size_t tlen, size_t qlen,
+                           size_t max_drift, size_t max_win, o_anchor *out, size_t cap,
+                           int *tail) {
+    anvec a = {0};
+    partition_windows(chain_in, n, tlen, qlen, max_drift, max_win, &a, tail);
+    size_t r = copy_anchors(&a, out, cap);
+    free(a.v);
+    return r;
+}
+
+size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const uint8_t *t,
+                        size_t tlen, o_anchor *out, size_t cap, int *tail) {
+    anvec a = {0};
+    anchor_windows(p, q, qlen, t, tlen, &a, tail);
+    size_t r = copy_anchors(&a, out, cap);
+    free(a.v);
+    return r;
+}
+
+/* src/align.rs:99-188: windows between chained anchors, each aligned by
+ * poa(beg, end) and followed by the anchor's own graph nodes. */
+static int seeded_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen,
+                        ob_base *out, size_t cap, size_t *out_len) {
+    size_t nn = g->n_nodes;
+    uint8_t *cons = xmalloc(nn + 1);
+    uint64_t *ids = xmalloc((nn + 1) * sizeof(uint64_t));
+    size_t clen = og_consensus(g, p, cons, ids, NULL, nn);
+    anvec an = {0};
+    int tail = 0;
+    anchor_windows(p, q, qlen, cons, clen, &an, &tail);
+    int rc = 0;
+    if (an.n == 0) {
+        rc = o_poa(p, g, 0, 1, q, qlen, 1, p->b, out, cap, out_len);
+        goto done;
+    }
+    /* Validate every anchor first: any invalid one means the whole query
+     * takes the full-graph fallback. */
+    size_t qbeg = 0, prev_te = (size_t)-1;
+    for (size_t a = 0; a < an.n; a++) {
+        const o_anchor *x = &an.v[a];
+        if (x->target_end + 1 < x->span) {
+            rc = fail("invalid minimizer target span");
+            goto done;
+        }
+        if (x->query_end + 1 < x->span) {
+            rc = fail("invalid minimizer query span");
+            goto done;
+        }
+        size_t ts = x->target_end + 1 - x->span, qs = x->query_end + 1 - x->span;
+        int valid = x->span > 0 && x->target_end < clen && x->query_end < qlen && qs >= qbeg &&
+                    (prev_te == (size_t)-1 || ts > prev_te);
+        for (size_t o = 0; valid && o < x->span; o++)
+            if (o_encode(p, cons[ts + o]) != o_encode(p, q[qs + o])) valid = 0;
+        if (!valid) {
+            rc = o_poa(p, g, 0, 1, q, qlen, 1, p->b, out, cap, out_len);
+            goto done;
+        }
+        qbeg = x->query_end + 1;
+        prev_te = x->target_end;
+    }
+    {
+        obvec res = {0};
+        ob_base *buf = NULL;
+        size_t bcap = 0;
+        size_t beg = 0;
+        qbeg = 0;
+        for (size_t a = 0; a <= an.n && rc == 0; a++) {
+            size_t end, qend, nanch = 0;
+            int adaptive;
+            const uint64_t *anch = NULL;
+            if (a < an.n) {
+                const o_anchor *x = &an.v[a];
+                size_t ts = x->target_end + 1 - x->span;
+                end = ids[ts];
+                qend = x->query_end + 1 - x->span;
+                adaptive = x->adaptive_before;
+                anch = ids + ts;
+                nanch = x->target_end - ts;
+            } else {
+                end = 1;
+                qend = qlen;
+                adaptive = tail;
+            }
+            size_t need = qend - qbeg + 8;
+            if (need > bcap) {
+                bcap = 2 * need;
+                buf = xrealloc(buf, bcap * sizeof(ob_base));
+            }
+            size_t len = 0;
+            rc = o_poa(p, g, beg, end, q + qbeg, qend - qbeg, adaptive,
+                       adaptive ? p->b : p->win_band, buf, bcap, &len);
+            if (rc) break;
+            if (len > bcap) {
+                rc = fail("oracle path buffer overflow");
+                break;
+            }
+            for (size_t k = 0; k < len; k++) ob_push(&res, buf[k].node, buf[k].base, buf[k].aligned);
+            for (size_t k = 0; k < nanch; k++)
+                ob_push(&res, (int64_t)anch[k], (int64_t)g->nodes[anch[k]].base, -1);
+            if (a < an.n) {
+                beg = ids[an.v[a].target_end];
+                qbeg = an.v[a].query_end + 1;
+            }
+        }
+        if (rc == 0) rc = emit(&res, out, cap, out_len);
+        free(buf);
+        free(res.v);
+    }
+done:
+    free(an.v);
+    free(cons);
+    free(ids);
+    return rc;
+}
+
+/* src/align.rs:81-188 (empty graph, unseeded and seeded branches). */
 int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen, ob_base *out,
             size_t cap, size_t *out_len) {
     if (g->n_nodes <= 2) {
@@ -1322,7 +2024,7 @@ int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen
         return 0;
     }
     if (!p->seeding) return o_poa(p, g, 0, 1, q, qlen, p->adaptive, p->b, out, cap, out_len);
-    return fail("minimizer seeding is not restated in the oracle");
+    return seeded_align(p, g, q, qlen, out, cap, out_len);
 }
 
 /* src/lib.rs:184-204 (input order). */

@@ -82,12 +82,43 @@ void o_insertion_prefix(const int32_t md[8], int32_t carry, int32_t open, int32_
 int o_i32_scores_safe(const op_params *p, const og_graph *g, size_t beg_id, size_t end_id,
                       size_t query_len);
 
+/* src/minimizer.rs:20-26 Anchor (adaptive_before 0/1). */
+typedef struct {
+    uint64_t target_end, query_end, span;
+    int64_t adaptive_before;
+} o_anchor;
+
+/*
+ * Minimizer seeding, src/minimizer.rs:32-488. Each returns the full count and
+ * writes up to `cap` entries. Minimizer positions are parity unpinned (the
+ * reference ranks k-mers inside the simd-minimizers crate).
+ */
+/* src/minimizer.rs:32-76: value = exact 2-bit k-mer, end = last base. */
+size_t o_sketch(const uint8_t *seq, size_t len, size_t k, size_t w, uint64_t *value,
+                uint64_t *end, size_t cap);
+/* src/minimizer.rs:78-110 */
+size_t o_repeat_intervals(const uint64_t *value, const uint64_t *end, const uint64_t *span,
+                          size_t n, size_t len, size_t max_dist, uint64_t *out_start,
+                          uint64_t *out_end, size_t cap);
+/* src/minimizer.rs:128-176 */
+size_t o_collect_anchors(const op_params *p, const uint8_t *q, size_t qlen, const uint8_t *t,
+                         size_t tlen, o_anchor *out, size_t cap);
+/* src/minimizer.rs:199-377 */
+size_t o_chain(const o_anchor *anchors, size_t n, size_t k, o_anchor *out, size_t cap);
+/* src/minimizer.rs:379-467 */
+size_t o_partition_windows(const o_anchor *chain, size_t n, size_t tlen, size_t qlen,
+                           size_t max_drift, size_t max_win, o_anchor *out, size_t cap,
+                           int *tail);
+/* src/minimizer.rs:475-488 */
+size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const uint8_t *t,
+                        size_t tlen, o_anchor *out, size_t cap, int *tail);
+
 /*
  * All alignment entry points write up to `cap` AlignedBase records to `out`,
  * set *out_len (the full length, even if it exceeds cap) and return 0, or
  * return 1 with the reference's error message in o_last_error().
  */
-/* src/align.rs:81-188 (unseeded and empty-graph branches). */
+/* src/align.rs:81-188 (empty-graph, unseeded and seeded branches). */
 int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen, ob_base *out,
             size_t cap, size_t *out_len);
 /* src/align.rs:192-211 */

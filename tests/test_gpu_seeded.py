@@ -1,0 +1,157 @@
+"""Seeded alignment on the device: DPs anchored between interior graph nodes,
+and the windows between chained minimizer anchors (src/align.rs:99-188,
+src/minimizer.rs), through the one-call path (mpoa_align / align_sequences:
+all windows of a read in one launch) and the engine (batch and server: the
+windows of every ready job share launches). Paths and rendered bytes equal
+the oracle's.
+
+Reference cases: src/align.rs:1525-1559, tests/avx512.rs:145-180.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import minipoa_hip as mh
+from helpers import Lcg, dna, validate_gfa, validate_msa
+from minipoa_hip import Params, records, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mh.Context(0)
+    yield c
+    c.close()
+
+
+def fnv1a(data, h=14695981039346656037):
+    for b in data:
+        h ^= b
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_interior_anchored_poa_matches_oracle(ctx, oracle):
+    """poa(beg, end) between interior nodes of a branched graph, the DP every
+    seeded window runs: path or error equal to the oracle's, for static,
+    adaptive and full bands."""
+    rng = random.Random(77)
+    params0 = Params()
+    tmpl = np.frombuffer(dna(700), dtype=np.uint8) % 4
+    reads = [synth.mutate(np.random.Generator(np.random.PCG64(s)), tmpl, 0.08) for s in range(6)]
+    og = oracle.align_sequences(params0, records(reads))
+    view = mh.view_from_graph(og, len(params0.alphabet))
+    cons = og.consensus(params0)
+    ids = cons[1]
+    assert len(ids) > 600
+    n = 0
+    for params, adaptive, band in [(Params(), False, 30), (Params(), True, 100), (Params(f=0), False, 100),
+                                   (Params(b=2), False, 2)]:
+        for _ in range(25):
+            i = rng.randrange(0, len(ids) - 50)
+            j = rng.randrange(i + 1, min(len(ids), i + 400))
+            beg, end = ids[i], ids[j]
+            q = bytes(reads[rng.randrange(6)][i:i + rng.randint(0, j - i + 20)])
+            want = oracle.result_or_error(oracle.poa, params, og, beg, end, q, adaptive, band)
+            got = mh.result_or_error(mh.poa, ctx, params, view, beg, end, q, adaptive, band)
+            assert got[0] == want[0], (beg, end, len(q), got, want)
+            if want[0] == "ok":
+                assert [tuple(x) for x in got[1]] == [tuple(x) for x in want[1]]
+                n += 1
+    assert n > 60
+
+
+def test_seeded_windows_on_a_linear_graph(ctx, oracle):
+    """src/align.rs:1525-1559: more than two windows; path == oracle's."""
+    p = Params(seeding=True, k=7, w=4, tr_dist=0, win_max=80, win_band=20)
+    r = Lcg(123981)
+    reference = bytes(b"ACGT"[r.next() % 4] for _ in range(2400))
+    q = bytearray(reference)
+    q[800:800] = b"TGCATGC"
+    del q[1600:1605]
+    q[2000] = ord("C") if q[2000] == ord("A") else ord("A")
+    q = bytes(q)
+    og = oracle.Graph(p)
+    og.add_path(0, oracle.align(p, og, reference))
+    og.toposort(False)
+    assert len(mh.anchor_windows(p, q, reference)[0]) > 2
+    want = oracle.align(p, og, q)
+    got = mh.align(ctx, p, mh.view_from_graph(og, len(p.alphabet)), q)
+    assert len(got) == len(q) + 1
+    assert [tuple(x) for x in got] == [tuple(x) for x in want]
+
+
+def test_seeded_workflow_bytes(ctx, oracle):
+    """tests/avx512.rs:145-180 (progressive=false) on the device."""
+    reference = dna(1409)
+    insertion = reference[:417] + dna(25) + reference[417:]
+    deletion = reference[:803] + reference[820:]
+    reads = records([reference, insertion, deletion])
+    p = Params(seeding=True, k=7, w=3, tr_dist=0, win_max=80, win_band=24)
+    g = mh.align_sequences(ctx, p, reads)
+    og = oracle.align_sequences(p, reads)
+    assert g.consensus(p)[0] == og.consensus(p)[0]
+    msa, gfa = g.msa(p, reads), g.gfa(p, reads)
+    assert msa == og.msa(p, reads)
+    assert gfa == og.gfa(p, reads)
+    validate_msa(msa, reads)
+    validate_gfa(gfa, reads)
+
+
+SEEDED_SETTINGS = [
+    Params(seeding=True),                                              # defaults: k19 w10, 500-base windows
+    Params(seeding=True, k=15, w=5, win_band=50, win_max=300),         # many short windows
+    Params(seeding=True, k=11, w=3, win_band=20, win_max=120, tr_dist=0, adaptive=True),
+    Params(seeding=True, k=15, w=5, win_band=40, win_max=200, f=0),    # full-row windows
+]
+
+
+@pytest.mark.parametrize("si", range(len(SEEDED_SETTINGS)))
+def test_seeded_batch_matches_oracle_bytes(ctx, oracle, si):
+    """Seeded jobs (and one job of unrelated reads, which has no chain and
+    takes the whole-graph fallback) through the batch engine."""
+    p = SEEDED_SETTINGS[si]
+    jo, ro, b = synth.seeded(seed=5 + si, njobs=5, length=2500, nreads=7, trim=200)
+    rng = np.random.Generator(np.random.PCG64(99))
+    unrelated = [synth.ACGT[rng.integers(0, 4, size=900)].tobytes() for _ in range(3)]
+    per_job = [synth.job_reads(jo, ro, b, j) for j in range(len(jo) - 1)] + [unrelated]
+    jo, ro, b = synth._pack(per_job)
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    for j, reads in enumerate(per_job):
+        seqs = records(reads)
+        og = oracle.align_sequences(p, seqs)
+        assert res.status(j) == 0, mh.last_error()
+        assert res.output(j, 0) == ">consensus sequence\n" + og.consensus(p)[0].decode() + "\n"
+        assert res.output(j, 1) == og.msa(p, seqs)
+        assert res.output(j, 2) == og.gfa(p, seqs)
+
+
+def test_seeded_server_jobs_match_oracle(ctx, oracle):
+    p = Params(seeding=True, k=15, w=5, win_band=60, win_max=300)
+    jo, ro, b = synth.seeded(seed=31, njobs=4, length=1800, nreads=6, trim=150)
+    per_job = [synth.job_reads(jo, ro, b, j) for j in range(len(jo) - 1)]
+    with mh.Server(ctx, p) as srv:
+        tickets = [srv.submit(reads) for reads in per_job]
+        for t, reads in zip(tickets, per_job):
+            srv.wait(t)
+            assert srv.status(t) == 0
+            seqs = records(reads)
+            og = oracle.align_sequences(p, seqs)
+            assert srv.output(t, 1) == og.msa(p, seqs)
+            assert srv.output(t, 2) == og.gfa(p, seqs)
+
+
+def test_seeded_long_read_sample_matches_oracle_hash(ctx, oracle):
+    """The seeded long-read workload (16.5 kb template, reads at 10% with
+    up to 500 bases trimmed at each end), first 12 reads: consensus + MSA
+    hash equal to the oracle's."""
+    jo, ro, b = synth.seeded(seed=20263, njobs=1, nreads=12)
+    p = Params(seeding=True)
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    cells, want = oracle.batch_run(p, jo, ro, b, 1)
+    got = fnv1a((res.output(0, 0) + res.output(0, 1)).encode())
+    assert want[0] != 0
+    assert got == int(want[0])
+    assert res.stats()["cells"] == cells
