@@ -16,6 +16,9 @@ Besides the headline workload, `legs` reports the other workloads once each,
 every one checked byte for byte against the oracle:
   c1  plain unseeded consensus: one 2,000 bp template, 10 reads at 10%,
       one DP per launch (the drop-in path), beside the oracle on one core;
+  c3  seeded long-read consensus (-S -r2): one 16,500 bp template, 100 reads
+      at 10% trimmed by up to 500 bases at each end; each read aligns as
+      ~28 minimizer-anchored windows that share one launch per step;
   c4  protein POA: 2,000 families over a 20-letter alphabet with a drawn
       symmetric matrix, 16 members at 30% divergence, full DP (-f0);
   c5  serving: Poisson arrivals of DNA jobs (template log-uniform in
@@ -106,8 +109,8 @@ def main():
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (counter scaling)")
-    ap.add_argument("--legs", default="c1,c4,c5",
-                    help="other workloads to report once each (c1, c4, c5; empty = none)")
+    ap.add_argument("--legs", default="c1,c3,c4,c5",
+                    help="other workloads to report once each (c1, c3, c4, c5; empty = none)")
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=0, help="serving launch slots (0 = library default)")
@@ -234,6 +237,8 @@ def main():
         if "c5" in legs:
             out["legs"]["c5"] = leg_c5(ctx, oracle, cores, args.serve_jobs, args.serve_rate,
                                        args.serve_slots)
+        if "c3" in legs:
+            out["legs"]["c3"] = leg_c3(ctx, oracle)
         # One DP per launch on the context's own stream last: after that
         # stream has run, the batched legs on the same context measured
         # 40% slower (C4 690 ms instead of 400-450 ms).
@@ -287,6 +292,41 @@ def leg_c1(ctx, oracle):
                         "one DP per launch",
             "cells": cells, "gpu_ms": round(dt * 1e3, 2), "gpu_gcups": round(cells / dt / 1e9, 4),
             "cpu_ms": round(ct * 1e3, 2), "cpu_gcups": round(cells / ct / 1e9, 4), "cpu_cores": 1,
+            "cpu_kind": "port", "identical": bool(same)}
+
+
+def leg_c3(ctx, oracle):
+    """Seeded long-read consensus (-S -r2): one job through the engine; every
+    read's anchored windows run side by side in one launch per step."""
+    import minipoa_hip as mh
+    from minipoa_hip import Params, synth
+
+    jo, ro, b = synth.seeded()
+    p = Params(seeding=True)
+    r = mh.batch_run(ctx, p, jo, ro, b)  # warm
+    del r
+    t = time.perf_counter()
+    r = mh.batch_run(ctx, p, jo, ro, b)
+    dt = time.perf_counter() - t
+    st = r.stats()
+    text = r.output(0, 0) + r.output(0, 1)
+    gfa = r.output(0, 2)
+    t = time.perf_counter()
+    cells, want = oracle.batch_run(p, jo, ro, b, 1)
+    ct = time.perf_counter() - t
+    h = 14695981039346656037
+    for c in text.encode():
+        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    same = h == int(want[0]) and st["cells"] == cells
+    nreads = int(jo[1] - jo[0])
+    log(f"c3: gpu {dt*1e3:.0f} ms {st['cells']/dt/1e9:.2f} GCUPS, {st['dp_calls']} DPs; "
+        f"oracle {ct:.1f}s on 1 core; identical {same}")
+    return {"workload": "seeded long-read consensus: 16500 bp template, 100 reads at 10% error, "
+                        "ends trimmed U[0,500], minimizer windows (k19 w10), -r2",
+            "cells": st["cells"], "dp_calls": st["dp_calls"], "windows_per_read": round(st["dp_calls"] / (nreads - 1), 1),
+            "gpu_ms": round(dt * 1e3, 1), "gpu_gcups": round(st["cells"] / dt / 1e9, 3),
+            "ms_per_read": round(dt * 1e3 / nreads, 2), "gfa_bytes": len(gfa),
+            "cpu_ms": round(ct * 1e3, 1), "cpu_gcups": round(cells / ct / 1e9, 3), "cpu_cores": 1,
             "cpu_kind": "port", "identical": bool(same)}
 
 

@@ -50,6 +50,36 @@ void TaskQueue::loop() {
     }
 }
 
+void par_for(TaskQueue &tq, size_t n, const std::function<void(size_t)> &f) {
+    if (n == 0) return;
+    struct State {
+        std::atomic<size_t> next{0}, done{0};
+        size_t n = 0;
+        const std::function<void(size_t)> *f = nullptr;
+        std::mutex mu;
+        std::condition_variable cv;
+    };
+    auto st = std::make_shared<State>();
+    st->n = n;
+    st->f = &f;
+    auto run = [st] {
+        for (;;) {
+            const size_t i = st->next++;
+            if (i >= st->n) return;
+            (*st->f)(i);  // f outlives every claimed index: the caller waits for them
+            if (++st->done == st->n) {
+                std::lock_guard<std::mutex> lk(st->mu);
+                st->cv.notify_all();
+            }
+        }
+    };
+    const size_t helpers = std::min<size_t>(tq.size() > 1 ? tq.size() - 1 : 0, n - 1);
+    for (size_t k = 0; k < helpers; k++) tq.push(run, true);
+    run();
+    std::unique_lock<std::mutex> lk(st->mu);
+    st->cv.wait(lk, [&] { return st->done.load() == st->n; });
+}
+
 // One DP of a job's step: the whole-graph DP, or one seeded window.
 struct Engine::Dp {
     Job *job = nullptr;
@@ -75,6 +105,9 @@ struct Engine::Job {
     std::vector<std::unique_ptr<Dp>> dp;  // the step's DPs are dp[0 .. ndp)
     uint32_t ndp = 0;
     std::vector<SeedWindow> win;          // the step's seeded windows
+    bool seeded_ahead = false;            // win (or seed_rc) already holds the next step's windows
+    int seed_rc = MPOA_OK;
+    std::string seed_err;
     uint32_t pending = 0;    // DPs of the step on the device (under the engine lock)
     uint64_t prev_used = 0, prev_n = 0;
     uint64_t cost = 0;       // expected cells of the step
@@ -359,11 +392,30 @@ void Engine::fail_job(Job &J, int status, const std::string &e) {
     finish(J);
 }
 
-// graph.rs:82-178 per read; false when the job ended with an error.
+// graph.rs:82-178 per read; false when the job ended with an error. With
+// seeding, the next read's windows (src/align.rs:99-162: consensus, anchors,
+// chain) are computed while the graph is re-sorted: seeding reads edges and
+// bases, the sort writes only ranks.
 bool Engine::fuse(Job &J) {
     Graph &g = *J.out.graph;
     std::string e = g.add_path(J.in.first + J.step, J.path.data(), J.path.size());
-    if (e.empty()) e = g.toposort(false);
+    J.seeded_ahead = false;
+    const uint32_t next = J.step + 1;
+    if (e.empty() && p_.seeding && next < J.in.nreads && g.size() > 2) {
+        const uint8_t *q = J.in.bases + J.in.read_off[next];
+        const uint64_t qlen = J.in.read_off[next + 1] - J.in.read_off[next];
+        par_for(tasks_, 2, [&](size_t i) {
+            if (i == 0) {
+                e = g.toposort(false);
+            } else {
+                J.seed_err.clear();
+                J.seed_rc = seed_windows(p_, GraphAcc{g}, q, qlen, J.win, J.seed_err);
+            }
+        });
+        J.seeded_ahead = true;
+    } else if (e.empty()) {
+        e = g.toposort(false);
+    }
     if (!e.empty()) {
         fail_job(J, MPOA_ERR_GRAPH, e);
         return false;
@@ -462,14 +514,18 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
     Graph &g = *J.out.graph;
     const uint8_t *q = J.in.bases + J.in.read_off[J.step];
     const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
-    J.win.clear();
     if (p_.seeding) {
-        std::string err;
-        const int rc = seed_windows(p_, GraphAcc{g}, q, qlen, J.win, err);
-        if (rc != MPOA_OK) {
-            fail_job(J, rc, err);
+        if (!J.seeded_ahead) {
+            J.seed_err.clear();
+            J.seed_rc = seed_windows(p_, GraphAcc{g}, q, qlen, J.win, J.seed_err);
+        }
+        J.seeded_ahead = false;
+        if (J.seed_rc != MPOA_OK) {
+            fail_job(J, J.seed_rc, J.seed_err);
             return false;
         }
+    } else {
+        J.win.clear();
     }
     J.ndp = uint32_t(std::max<size_t>(1, J.win.size()));
     while (J.dp.size() < J.ndp) {
@@ -479,19 +535,25 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
     }
     launch.clear();
     J.cost = 0;
-    for (uint32_t k = 0; k < J.ndp; k++) {
+    auto plan_one = [&](size_t k) {
         Dp &D = *J.dp[k];
         D.retry = false;
         D.fresh = false;
         D.status = MPOA_OK;
         D.err.clear();
-        JobPlan &P = D.plan;
         if (J.win.empty()) {
-            plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, P);
+            plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, D.plan);
         } else {
             const SeedWindow &w = J.win[k];
-            plan_job(p_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, w.adaptive, w.band, 0, P);
+            plan_job(p_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, w.adaptive, w.band, 0, D.plan);
         }
+    };
+    // Windows plan independently (each writes its own plan only).
+    if (J.ndp > 1) par_for(tasks_, J.ndp, plan_one);
+    else plan_one(0);
+    for (uint32_t k = 0; k < J.ndp; k++) {
+        Dp &D = *J.dp[k];
+        JobPlan &P = D.plan;
         if (P.status != MPOA_OK) {
             D.fresh = true;
             continue;

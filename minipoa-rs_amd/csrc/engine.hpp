@@ -51,6 +51,11 @@ private:
     bool stop_ = false;
 };
 
+// Runs f(0) .. f(n-1) on the calling thread and on idle workers of `tq`, and
+// returns once every index ran. Helpers that start after the indices are
+// taken return at once, so the caller never waits on a queued task.
+void par_for(TaskQueue &tq, size_t n, const std::function<void(size_t)> &f);
+
 // One job: reads read_off[k] .. read_off[k+1] of `bases`, k < nreads, aligned
 // progressively into `graph` (a new graph when null). `first` is the number
 // of sequences already in the graph (extend_graph's `existing`): read k gets
