@@ -238,7 +238,7 @@ def main():
             out["legs"]["c5"] = leg_c5(ctx, oracle, cores, args.serve_jobs, args.serve_rate,
                                        args.serve_slots)
         if "c3" in legs:
-            out["legs"]["c3"] = leg_c3(ctx, oracle)
+            out["legs"]["c3"] = leg_c3(ctx, oracle, cores)
         # One DP per launch on the context's own stream last: after that
         # stream has run, the batched legs on the same context measured
         # 40% slower (C4 690 ms instead of 400-450 ms).
@@ -295,7 +295,7 @@ def leg_c1(ctx, oracle):
             "cpu_kind": "port", "identical": bool(same)}
 
 
-def leg_c3(ctx, oracle):
+def leg_c3(ctx, oracle, cores):
     """Seeded long-read consensus (-S -r2): one job through the engine; every
     read's anchored windows run side by side in one launch per step."""
     import minipoa_hip as mh
@@ -314,19 +314,26 @@ def leg_c3(ctx, oracle):
     t = time.perf_counter()
     cells, want = oracle.batch_run(p, jo, ro, b, 1)
     ct = time.perf_counter() - t
+    # The reference's -t N: the windows of each read on N threads
+    # (src/parallel.rs:46-67), results in window order.
+    pt = Params(seeding=True, threads=cores)
+    t = time.perf_counter()
+    cells_t, want_t = oracle.batch_run(pt, jo, ro, b, 1)
+    ctt = time.perf_counter() - t
     h = 14695981039346656037
     for c in text.encode():
         h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
-    same = h == int(want[0]) and st["cells"] == cells
+    same = h == int(want[0]) == int(want_t[0]) and st["cells"] == cells == cells_t
     nreads = int(jo[1] - jo[0])
     log(f"c3: gpu {dt*1e3:.0f} ms {st['cells']/dt/1e9:.2f} GCUPS, {st['dp_calls']} DPs; "
-        f"oracle {ct:.1f}s on 1 core; identical {same}")
+        f"oracle {ct:.1f}s on 1 core, {ctt:.1f}s with -t {cores}; identical {same}")
     return {"workload": "seeded long-read consensus: 16500 bp template, 100 reads at 10% error, "
                         "ends trimmed U[0,500], minimizer windows (k19 w10), -r2",
             "cells": st["cells"], "dp_calls": st["dp_calls"], "windows_per_read": round(st["dp_calls"] / (nreads - 1), 1),
             "gpu_ms": round(dt * 1e3, 1), "gpu_gcups": round(st["cells"] / dt / 1e9, 3),
             "ms_per_read": round(dt * 1e3 / nreads, 2), "gfa_bytes": len(gfa),
             "cpu_ms": round(ct * 1e3, 1), "cpu_gcups": round(cells / ct / 1e9, 3), "cpu_cores": 1,
+            "cpu_threads_ms": round(ctt * 1e3, 1), "cpu_threads": cores,
             "cpu_kind": "port", "identical": bool(same)}
 
 

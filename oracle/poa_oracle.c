@@ -1910,8 +1910,63 @@ size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const
     return r;
 }
 
+/* One seeded window (src/align.rs:104-111) and its outcome. */
+typedef struct {
+    size_t beg, end, qbeg, qend;
+    int adaptive;
+    const uint64_t *anchor;
+    size_t nanchor;
+    int rc;
+    char err[512];
+    obvec path;
+    uint64_t cells;
+} o_window;
+
+typedef struct {
+    const op_params *p;
+    const og_graph *g;
+    const uint8_t *q;
+    o_window *w;
+    size_t n;
+    atomic_size_t next;
+} o_winpool;
+
+/* src/align.rs:162-181 `process`: poa over the window, then its anchors. */
+static void process_window(const op_params *p, const og_graph *g, const uint8_t *q, o_window *w) {
+    size_t qn = w->qend - w->qbeg;
+    size_t cap = qn + 8;
+    ob_base *buf = xmalloc(cap * sizeof(ob_base));
+    size_t len = 0;
+    uint64_t c0 = g_cells;
+    w->rc = o_poa(p, g, w->beg, w->end, q + w->qbeg, qn, w->adaptive,
+                  w->adaptive ? p->b : p->win_band, buf, cap, &len);
+    w->cells = g_cells - c0;
+    if (w->rc == 0 && len > cap) w->rc = fail("oracle path buffer overflow");
+    if (w->rc) {
+        snprintf(w->err, sizeof w->err, "%s", g_err);
+    } else {
+        for (size_t k = 0; k < len; k++) ob_push(&w->path, buf[k].node, buf[k].base, buf[k].aligned);
+        for (size_t k = 0; k < w->nanchor; k++)
+            ob_push(&w->path, (int64_t)w->anchor[k], (int64_t)g->nodes[w->anchor[k]].base, -1);
+    }
+    free(buf);
+}
+
+static void *window_worker(void *arg) {
+    o_winpool *wp = arg;
+    for (;;) {
+        size_t i = atomic_fetch_add(&wp->next, 1);
+        if (i >= wp->n) break;
+        process_window(wp->p, wp->g, wp->q, &wp->w[i]);
+    }
+    o_arena_clear();
+    return NULL;
+}
+
 /* src/align.rs:99-188: windows between chained anchors, each aligned by
- * poa(beg, end) and followed by the anchor's own graph nodes. */
+ * poa(beg, end) and followed by the anchor's own graph nodes. With
+ * p->threads > 1 the windows run on that many threads, results in window
+ * order and the first error in window order (src/parallel.rs:46-67). */
 static int seeded_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen,
                         ob_base *out, size_t cap, size_t *out_len) {
     size_t nn = g->n_nodes;
@@ -1922,6 +1977,8 @@ static int seeded_align(const op_params *p, const og_graph *g, const uint8_t *q,
     int tail = 0;
     anchor_windows(p, q, qlen, cons, clen, &an, &tail);
     int rc = 0;
+    o_window *w = NULL;
+    size_t nw = 0;
     if (an.n == 0) {
         rc = o_poa(p, g, 0, 1, q, qlen, 1, p->b, out, cap, out_len);
         goto done;
@@ -1951,55 +2008,69 @@ static int seeded_align(const op_params *p, const og_graph *g, const uint8_t *q,
         qbeg = x->query_end + 1;
         prev_te = x->target_end;
     }
+    nw = an.n + 1;
+    w = xcalloc(nw, sizeof(o_window));
     {
-        obvec res = {0};
-        ob_base *buf = NULL;
-        size_t bcap = 0;
         size_t beg = 0;
         qbeg = 0;
-        for (size_t a = 0; a <= an.n && rc == 0; a++) {
-            size_t end, qend, nanch = 0;
-            int adaptive;
-            const uint64_t *anch = NULL;
+        for (size_t a = 0; a < nw; a++) {
+            o_window *x = &w[a];
+            x->beg = beg;
+            x->qbeg = qbeg;
             if (a < an.n) {
-                const o_anchor *x = &an.v[a];
-                size_t ts = x->target_end + 1 - x->span;
-                end = ids[ts];
-                qend = x->query_end + 1 - x->span;
-                adaptive = x->adaptive_before;
-                anch = ids + ts;
-                nanch = x->target_end - ts;
+                const o_anchor *y = &an.v[a];
+                size_t ts = y->target_end + 1 - y->span;
+                x->end = ids[ts];
+                x->qend = y->query_end + 1 - y->span;
+                x->adaptive = (int)y->adaptive_before;
+                x->anchor = ids + ts;
+                x->nanchor = y->target_end - ts;
+                beg = ids[y->target_end];
+                qbeg = y->query_end + 1;
             } else {
-                end = 1;
-                qend = qlen;
-                adaptive = tail;
-            }
-            size_t need = qend - qbeg + 8;
-            if (need > bcap) {
-                bcap = 2 * need;
-                buf = xrealloc(buf, bcap * sizeof(ob_base));
-            }
-            size_t len = 0;
-            rc = o_poa(p, g, beg, end, q + qbeg, qend - qbeg, adaptive,
-                       adaptive ? p->b : p->win_band, buf, bcap, &len);
-            if (rc) break;
-            if (len > bcap) {
-                rc = fail("oracle path buffer overflow");
-                break;
-            }
-            for (size_t k = 0; k < len; k++) ob_push(&res, buf[k].node, buf[k].base, buf[k].aligned);
-            for (size_t k = 0; k < nanch; k++)
-                ob_push(&res, (int64_t)anch[k], (int64_t)g->nodes[anch[k]].base, -1);
-            if (a < an.n) {
-                beg = ids[an.v[a].target_end];
-                qbeg = an.v[a].query_end + 1;
+                x->end = 1;
+                x->qend = qlen;
+                x->adaptive = tail;
             }
         }
+    }
+    {
+        size_t workers = p->threads > 1 ? (p->threads < nw ? p->threads : nw) : 1;
+        if (workers <= 1) {
+            /* Sequential map stops at the first error. */
+            for (size_t a = 0; a < nw; a++) {
+                process_window(p, g, q, &w[a]);
+                if (w[a].rc) break;
+            }
+        } else {
+            o_winpool wp;
+            wp.p = p;
+            wp.g = g;
+            wp.q = q;
+            wp.w = w;
+            wp.n = nw;
+            atomic_init(&wp.next, 0);
+            pthread_t *th = xmalloc(workers * sizeof(pthread_t));
+            for (size_t t = 0; t < workers; t++) pthread_create(&th[t], NULL, window_worker, &wp);
+            for (size_t t = 0; t < workers; t++) pthread_join(th[t], NULL);
+            free(th);
+            for (size_t a = 0; a < nw; a++) g_cells += w[a].cells;
+        }
+        obvec res = {0};
+        for (size_t a = 0; a < nw && rc == 0; a++) {
+            if (w[a].rc) {
+                rc = fail("%s", w[a].err);
+                break;
+            }
+            for (size_t k = 0; k < w[a].path.n; k++)
+                ob_push(&res, w[a].path.v[k].node, w[a].path.v[k].base, w[a].path.v[k].aligned);
+        }
         if (rc == 0) rc = emit(&res, out, cap, out_len);
-        free(buf);
         free(res.v);
     }
 done:
+    for (size_t a = 0; a < nw; a++) free(w[a].path.v);
+    free(w);
     free(an.v);
     free(cons);
     free(ids);

@@ -118,7 +118,9 @@ size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const
  * set *out_len (the full length, even if it exceeds cap) and return 0, or
  * return 1 with the reference's error message in o_last_error().
  */
-/* src/align.rs:81-188 (empty-graph, unseeded and seeded branches). */
+/* src/align.rs:81-188 (empty-graph, unseeded and seeded branches). Seeded
+ * windows run on p->threads threads when it is above 1, results in window
+ * order (src/parallel.rs:46-67). */
 int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen, ob_base *out,
             size_t cap, size_t *out_len);
 /* src/align.rs:192-211 */
