@@ -173,6 +173,12 @@ int mpoa_chain(const mpoa_anchor *anchors, uint64_t n, uint64_t k, mpoa_anchor *
 int mpoa_partition_windows(const mpoa_anchor *chain, uint64_t n, uint64_t tlen, uint64_t qlen,
                            uint64_t max_drift, uint64_t max_win, mpoa_anchor *out, uint64_t cap,
                            uint64_t *out_len, int *tail_adaptive);
+/* src/minimizer.rs:489-578 `guide_order`: order[0..nseq) is the -p
+ * progressive order (input order unless params.progressive, nseq >= 2 and a
+ * DNA alphabet). mpoa_align_sequences, mpoa_extend_graph (existing == 0),
+ * batch jobs and served jobs without a graph align in this order. */
+int mpoa_guide_order(const mpoa_params *params, const uint8_t *const *seqs, const uint64_t *lens,
+                     uint64_t nseq, uint64_t *order);
 /* src/minimizer.rs:475-488 `anchor_windows`. */
 int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
                         const uint8_t *consensus, uint64_t clen, mpoa_anchor *out, uint64_t cap,
@@ -214,7 +220,8 @@ int mpoa_params_validate(const mpoa_params *params);
  * text returns MPOA_ERR_INVALID_PARAMS and writes nothing. */
 int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t *matrix, uint32_t cap,
                      uint32_t *alphabet_len);
-/* src/lib.rs:177-204 `align_sequences` (input order) on the device. */
+/* src/lib.rs:177-204 `align_sequences` (input order, or the guide order with
+ * params.progressive) on the device. */
 int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
                          const uint64_t *lens, uint64_t nseq, mpoa_graph **out);
 /* src/lib.rs:184-204 `extend_graph` from sequence index `existing` (the -i

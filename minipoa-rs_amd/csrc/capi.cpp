@@ -544,10 +544,16 @@ int mpoa_extend_graph(mpoa_ctx *ctx, const mpoa_params *params, mpoa_graph *g,
     Params p;
     int rc = MPOA_OK;
     if (!check_params(params, p, rc)) return rc;
-    if (p.progressive && existing == 0)
-        return fail(MPOA_ERR_UNSUPPORTED, "guide-tree order (-p) is not implemented by this backend");
+    // src/lib.rs:189-193: the guide order with -p on a new graph, else input
+    // order; the read keeps its input index as sequence id.
+    std::vector<uint32_t> order;
+    if (p.progressive && existing == 0) {
+        if (nseq > UINT32_MAX) return fail(MPOA_ERR_INVALID_ARGUMENT, "too many sequences");
+        guide_order(p, seqs, lens, nseq, order);
+    }
     std::vector<AlignedBase> path;
-    for (uint64_t id = existing; id < nseq; id++) {
+    for (uint64_t index = existing; index < nseq; index++) {
+        const uint64_t id = order.empty() ? index : order[index];
         rc = align_one(*ctx->c, p, GraphAcc{g->g}, seqs[id], lens[id], path);
         if (rc != MPOA_OK) return rc;
         std::string e = g->g.add_path(id, path.data(), path.size());
@@ -585,8 +591,6 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     Params p;
     int rc = MPOA_OK;
     if (!check_params(params, p, rc)) return rc;
-    if (p.progressive)
-        return fail(MPOA_ERR_UNSUPPORTED, "guide-ordered batches (-p) are not implemented");
     auto t_all = std::chrono::steady_clock::now();
     auto B = std::make_unique<mpoa_batch>();
     B->params = p;
@@ -658,8 +662,6 @@ int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_serv
     Params p;
     int rc = MPOA_OK;
     if (!check_params(params, p, rc)) return rc;
-    if (p.progressive)
-        return fail(MPOA_ERR_UNSUPPORTED, "guide-ordered jobs (-p) are not implemented");
     // Four slots by default: two for long DPs, two for short ones.
     const int nslots = engine_slots(opts && opts->slots ? int(opts->slots) : 4);
     rc = ensure_engine(ctx, nslots);
@@ -700,6 +702,7 @@ int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *rea
     if (extend) ji.graph = std::make_unique<Graph>(extend->g);
     const uint64_t L = nreads ? ji.own_off[1] : 0;
     ji.work = L * nreads * nreads;
+    ji.guide = s->params.progressive && existing == 0;
     *ticket = s->engine->submit(std::move(ji));
     return MPOA_OK;
 }
@@ -850,6 +853,19 @@ int mpoa_partition_windows(const mpoa_anchor *chain_in, uint64_t n, uint64_t tle
     const bool tail = partition_windows(in, tlen, qlen, max_drift, max_win, w);
     if (tail_adaptive) *tail_adaptive = tail ? 1 : 0;
     return anchors_out(w, out, cap, out_len);
+}
+
+int mpoa_guide_order(const mpoa_params *params, const uint8_t *const *seqs, const uint64_t *lens,
+                     uint64_t nseq, uint64_t *order) {
+    if (nseq && (!seqs || !lens || !order)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nseq > UINT32_MAX) return fail(MPOA_ERR_INVALID_ARGUMENT, "too many sequences");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    std::vector<uint32_t> o;
+    guide_order(p, seqs, lens, nseq, o);
+    for (uint64_t i = 0; i < nseq; i++) order[i] = o[i];
+    return MPOA_OK;
 }
 
 int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_t qlen,

@@ -115,6 +115,15 @@ struct Engine::Job {
     uint64_t footprint = 0;  // bytes counted in active_bytes_ (under the engine lock)
     bool admitted = false;
     std::vector<AlignedBase> path;
+    std::vector<uint32_t> order;  // -p: read aligned at each step (empty: input order)
+
+    // The read of step s: its index in `in` and its bases.
+    uint32_t read_index(uint32_t s) const { return order.empty() ? s : order[s]; }
+    const uint8_t *read(uint32_t s, uint64_t &len) const {
+        const uint32_t r = read_index(s);
+        len = in.read_off[r + 1] - in.read_off[r];
+        return in.bases + in.read_off[r];
+    }
 };
 
 // One line per launch for MPOA_TRACE=<file>: scheduling diagnostics.
@@ -398,12 +407,12 @@ void Engine::fail_job(Job &J, int status, const std::string &e) {
 // bases, the sort writes only ranks.
 bool Engine::fuse(Job &J) {
     Graph &g = *J.out.graph;
-    std::string e = g.add_path(J.in.first + J.step, J.path.data(), J.path.size());
+    std::string e = g.add_path(J.in.first + J.read_index(J.step), J.path.data(), J.path.size());
     J.seeded_ahead = false;
     const uint32_t next = J.step + 1;
     if (e.empty() && p_.seeding && next < J.in.nreads && g.size() > 2) {
-        const uint8_t *q = J.in.bases + J.in.read_off[next];
-        const uint64_t qlen = J.in.read_off[next + 1] - J.in.read_off[next];
+        uint64_t qlen;
+        const uint8_t *q = J.read(next, qlen);
         par_for(tasks_, 2, [&](size_t i) {
             if (i == 0) {
                 e = g.toposort(false);
@@ -443,8 +452,8 @@ void Engine::render(Job &J) {
 // status and path. False when the job ended with an error.
 bool Engine::consume(Job &J, std::vector<Dp *> &again) {
     Graph &g = *J.out.graph;
-    const uint8_t *q = J.in.bases + J.in.read_off[J.step];
-    const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+    uint64_t qlen;
+    const uint8_t *q = J.read(J.step, qlen);
     again.clear();
     for (uint32_t k = 0; k < J.ndp; k++) {
         Dp &D = *J.dp[k];
@@ -512,8 +521,8 @@ bool Engine::consume(Job &J, std::vector<Dp *> &again) {
 // False when the job ended with an error.
 bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
     Graph &g = *J.out.graph;
-    const uint8_t *q = J.in.bases + J.in.read_off[J.step];
-    const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+    uint64_t qlen;
+    const uint8_t *q = J.read(J.step, qlen);
     if (p_.seeding) {
         if (!J.seeded_ahead) {
             J.seed_err.clear();
@@ -579,6 +588,16 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
 void Engine::advance(Job &J, bool have) {
     Graph &g = *J.out.graph;
     std::vector<Dp *> again;
+    if (J.in.guide && J.step == 0 && !have && J.order.empty() && J.in.nreads > 0) {
+        // src/lib.rs:189-193: the guide order of all the job's reads.
+        std::vector<const uint8_t *> seqs(J.in.nreads);
+        std::vector<uint64_t> lens(J.in.nreads);
+        for (uint32_t r = 0; r < J.in.nreads; r++) {
+            seqs[r] = J.in.bases + J.in.read_off[r];
+            lens[r] = J.in.read_off[r + 1] - J.in.read_off[r];
+        }
+        guide_order(p_, seqs.data(), lens.data(), J.in.nreads, J.order);
+    }
     for (;;) {
         if (have) {
             have = false;
@@ -594,8 +613,8 @@ void Engine::advance(Job &J, bool have) {
             finish(J);
             return;
         }
-        const uint8_t *q = J.in.bases + J.in.read_off[J.step];
-        const uint64_t qlen = J.in.read_off[J.step + 1] - J.in.read_off[J.step];
+        uint64_t qlen;
+        const uint8_t *q = J.read(J.step, qlen);
         if (g.size() <= 2) {
             J.path.clear();
             J.path.push_back(AlignedBase{0, int64_t(p_.encode('N')), -1});
@@ -797,6 +816,7 @@ int run_engine(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, con
             ji.read_off = in.read_off + in.job_off[j];
             ji.nreads = uint32_t(in.job_off[j + 1] - in.job_off[j]);
             ji.crit = crit[j] != 0;
+            ji.guide = p.progressive;
             ji.work = work[j];
             id[j] = E.submit(std::move(ji));
         }

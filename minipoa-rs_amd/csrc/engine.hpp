@@ -68,6 +68,7 @@ struct JobIn {
     std::unique_ptr<Graph> graph;
     std::vector<uint8_t> own_bases;
     std::vector<uint64_t> own_off;
+    bool guide = false;  // align in the -p guide order (src/minimizer.rs:489-578); first must be 0
     bool crit = false;   // runs in the priority slot (batch mode)
     uint64_t work = 0;   // L x R^2: orders the priority slot's jobs
 };

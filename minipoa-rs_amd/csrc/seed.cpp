@@ -369,6 +369,84 @@ bool partition_windows(const std::vector<Anchor> &ch, uint64_t tlen, uint64_t ql
     return tail;
 }
 
+void guide_order(const Params &p, const uint8_t *const *seqs, const uint64_t *lens, uint64_t n,
+                 std::vector<uint32_t> &order) {
+    order.resize(n);
+    for (uint64_t i = 0; i < n; i++) order[i] = uint32_t(i);
+    if (!p.progressive || n < 2 || p.alpha() > 5) return;
+    // Every (k-mer value, read) occurrence, sorted: a value's group lists
+    // its reads in increasing id, once per occurrence.
+    std::vector<std::pair<uint64_t, uint32_t>> occ;
+    std::vector<Minimizer> mm;
+    for (uint64_t r = 0; r < n; r++) {
+        sketch(seqs[r], lens[r], p.k, p.w, mm);
+        for (const Minimizer &m : mm) occ.emplace_back(m.value, uint32_t(r));
+    }
+    if (occ.empty()) return;
+    std::sort(occ.begin(), occ.end());
+    std::vector<uint64_t> total(n, 0), shared(n * n, 0);
+    std::vector<std::pair<uint32_t, uint64_t>> grp;  // (read, count)
+    for (size_t a = 0; a < occ.size();) {
+        size_t b = a;
+        grp.clear();
+        for (; b < occ.size() && occ[b].first == occ[a].first; b++) {
+            if (!grp.empty() && grp.back().first == occ[b].second) grp.back().second++;
+            else grp.emplace_back(occ[b].second, 1);
+        }
+        a = b;
+        // Values shared by more than mm_filter_ratio of the reads are too
+        // frequent to tell reads apart.
+        if (double(grp.size()) > double(n) * p.mm_filter_ratio) continue;
+        for (size_t x = 0; x < grp.size(); x++) {
+            total[grp[x].first] += grp[x].second;
+            for (size_t y = 0; y < x; y++) {
+                const uint64_t c = std::min(grp[x].second, grp[y].second);
+                shared[size_t(grp[x].first) * n + grp[y].first] += c;
+                shared[size_t(grp[y].first) * n + grp[x].first] += c;
+            }
+        }
+    }
+    uint64_t longest = 1;
+    for (uint64_t i = 0; i < n; i++) longest = std::max(longest, lens[i]);
+    const double max_len = double(longest);
+    std::vector<double> sim(n * n, 0.0);
+    double best = -1.0;
+    uint64_t bi = 0, bj = 1;
+    for (uint64_t i = 0; i < n; i++) {
+        for (uint64_t j = 0; j < n; j++) {
+            if (i == j) continue;
+            const uint64_t common = shared[i * n + j];
+            const uint64_t uni = total[i] + total[j] - common;
+            // Multiset Jaccard, weighted toward the longer read of the pair
+            // (evaluated in the reference's order of operations).
+            const double jac = uni == 0 ? 0.0
+                                        : double(common) / double(uni) *
+                                              (0.51 * double(lens[i]) / max_len + 0.49 * double(lens[j]) / max_len);
+            sim[i * n + j] = jac;
+            if (jac > best) {
+                best = jac;
+                bi = i;
+                bj = j;
+            }
+        }
+    }
+    std::vector<uint8_t> placed(n, 0);
+    std::vector<double> sum(n);
+    order[0] = uint32_t(bi);
+    order[1] = uint32_t(bj);
+    placed[bi] = placed[bj] = 1;
+    for (uint64_t v = 0; v < n; v++) sum[v] = sim[bi * n + v] + sim[bj * n + v];
+    for (uint64_t k = 2; k < n; k++) {
+        uint64_t next = n;
+        for (uint64_t v = 0; v < n; v++)
+            if (!placed[v] && (next == n || sum[v] > sum[next])) next = v;
+        order[k] = uint32_t(next);
+        placed[next] = 1;
+        for (uint64_t v = 0; v < n; v++)
+            if (!placed[v]) sum[v] += sim[next * n + v];
+    }
+}
+
 bool anchor_windows(const Params &p, const uint8_t *q, uint64_t qlen, const uint8_t *cons,
                     uint64_t clen, std::vector<Anchor> &out) {
     std::vector<Anchor> m, c;

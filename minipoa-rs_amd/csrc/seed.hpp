@@ -47,6 +47,11 @@ bool partition_windows(const std::vector<Anchor> &chain, uint64_t tlen, uint64_t
 bool anchor_windows(const Params &p, const uint8_t *q, uint64_t qlen, const uint8_t *cons,
                     uint64_t clen, std::vector<Anchor> &out);
 
+// src/minimizer.rs:489-578: the progressive guide order of n reads (input
+// order unless p.progressive, n >= 2 and at most five symbols).
+void guide_order(const Params &p, const uint8_t *const *seqs, const uint64_t *lens, uint64_t n,
+                 std::vector<uint32_t> &order);
+
 // One anchored sub-DP of a seeded alignment: poa(beg, end) over
 // query[qbeg, qend), followed in the result by the anchor's own graph nodes.
 struct SeedWindow {

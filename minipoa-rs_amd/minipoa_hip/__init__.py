@@ -140,6 +140,7 @@ SIGNATURES = {
     "mpoa_chain": (C.c_int, [C.POINTER(_Anchor), _u64, _u64, C.POINTER(_Anchor), _u64, U64P]),
     "mpoa_partition_windows": (C.c_int, [C.POINTER(_Anchor), _u64, _u64, _u64, _u64, _u64,
                                          C.POINTER(_Anchor), _u64, U64P, C.POINTER(C.c_int)]),
+    "mpoa_guide_order": (C.c_int, [_P, C.POINTER(C.c_char_p), U64P, _u64, U64P]),
     "mpoa_anchor_windows": (C.c_int, [_P, C.c_char_p, _u64, C.c_char_p, _u64, C.POINTER(_Anchor),
                                       _u64, U64P, C.POINTER(C.c_int)]),
     "mpoa_load_matrix": (C.c_int, [C.c_char_p, _u64, C.POINTER(C.c_uint8), C.POINTER(C.c_int32),
@@ -547,6 +548,17 @@ def partition_windows(chain_, tlen, qlen, max_drift, max_win):
     return res, bool(tail.value)
 
 
+def guide_order(params, seqs):
+    """src/minimizer.rs:489-578 (seqs: Sequence records or byte strings)."""
+    raw = [bytes(getattr(s, "seq", s)) for s in seqs]
+    cp = CParams(params)
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_uint64 * max(1, len(raw)))(*[len(r) for r in raw])
+    out = (C.c_uint64 * max(1, len(raw)))()
+    _check(lib().mpoa_guide_order(cp.ref, arr, lens, len(raw), out))
+    return list(out)[:len(raw)]
+
+
 def anchor_windows(params, query, consensus):
     """src/minimizer.rs:475-488 -> (boundaries, tail_adaptive)."""
     cp, q, t = CParams(params), bytes(query), bytes(consensus)
@@ -718,5 +730,5 @@ __all__ = [
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
     "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server", "load_matrix",
     "Minimizer", "Anchor", "sketch", "collect_anchors", "chain", "partition_windows",
-    "anchor_windows",
+    "anchor_windows", "guide_order",
 ]

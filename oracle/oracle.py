@@ -138,6 +138,8 @@ def lib():
         L.o_anchor_windows.argtypes = [P, C.c_char_p, sz, C.c_char_p, sz, A, sz,
                                        C.POINTER(C.c_int)]
         L.o_anchor_windows.restype = sz
+        L.o_guide_order.argtypes = [P, C.POINTER(C.c_char_p), C.POINTER(sz), sz, C.POINTER(sz)]
+        L.o_guide_order.restype = None
         _lib = L
     return _lib
 
@@ -476,3 +478,14 @@ def anchor_windows(params, query, consensus):
         return n, lambda n: _anchors_out(out, n)
     res = _grow(call)
     return res, bool(tail.value)
+
+
+def guide_order(params, seqs):
+    """src/minimizer.rs:489-578 (seqs: Sequence records or byte strings)."""
+    raw = [bytes(getattr(s, "seq", s)) for s in seqs]
+    P = _P(params)
+    arr = (C.c_char_p * max(1, len(raw)))(*raw)
+    lens = (C.c_size_t * max(1, len(raw)))(*[len(r) for r in raw])
+    out = (C.c_size_t * max(1, len(raw)))()
+    lib().o_guide_order(P.ref, arr, lens, len(raw), out)
+    return list(out)[:len(raw)]

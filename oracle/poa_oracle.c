@@ -1910,6 +1910,124 @@ size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const
     return r;
 }
 
+/* src/minimizer.rs:489-578 guide_order: minimizer multiset Jaccard
+ * similarity weighted 0.51/0.49 by length, the most similar ordered pair
+ * first, then the read with the largest summed similarity to those placed
+ * (ties to the lower index). Inherits the sketch's unpinned positions. */
+typedef struct {
+    uint64_t value;
+    size_t rid;
+} o_vr;
+
+static int vr_cmp(const void *a, const void *b) {
+    const o_vr *x = a, *y = b;
+    if (x->value != y->value) return x->value < y->value ? -1 : 1;
+    return x->rid < y->rid ? -1 : (x->rid > y->rid ? 1 : 0);
+}
+
+void o_guide_order(const op_params *p, const uint8_t *const *seqs, const size_t *lens, size_t n,
+                   size_t *order) {
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    if (!p->progressive || n < 2 || p->alphabet_len > 5) return;
+    /* (value, read) pairs sorted: each value's group lists its reads in
+     * increasing id, a read repeated once per occurrence. */
+    o_vr *vr = NULL;
+    size_t nvr = 0, cvr = 0;
+    mmvec mm = {0};
+    for (size_t r = 0; r < n; r++) {
+        sketch_into(seqs[r], lens[r], p->k, p->w, &mm);
+        for (size_t i = 0; i < mm.n; i++) {
+            if (nvr == cvr) {
+                cvr = cvr ? 2 * cvr : 1024;
+                vr = xrealloc(vr, cvr * sizeof(o_vr));
+            }
+            vr[nvr].value = mm.v[i].value;
+            vr[nvr].rid = r;
+            nvr++;
+        }
+    }
+    free(mm.v);
+    if (nvr == 0) {
+        free(vr);
+        return;
+    }
+    qsort(vr, nvr, sizeof(o_vr), vr_cmp);
+    size_t *total = xcalloc(n, sizeof(size_t));
+    size_t *shared = xcalloc(n * n, sizeof(size_t));
+    size_t *rid = xmalloc(n * sizeof(size_t)), *cnt = xmalloc(n * sizeof(size_t));
+    for (size_t a = 0; a < nvr;) {
+        size_t b = a;
+        while (b < nvr && vr[b].value == vr[a].value) b++;
+        /* distinct reads of the group with their counts, increasing id */
+        size_t nd = 0;
+        for (size_t x = a; x < b; x++) {
+            if (nd > 0 && rid[nd - 1] == vr[x].rid) {
+                cnt[nd - 1]++;
+            } else {
+                rid[nd] = vr[x].rid;
+                cnt[nd] = 1;
+                nd++;
+            }
+        }
+        if (!((double)nd > (double)n * p->mm_filter_ratio)) {
+            for (size_t x = 0; x < nd; x++) {
+                total[rid[x]] += cnt[x];
+                for (size_t y = 0; y < x; y++) {
+                    size_t m = cnt[x] < cnt[y] ? cnt[x] : cnt[y];
+                    shared[rid[x] * n + rid[y]] += m;
+                    shared[rid[y] * n + rid[x]] += m;
+                }
+            }
+        }
+        a = b;
+    }
+    size_t maxl = 1;
+    for (size_t i = 0; i < n; i++)
+        if (lens[i] > maxl) maxl = lens[i];
+    double max_len = (double)maxl;
+    double *sim = xcalloc(n * n, sizeof(double));
+    double best = -1.0;
+    size_t bi = 0, bj = 1;
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < n; j++) {
+            if (i == j) continue;
+            size_t common = shared[i * n + j];
+            size_t uni = total[i] + total[j] - common;
+            double jac = uni == 0 ? 0.0
+                                  : (double)common / (double)uni *
+                                        (0.51 * (double)lens[i] / max_len + 0.49 * (double)lens[j] / max_len);
+            sim[i * n + j] = jac;
+            if (jac > best) {
+                best = jac;
+                bi = i;
+                bj = j;
+            }
+        }
+    uint8_t *used = xcalloc(n, 1);
+    double *sums = xmalloc(n * sizeof(double));
+    order[0] = bi;
+    order[1] = bj;
+    used[bi] = used[bj] = 1;
+    for (size_t v = 0; v < n; v++) sums[v] = sim[bi * n + v] + sim[bj * n + v];
+    for (size_t k = 2; k < n; k++) {
+        size_t next = (size_t)-1;
+        for (size_t v = 0; v < n; v++)
+            if (!used[v] && (next == (size_t)-1 || sums[v] > sums[next])) next = v;
+        order[k] = next;
+        used[next] = 1;
+        for (size_t v = 0; v < n; v++)
+            if (!used[v]) sums[v] += sim[next * n + v];
+    }
+    free(sums);
+    free(used);
+    free(sim);
+    free(rid);
+    free(cnt);
+    free(shared);
+    free(total);
+    free(vr);
+}
+
 /* One seeded window (src/align.rs:104-111) and its outcome. */
 typedef struct {
     size_t beg, end, qbeg, qend;
@@ -2101,10 +2219,16 @@ int o_align(const op_params *p, const og_graph *g, const uint8_t *q, size_t qlen
 /* src/lib.rs:184-204 (input order). */
 int o_extend_graph(const op_params *p, og_graph *g, const uint8_t *const *seqs,
                    const size_t *lens, size_t nseq, size_t existing) {
-    if (p->progressive && existing == 0) return fail("guide order (-p) is not restated in the oracle");
+    size_t *order = xmalloc((nseq ? nseq : 1) * sizeof(size_t));
+    if (p->progressive && existing == 0) {
+        o_guide_order(p, seqs, lens, nseq, order);
+    } else {
+        for (size_t i = 0; i < nseq; i++) order[i] = i;
+    }
     size_t cap = 0;
     ob_base *buf = NULL;
-    for (size_t id = existing; id < nseq; id++) {
+    for (size_t index = existing; index < nseq; index++) {
+        size_t id = order[index];
         size_t need = lens[id] + 8;
         if (need > cap) {
             cap = need * 2;
@@ -2113,18 +2237,22 @@ int o_extend_graph(const op_params *p, og_graph *g, const uint8_t *const *seqs,
         size_t len = 0;
         if (o_align(p, g, seqs[id], lens[id], buf, cap, &len)) {
             free(buf);
+            free(order);
             return 1;
         }
         if (len > cap) {
             free(buf);
+            free(order);
             return fail("oracle path buffer overflow");
         }
         if (og_add_path(g, id, buf, len) || og_toposort(g, 0)) {
             free(buf);
+            free(order);
             return 1;
         }
     }
     free(buf);
+    free(order);
     return 0;
 }
 

@@ -109,6 +109,10 @@ size_t o_chain(const o_anchor *anchors, size_t n, size_t k, o_anchor *out, size_
 size_t o_partition_windows(const o_anchor *chain, size_t n, size_t tlen, size_t qlen,
                            size_t max_drift, size_t max_win, o_anchor *out, size_t cap,
                            int *tail);
+/* src/minimizer.rs:489-578: order[0..n) = the progressive guide order
+ * (input order unless p->progressive, n >= 2 and a DNA alphabet). */
+void o_guide_order(const op_params *p, const uint8_t *const *seqs, const size_t *lens, size_t n,
+                   size_t *order);
 /* src/minimizer.rs:475-488 */
 size_t o_anchor_windows(const op_params *p, const uint8_t *q, size_t qlen, const uint8_t *t,
                         size_t tlen, o_anchor *out, size_t cap, int *tail);
@@ -149,7 +153,7 @@ void o_reset_cells(void);
 size_t o_arena_capacity(void);
 void o_arena_clear(void);
 
-/* src/lib.rs:177-204 align_sequences (in input order; -p is not restated). */
+/* src/lib.rs:177-204 align_sequences (input order, or the guide order with -p). */
 og_graph *o_align_sequences(const op_params *p, const uint8_t *const *seqs,
                             const size_t *lens, size_t nseq);
 /* Continue progressive alignment of seqs[existing..] into g (src/lib.rs:184-204). */

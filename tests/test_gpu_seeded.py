@@ -155,3 +155,41 @@ def test_seeded_long_read_sample_matches_oracle_hash(ctx, oracle):
     assert want[0] != 0
     assert got == int(want[0])
     assert res.stats()["cells"] == cells
+
+
+def guided_reads(seed, n=7):
+    """Reads of two diverged templates, interleaved: the -p order groups them."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = [rng.integers(0, 4, size=L).astype(np.uint8) for L in (1800, 1500)]
+    return [synth.mutate(rng, t[k % 2], 0.06) for k in range(n)]
+
+
+@pytest.mark.parametrize("seeding", [False, True])
+def test_progressive_guide_order_matches_oracle_bytes(ctx, oracle, seeding):
+    """-p (src/lib.rs:189-193, src/minimizer.rs:489-578) through the one-call
+    path, the batch engine and the server."""
+    p = Params(progressive=True, seeding=seeding, k=15, w=5, win_band=60, win_max=300,
+               mm_filter_ratio=0.6)
+    per_job = [guided_reads(s) for s in (1, 2, 3)]
+    assert any(oracle.guide_order(p, r) != list(range(len(r))) for r in per_job)
+    want = []
+    for reads in per_job:
+        seqs = records(reads)
+        og = oracle.align_sequences(p, seqs)
+        want.append((">consensus sequence\n" + og.consensus(p)[0].decode() + "\n", og.msa(p, seqs),
+                     og.gfa(p, seqs)))
+    for reads, w in zip(per_job, want):
+        seqs = records(reads)
+        g = mh.align_sequences(ctx, p, seqs)
+        assert (g.msa(p, seqs), g.gfa(p, seqs)) == w[1:]
+    jo, ro, b = synth._pack(per_job)
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    for j, w in enumerate(want):
+        assert res.status(j) == 0, mh.last_error()
+        assert (res.output(j, 0), res.output(j, 1), res.output(j, 2)) == w
+    with mh.Server(ctx, p) as srv:
+        tickets = [srv.submit(r) for r in per_job]
+        for t, w in zip(tickets, want):
+            srv.wait(t)
+            assert srv.status(t) == 0
+            assert (srv.output(t, 0), srv.output(t, 1), srv.output(t, 2)) == w

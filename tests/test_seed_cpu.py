@@ -158,3 +158,32 @@ def test_protein_alphabet_has_no_anchors():
     assert mh.collect_anchors(p, b"ACDEFGHIKL" * 20, b"ACDEFGHIKL" * 20) == []
     # The same text under the DNA alphabet does seed.
     assert mh.collect_anchors(Params(k=5, w=1, tr_dist=0), b"ACGTTGCAAC" * 3, b"ACGTTGCAAC" * 3)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_guide_order_matches_oracle(oracle, seed):
+    """src/minimizer.rs:489-578: reads of three diverged templates, shuffled,
+    with ambiguous runs and one short read without minimizers."""
+    from minipoa_hip.params import Sequence
+    rng = random.Random(5000 + seed)
+    templates = [random_dna(rng, rng.randint(400, 3000)) for _ in range(3)]
+    seqs = [Sequence(seq=messy(rng, mutate(rng, rng.choice(templates), rng.choice([0.02, 0.1]))))
+            for _ in range(rng.randint(2, 14))]
+    seqs.append(Sequence(seq=b"ACG"))
+    rng.shuffle(seqs)
+    for p in (Params(progressive=True), Params(progressive=True, k=11, w=4),
+              Params(progressive=True, k=15, w=5, mm_filter_ratio=0.5),
+              Params(progressive=False), Params(progressive=True, mm_filter_ratio=0.0)):
+        assert mh.guide_order(p, seqs) == oracle.guide_order(p, seqs), p
+
+
+def test_guide_order_reference_cases():
+    """src/minimizer.rs:753-776 on the library."""
+    from minipoa_hip.params import Sequence
+    from test_seed_oracle import xorshift_prefix_records
+    seqs = xorshift_prefix_records()
+    p = Params(progressive=True)
+    order = mh.guide_order(p, seqs)
+    assert order == mh.guide_order(p, seqs)
+    assert sorted(order) == list(range(12))
+    assert mh.guide_order(p, [Sequence(seq=b"ACGT")] * 3) == [0, 1, 2]

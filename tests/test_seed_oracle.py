@@ -212,3 +212,107 @@ def test_seeded_workflow_renders_every_read(oracle):
     validate_msa(g.msa(p, reads), reads)
     validate_gfa(g.gfa(p, reads), reads)
     assert g.consensus(p)
+
+
+def xorshift_prefix_records(n_reads=12, base=600):
+    """src/minimizer.rs:753-762: random_dna(600 + i) restarts the xorshift
+    stream each call, so the reads are prefixes of one sequence."""
+    from minipoa_hip.params import Sequence
+    return [Sequence(name=str(i), seq=xorshift_dna(base + i)) for i in range(n_reads)]
+
+
+def guide_order_by_definition(oracle, p, seqs):
+    """src/minimizer.rs:505-578 from the oracle's (pinned) sketch with plain
+    dictionaries: multiset Jaccard x (0.51 len_i + 0.49 len_j) / max len,
+    best ordered pair, then greedy by summed similarity, ties to lower index."""
+    from collections import Counter, defaultdict
+    n = len(seqs)
+    if not p.progressive or n < 2 or len(p.alphabet) > 5:
+        return list(range(n))
+    groups = defaultdict(Counter)
+    for rid, s in enumerate(seqs):
+        for m in oracle.sketch(s.seq, p.k, p.w):
+            groups[m.value][rid] += 1
+    if not groups:
+        return list(range(n))
+    total = [0] * n
+    shared = [[0] * n for _ in range(n)]
+    for counts in groups.values():
+        if len(counts) > n * p.mm_filter_ratio:
+            continue
+        for a, ca in counts.items():
+            total[a] += ca
+            for b, cb in counts.items():
+                if b < a:
+                    shared[a][b] += min(ca, cb)
+                    shared[b][a] += min(ca, cb)
+    max_len = float(max(1, max(len(s.seq) for s in seqs)))
+    sim = [[0.0] * n for _ in range(n)]
+    best = (-1.0, 0, 1)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            c = shared[i][j]
+            u = total[i] + total[j] - c
+            jac = 0.0 if u == 0 else c / u * (0.51 * len(seqs[i].seq) / max_len
+                                               + 0.49 * len(seqs[j].seq) / max_len)
+            sim[i][j] = jac
+            if jac > best[0]:
+                best = (jac, i, j)
+    order = [best[1], best[2]]
+    used = set(order)
+    sums = [sim[best[1]][v] + sim[best[2]][v] for v in range(n)]
+    while len(order) < n:
+        nxt = min((v for v in range(n) if v not in used), key=lambda v: (-sums[v], v))
+        order.append(nxt)
+        used.add(nxt)
+        for v in range(n):
+            if v not in used:
+                sums[v] += sim[nxt][v]
+    return order
+
+
+def test_guide_order_is_permutation_and_deterministic(oracle):
+    """src/minimizer.rs:753-776"""
+    from minipoa_hip.params import Sequence
+    seqs = xorshift_prefix_records()
+    p = Params(progressive=True)
+    order = oracle.guide_order(p, seqs)
+    assert order == oracle.guide_order(p, seqs)
+    assert sorted(order) == list(range(len(seqs)))
+    short = [Sequence(seq=b"ACGT")] * 3
+    assert oracle.guide_order(p, short) == [0, 1, 2]
+    assert oracle.guide_order(Params(progressive=False), seqs) == list(range(12))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_guide_order_matches_its_definition(oracle, seed):
+    """Diverged copies of two templates in shuffled order: the oracle's
+    grouping and greedy order equal the dictionary computation, and reads of
+    one template come out next to each other."""
+    import random as _r
+    from minipoa_hip.params import Sequence
+    rng = _r.Random(seed)
+    t1, t2 = xorshift_dna(1500), bytes(rng.choice(b"ACGT") for _ in range(1300))
+
+    def mut(t):
+        return bytes(c if rng.random() > 0.05 else rng.choice(b"ACGT") for c in t)
+
+    seqs = [Sequence(seq=mut(t1 if k % 2 else t2)) for k in range(9)]
+    rng.shuffle(seqs)
+    for p in (Params(progressive=True, k=15, w=5), Params(progressive=True, k=11, w=3, mm_filter_ratio=0.9),
+              Params(progressive=True, mm_filter_ratio=0.0)):
+        assert oracle.guide_order(p, seqs) == guide_order_by_definition(oracle, p, seqs)
+
+
+def test_seeded_progressive_workflow_renders_every_read(oracle):
+    """tests/avx512.rs:145-180 with progressive = true."""
+    reference = dna(1409)
+    insertion = reference[:417] + dna(25) + reference[417:]
+    deletion = reference[:803] + reference[820:]
+    reads = records([reference, insertion, deletion])
+    p = Params(seeding=True, k=7, w=3, tr_dist=0, win_max=80, win_band=24, progressive=True)
+    g = oracle.align_sequences(p, reads)
+    validate_msa(g.msa(p, reads), reads)
+    validate_gfa(g.gfa(p, reads), reads)
