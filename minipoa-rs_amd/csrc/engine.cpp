@@ -657,6 +657,13 @@ void Engine::slot_main(size_t si) {
     // ones; the others take short DPs only.
     const size_t ord = o_.crit_lane ? si - 1 : si;
     const int role = !crit_slot && ord < o_.long_slots ? 1 : 0;
+    // Serving (long and short lanes): launches group DPs of similar rows,
+    // and within each lane every second slot starts from the smallest DPs,
+    // so short chains are not held behind the largest jobs' launches.
+    const bool by_size = o_.long_rows != 0 && !crit_slot;
+    const size_t lane_first = role == 1 ? 0 : o_.long_slots;
+    const bool from_small = by_size && (ord - lane_first) % 2 == 1;
+    std::vector<uint8_t> taken;
     std::vector<Dp *> batch;
     std::vector<Job *> back;
     std::vector<JobPlan *> plans;
@@ -688,25 +695,43 @@ void Engine::slot_main(size_t si) {
             std::vector<Dp *> &q = crit_slot ? ready_crit_ : (role == 1 && !ready_long_.empty() ? ready_long_ : ready_);
             const size_t idle = std::max<size_t>(1, &q == &ready_long_ ? idle_[1] : idle_[0] + (role == 1 ? 1 : 0));
             if (!crit_slot) {
-                // Longest remaining chain first (remaining reads times the
-                // current step's cells); neighbours in that order have
-                // similar step sizes, and a launch lasts as long as its
-                // longest job.
-                std::sort(q.begin(), q.end(), [](const Dp *x, const Dp *y) { return x->rem > y->rem; });
-            }
-            const uint64_t want = crit_slot ? q.size() : std::min<uint64_t>(max_launch_, (q.size() + idle - 1) / idle);
-            const bool i64 = q[0]->plan.use_i64;  // one score width per launch
-            size_t w = 0;
-            uint64_t arena = 0;
-            for (size_t k = 0; k < q.size(); k++) {
-                const uint64_t a = job_arena_bytes(q[k]->plan, i64);
-                if (batch.size() < want && q[k]->plan.use_i64 == i64 && (batch.empty() || arena + a <= arena_budget_)) {
-                    batch.push_back(q[k]);
-                    arena += a;
+                if (by_size) {
+                    // Serving: by rows, so that a launch holds DPs of similar
+                    // length (a DP waits for the longest one in its launch).
+                    std::sort(q.begin(), q.end(), [](const Dp *x, const Dp *y) {
+                        return x->plan.n != y->plan.n ? x->plan.n > y->plan.n : x->rem > y->rem;
+                    });
                 } else {
-                    q[w++] = q[k];
+                    // Longest remaining chain first (remaining reads times
+                    // the current step's cells); neighbours in that order
+                    // have similar step sizes, and a launch lasts as long as
+                    // its longest job.
+                    std::sort(q.begin(), q.end(), [](const Dp *x, const Dp *y) { return x->rem > y->rem; });
                 }
             }
+            const uint64_t want = crit_slot ? q.size() : std::min<uint64_t>(max_launch_, (q.size() + idle - 1) / idle);
+            const size_t nq = q.size();
+            const Dp *first = from_small ? q[nq - 1] : q[0];
+            const bool i64 = first->plan.use_i64;  // one score width per launch
+            const uint64_t n0 = first->plan.n;
+            size_t w = 0;
+            uint64_t arena = 0;
+            taken.assign(nq, 0);
+            for (size_t t = 0; t < nq; t++) {
+                const size_t k = from_small ? nq - 1 - t : t;
+                const Dp *D = q[k];
+                const uint64_t a = job_arena_bytes(D->plan, i64);
+                // by_size: within a factor of two of the launch's first DP.
+                const bool similar = !by_size || (from_small ? D->plan.n <= 2 * n0 : 2 * uint64_t(D->plan.n) >= n0);
+                if (batch.size() < want && D->plan.use_i64 == i64 && similar &&
+                    (batch.empty() || arena + a <= arena_budget_)) {
+                    batch.push_back(q[k]);
+                    taken[k] = 1;
+                    arena += a;
+                }
+            }
+            for (size_t k = 0; k < nq; k++)
+                if (!taken[k]) q[w++] = q[k];
             q.resize(w);
             if (crit_slot) {
                 crit_busy_ = true;
