@@ -193,3 +193,71 @@ def test_progressive_guide_order_matches_oracle_bytes(ctx, oracle, seeding):
             srv.wait(t)
             assert srv.status(t) == 0
             assert (srv.output(t, 0), srv.output(t, 1), srv.output(t, 2)) == w
+
+
+def _oracle_bytes(oracle, p, reads):
+    seqs = records(reads)
+    og = oracle.align_sequences(p, seqs)
+    return (">consensus sequence\n" + og.consensus(p)[0].decode() + "\n", og.msa(p, seqs), og.gfa(p, seqs))
+
+
+def test_seeded_reads_with_case_ambiguity_and_rna(ctx, oracle):
+    """Seeding skips ambiguous runs and treats lower case and U as their
+    bases (src/minimizer.rs:28-76); the DP encodes them as the reference
+    does. Lower-case/N reads and an RNA job through the batch."""
+    rng = np.random.Generator(np.random.PCG64(41))
+    t = rng.integers(0, 4, size=2200).astype(np.uint8)
+    messy = []
+    for k in range(6):
+        r = bytearray(synth.mutate(rng, t, 0.05))
+        a = int(rng.integers(100, len(r) - 200))
+        r[a:a + 25] = b"N" * 25
+        for i in rng.choice(len(r), size=len(r) // 10, replace=False):
+            r[int(i)] |= 0x20
+        messy.append(bytes(r))
+    rna = [synth.mutate(rng, t[:1500], 0.05).replace(b"T", b"U") for _ in range(5)]
+    for reads, p in ((messy, Params(seeding=True, k=15, w=5, win_band=60, win_max=300)),
+                     (rna, Params(seeding=True, k=15, w=5, win_band=60, win_max=300, is_rna=True))):
+        assert mh.anchor_windows(p, reads[1], reads[0])[0]
+        jo, ro, b = synth._pack([reads])
+        res = mh.batch_run(ctx, p, jo, ro, b)
+        assert res.status(0) == 0, mh.last_error()
+        assert (res.output(0, 0), res.output(0, 1), res.output(0, 2)) == _oracle_bytes(oracle, p, reads)
+
+
+def test_seeded_windows_split_between_i32_and_i64(ctx, oracle):
+    """Scores near the i32 guard (src/align.rs:493-529): each window DP is
+    guarded on its own rows and query, so the windows of one step split
+    between the i32 and the i64 kernels (separate launches), and the result
+    equals the oracle's."""
+    # max |score| 110,000: rows + query <= 2^26 / 110,000 - 20 = 590 stays
+    # on the i32 kernel; 9 of this job's 69 windows are larger.
+    mat = [110000 if a == b else -110000 for a in range(5) for b in range(5)]
+    mat[24] = 110000
+    p = Params(seeding=True, k=15, w=5, win_band=60, win_max=300, match_score=110000, mismatch=-110000,
+               gap_open=-20000, gap_ext=-10000, matrix=mat)
+    rng = np.random.Generator(np.random.PCG64(43))
+    t = rng.integers(0, 4, size=2600).astype(np.uint8)
+    reads = [synth.mutate(rng, t, 0.06) for _ in range(5)]
+    jo, ro, b = synth._pack([reads])
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    st = res.stats()
+    assert res.status(0) == 0, mh.last_error()
+    assert 0 < st["i64_calls"] < st["dp_calls"]
+    assert (res.output(0, 0), res.output(0, 1), res.output(0, 2)) == _oracle_bytes(oracle, p, reads)
+
+
+def test_seeded_incremental_extension_matches_oracle(ctx, oracle):
+    """A served job extends a graph of earlier reads with seeding on
+    (src/lib.rs:184-204): equal to aligning all reads at once."""
+    p = Params(seeding=True, k=15, w=5, win_band=60, win_max=300)
+    rng = np.random.Generator(np.random.PCG64(47))
+    t = rng.integers(0, 4, size=2000).astype(np.uint8)
+    reads = [synth.mutate(rng, t, 0.08) for _ in range(10)]
+    with mh.Server(ctx, p) as srv:
+        t0 = srv.submit(reads[:6])
+        srv.wait(t0)
+        t1 = srv.submit(reads[6:], graph=srv.graph(t0), existing=6)
+        srv.wait(t1)
+        assert srv.status(t1) == 0
+        assert (srv.output(t1, 0), srv.output(t1, 1), srv.output(t1, 2)) == _oracle_bytes(oracle, p, reads)
