@@ -188,6 +188,37 @@ std::string Graph::add_path(uint64_t seq_id, const AlignedBase *a, size_t len) {
 // `grouped` ranks whole column groups (parents) for MSA rendering.
 std::string Graph::toposort(bool grouped) {
     const size_t nn = nodes_.size();
+    if (!grouped) {
+        // The same LIFO Kahn walk as below, on a flat copy of the out-edges
+        // and in-degrees made by one sequential pass: the walk then touches
+        // 4-byte entries instead of whole node records in pop order, and
+        // ranks go back to the nodes in id order.
+        ts_off_.resize(nn + 1);
+        ts_deg_.resize(nn);
+        ts_adj_.clear();
+        ts_stack_.clear();
+        for (size_t id = 0; id < nn; id++) {
+            const Node &n = nodes_[id];
+            ts_off_[id] = uint32_t(ts_adj_.size());
+            for (const Edge &ed : n.out) ts_adj_.push_back(ed.node);
+            ts_deg_[id] = uint32_t(n.in.size());
+            if (ts_deg_[id] == 0) ts_stack_.push_back(uint32_t(id));
+        }
+        ts_off_[nn] = uint32_t(ts_adj_.size());
+        rank_.clear();
+        rank_.reserve(nn);
+        while (!ts_stack_.empty()) {
+            const uint32_t id = ts_stack_.back();
+            ts_stack_.pop_back();
+            rank_flat_[id] = uint32_t(rank_.size());
+            rank_.push_back(id);
+            for (uint32_t k = ts_off_[id]; k < ts_off_[id + 1]; k++)
+                if (--ts_deg_[ts_adj_[k]] == 0) ts_stack_.push_back(ts_adj_[k]);
+        }
+        for (size_t id = 0; id < nn; id++) nodes_[id].rank = rank_flat_[id];
+        if (rank_.size() != nn) return "graph contains a cycle (possibly between alignment columns)";
+        return {};
+    }
     std::vector<uint32_t> degree(nn, 0);
     std::vector<uint32_t> stack;
     stack.reserve(64);

@@ -137,6 +137,8 @@ private:
     std::vector<uint32_t> rank_;
     std::vector<uint32_t> rank_flat_;
     std::vector<std::pair<uint64_t, std::vector<uint32_t>>> paths_;
+    // toposort(false) scratch: out-edges as flat arrays, in-degrees, stack
+    std::vector<uint32_t> ts_off_, ts_adj_, ts_deg_, ts_stack_;
     // view buffers
     std::vector<uint64_t> v_rank_, v_parent_, v_base_, v_in_off_, v_in_, v_in_w_, v_out_off_,
         v_out_, v_out_w_, v_order_;

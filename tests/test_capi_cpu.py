@@ -175,3 +175,25 @@ def test_library_graph_rna_decoding(oracle):
     og.toposort(False)
     lg.toposort(False)
     assert lg.consensus(p)[0] == og.consensus(p)[0] == b"ACGU"
+
+
+def test_library_ranks_match_oracle_on_a_large_graph(oracle):
+    """toposort(false) (src/graph.rs:124-178) on a branchy graph of 12 reads
+    of 2.5 kb at 10% error: the library's ranks equal the oracle's after
+    every read."""
+    import numpy as np
+
+    from minipoa_hip import synth
+    p = Params(adaptive=True)
+    rng = np.random.Generator(np.random.PCG64(77))
+    t = rng.integers(0, 4, size=2500).astype(np.uint8)
+    og, lg = oracle.Graph(p), mh.Graph(p)
+    for sid in range(12):
+        r = synth.mutate(rng, t, 0.10)
+        path = oracle.align(p, og, r)
+        og.add_path(sid, path)
+        og.toposort(False)
+        lg.add_path(sid, path)
+        lg.toposort(False)
+        assert lg.rank == og.rank, sid
+    assert lg.n_nodes > 4000
