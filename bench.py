@@ -18,7 +18,8 @@ every one checked byte for byte against the oracle:
       one DP per launch (the drop-in path), beside the oracle on one core;
   c3  seeded long-read consensus (-S -r2): one 16,500 bp template, 100 reads
       at 10% trimmed by up to 500 bases at each end; each read aligns as
-      ~28 minimizer-anchored windows that share one launch per step;
+      ~28 minimizer-anchored windows that share one launch per step; then
+      16 such jobs in one batch beside the oracle on all host cores;
   c4  protein POA: 2,000 families over a 20-letter alphabet with a drawn
       symmetric matrix, 16 members at 30% divergence, full DP (-f0);
   c5  serving: Poisson arrivals of DNA jobs (template log-uniform in
@@ -262,6 +263,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def fnv1a_one(text):
+    """FNV-1a 64 of one long text (a plain loop: megabytes in well under a second)."""
+    h = 14695981039346656037
+    for c in text.encode():
+        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
 def hashes_of(get_text, n):
     import numpy as np
 
@@ -298,6 +307,8 @@ def leg_c1(ctx, oracle):
 def leg_c3(ctx, oracle, cores):
     """Seeded long-read consensus (-S -r2): one job through the engine; every
     read's anchored windows run side by side in one launch per step."""
+    import numpy as np
+
     import minipoa_hip as mh
     from minipoa_hip import Params, synth
 
@@ -320,13 +331,27 @@ def leg_c3(ctx, oracle, cores):
     t = time.perf_counter()
     cells_t, want_t = oracle.batch_run(pt, jo, ro, b, 1)
     ctt = time.perf_counter() - t
-    h = 14695981039346656037
-    for c in text.encode():
-        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
-    same = h == int(want[0]) == int(want_t[0]) and st["cells"] == cells == cells_t
+    same = fnv1a_one(text) == int(want[0]) == int(want_t[0]) and st["cells"] == cells == cells_t
     nreads = int(jo[1] - jo[0])
     log(f"c3: gpu {dt*1e3:.0f} ms {st['cells']/dt/1e9:.2f} GCUPS, {st['dp_calls']} DPs; "
         f"oracle {ct:.1f}s on 1 core, {ctt:.1f}s with -t {cores}; identical {same}")
+    # Many such jobs at once: every ready job's windows share launches.
+    nb = 16
+    bj, br, bb = synth.seeded(seed=20264, njobs=nb)
+    rb = mh.batch_run(ctx, p, bj, br, bb)  # warm
+    del rb
+    t = time.perf_counter()
+    rb = mh.batch_run(ctx, p, bj, br, bb)
+    bdt = time.perf_counter() - t
+    bst = rb.stats()
+    got = np.array([fnv1a_one(rb.output(k, 0) + rb.output(k, 1)) for k in range(nb)], dtype=np.uint64)
+    del rb
+    t = time.perf_counter()
+    bcells, bwant = oracle.batch_run(p, bj, br, bb, cores)
+    bct = time.perf_counter() - t
+    bsame = int((got == bwant).sum())
+    log(f"c3 batch: {nb} jobs gpu {bdt*1e3:.0f} ms {bst['cells']/bdt/1e9:.2f} GCUPS; oracle {bct:.1f}s "
+        f"on {cores} cores ({bcells/bct/1e9:.2f} GCUPS); bit-exact {bsame}/{nb}")
     return {"workload": "seeded long-read consensus: 16500 bp template, 100 reads at 10% error, "
                         "ends trimmed U[0,500], minimizer windows (k19 w10), -r2",
             "cells": st["cells"], "dp_calls": st["dp_calls"], "windows_per_read": round(st["dp_calls"] / (nreads - 1), 1),
@@ -334,7 +359,12 @@ def leg_c3(ctx, oracle, cores):
             "ms_per_read": round(dt * 1e3 / nreads, 2), "gfa_bytes": len(gfa),
             "cpu_ms": round(ct * 1e3, 1), "cpu_gcups": round(cells / ct / 1e9, 3), "cpu_cores": 1,
             "cpu_threads_ms": round(ctt * 1e3, 1), "cpu_threads": cores,
-            "cpu_kind": "port", "identical": bool(same)}
+            "cpu_kind": "port", "identical": bool(same),
+            "batch": {"jobs": nb, "cells": bst["cells"], "gpu_ms": round(bdt * 1e3, 1),
+                      "gpu_gcups": round(bst["cells"] / bdt / 1e9, 3),
+                      "cpu_ms": round(bct * 1e3, 1), "cpu_gcups": round(bcells / bct / 1e9, 3),
+                      "cpu_cores": cores, "bit_exact": bsame,
+                      "what": "16 such jobs (seed 20264) in one batch; oracle job-parallel on host cores"}}
 
 
 def leg_c4(ctx, oracle, cores):

@@ -1449,7 +1449,8 @@ static int iv_cmp(const void *a, const void *b) {
 }
 
 /* Sort order of minimizer indices by (value, index). */
-static const o_mm *g_sort_mm;
+/* Per-thread: the job-parallel batch driver sorts on several threads. */
+static __thread const o_mm *g_sort_mm;
 static int mmidx_cmp(const void *a, const void *b) {
     size_t i = *(const size_t *)a, j = *(const size_t *)b;
     uint64_t vi = g_sort_mm[i].value, vj = g_sort_mm[j].value;
@@ -1631,7 +1632,7 @@ typedef struct {
     size_t n;
 } o_local;
 
-static const o_anchor *g_sort_an;
+static __thread const o_anchor *g_sort_an;
 static int local_cmp(const void *a, const void *b) {
     const o_local *x = a, *y = b;
     return anchor_cmp(&g_sort_an[x->v[x->n - 1]], &g_sort_an[y->v[y->n - 1]]);

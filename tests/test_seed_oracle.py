@@ -316,3 +316,16 @@ def test_seeded_progressive_workflow_renders_every_read(oracle):
     g = oracle.align_sequences(p, reads)
     validate_msa(g.msa(p, reads), reads)
     validate_gfa(g.gfa(p, reads), reads)
+
+
+def test_seeded_batch_driver_is_thread_safe(oracle):
+    """The job-parallel driver (the CPU baseline) gives every seeded job the
+    same bytes and cells on one thread and on eight: the seeding's sorts keep
+    no state shared between threads."""
+    from minipoa_hip import synth
+    jo, ro, b = synth.seeded(seed=20264, njobs=8, length=3000, nreads=8)
+    p = Params(seeding=True)
+    c1, h1 = oracle.batch_run(p, jo, ro, b, 1)
+    c8, h8 = oracle.batch_run(p, jo, ro, b, 8)
+    assert c1 == c8
+    assert (h1 == h8).all() and (h1 != 0).all()
