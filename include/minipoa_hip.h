@@ -6,7 +6,9 @@
  * and no GPU types cross this boundary, so a Rust `-sys` crate can bind it
  * (minipoa itself is `#![forbid(unsafe_code)]`, src/lib.rs:2). Every
  * function returns MPOA_OK (0) or a status; the message of the calling
- * thread's last failure is mpoa_last_error().
+ * thread's last failure is mpoa_last_error(). A host allocation failure
+ * inside the library returns MPOA_ERR_ALLOC (in a batch or server job: that
+ * job's status).
  *
  * usize fields of the reference are uint64_t here; Option<usize> is int64_t
  * with -1 for None.

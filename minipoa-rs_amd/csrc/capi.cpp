@@ -7,6 +7,8 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -52,6 +54,20 @@ namespace {
 int fail(int status, const std::string &msg) {
     set_error(msg);
     return status;
+}
+
+// No C++ exception crosses the C ABI: an entry point that throws (host
+// allocation, in practice) returns a status with the message set.
+int exception_status() {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(MPOA_ERR_ALLOC, "host memory allocation failed");
+    } catch (const std::exception &e) {
+        return fail(MPOA_ERR_INVALID_ARGUMENT, std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "unexpected error");
+    }
 }
 
 // Reference-visible DP errors (src/align.rs returns Err for each of them), on
@@ -277,7 +293,7 @@ extern "C" {
 const char *mpoa_backend_name(void) { return "hip-gfx950 (wave64 x i32)"; }
 const char *mpoa_last_error(void) { return last_error(); }
 
-int mpoa_ctx_create(int device, mpoa_ctx **out) {
+int mpoa_ctx_create(int device, mpoa_ctx **out) try {
     if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
     Ctx *c = nullptr;
@@ -287,7 +303,7 @@ int mpoa_ctx_create(int device, mpoa_ctx **out) {
     h->c = c;
     *out = h;
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 void mpoa_ctx_destroy(mpoa_ctx *ctx) {
     if (!ctx) return;
@@ -296,7 +312,7 @@ void mpoa_ctx_destroy(mpoa_ctx *ctx) {
     delete ctx;
 }
 
-uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) {
+uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) try {
     if (!ctx) return 0;
     uint64_t total = 0;
     auto add = [&](const Ctx &c) {
@@ -305,10 +321,13 @@ uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) {
     add(*ctx->c);
     for (const auto &l : ctx->lanes) add(*l);
     return total;
+} catch (...) {
+    (void)exception_status();
+    return 0;
 }
 
 int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t *matrix, uint32_t cap,
-                     uint32_t *alphabet_len) {
+                     uint32_t *alphabet_len) try {
     if (!text || !alphabet_len || (cap && (!alphabet || !matrix)))
         return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *alphabet_len = 0;
@@ -376,21 +395,21 @@ int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t 
     for (size_t c = 0; c < n; c++) alphabet[c] = (uint8_t)head[c][0];
     memcpy(matrix, m.data(), m.size() * sizeof(int32_t));
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_params_validate(const mpoa_params *params) {
+int mpoa_params_validate(const mpoa_params *params) try {
     Params p;
     int rc = MPOA_OK;
     check_params(params, p, rc);
     return rc;
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_i32_scores_safe(const mpoa_params *params, const mpoa_graph_view *graph,
-                         uint64_t beg_id, uint64_t end_id, uint64_t query_len) {
+                         uint64_t beg_id, uint64_t end_id, uint64_t query_len) try {
     if (!params || !graph) return 0;
     Params p = Params::from_c(*params);
     return i32_scores_safe(p, ViewAcc{*graph}, beg_id, end_id, query_len) ? 1 : 0;
-}
+} catch (...) { return exception_status(); }
 
 static int poa_entry(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                      uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
@@ -419,32 +438,32 @@ static int poa_entry(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_
 int mpoa_poa(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
              uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen, int adaptive,
              uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap, uint64_t *out_len,
-             int64_t *out_score) {
+             int64_t *out_score) try {
     return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, 0, out,
                      out_cap, out_len, out_score);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_poa_inner(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                    uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
                    int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
-                   uint64_t *out_len, int64_t *out_score) {
+                   uint64_t *out_len, int64_t *out_score) try {
     return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, 1, out,
                      out_cap, out_len, out_score);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_poa_kernel(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                     uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
                     int adaptive, uint64_t band_b, int score_bits, mpoa_aligned_base *out,
-                    uint64_t out_cap, uint64_t *out_len, int64_t *out_score) {
+                    uint64_t out_cap, uint64_t *out_len, int64_t *out_score) try {
     if (score_bits != 32 && score_bits != 64)
         return fail(MPOA_ERR_INVALID_ARGUMENT, "score_bits must be 32 or 64");
     return poa_entry(ctx, params, graph, beg_id, end_id, query, qlen, adaptive, band_b, score_bits,
                      out, out_cap, out_len, out_score);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_align(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                const uint8_t *query, uint64_t qlen, mpoa_aligned_base *out, uint64_t out_cap,
-               uint64_t *out_len) {
+               uint64_t *out_len) try {
     if (!ctx || (!query && qlen)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     Params p;
     int rc = MPOA_OK;
@@ -453,11 +472,11 @@ int mpoa_align(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *
     rc = align_one(*ctx->c, p, ViewAcc{*graph}, query, qlen, v);
     if (rc != MPOA_OK) return rc;
     return emit(v, out, out_cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 // ---- library-owned graph ----
 
-mpoa_graph *mpoa_graph_new(const mpoa_params *params) {
+mpoa_graph *mpoa_graph_new(const mpoa_params *params) try {
     if (!params) return nullptr;
     Params p = Params::from_c(*params);
     if (p.alpha() == 0) {
@@ -465,36 +484,45 @@ mpoa_graph *mpoa_graph_new(const mpoa_params *params) {
         return nullptr;
     }
     return new mpoa_graph(p);
+} catch (...) {
+    (void)exception_status();
+    return nullptr;
 }
 
 void mpoa_graph_free(mpoa_graph *g) { delete g; }
 
-uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base) { return g->g.add_node(uint32_t(base)); }
-
-void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) {
-    g->g.add_edge(uint32_t(from), uint32_t(to));
+uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base) try { return g->g.add_node(uint32_t(base)); } catch (...) {
+    (void)exception_status();
+    return 0;
 }
 
-int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *a, uint64_t len) {
+void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) try {
+    g->g.add_edge(uint32_t(from), uint32_t(to));
+} catch (...) { (void)exception_status(); }
+
+int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *a, uint64_t len) try {
     std::string e = g->g.add_path(seq_id, a, len);
     return e.empty() ? MPOA_OK : fail(MPOA_ERR_GRAPH, e);
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_graph_toposort(mpoa_graph *g, int grouped) {
+int mpoa_graph_toposort(mpoa_graph *g, int grouped) try {
     std::string e = g->g.toposort(grouped != 0);
     return e.empty() ? MPOA_OK : fail(MPOA_ERR_GRAPH, e);
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out) {
+int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out) try {
     if (!g || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = g->g.view();
     return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+uint64_t mpoa_graph_n_paths(const mpoa_graph *g) try { return g->g.paths().size(); } catch (...) {
+    (void)exception_status();
+    return 0;
 }
 
-uint64_t mpoa_graph_n_paths(const mpoa_graph *g) { return g->g.paths().size(); }
-
 uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, uint8_t *seq,
-                              uint64_t *ids, uint64_t *coverage, uint64_t cap) {
+                              uint64_t *ids, uint64_t *coverage, uint64_t cap) try {
     Params p = Params::from_c(*params);
     std::vector<uint32_t> v;
     g->g.consensus(v);
@@ -509,11 +537,14 @@ uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, ui
         }
     }
     return v.size();
+} catch (...) {
+    (void)exception_status();
+    return 0;
 }
 
 int mpoa_graph_msa(mpoa_graph *g, const mpoa_params *params, const char *const *names,
                    const char *const *comments, uint64_t nseq, char *out, uint64_t cap,
-                   uint64_t *out_len) {
+                   uint64_t *out_len) try {
     Params p = Params::from_c(*params);
     std::vector<std::string> nm(nseq), cm(nseq);
     for (uint64_t k = 0; k < nseq; k++) {
@@ -524,10 +555,10 @@ int mpoa_graph_msa(mpoa_graph *g, const mpoa_params *params, const char *const *
     std::string e = g->g.msa(p, nm, cm, text);
     if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
     return emit_text(text, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_graph_gfa(const mpoa_graph *g, const mpoa_params *params, const char *const *names,
-                   uint64_t nseq, char *out, uint64_t cap, uint64_t *out_len) {
+                   uint64_t nseq, char *out, uint64_t cap, uint64_t *out_len) try {
     Params p = Params::from_c(*params);
     std::vector<std::string> nm(nseq);
     for (uint64_t k = 0; k < nseq; k++) nm[k] = names && names[k] ? names[k] : "";
@@ -535,11 +566,11 @@ int mpoa_graph_gfa(const mpoa_graph *g, const mpoa_params *params, const char *c
     std::string e = g->g.gfa(p, nm, text);
     if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
     return emit_text(text, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_extend_graph(mpoa_ctx *ctx, const mpoa_params *params, mpoa_graph *g,
                       const uint8_t *const *seqs, const uint64_t *lens, uint64_t nseq,
-                      uint64_t existing) {
+                      uint64_t existing) try {
     if (!ctx || !g) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     Params p;
     int rc = MPOA_OK;
@@ -561,10 +592,10 @@ int mpoa_extend_graph(mpoa_ctx *ctx, const mpoa_params *params, mpoa_graph *g,
         if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
     }
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
-                         const uint64_t *lens, uint64_t nseq, mpoa_graph **out) {
+                         const uint64_t *lens, uint64_t nseq, mpoa_graph **out) try {
     if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
     Params p;
@@ -578,13 +609,13 @@ int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t
     }
     *out = g;
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 // ---- batched progressive POA ----
 
 int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
                    const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
-                   mpoa_batch **out) {
+                   mpoa_batch **out) try {
     if (!ctx || !out || (njobs && (!job_off || !read_off || !bases)))
         return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
@@ -622,24 +653,24 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     S.wall_ms = ms_since(t_all);
     *out = B.release();
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 void mpoa_batch_free(mpoa_batch *b) { delete b; }
 
-int mpoa_batch_stats_get(const mpoa_batch *b, mpoa_batch_stats *out) {
+int mpoa_batch_stats_get(const mpoa_batch *b, mpoa_batch_stats *out) try {
     if (!b || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = b->stats;
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job) {
+int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job) try {
     if (!b || job >= b->status.size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "job out of range");
     if (b->status[job] != MPOA_OK) set_error(b->err[job]);
     return b->status[job];
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out, uint64_t cap,
-                          uint64_t *out_len) {
+                          uint64_t *out_len) try {
     if (!b || job >= b->status.size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "job out of range");
     if (b->status[job] != MPOA_OK) return fail(b->status[job], b->err[job]);
     if (mode == 0) return emit_text(b->out0[job], out, cap, out_len);
@@ -651,12 +682,12 @@ int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out
         return emit_text(text, out, cap, out_len);
     }
     return fail(MPOA_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
-}
+} catch (...) { return exception_status(); }
 
 // ---- serving: jobs admitted as they arrive ----
 
 int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_server_opts *opts,
-                       mpoa_server **out) {
+                       mpoa_server **out) try {
     if (!ctx || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     Params p;
@@ -682,10 +713,10 @@ int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_serv
     S->engine = std::make_unique<Engine>(slots, *ctx->tasks, p, o);
     *out = S.release();
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *read_off, uint64_t nreads,
-                       const mpoa_graph *extend, uint64_t existing, uint64_t *ticket) {
+                       const mpoa_graph *extend, uint64_t existing, uint64_t *ticket) try {
     if (!s || !ticket || (nreads && (!bases || !read_off)))
         return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (nreads > UINT32_MAX || existing > UINT32_MAX) return fail(MPOA_ERR_INVALID_ARGUMENT, "too many reads");
@@ -705,29 +736,29 @@ int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *rea
     ji.guide = s->params.progressive && existing == 0;
     *ticket = s->engine->submit(std::move(ji));
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_done(mpoa_server *s, uint64_t ticket) {
+int mpoa_server_done(mpoa_server *s, uint64_t ticket) try {
     if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
     return s->engine->done(ticket) ? 1 : 0;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_wait(mpoa_server *s, uint64_t ticket) {
+int mpoa_server_wait(mpoa_server *s, uint64_t ticket) try {
     if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
     if (!s->engine->wait(ticket)) return fail(s->engine->error(), "the engine stopped: " + std::string(last_error()));
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_job_status(mpoa_server *s, uint64_t ticket) {
+int mpoa_server_job_status(mpoa_server *s, uint64_t ticket) try {
     if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
     if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
     const JobOut &o = s->engine->out(ticket);
     if (o.status != MPOA_OK) set_error(o.err);
     return o.status;
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out, uint64_t cap,
-                           uint64_t *out_len) {
+                           uint64_t *out_len) try {
     const int st = mpoa_server_job_status(s, ticket);
     if (st != MPOA_OK) return st;
     JobOut &o = s->engine->out(ticket);
@@ -740,9 +771,9 @@ int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out,
         return emit_text(text, out, cap, out_len);
     }
     return fail(MPOA_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out) {
+int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out) try {
     if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
     if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
@@ -752,9 +783,9 @@ int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out) 
     out->done_ms = o.t_done;
     out->cells = o.cells;
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) {
+int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) try {
     if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     const int st = mpoa_server_job_status(s, ticket);
@@ -763,15 +794,15 @@ int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) {
     g->g = *s->engine->out(ticket).graph;
     *out = g;
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
-int mpoa_server_stats_get(mpoa_server *s, mpoa_batch_stats *out) {
+int mpoa_server_stats_get(mpoa_server *s, mpoa_batch_stats *out) try {
     if (!s || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = mpoa_batch_stats{};
     fill_stats(s->engine->stats(), *out);
     out->wall_ms = s->engine->now_ms();
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 double mpoa_server_now_ms(mpoa_server *s) { return s ? s->engine->now_ms() : 0.0; }
 
@@ -813,7 +844,7 @@ int anchors_out(const std::vector<Anchor> &v, mpoa_anchor *out, uint64_t cap, ui
 extern "C" {
 
 int mpoa_sketch(const uint8_t *seq, uint64_t len, uint64_t k, uint64_t w, uint64_t *value,
-                uint64_t *end, uint64_t cap, uint64_t *out_len) {
+                uint64_t *end, uint64_t cap, uint64_t *out_len) try {
     if ((!seq && len) || (cap && (!value || !end))) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     std::vector<Minimizer> mm;
     sketch(seq, len, k, w, mm);
@@ -821,11 +852,11 @@ int mpoa_sketch(const uint8_t *seq, uint64_t len, uint64_t k, uint64_t w, uint64
         value[i] = m.value;
         end[i] = m.end;
     });
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_collect_anchors(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
                          const uint8_t *target, uint64_t tlen, mpoa_anchor *out, uint64_t cap,
-                         uint64_t *out_len) {
+                         uint64_t *out_len) try {
     if ((!query && qlen) || (!target && tlen) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     Params p;
     int rc = MPOA_OK;
@@ -833,30 +864,30 @@ int mpoa_collect_anchors(const mpoa_params *params, const uint8_t *query, uint64
     std::vector<Anchor> a;
     collect_anchors(p, query, qlen, target, tlen, a);
     return anchors_out(a, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_chain(const mpoa_anchor *anchors, uint64_t n, uint64_t k, mpoa_anchor *out, uint64_t cap,
-               uint64_t *out_len) {
+               uint64_t *out_len) try {
     if ((!anchors && n) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     std::vector<Anchor> in, c;
     anchors_in(anchors, n, in);
     chain(in, k, c);
     return anchors_out(c, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_partition_windows(const mpoa_anchor *chain_in, uint64_t n, uint64_t tlen, uint64_t qlen,
                            uint64_t max_drift, uint64_t max_win, mpoa_anchor *out, uint64_t cap,
-                           uint64_t *out_len, int *tail_adaptive) {
+                           uint64_t *out_len, int *tail_adaptive) try {
     if ((!chain_in && n) || (cap && !out)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     std::vector<Anchor> in, w;
     anchors_in(chain_in, n, in);
     const bool tail = partition_windows(in, tlen, qlen, max_drift, max_win, w);
     if (tail_adaptive) *tail_adaptive = tail ? 1 : 0;
     return anchors_out(w, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_guide_order(const mpoa_params *params, const uint8_t *const *seqs, const uint64_t *lens,
-                     uint64_t nseq, uint64_t *order) {
+                     uint64_t nseq, uint64_t *order) try {
     if (nseq && (!seqs || !lens || !order)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     if (nseq > UINT32_MAX) return fail(MPOA_ERR_INVALID_ARGUMENT, "too many sequences");
     Params p;
@@ -866,11 +897,11 @@ int mpoa_guide_order(const mpoa_params *params, const uint8_t *const *seqs, cons
     guide_order(p, seqs, lens, nseq, o);
     for (uint64_t i = 0; i < nseq; i++) order[i] = o[i];
     return MPOA_OK;
-}
+} catch (...) { return exception_status(); }
 
 int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_t qlen,
                         const uint8_t *consensus, uint64_t clen, mpoa_anchor *out, uint64_t cap,
-                        uint64_t *out_len, int *tail_adaptive) {
+                        uint64_t *out_len, int *tail_adaptive) try {
     if ((!query && qlen) || (!consensus && clen) || (cap && !out))
         return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     Params p;
@@ -880,6 +911,6 @@ int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_
     const bool tail = anchor_windows(p, query, qlen, consensus, clen, w);
     if (tail_adaptive) *tail_adaptive = tail ? 1 : 0;
     return anchors_out(w, out, cap, out_len);
-}
+} catch (...) { return exception_status(); }
 
 }  // extern "C"

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
+#include <new>
 
 #include "minipoa_hip.h"
 #include "plan.hpp"
@@ -639,7 +641,14 @@ void Engine::task(Job &J, bool have) {
     }
     tasks_.push([this, jp = &J, have] {
         const auto t0 = std::chrono::steady_clock::now();
-        advance(*jp, have);
+        try {
+            advance(*jp, have);
+        } catch (const std::bad_alloc &) {
+            // The job's step did not complete: it ends with an error.
+            if (!jp->out.done) fail_job(*jp, MPOA_ERR_ALLOC, "host memory allocation failed");
+        } catch (const std::exception &e) {
+            if (!jp->out.done) fail_job(*jp, MPOA_ERR_INVALID_ARGUMENT, std::string("unexpected error: ") + e.what());
+        }
         task_ns_ += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
         std::lock_guard<std::mutex> lk(mu_);
         outstanding_--;
@@ -651,6 +660,18 @@ void Engine::task(Job &J, bool have) {
 // stream and hand its jobs back to the host workers. A slot's packing and
 // waiting never hold up another slot.
 void Engine::slot_main(size_t si) {
+    try {
+        slot_loop(si);
+    } catch (...) {
+        // A slot that cannot go on (host allocation) stops the engine.
+        std::lock_guard<std::mutex> lk(mu_);
+        rc_ = MPOA_ERR_ALLOC;
+        stop_ = true;
+        cv_.notify_all();
+    }
+}
+
+void Engine::slot_loop(size_t si) {
     Ctx &c = *slots_[si];
     const bool crit_slot = o_.crit_lane && si == 0;
     // Ordinary slots: the first long_slots take long DPs first, then short

@@ -134,6 +134,7 @@ private:
     void task(Job &J, bool have);
     void admit_locked(Job &J);
     void slot_main(size_t si);
+    void slot_loop(size_t si);
     uint64_t footprint_estimate(const Job &J) const;
 
     std::vector<Ctx *> slots_;
