@@ -261,3 +261,36 @@ def test_seeded_incremental_extension_matches_oracle(ctx, oracle):
         srv.wait(t1)
         assert srv.status(t1) == 0
         assert (srv.output(t1, 0), srv.output(t1, 1), srv.output(t1, 2)) == _oracle_bytes(oracle, p, reads)
+
+
+def test_seeded_edge_reads_match_oracle(ctx, oracle):
+    """Seeded jobs with an empty read, a read shorter than k, an all-N read
+    (no minimizers: the whole-graph fallback), an exact copy (dense
+    anchors), a copy with an inverted segment (the chain breaks: adaptive
+    windows) and a truncated copy, through the batch and the one-call path;
+    and anchored DPs with an empty query between interior nodes (a window
+    whose query gap is zero)."""
+    rng = np.random.Generator(np.random.PCG64(53))
+    t = synth.ACGT[rng.integers(0, 4, size=1500)].tobytes()
+    comp = bytes.maketrans(b"ACGT", b"TGCA")
+    inv = t[:500] + t[500:900][::-1].translate(comp) + t[900:]
+    reads = [t, b"", b"ACG", b"N" * 300, t, inv, t[100:1400]]
+    p = Params(seeding=True, k=11, w=3, win_band=40, win_max=150, tr_dist=0)
+    want = _oracle_bytes(oracle, p, reads)
+    jo, ro, b = synth._pack([reads])
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    assert res.status(0) == 0, mh.last_error()
+    assert (res.output(0, 0), res.output(0, 1), res.output(0, 2)) == want
+    seqs = records(reads)
+    g = mh.align_sequences(ctx, p, seqs)
+    assert (g.msa(p, seqs), g.gfa(p, seqs)) == want[1:]
+    og = oracle.align_sequences(p, seqs)
+    view = mh.view_from_graph(og, len(p.alphabet))
+    ids = og.consensus(p)[1]
+    for i, j in [(0, 1), (10, 11), (100, 140), (600, 1100)]:
+        for adaptive, band in [(False, 40), (True, 100)]:
+            want1 = oracle.result_or_error(oracle.poa, p, og, ids[i], ids[j], b"", adaptive, band)
+            got1 = mh.result_or_error(mh.poa, ctx, p, view, ids[i], ids[j], b"", adaptive, band)
+            assert got1[0] == want1[0], (i, j, got1, want1)
+            if want1[0] == "ok":
+                assert [tuple(x) for x in got1[1]] == [tuple(x) for x in want1[1]]
