@@ -294,3 +294,28 @@ def test_seeded_edge_reads_match_oracle(ctx, oracle):
             assert got1[0] == want1[0], (i, j, got1, want1)
             if want1[0] == "ok":
                 assert [tuple(x) for x in got1[1]] == [tuple(x) for x in want1[1]]
+
+
+def test_seeded_window_retries_with_full_matrix(ctx, oracle):
+    """With win_band 0 the static band of many windows (0 + m/40) misses the
+    path around 40-base indels: each such window's banded DP fails and poa
+    retries it with the full matrix (src/align.rs:202-207) -- 77 of 425
+    windows on the oracle for these reads. Batch and one-call path equal the
+    oracle's bytes."""
+    rng = np.random.Generator(np.random.PCG64(61))
+    t = rng.integers(0, 4, size=3000).astype(np.uint8)
+    reads = []
+    for _ in range(6):
+        r = synth.mutate(rng, t, 0.03)
+        a, b = int(rng.integers(300, 1300)), int(rng.integers(1600, 2600))
+        reads.append(r[:a] + r[a + 40:b] + synth.ACGT[rng.integers(0, 4, size=40)].tobytes() + r[b:])
+    p = Params(seeding=True, k=15, w=5, win_band=0, win_max=300)
+    want = _oracle_bytes(oracle, p, reads)
+    jo, ro, b = synth._pack([reads])
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    assert res.status(0) == 0, mh.last_error()
+    assert res.stats()["retries"] > 0
+    assert (res.output(0, 0), res.output(0, 1), res.output(0, 2)) == want
+    seqs = records(reads)
+    g = mh.align_sequences(ctx, p, seqs)
+    assert (g.msa(p, seqs), g.gfa(p, seqs)) == want[1:]
