@@ -319,3 +319,16 @@ def test_seeded_window_retries_with_full_matrix(ctx, oracle):
     seqs = records(reads)
     g = mh.align_sequences(ctx, p, seqs)
     assert (g.msa(p, seqs), g.gfa(p, seqs)) == want[1:]
+
+
+def test_seeded_protein_family_takes_the_whole_graph_fallback(ctx, oracle):
+    """Seeding with a 20-letter alphabet has no anchors (src/minimizer.rs:
+    129-131), so every read aligns to the whole graph with the adaptive band
+    (src/align.rs:101-103); bytes equal the oracle's."""
+    sp, jo, ro, b = synth.protein_families(nfam=3, lmin=120, lmax=200, members=6)
+    p = Params(seeding=True, alphabet=sp.alphabet, matrix=sp.matrix, f=40)
+    res = mh.batch_run(ctx, p, jo, ro, b)
+    for j in range(len(jo) - 1):
+        reads = synth.job_reads(jo, ro, b, j)
+        assert res.status(j) == 0, mh.last_error()
+        assert (res.output(j, 0), res.output(j, 1), res.output(j, 2)) == _oracle_bytes(oracle, p, reads)
