@@ -10,6 +10,7 @@
 
 #include "graph.hpp"
 #include "minipoa_hip.h"
+#include "plan.hpp"
 
 namespace mpoa {
 
@@ -101,6 +102,12 @@ void heaviest_path(const G &g, std::vector<uint32_t> &ids) {
     std::reverse(ids.begin(), ids.end());
 }
 
+// The consensus path's ids: the library graph's own (flat-array) walk, or
+// the generic walk over a caller's view.
+template <class G>
+void consensus_ids(const G &g, std::vector<uint32_t> &ids) { heaviest_path(g, ids); }
+inline void consensus_ids(const GraphAcc &g, std::vector<uint32_t> &ids) { g.g.consensus(ids); }
+
 // src/align.rs:99-162: the windows of a seeded alignment of q against the
 // graph's consensus. An empty chain or an unusable anchor gives one
 // whole-graph window with the adaptive band. Returns MPOA_OK, or
@@ -110,7 +117,7 @@ int seed_windows(const Params &p, const G &g, const uint8_t *q, uint64_t qlen,
                  std::vector<SeedWindow> &out, std::string &err) {
     out.clear();
     std::vector<uint32_t> ids;
-    heaviest_path(g, ids);
+    consensus_ids(g, ids);
     std::string cons(ids.size(), 'N');
     for (size_t k = 0; k < ids.size(); k++) cons[k] = char(p.decode(g.base(ids[k])));
     std::vector<Anchor> an;
