@@ -423,12 +423,17 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     FillShape shape = ctx.force_shape.cols ? ctx.force_shape : fill_shape_for_width(wexp);
     P.ring_bytes = ring_bytes_for_width(wexp, i64);
     // A launch of few DPs is bound by their row chains, not by how many
-    // workgroups fit a CU: give it the largest ring (clamped to the LDS left
-    // below), so fewer predecessor rows come from HBM. Measured on one
-    // seeded 100-read job (~7 window DPs per launch): 48 KB instead of 16 KB
-    // cut the fill time by 16% and the run by 12%.
+    // workgroups fit a CU. It gets the largest ring (clamped to the LDS left
+    // below), so fewer predecessor rows come from HBM: on one seeded 100-read
+    // job (~7 window DPs per launch) 48 KB instead of 16 KB cut the fill
+    // time by 16%. Narrow rows also take two waves of four columns per lane
+    // instead of one wave of eight (a shorter row chain): one 2 kb job of
+    // 10 reads, one DP per launch, ran in 72 ms instead of 84.
     constexpr size_t kFewDps = 64;
-    if (N <= kFewDps) P.ring_bytes = std::max<uint32_t>(P.ring_bytes, 49152u);
+    if (N <= kFewDps) {
+        P.ring_bytes = std::max<uint32_t>(P.ring_bytes, 49152u);
+        if (!ctx.force_shape.cols && !i64 && wexp <= 384u) shape = FillShape{4, 2};
+    }
     TuneEntry tuned{};
     if (!ctx.force_shape.cols && !i64 && P.seq_lds != 0 && tuned_for(wexp, tuned)) {
         shape = tuned.shape;
