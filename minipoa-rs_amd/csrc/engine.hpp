@@ -59,7 +59,8 @@ void par_for(TaskQueue &tq, size_t n, const std::function<void(size_t)> &f);
 // One job: reads read_off[k] .. read_off[k+1] of `bases`, k < nreads, aligned
 // progressively into `graph` (a new graph when null). `first` is the number
 // of sequences already in the graph (extend_graph's `existing`): read k gets
-// sequence id first + k. The reads are borrowed unless `own` holds them.
+// sequence id first + k, also when `guide` changes the order in which the
+// reads are aligned. The reads are borrowed unless `own` holds them.
 struct JobIn {
     const uint8_t *bases = nullptr;
     const uint64_t *read_off = nullptr;
