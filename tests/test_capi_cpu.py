@@ -197,3 +197,27 @@ def test_library_ranks_match_oracle_on_a_large_graph(oracle):
         lg.toposort(False)
         assert lg.rank == og.rank, sid
     assert lg.n_nodes > 4000
+
+
+def test_seeding_entry_points_check_arguments():
+    """The seeding exports reject NULL buffers with a status, report the full
+    count when the buffer is too small, and accept empty inputs."""
+    import ctypes as C
+    L = mh.lib()
+    n = C.c_uint64(0)
+    u64 = C.POINTER(C.c_uint64)
+    # NULL sequence with a length; NULL outputs with a capacity.
+    assert L.mpoa_sketch(None, 5, 3, 2, None, None, 0, C.byref(n)) == 17
+    assert L.mpoa_sketch(b"ACGTACGT", 8, 3, 1, None, None, 4, C.byref(n)) == 17
+    # Too small: status 16 and the count needed.
+    v = (C.c_uint64 * 1)()
+    e = (C.c_uint64 * 1)()
+    assert L.mpoa_sketch(b"ACGTACGTAC", 10, 3, 1, v, e, 1, C.byref(n)) == 16
+    assert n.value == 8
+    # Empty input: no minimizers.
+    assert L.mpoa_sketch(b"", 0, 3, 2, v, e, 1, C.byref(n)) == 0 and n.value == 0
+    cp = mh.CParams(Params())
+    assert L.mpoa_guide_order(cp.ref, None, None, 3, None) == 17
+    assert L.mpoa_guide_order(cp.ref, None, None, 0, None) == 0
+    assert L.mpoa_chain(None, 2, 19, None, 0, C.byref(n)) == 17
+    assert L.mpoa_anchor_windows(None, b"A", 1, b"A", 1, None, 0, C.byref(n), None) == 17
