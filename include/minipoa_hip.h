@@ -190,8 +190,12 @@ int mpoa_anchor_windows(const mpoa_params *params, const uint8_t *query, uint64_
 
 mpoa_graph *mpoa_graph_new(const mpoa_params *params);                 /* graph.rs:30-40 */
 void mpoa_graph_free(mpoa_graph *g);
-uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base);            /* graph.rs:42-61 */
-void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to);   /* graph.rs:63-80 */
+/* graph.rs:42-61: the new node's id, or UINT64_MAX (message set) when base
+ * is outside the alphabet (the reference panics on that index). */
+uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base);
+/* graph.rs:63-80: MPOA_ERR_INVALID_ARGUMENT, graph unchanged, when an id is
+ * not a node of g (the reference panics on that index). */
+int mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to);
 int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *alignment,
                         uint64_t len);                                 /* graph.rs:82-122 */
 int mpoa_graph_toposort(mpoa_graph *g, int grouped);                   /* graph.rs:124-178 */

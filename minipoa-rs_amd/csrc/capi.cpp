@@ -491,14 +491,35 @@ mpoa_graph *mpoa_graph_new(const mpoa_params *params) try {
 
 void mpoa_graph_free(mpoa_graph *g) { delete g; }
 
-uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base) try { return g->g.add_node(uint32_t(base)); } catch (...) {
+// The reference panics on an out-of-range base or node id (Rust bounds
+// checks, graph.rs:42-80); here they are refused before the graph changes.
+uint64_t mpoa_graph_add_node(mpoa_graph *g, uint64_t base) try {
+    if (!g) {
+        set_error("graph is NULL");
+        return UINT64_MAX;
+    }
+    if (base >= g->g.alpha()) {
+        set_error("node base outside the alphabet");
+        return UINT64_MAX;
+    }
+    if (g->g.size() >= uint64_t(UINT32_MAX)) {
+        set_error("graph holds the maximum number of nodes");
+        return UINT64_MAX;
+    }
+    return g->g.add_node(uint32_t(base));
+} catch (...) {
     (void)exception_status();
-    return 0;
+    return UINT64_MAX;
 }
 
-void mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) try {
+int mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) try {
+    if (!g) return fail(MPOA_ERR_INVALID_ARGUMENT, "graph is NULL");
+    if (from >= g->g.size() || to >= g->g.size())
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "edge references an unknown node");
     g->g.add_edge(uint32_t(from), uint32_t(to));
-} catch (...) { (void)exception_status(); }
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
 
 int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *a, uint64_t len) try {
     std::string e = g->g.add_path(seq_id, a, len);

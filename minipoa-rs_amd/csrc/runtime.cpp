@@ -434,8 +434,10 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         P.ring_bytes = std::max<uint32_t>(P.ring_bytes, 49152u);
         if (!ctx.force_shape.cols && !i64 && wexp <= 384u) shape = FillShape{4, 2};
     }
+    // The tuning table is measured on full launches (calibration batches of
+    // hundreds of jobs); launches of few DPs keep the rule above.
     TuneEntry tuned{};
-    if (!ctx.force_shape.cols && !i64 && P.seq_lds != 0 && tuned_for(wexp, tuned)) {
+    if (N > kFewDps && !ctx.force_shape.cols && !i64 && P.seq_lds != 0 && tuned_for(wexp, tuned)) {
         shape = tuned.shape;
         P.ring_bytes = tuned.ring_bytes;
     }

@@ -28,6 +28,7 @@ STATUS = {
     19: "MPOA_ERR_SEED_SPAN",
 }
 BUFFER_TOO_SMALL = 16
+INVALID_ARGUMENT = 17
 
 
 class HipPoaError(RuntimeError):
@@ -124,7 +125,7 @@ SIGNATURES = {
     "mpoa_graph_new": (_vp, [_P]),
     "mpoa_graph_free": (None, [_vp]),
     "mpoa_graph_add_node": (_u64, [_vp, _u64]),
-    "mpoa_graph_add_edge": (None, [_vp, _u64, _u64]),
+    "mpoa_graph_add_edge": (C.c_int, [_vp, _u64, _u64]),
     "mpoa_graph_add_path": (C.c_int, [_vp, _u64, _B, _u64]),
     "mpoa_graph_toposort": (C.c_int, [_vp, C.c_int]),
     "mpoa_graph_view_of": (C.c_int, [_vp, _V]),
@@ -322,10 +323,13 @@ class Graph:
             self._h = None
 
     def add_node(self, base):
-        return lib().mpoa_graph_add_node(self._h, base)
+        nid = lib().mpoa_graph_add_node(self._h, base)
+        if nid == (1 << 64) - 1:
+            raise HipPoaError(INVALID_ARGUMENT, last_error())
+        return nid
 
     def add_edge(self, a, b):
-        lib().mpoa_graph_add_edge(self._h, a, b)
+        _check(lib().mpoa_graph_add_edge(self._h, a, b))
 
     def add_path(self, seq_id, path):
         arr = (_Base * max(1, len(path)))()

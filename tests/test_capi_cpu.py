@@ -116,6 +116,35 @@ def test_add_path_rejects_bad_records():
         g.add_path(0, [AlignedBase(7, 0, None)])
 
 
+def test_graph_node_and_edge_exports_check_arguments():
+    """The reference panics (Rust bounds checks, src/graph.rs:42-80) on a
+    base outside the alphabet or an unknown node id; the C ABI refuses them
+    and leaves the graph unchanged."""
+    import ctypes as C
+    L = mh.lib()
+    p = Params()
+    g = mh.Graph(p)
+    alpha = len(p.alphabet)
+    assert L.mpoa_graph_add_node(g._h, alpha) == (1 << 64) - 1
+    assert "alphabet" in mh.last_error()
+    assert L.mpoa_graph_add_node(g._h, (1 << 32) + 1) == (1 << 64) - 1
+    assert L.mpoa_graph_add_node(None, 0) == (1 << 64) - 1
+    with pytest.raises(mh.HipPoaError):
+        g.add_node(alpha)
+    assert g.n_nodes == 2
+    a = g.add_node(0)
+    assert a == 2
+    assert L.mpoa_graph_add_edge(g._h, 0, 3) == 17
+    assert L.mpoa_graph_add_edge(g._h, (1 << 32) + 2, a) == 17
+    assert L.mpoa_graph_add_edge(None, 0, 1) == 17
+    with pytest.raises(mh.HipPoaError, match="unknown node"):
+        g.add_edge(a, 99)
+    g.add_edge(0, a)
+    g.add_edge(a, 1)
+    assert g.n_nodes == 3
+    assert g.node(a)["incoming"] == [0] and g.node(a)["outgoing"] == [1]
+
+
 def _same_graph(a, b, alpha):
     assert a.n_nodes == b.n_nodes
     assert a.rank == b.rank
