@@ -44,7 +44,8 @@ typedef enum {
     MPOA_ERR_BUFFER_TOO_SMALL = 16,       /* *out_len holds the size needed */
     MPOA_ERR_INVALID_ARGUMENT = 17,
     MPOA_ERR_PENDING = 18,                /* a served job has not finished yet */
-    MPOA_ERR_SEED_SPAN = 19               /* align.rs:116-130 invalid minimizer span */
+    MPOA_ERR_SEED_SPAN = 19,              /* align.rs:116-130 invalid minimizer span */
+    MPOA_ERR_SCRATCH_LIMIT = 20           /* a DP's buffers exceed mpoa_poa_noalloc's scratch */
 } mpoa_status;
 
 /* src/lib.rs:47-71 Params. `alphabet` and `matrix` are borrowed for the
@@ -123,6 +124,22 @@ int mpoa_poa(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *gr
              uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
              int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
              uint64_t *out_len, int64_t *out_score);
+/* No counterpart in the reference: mpoa_poa for callers that cannot allocate
+ * device memory (code around HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors as
+ * mpoa_poa, but every device buffer of its launches (inputs, results, row
+ * records, score arena) is carved from a fixed 64 MiB array that each GPU
+ * the library runs on holds from its first context until the process ends,
+ * and host staging uses pageable memory: the call makes no hipMalloc,
+ * hipFree or hipHostMalloc. A DP whose buffers exceed 64 MiB returns
+ * MPOA_ERR_SCRATCH_LIMIT (for the banded attempt or for poa's full-DP
+ * retry). The array keeps the last call's contents; calls on one GPU share
+ * it, so the library runs them one at a time (a per-device lock), and two
+ * processes each have their own copy. */
+int mpoa_poa_noalloc(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                     uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                     int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
+                     uint64_t *out_len, int64_t *out_score);
 /* src/align.rs:531-545 `poa_inner`: one attempt, no retry; the i32 kernel
  * under the score guard (src/align.rs:493-529), else the i64 kernel. */
 int mpoa_poa_inner(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,

@@ -77,7 +77,7 @@ bool is_dp_error(int s) { return s >= MPOA_ERR_ANCHOR_NOT_IN_GRAPH && s <= MPOA_
 template <class G>
 int poa_once(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end, const uint8_t *q,
              uint64_t qlen, bool adaptive, uint64_t band_b, int force_bits,
-             std::vector<AlignedBase> &out, int64_t *score) {
+             std::vector<AlignedBase> &out, int64_t *score, const FixedScratch *fx = nullptr) {
     JobPlan J;
     plan_job(p, g, beg, end, q, qlen, adaptive, band_b, force_bits, J);
     if (J.status != MPOA_OK) return fail(J.status, J.err);
@@ -85,7 +85,7 @@ int poa_once(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end, 
     std::vector<DevResult> res;
     std::vector<uint32_t> tb;
     std::vector<uint64_t> off;
-    int rc = run_plans(ctx, p, plans, res, tb, off, nullptr);
+    int rc = run_plans(ctx, p, plans, res, tb, off, nullptr, fx);
     if (rc != MPOA_OK) return rc;
     std::string msg;
     int s = device_status(res[0], msg);
@@ -99,12 +99,12 @@ int poa_once(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end, 
 template <class G>
 int poa_retry(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end,
               const uint8_t *q, uint64_t qlen, bool adaptive, uint64_t band_b,
-              std::vector<AlignedBase> &out, int64_t *score) {
-    int rc = poa_once(ctx, p, g, beg, end, q, qlen, adaptive, band_b, 0, out, score);
+              std::vector<AlignedBase> &out, int64_t *score, const FixedScratch *fx = nullptr) {
+    int rc = poa_once(ctx, p, g, beg, end, q, qlen, adaptive, band_b, 0, out, score, fx);
     if (rc != MPOA_OK && p.f != 0 && is_dp_error(rc)) {
         Params full = p;
         full.f = 0;
-        rc = poa_once(ctx, full, g, beg, end, q, qlen, false, band_b, 0, out, score);
+        rc = poa_once(ctx, full, g, beg, end, q, qlen, false, band_b, 0, out, score, fx);
     }
     return rc;
 }
@@ -443,6 +443,33 @@ int mpoa_poa(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *gr
                      out_cap, out_len, out_score);
 } catch (...) { return exception_status(); }
 
+// One lock per device: calls of the no-allocation entry point on one GPU
+// share its scratch array, so they run one at a time within the process.
+static std::mutex g_noalloc_mu[kScratchDevices];
+
+int mpoa_poa_noalloc(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
+                     uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
+                     int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
+                     uint64_t *out_len, int64_t *out_score) try {
+    if (!ctx || (!query && qlen)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc) || !view_ok(graph, p, rc)) return rc;
+    const int dev = ctx->c->device;
+    FixedScratch fx;
+    fx.dev = static_cast<char *>(scratch_device_ptr(dev));
+    fx.bytes = kScratchBytes;
+    if (!fx.dev) return fail(MPOA_ERR_DEVICE, "the device scratch of this GPU is not available");
+    std::vector<AlignedBase> v;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu[dev]);
+        rc = poa_retry(*ctx->c, p, ViewAcc{*graph}, beg_id, end_id, query, qlen, adaptive != 0,
+                       band_b, v, out_score, &fx);
+    }
+    if (rc != MPOA_OK) return rc;
+    return emit(v, out, out_cap, out_len);
+} catch (...) { return exception_status(); }
+
 int mpoa_poa_inner(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_graph_view *graph,
                    uint64_t beg_id, uint64_t end_id, const uint8_t *query, uint64_t qlen,
                    int adaptive, uint64_t band_b, mpoa_aligned_base *out, uint64_t out_cap,
@@ -519,7 +546,6 @@ int mpoa_graph_add_edge(mpoa_graph *g, uint64_t from, uint64_t to) try {
     g->g.add_edge(uint32_t(from), uint32_t(to));
     return MPOA_OK;
 } catch (...) { return exception_status(); }
-
 
 int mpoa_graph_add_path(mpoa_graph *g, uint64_t seq_id, const mpoa_aligned_base *a, uint64_t len) try {
     std::string e = g->g.add_path(seq_id, a, len);

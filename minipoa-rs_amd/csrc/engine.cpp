@@ -776,7 +776,7 @@ void Engine::slot_loop(size_t si) {
             cv_.notify_all();
             return;
         }
-        const DevResult *hr = launch_results(c);
+        const DevResult *hr = launch_results(li);
         for (size_t k = 0; k < batch.size(); k++) {
             Dp &D = *batch[k];
             D.res = hr[k];

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "dp_types.hpp"
 #include "kernels.hpp"
@@ -1201,6 +1202,48 @@ void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s
         if (lt) hipLaunchKernelGGL((poa_traceback<int32_t, true>), grid, dim3(64), lds, s, P, njobs);
         else hipLaunchKernelGGL((poa_traceback<int32_t, false>), grid, dim3(64), 0, s, P, njobs);
     }
+}
+
+// Device scratch of the no-allocation entry point: one fixed array per GPU,
+// loaded with this code object and held until the process ends. Its name in
+// host code is a host-side stand-in, not a device address; the address is
+// looked up once per device (hipGetSymbolAddress, device current) and only
+// that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices];
+bool g_scratch_done[kScratchDevices];
+hipError_t g_scratch_err[kScratchDevices];
+}  // namespace
+
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= int(kScratchDevices)) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return g_scratch_err[device];
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    void *ptr = nullptr;
+    if (e == hipSuccess) {
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+    g_scratch_err[device] = e;
+    g_scratch_done[device] = true;
+    return e;
+}
+
+void *scratch_device_ptr(int device) {
+    if (device < 0 || device >= int(kScratchDevices)) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
 }
 
 }  // namespace mpoa

@@ -53,4 +53,13 @@ uint32_t ring_bytes_for_width(uint32_t width, bool i64);
 int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStream_t s);
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s);
 
+// Fixed device scratch of the no-allocation entry point (64 MiB per GPU,
+// a static device array of kernels.hip). scratch_init looks up its address
+// on `device` (once per device; later calls return the first result) and
+// scratch_device_ptr returns it (nullptr before a successful scratch_init).
+constexpr uint64_t kScratchBytes = uint64_t(64) << 20;
+constexpr uint32_t kScratchDevices = 64;
+hipError_t scratch_init(int device);
+void *scratch_device_ptr(int device);
+
 }  // namespace mpoa

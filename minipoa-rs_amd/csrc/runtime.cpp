@@ -189,6 +189,9 @@ int ctx_create(int device, Ctx **out, unsigned threads) {
     Ctx *c = new Ctx();
     c->device = device;
     hipError_t e = hipSetDevice(device);
+    // The no-allocation entry point's scratch address on this device (looked
+    // up once per device, at the first context).
+    if (e == hipSuccess) e = scratch_init(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (auto &ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
@@ -271,7 +274,7 @@ bool tuned_for(uint32_t width, TuneEntry &out) {
 }  // namespace
 
 int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
-                 bool full_arena, LaunchInfo &li) {
+                 bool full_arena, LaunchInfo &li, const FixedScratch *fx) {
     const size_t N = plans.size();
     const size_t tsz = i64 ? 8 : 4;
     li.n = N;
@@ -312,8 +315,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     // their estimate take chunks of at least three full-width rows).
     const uint64_t chunk = align_up(std::max<uint64_t>(65536, 3ull * maxpad), 64);
     const uint64_t pool_beg = arena;
-    const uint64_t pool_len = std::max<uint64_t>(arena / 2, 4 * chunk);
-    arena += pool_len;
+    uint64_t pool_len = std::max<uint64_t>(arena / 2, 4 * chunk);
     const size_t alpha = p.alpha();
     size_t b_ctr = 0;  // the pool's next-chunk counter
     size_t b_jobs = 256;
@@ -323,6 +325,27 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     size_t b_seq = align_up(b_preds + preds * sizeof(PredE), 256);
     size_t b_mat = align_up(b_seq + seq, 256);
     size_t b_end = align_up(b_mat + alpha * alpha * sizeof(int32_t), 256);
+
+    // Fixed scratch: [inputs][results][row meta][adaptive records][traceback
+    // records][arena], the pool taking whatever the scratch has left.
+    const uint64_t s_res = align_up(b_end, 256);
+    const uint64_t s_meta = align_up(s_res + N * sizeof(DevResult), 256);
+    const uint64_t s_adapt = align_up(s_meta + std::max<uint64_t>(meta, 1) * sizeof(RowMeta), 256);
+    const uint64_t s_tb = align_up(s_adapt + std::max<uint64_t>(adapt, 1) * sizeof(RowRec), 256);
+    const uint64_t s_arena = align_up(s_tb + std::max<uint64_t>(tbn, 1) * sizeof(uint32_t), 256);
+    if (fx) {
+        const uint64_t need = s_arena + arena * tsz;
+        if (need > fx->bytes) {
+            char buf[200];
+            snprintf(buf, sizeof buf,
+                     "the DP needs %llu bytes of device buffers; the no-allocation scratch holds %llu",
+                     (unsigned long long)need, (unsigned long long)fx->bytes);
+            set_error(buf);
+            return MPOA_ERR_SCRATCH_LIMIT;
+        }
+        pool_len = (fx->bytes - s_arena) / tsz - arena;
+    }
+    arena += pool_len;
 
     HIP_TRY(hipSetDevice(ctx.device));
     li.grown = 0;
@@ -335,8 +358,17 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         nbuf++;
         return e;
     };
-    HIP_TRY(grow(ctx.h_in, b_end));
-    char *h = static_cast<char *>(ctx.h_in.p);
+    // Host staging: the context's pinned buffers, or pageable memory of the
+    // calling thread for a fixed-scratch launch.
+    static thread_local std::vector<char> t_in, t_res, t_tb;
+    char *h;
+    if (fx) {
+        if (t_in.size() < b_end) t_in.resize(b_end);
+        h = t_in.data();
+    } else {
+        HIP_TRY(grow(ctx.h_in, b_end));
+        h = static_cast<char *>(ctx.h_in.p);
+    }
     ctx.pool->parallel_for(N, [&](size_t k) {
         const JobPlan &J = *plans[k];
         const Off &o = off[k];
@@ -373,30 +405,53 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     const unsigned long long ctr0 = pool_beg;
     memcpy(h + b_ctr, &ctr0, sizeof ctr0);
 
-    HIP_TRY(grow(ctx.d_in, b_end));
-    HIP_TRY(grow(ctx.d_res, N * sizeof(DevResult)));
-    HIP_TRY(grow(ctx.d_meta, std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
-    HIP_TRY(grow(ctx.d_adapt, std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
-    HIP_TRY(grow(ctx.d_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
-    HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 1) * tsz));
+    char *d;
+    void *d_res, *d_meta, *d_adapt, *d_tb, *d_arena;
+    if (fx) {
+        if (t_res.size() < N * sizeof(DevResult)) t_res.resize(N * sizeof(DevResult));
+        if (t_tb.size() < std::max<uint64_t>(tbn, 1) * sizeof(uint32_t))
+            t_tb.resize(std::max<uint64_t>(tbn, 1) * sizeof(uint32_t));
+        li.hres = t_res.data();
+        li.htb = t_tb.data();
+        d = fx->dev;
+        d_res = fx->dev + s_res;
+        d_meta = fx->dev + s_meta;
+        d_adapt = fx->dev + s_adapt;
+        d_tb = fx->dev + s_tb;
+        d_arena = fx->dev + s_arena;
+    } else {
+        HIP_TRY(grow(ctx.d_in, b_end));
+        HIP_TRY(grow(ctx.d_res, N * sizeof(DevResult)));
+        HIP_TRY(grow(ctx.d_meta, std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
+        HIP_TRY(grow(ctx.d_adapt, std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
+        HIP_TRY(grow(ctx.d_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+        HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 1) * tsz));
+        HIP_TRY(grow(ctx.h_res, N * sizeof(DevResult)));
+        HIP_TRY(grow(ctx.h_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
+        li.hres = ctx.h_res.p;
+        li.htb = ctx.h_tb.p;
+        d = static_cast<char *>(ctx.d_in.p);
+        d_res = ctx.d_res.p;
+        d_meta = ctx.d_meta.p;
+        d_adapt = ctx.d_adapt.p;
+        d_tb = ctx.d_tb.p;
+        d_arena = ctx.d_arena.p;
+    }
     li.arena_bytes = std::max<uint64_t>(arena, 1) * tsz;
     li.rows = meta;
-    HIP_TRY(grow(ctx.h_res, N * sizeof(DevResult)));
-    HIP_TRY(grow(ctx.h_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
 
-    char *d = static_cast<char *>(ctx.d_in.p);
     DevPools P;
     P.jobs = reinterpret_cast<const DevJob *>(d + b_jobs);
-    P.res = static_cast<DevResult *>(ctx.d_res.p);
+    P.res = static_cast<DevResult *>(d_res);
     P.rows = reinterpret_cast<const RowIn *>(d + b_rows);
     P.aux = reinterpret_cast<const RowAux *>(d + b_aux);
     P.preds = reinterpret_cast<const PredE *>(d + b_preds);
     P.seq = reinterpret_cast<const uint8_t *>(d + b_seq);
     P.matrix = reinterpret_cast<const int32_t *>(d + b_mat);
-    P.arena = ctx.d_arena.p;
-    P.meta = static_cast<RowMeta *>(ctx.d_meta.p);
-    P.adapt = static_cast<RowRec *>(ctx.d_adapt.p);
-    P.tb = static_cast<uint32_t *>(ctx.d_tb.p);
+    P.arena = d_arena;
+    P.meta = static_cast<RowMeta *>(d_meta);
+    P.adapt = static_cast<RowRec *>(d_adapt);
+    P.tb = static_cast<uint32_t *>(d_tb);
     P.pool_next = reinterpret_cast<unsigned long long *>(d + b_ctr);
     P.pool_end = pool_beg + pool_len;
     P.chunk = uint32_t(chunk);
@@ -469,10 +524,10 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     launch_traceback(P, uint32_t(N), i64, ctx.stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx.ev[2], ctx.stream));
-    HIP_TRY(hipMemcpyAsync(ctx.h_res.p, P.res, N * sizeof(DevResult), hipMemcpyDeviceToHost,
-                           ctx.stream));
-    HIP_TRY(hipMemcpyAsync(ctx.h_tb.p, P.tb, tbn * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           ctx.stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<void *>(li.hres), P.res, N * sizeof(DevResult),
+                           hipMemcpyDeviceToHost, ctx.stream));
+    HIP_TRY(hipMemcpyAsync(const_cast<void *>(li.htb), P.tb, tbn * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ctx.stream));
     HIP_TRY(hipEventRecord(ctx.ev[3], ctx.stream));
     for (size_t k = 0; k < N; k++) li.tb_at[k] = off[k].tb;
     return MPOA_OK;
@@ -509,7 +564,7 @@ int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
     HIP_TRY(hipEventElapsedTime(&li.d2h_ms, ctx.ev[2], ctx.ev[3]));
     li.fill_ms = fill_ms;
     li.all_ms = all_ms;
-    const DevResult *hr = launch_results(ctx);
+    const DevResult *hr = launch_results(li);
     if (st) {
         // Overflowed attempts are re-run with the full arena; count the cells
         // of the run that completes.
@@ -527,24 +582,24 @@ int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
     return MPOA_OK;
 }
 
-const DevResult *launch_results(const Ctx &ctx) { return static_cast<const DevResult *>(ctx.h_res.p); }
-const uint32_t *launch_tb(const Ctx &ctx, const LaunchInfo &li, size_t k) {
-    return static_cast<const uint32_t *>(ctx.h_tb.p) + li.tb_at[k];
+const DevResult *launch_results(const LaunchInfo &li) { return static_cast<const DevResult *>(li.hres); }
+const uint32_t *launch_tb(const Ctx &, const LaunchInfo &li, size_t k) {
+    return static_cast<const uint32_t *>(li.htb) + li.tb_at[k];
 }
 
 // One synchronous launch over plans[idx[*]] that share the score width.
 static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
                      const std::vector<size_t> &idx, bool i64, bool full_arena,
                      std::vector<DevResult> &res, std::vector<uint32_t> &tb,
-                     const std::vector<uint64_t> &tb_off, RunStats *st) {
+                     const std::vector<uint64_t> &tb_off, RunStats *st, const FixedScratch *fx) {
     std::vector<JobPlan *> group(idx.size());
     for (size_t k = 0; k < idx.size(); k++) group[k] = plans[idx[k]];
     LaunchInfo li;
-    int rc = launch_begin(ctx, p, group, i64, full_arena, li);
+    int rc = launch_begin(ctx, p, group, i64, full_arena, li, fx);
     if (rc != MPOA_OK) return rc;
     rc = launch_finish(ctx, li, st);
     if (rc != MPOA_OK) return rc;
-    const DevResult *hr = launch_results(ctx);
+    const DevResult *hr = launch_results(li);
     for (size_t k = 0; k < idx.size(); k++) res[idx[k]] = hr[k];
     ctx.pool->parallel_for(idx.size(), [&](size_t k) {
         if (hr[k].status == kOk)
@@ -556,7 +611,7 @@ static int run_group(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
 
 int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
               std::vector<DevResult> &res, std::vector<uint32_t> &tb,
-              std::vector<uint64_t> &tb_off, RunStats *st) {
+              std::vector<uint64_t> &tb_off, RunStats *st, const FixedScratch *fx) {
     const size_t N = plans.size();
     res.assign(N, DevResult{});
     tb_off.assign(N + 1, 0);
@@ -577,7 +632,7 @@ int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
             std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
                 return plans[x]->static_cells > plans[y]->static_cells;
             });
-            int rc = run_group(ctx, p, plans, idx, wide == 1, pass == 1, res, tb, tb_off, st);
+            int rc = run_group(ctx, p, plans, idx, wide == 1, pass == 1, res, tb, tb_off, st, fx);
             if (rc != MPOA_OK) return rc;
         }
     }

@@ -80,6 +80,15 @@ struct Ctx {
     ~Ctx();
 };
 
+// Device memory that a launch carves all of its device buffers from instead
+// of the context's own growable buffers (the no-allocation entry point's
+// scratch). Host staging then uses pageable memory of the calling thread, so
+// such a launch calls no HIP allocator at all.
+struct FixedScratch {
+    char *dev = nullptr;
+    uint64_t bytes = 0;
+};
+
 // threads: host workers of the context's pool (0 = default_threads()).
 int ctx_create(int device, Ctx **out, unsigned threads = 0);
 unsigned default_threads();
@@ -87,9 +96,12 @@ unsigned default_threads();
 // Runs every plan with status MPOA_OK on the device (i32 and i64 groups in
 // separate launches; arena overflows are re-run with the full-width bound).
 // Fills res[k] and tb (flattened, tb_off[k] .. tb_off[k] + m).
+// With fx, every launch takes its device buffers from fx (a DP whose
+// buffers do not fit returns MPOA_ERR_SCRATCH_LIMIT).
 int run_plans(Ctx &ctx, const Params &p, std::vector<JobPlan *> &plans,
               std::vector<DevResult> &res, std::vector<uint32_t> &tb,
-              std::vector<uint64_t> &tb_off, RunStats *stats);
+              std::vector<uint64_t> &tb_off, RunStats *stats,
+              const FixedScratch *fx = nullptr);
 
 // An asynchronous launch: one fill and one traceback over plans that share
 // the score width, with staging in and results out on ctx's stream. The plans
@@ -107,9 +119,11 @@ struct LaunchInfo {
     uint64_t arena_bytes = 0;     // score arena the launch asked for
     float fill_ms = 0, all_ms = 0;  // device time of the fill, and of the fill and traceback
     float h2d_ms = 0, d2h_ms = 0;   // device time of the input and result copies
+    const void *hres = nullptr;     // host copies of the results and traceback records
+    const void *htb = nullptr;
 };
 int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
-                 bool full_arena, LaunchInfo &li);
+                 bool full_arena, LaunchInfo &li, const FixedScratch *fx = nullptr);
 // Score-arena bytes job J adds to a launch, its share of the overflow pool
 // included.
 uint64_t job_arena_bytes(const JobPlan &J, bool i64);
@@ -119,7 +133,7 @@ uint64_t device_free_bytes(int device);
 bool launch_done(Ctx &ctx);
 // Waits for the launch and adds its timings and cells to *st.
 int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st);
-const DevResult *launch_results(const Ctx &ctx);
+const DevResult *launch_results(const LaunchInfo &li);
 const uint32_t *launch_tb(const Ctx &ctx, const LaunchInfo &li, size_t k);
 
 // Maps a failed DevResult to (status, message) as the reference reports it.

@@ -25,10 +25,11 @@ STATUS = {
     10: "MPOA_ERR_ALLOC", 11: "MPOA_ERR_INVALID_PARAMS", 12: "MPOA_ERR_GRAPH",
     13: "MPOA_ERR_DEVICE", 14: "MPOA_ERR_NO_DEVICE", 15: "MPOA_ERR_UNSUPPORTED",
     16: "MPOA_ERR_BUFFER_TOO_SMALL", 17: "MPOA_ERR_INVALID_ARGUMENT", 18: "MPOA_ERR_PENDING",
-    19: "MPOA_ERR_SEED_SPAN",
+    19: "MPOA_ERR_SEED_SPAN", 20: "MPOA_ERR_SCRATCH_LIMIT",
 }
 BUFFER_TOO_SMALL = 16
 INVALID_ARGUMENT = 17
+SCRATCH_LIMIT = 20
 
 
 class HipPoaError(RuntimeError):
@@ -118,6 +119,8 @@ SIGNATURES = {
                            U64P, C.POINTER(C.c_int64)]),
     "mpoa_poa_inner": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64, _B,
                                  _u64, U64P, C.POINTER(C.c_int64)]),
+    "mpoa_poa_noalloc": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64, _B,
+                                   _u64, U64P, C.POINTER(C.c_int64)]),
     "mpoa_poa_kernel": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64,
                                   C.c_int, _B, _u64, U64P, C.POINTER(C.c_int64)]),
     "mpoa_i32_scores_safe": (C.c_int, [_P, _V, _u64, _u64, _u64]),
@@ -426,6 +429,19 @@ def poa_inner(ctx, params, graph, beg, end, query, adaptive, band, score=False):
     sc = C.c_int64(0)
     out = _path_call(lib().mpoa_poa_inner, ((ctx._h, cp.ref, vref, beg, end, q, len(q),
                                              int(bool(adaptive)), band), (C.byref(sc),)),
+                     len(q) + 16)
+    return (out, sc.value) if score else out
+
+
+def poa_noalloc(ctx, params, graph, beg, end, query, adaptive, band, score=False):
+    """`poa` as mpoa_poa_noalloc runs it: the same DP with every device
+    buffer taken from the library's fixed per-GPU scratch (no allocation)."""
+    cp = CParams(params)
+    vref, keep = _as_view(graph)
+    q = bytes(query)
+    sc = C.c_int64(0)
+    out = _path_call(lib().mpoa_poa_noalloc, ((ctx._h, cp.ref, vref, beg, end, q, len(q),
+                                               int(bool(adaptive)), band), (C.byref(sc),)),
                      len(q) + 16)
     return (out, sc.value) if score else out
 
