@@ -35,12 +35,19 @@ from pathlib import Path
 
 # At least eight hardware queues per process (HIP opens four by default),
 # set before HIP starts: the engine then runs five launch slots, each stream
-# on a queue of its own. A larger setting is kept.
+# on a queue of its own. A larger setting is kept. `--hw-queues N` sets N
+# exactly (4: the library default a drop-in caller gets, three slots).
 try:
     _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
 except ValueError:
     _queues = 4
-os.environ["GPU_MAX_HW_QUEUES"] = str(max(_queues, 8))
+_queues = max(_queues, 8)
+for _k, _a in enumerate(sys.argv):
+    if _a == "--hw-queues" and _k + 1 < len(sys.argv):
+        _queues = int(sys.argv[_k + 1])
+    elif _a.startswith("--hw-queues="):
+        _queues = int(_a.split("=", 1)[1])
+os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(_queues, 1), 32))
 
 ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "minipoa-rs_amd"), str(ROOT / "oracle")]
@@ -48,11 +55,52 @@ sys.path[:0] = [str(ROOT / "minipoa-rs_amd"), str(ROOT / "oracle")]
 METRIC = ("aggregate GCUPS + POA jobs/sec (batched vs plain, bit-exact); "
           "p50/p99 job latency")
 HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 12  # M, D and I (i32) stored per cell for the traceback
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s, nominal
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """Host cores this process may use: the affinity mask, bounded by the
+    cgroup CPU quota when one is set (a quota of q cores time-slices more
+    threads onto q cores). Returns (cores, description)."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    cores = affinity if quota is None else max(1, min(affinity, int(quota)))
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, {"os_cpu_count": os.cpu_count(), "affinity": affinity,
+                   "cgroup_quota_cores": quota, "model": model}
+
+
+def int_valu_peak():
+    """Measured int32 VALU rate (lane-ops/s) of device 0: dependency-free
+    add/max chains (minipoa-rs_amd/csrc/probe_valu.hip), best of 5 runs of
+    2,048 workgroups x 256 lanes x 8 chains x 32,768 steps."""
+    import ctypes
+
+    lib = ctypes.CDLL(str(ROOT / "minipoa-rs_amd" / "libmpoa_probe.so"))
+    ops, ms = ctypes.c_double(0), ctypes.c_double(0)
+    rc = lib.mpoa_probe_int_valu(0, 2048, 32768, 5, ctypes.byref(ops), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"int-VALU probe failed (HIP error {rc})")
+    log(f"int-VALU probe: {ops.value/1e12:.2f} T lane-ops/s ({ms.value:.2f} ms per run)")
+    return ops.value
 
 
 def fnv1a_many(texts):
@@ -110,6 +158,10 @@ def main():
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (counter scaling)")
+    ap.add_argument("--hw-queues", type=int, default=None,
+                    help="hardware queues per process (read before HIP starts; default 8)")
+    ap.add_argument("--no-valu-probe", dest="valu_probe", action="store_false",
+                    help="skip the int-VALU rate probe (the roofline then uses the nominal peak)")
     ap.add_argument("--legs", default="c1,c3,c4,c5",
                     help="other workloads to report once each (c1, c3, c4, c5; empty = none)")
     ap.add_argument("--serve-jobs", type=int, default=96)
@@ -134,6 +186,7 @@ def main():
     ctx = mh.Context(0)
     if args.calib_copy_mib > 0:
         copy_calibration(args.calib_copy_mib)
+    valu_peak = int_valu_peak() if args.valu_probe else None
 
     for w in range(args.warmup):
         t = time.time()
@@ -162,9 +215,13 @@ def main():
     # Dominant kernel: the fill. Algorithmic bytes: the M, D and I scores
     # (4 B each) written per cell for the traceback. Predecessor rows are
     # re-read from the LDS ring, so they add no HBM bytes by design.
+    # Algorithmic int ops per cell: 6 (profile add, md max, I add/add/max,
+    # M max) + 5 per predecessor overlap (M max, D add/add/max/max) + 2 for
+    # the adaptive argmax.
     fill_s = st["fill_ms"] / 1e3
     launches = st["fill_launches"]
-    alg_bytes = 12 * cells
+    wall_s = total / len(times)
+    alg_bytes = BYTES_PER_CELL * cells
     int_ops = 6 * cells + 5 * st["pred_cells"] + 2 * cells
     achieved_gbs = alg_bytes / fill_s / 1e9
     traffic = None
@@ -172,19 +229,34 @@ def main():
     if tfile.exists():
         # HBM bytes per cell measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
         # passes over this workload (copy-kernel scaled), times this run's
-        # cells per launch.
+        # cells per launch: profile-derived, not measured in this run.
         t = json.loads(tfile.read_text())
         traffic = int((t["fetch_bytes_per_cell"] + t["write_bytes_per_cell"]) * cells
                       / max(1, launches))
+    peak_t = (valu_peak or VALU_PEAK_TOPS * 1e12) / 1e12
     roofline = {
         "bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "traffic_source": "profile-derived (profiles/poa_fill_traffic.json x this run's cells)",
         "kernel": "poa_fill", "launches": launches,
         "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3),
         "bytes_per_launch": alg_bytes // max(1, launches),
+        "bytes_per_cell": BYTES_PER_CELL,
+        # Over the wall: the step's algorithmic bytes per wall second (the
+        # per-launch figure divides one launch's bytes by its own time, and
+        # launches of the engine's streams overlap).
+        "wall_gbs": round(alg_bytes / wall_s / 1e9, 1),
+        "wall_frac": round(alg_bytes / wall_s / 1e9 / HBM_PEAK_GBS, 4),
+        # The DP is integer max-plus: int VALU is the binding roof.
         "valu": {"achieved_tops": round(int_ops / fill_s / 1e12, 3),
-                 "peak_tops": round(VALU_PEAK_TOPS, 1),
-                 "frac": round(int_ops / fill_s / 1e12 / VALU_PEAK_TOPS, 4)},
+                 "peak_tops": round(peak_t, 2),
+                 "peak_source": "measured on this device (add/max probe)" if valu_peak
+                 else "nominal 256 CU x 64 lanes x 2.4 GHz",
+                 "peak_nominal_tops": round(VALU_PEAK_TOPS, 1),
+                 "frac": round(int_ops / fill_s / 1e12 / peak_t, 4),
+                 "wall_tops": round(int_ops / wall_s / 1e12, 3),
+                 "wall_frac": round(int_ops / wall_s / 1e12 / peak_t, 4),
+                 "ops_per_cell": round(int_ops / max(1, cells), 2)},
     }
 
     out = {
@@ -207,8 +279,7 @@ def main():
         import oracle
 
         n = min(args.cpu_jobs, args.jobs)
-        cores = len(os.sched_getaffinity(0))
-        cores = min(cores, 16)
+        cores, host = host_cores()
         sj, sr, sb = synth.subset(job_off, read_off, bases, range(n))
         t = time.perf_counter()
         ccells, hashes = oracle.batch_run(params, sj, sr, sb, cores)
@@ -220,7 +291,7 @@ def main():
         exact = int((got == hashes).sum())
         out["cpu_baseline"] = {
             "value": round(ccells / dt / 1e9, 4), "unit": "GCUPS", "cores": cores,
-            "kind": "port",
+            "kind": "port", "host": host, "jobs_per_s": round(n / dt, 1),
             "sample": f"first {n} of the {args.jobs} jobs ({ccells/1e9:.2f} G cells), "
                       f"oracle 8-lane i32 restatement, {dt:.1f}s"}
         out["bit_exact"] = {"jobs_checked": n, "identical": exact,
@@ -231,7 +302,7 @@ def main():
     if legs:
         import oracle
 
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores, _ = host_cores()
         out["legs"] = {}
         if "c4" in legs:
             out["legs"]["c4"] = leg_c4(ctx, oracle, cores)
@@ -323,7 +394,7 @@ def leg_c3(ctx, oracle, cores):
     text = r.output(0, 0) + r.output(0, 1)
     gfa = r.output(0, 2)
     t = time.perf_counter()
-    cells, want = oracle.batch_run(p, jo, ro, b, 1)
+    cells, want, want_gfa = oracle.batch_run(p, jo, ro, b, 1, gfa=True)
     ct = time.perf_counter() - t
     # The reference's -t N: the windows of each read on N threads
     # (src/parallel.rs:46-67), results in window order.
@@ -332,9 +403,11 @@ def leg_c3(ctx, oracle, cores):
     cells_t, want_t = oracle.batch_run(pt, jo, ro, b, 1)
     ctt = time.perf_counter() - t
     same = fnv1a_one(text) == int(want[0]) == int(want_t[0]) and st["cells"] == cells == cells_t
+    same_gfa = fnv1a_one(gfa) == int(want_gfa[0])
+    same = same and same_gfa
     nreads = int(jo[1] - jo[0])
     log(f"c3: gpu {dt*1e3:.0f} ms {st['cells']/dt/1e9:.2f} GCUPS, {st['dp_calls']} DPs; "
-        f"oracle {ct:.1f}s on 1 core, {ctt:.1f}s with -t {cores}; identical {same}")
+        f"oracle {ct:.1f}s on 1 core, {ctt:.1f}s with -t {cores}; identical {same} (GFA {same_gfa})")
     # Many such jobs at once: every ready job's windows share launches.
     nb = 16
     bj, br, bb = synth.seeded(seed=20264, njobs=nb)
@@ -357,6 +430,7 @@ def leg_c3(ctx, oracle, cores):
             "cells": st["cells"], "dp_calls": st["dp_calls"], "windows_per_read": round(st["dp_calls"] / (nreads - 1), 1),
             "gpu_ms": round(dt * 1e3, 1), "gpu_gcups": round(st["cells"] / dt / 1e9, 3),
             "ms_per_read": round(dt * 1e3 / nreads, 2), "gfa_bytes": len(gfa),
+            "gfa_identical": bool(same_gfa),
             "cpu_ms": round(ct * 1e3, 1), "cpu_gcups": round(cells / ct / 1e9, 3), "cpu_cores": 1,
             "cpu_threads_ms": round(ctt * 1e3, 1), "cpu_threads": cores,
             "cpu_kind": "port", "identical": bool(same),

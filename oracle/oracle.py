@@ -123,6 +123,10 @@ def lib():
         L.o_batch_run.argtypes = [P, sz, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]
         L.o_batch_run.restype = C.c_uint64
+        L.o_batch_run_gfa.argtypes = [P, sz, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                      C.c_void_p, C.c_int, C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+        L.o_batch_run_gfa.restype = C.c_uint64
         u64p = C.POINTER(C.c_uint64)
         A = C.POINTER(_Anchor)
         L.o_sketch.argtypes = [C.c_char_p, sz, sz, sz, u64p, u64p, sz]
@@ -363,8 +367,10 @@ def extend_graph(params, graph, seqs, existing):
         raise _err()
 
 
-def batch_run(params, job_off, read_off, bases, nthreads):
-    """Job-parallel progressive POA + consensus/MSA hash per job (numpy arrays)."""
+def batch_run(params, job_off, read_off, bases, nthreads, gfa=False):
+    """Job-parallel progressive POA + consensus/MSA hash per job (numpy
+    arrays). With gfa: also the FNV-1a 64 of each job's GFA text, returned
+    as a third value."""
     import numpy as np
 
     job_off = np.ascontiguousarray(job_off, dtype=np.uint64)
@@ -372,11 +378,13 @@ def batch_run(params, job_off, read_off, bases, nthreads):
     bases = np.ascontiguousarray(bases, dtype=np.uint8)
     njobs = len(job_off) - 1
     out = np.zeros(njobs, dtype=np.uint64)
+    outg = np.zeros(njobs, dtype=np.uint64)
     u64p = C.POINTER(C.c_uint64)
-    cells_ = lib().o_batch_run(
+    cells_ = lib().o_batch_run_gfa(
         _P(params).ref, njobs, job_off.ctypes.data_as(u64p), read_off.ctypes.data_as(u64p),
-        bases.ctypes.data_as(C.c_void_p), int(nthreads), out.ctypes.data_as(u64p))
-    return int(cells_), out
+        bases.ctypes.data_as(C.c_void_p), int(nthreads), out.ctypes.data_as(u64p),
+        outg.ctypes.data_as(u64p) if gfa else None)
+    return (int(cells_), out, outg) if gfa else (int(cells_), out)
 
 
 # ---- minimizer seeding (src/minimizer.rs:32-488) ----

@@ -2286,11 +2286,12 @@ typedef struct {
     const uint64_t *job_off, *read_off;
     const uint8_t *bases;
     uint64_t *out_hash;
+    uint64_t *out_gfa; /* NULL: no GFA */
     atomic_size_t next;
     atomic_uint_fast64_t cells;
 } batch_ctx;
 
-static uint64_t run_job(const op_params *p, const batch_ctx *c, size_t job) {
+static uint64_t run_job(const op_params *p, const batch_ctx *c, size_t job, uint64_t *gfa_hash) {
     size_t r0 = c->job_off[job], r1 = c->job_off[job + 1], nr = r1 - r0;
     const uint8_t **seqs = xmalloc((nr + 1) * sizeof(uint8_t *));
     size_t *lens = xmalloc((nr + 1) * sizeof(size_t));
@@ -2318,6 +2319,12 @@ static uint64_t run_job(const op_params *p, const batch_ctx *c, size_t job) {
             h = fnv1a(h, msa, strlen(msa));
             if (h == 0) h = 1;
         }
+        char *gfa = NULL;
+        if (gfa_hash && og_gfa(g, p, (const char *const *)names, nr, &gfa) == 0) {
+            uint64_t x = fnv1a(UINT64_C(14695981039346656037), gfa, strlen(gfa));
+            *gfa_hash = x ? x : 1;
+        }
+        free(gfa);
         free(msa);
         free(cons);
         og_free(g);
@@ -2336,7 +2343,7 @@ static void *batch_worker(void *arg) {
     for (;;) {
         size_t job = atomic_fetch_add(&c->next, 1);
         if (job >= c->njobs) break;
-        c->out_hash[job] = run_job(c->p, c, job);
+        c->out_hash[job] = run_job(c->p, c, job, c->out_gfa ? c->out_gfa + job : NULL);
     }
     atomic_fetch_add(&c->cells, g_cells);
     o_arena_clear();
@@ -2346,6 +2353,12 @@ static void *batch_worker(void *arg) {
 uint64_t o_batch_run(const op_params *p, size_t njobs, const uint64_t *job_off,
                      const uint64_t *read_off, const uint8_t *bases, int nthreads,
                      uint64_t *out_hash) {
+    return o_batch_run_gfa(p, njobs, job_off, read_off, bases, nthreads, out_hash, NULL);
+}
+
+uint64_t o_batch_run_gfa(const op_params *p, size_t njobs, const uint64_t *job_off,
+                         const uint64_t *read_off, const uint8_t *bases, int nthreads,
+                         uint64_t *out_hash, uint64_t *out_gfa) {
     batch_ctx c;
     c.p = p;
     c.njobs = njobs;
@@ -2353,6 +2366,8 @@ uint64_t o_batch_run(const op_params *p, size_t njobs, const uint64_t *job_off,
     c.read_off = read_off;
     c.bases = bases;
     c.out_hash = out_hash;
+    c.out_gfa = out_gfa;
+    if (out_gfa) memset(out_gfa, 0, njobs * sizeof(uint64_t));
     atomic_init(&c.next, 0);
     atomic_init(&c.cells, 0);
     if (nthreads < 1) nthreads = 1;

@@ -169,6 +169,10 @@ int o_extend_graph(const op_params *p, og_graph *g, const uint8_t *const *seqs,
 uint64_t o_batch_run(const op_params *p, size_t njobs, const uint64_t *job_off,
                      const uint64_t *read_off, const uint8_t *bases, int nthreads,
                      uint64_t *out_hash);
+/* The same, plus out_gfa[i]: the FNV-1a 64 of job i's GFA text (-r2), or 0. */
+uint64_t o_batch_run_gfa(const op_params *p, size_t njobs, const uint64_t *job_off,
+                         const uint64_t *read_off, const uint8_t *bases, int nthreads,
+                         uint64_t *out_hash, uint64_t *out_gfa);
 
 #ifdef __cplusplus
 }
