@@ -204,7 +204,10 @@ def main():
         stats.append(res.stats())
         log(f"step {s}: {dt:.3f}s cells={stats[-1]['cells']/1e9:.2f}G "
             f"fill={stats[-1]['fill_ms']:.0f}ms device={stats[-1]['device_ms']:.0f}ms "
-            f"host={stats[-1]['host_ms']:.0f}ms launches={stats[-1]['fill_launches']}")
+            f"host={stats[-1]['host_ms']:.0f}ms (plan {stats[-1]['plan_ms']:.0f} "
+            f"consume {stats[-1]['consume_ms']:.0f} fuse {stats[-1]['fuse_ms']:.0f} "
+            f"render {stats[-1]['render_ms']:.0f} pack/slot {stats[-1]['pack_ms']:.0f}) "
+            f"launches={stats[-1]['fill_launches']}")
     st = stats[-1]
     failed = sum(res.status(j) != 0 for j in range(args.jobs))
     total = sum(times)

@@ -219,6 +219,10 @@ int mpoa_graph_toposort(mpoa_graph *g, int grouped);                   /* graph.
 /* A view of the graph, valid until the graph is next modified or freed. */
 int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out);
 uint64_t mpoa_graph_n_paths(const mpoa_graph *g);
+/* graph.paths[k] (src/graph.rs:20-27): its sequence id and node ids (up to
+ * cap; *out_len is the full length). */
+int mpoa_graph_path(const mpoa_graph *g, uint64_t k, uint64_t *seq_id, uint64_t *ids, uint64_t cap,
+                    uint64_t *out_len);
 /* graph.rs:181-237: writes up to cap bases (decoded) / node ids / coverage
  * (any pointer may be NULL) and returns the consensus length. */
 uint64_t mpoa_graph_consensus(const mpoa_graph *g, const mpoa_params *params, uint8_t *seq,
@@ -243,6 +247,34 @@ int mpoa_params_validate(const mpoa_params *params);
  * text returns MPOA_ERR_INVALID_PARAMS and writes nothing. */
 int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t *matrix, uint32_t cap,
                      uint32_t *alphabet_len);
+/* ---- input formats: the reference's io module (src/io.rs is not in its
+ * tree; behaviour from tests/io_cli.rs and README.md) ----
+ * Records (src/lib.rs:39-45 Sequence) live in an mpoa_records handle; the
+ * pointers mpoa_record_get returns stay valid until mpoa_records_free. */
+typedef struct mpoa_records mpoa_records;
+/* io::read_sequences / io::parse_sequences (tests/io_cli.rs:34-87): FASTA or
+ * FASTQ text (multi-line records, CRLF, header comments, empty records,
+ * quality lines that start with '@' or '+'), gzip-compressed when the data
+ * starts with the gzip magic (concatenated members). Malformed input returns
+ * MPOA_ERR_INVALID_ARGUMENT with the reason; empty input gives no records. */
+int mpoa_read_sequences(const uint8_t *data, uint64_t len, mpoa_records **out);
+uint64_t mpoa_records_len(const mpoa_records *r);
+int mpoa_record_get(const mpoa_records *r, uint64_t k, const char **name, uint64_t *name_len,
+                    const char **comment, uint64_t *comment_len, const uint8_t **seq,
+                    uint64_t *seq_len, const uint8_t **qual, uint64_t *qual_len);
+void mpoa_records_free(mpoa_records *r);
+/* io::read_gfa for -i (src/lib.rs:214-216; tests/io_cli.rs:374-419, 563-610):
+ * a forward, acyclic GFA1 graph (H/S/L/P, explicit sequences, 0M overlaps)
+ * as a graph plus one record per path (in file order; sequence id k is the
+ * k-th path). Each segment becomes one node per base, chained; links add
+ * edges; paths are replayed as reads, so edge weights count the paths that
+ * use them (a link no path uses weighs 1); nodes without a predecessor or
+ * successor are joined to the source or sink. Every node is its own
+ * alignment column (GFA carries no column groups). Continue it with
+ * mpoa_extend_graph(existing = number of records). Errors: MPOA_ERR_GRAPH. */
+int mpoa_read_gfa(const mpoa_params *params, const uint8_t *data, uint64_t len, mpoa_graph **graph,
+                  mpoa_records **records);
+
 /* src/lib.rs:177-204 `align_sequences` (input order, or the guide order with
  * params.progressive) on the device. */
 int mpoa_align_sequences(mpoa_ctx *ctx, const mpoa_params *params, const uint8_t *const *seqs,
@@ -269,6 +301,10 @@ typedef struct {
     double fill_ms;        /* fill kernel time alone (HIP events, summed) */
     double host_ms;        /* host preparation + fusion time */
     double wall_ms;
+    /* Parts of host_ms: planning, result consumption (path assembly),
+     * fusion (add_path + toposort) and rendering, per host worker; and the
+     * launch slots' input packing, per slot. */
+    double plan_ms, consume_ms, fuse_ms, render_ms, pack_ms;
 } mpoa_batch_stats;
 
 typedef struct mpoa_batch mpoa_batch;

@@ -17,6 +17,7 @@
 
 #include "engine.hpp"
 #include "graph.hpp"
+#include "io.hpp"
 #include "minipoa_hip.h"
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -34,6 +35,9 @@ struct mpoa_ctx {
 struct mpoa_graph {
     Graph g;
     explicit mpoa_graph(const Params &p) : g(p) {}
+};
+struct mpoa_records {
+    std::vector<SeqRecord> r;
 };
 struct mpoa_server {
     mpoa_ctx *ctx = nullptr;
@@ -284,6 +288,11 @@ void fill_stats(const EngineStats &es, mpoa_batch_stats &S) {
     S.device_ms = es.run.device_ms;
     S.fill_ms = es.run.fill_ms;
     S.host_ms = es.host_ms;
+    S.plan_ms = es.plan_ms;
+    S.consume_ms = es.consume_ms;
+    S.fuse_ms = es.fuse_ms;
+    S.render_ms = es.render_ms;
+    S.pack_ms = es.pack_ms;
 }
 
 }  // namespace
@@ -394,6 +403,60 @@ int mpoa_load_matrix(const char *text, uint64_t len, uint8_t *alphabet, int32_t 
     if (n > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "alphabet buffer is too small");
     for (size_t c = 0; c < n; c++) alphabet[c] = (uint8_t)head[c][0];
     memcpy(matrix, m.data(), m.size() * sizeof(int32_t));
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+// ---- input formats (the reference's io module) ----
+
+int mpoa_read_sequences(const uint8_t *data, uint64_t len, mpoa_records **out) try {
+    if (!out || (!data && len)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    std::vector<uint8_t> text;
+    std::string e = gunzip_if_gzip(data, len, text);
+    auto r = std::make_unique<mpoa_records>();
+    if (e.empty()) e = parse_sequences(text.data(), text.size(), r->r);
+    if (!e.empty()) return fail(MPOA_ERR_INVALID_ARGUMENT, e);
+    *out = r.release();
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+uint64_t mpoa_records_len(const mpoa_records *r) { return r ? r->r.size() : 0; }
+
+int mpoa_record_get(const mpoa_records *r, uint64_t k, const char **name, uint64_t *name_len,
+                    const char **comment, uint64_t *comment_len, const uint8_t **seq,
+                    uint64_t *seq_len, const uint8_t **qual, uint64_t *qual_len) try {
+    if (!r || k >= r->r.size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "record out of range");
+    const SeqRecord &x = r->r[k];
+    if (name) *name = x.name.c_str();
+    if (name_len) *name_len = x.name.size();
+    if (comment) *comment = x.comment.c_str();
+    if (comment_len) *comment_len = x.comment.size();
+    if (seq) *seq = x.seq.data();
+    if (seq_len) *seq_len = x.seq.size();
+    if (qual) *qual = x.qual.data();
+    if (qual_len) *qual_len = x.qual.size();
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+void mpoa_records_free(mpoa_records *r) { delete r; }
+
+int mpoa_read_gfa(const mpoa_params *params, const uint8_t *data, uint64_t len, mpoa_graph **graph,
+                  mpoa_records **records) try {
+    if (!graph || !records || (!data && len)) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *graph = nullptr;
+    *records = nullptr;
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    std::vector<uint8_t> text;
+    std::string e = gunzip_if_gzip(data, len, text);
+    if (!e.empty()) return fail(MPOA_ERR_INVALID_ARGUMENT, e);
+    auto g = std::make_unique<mpoa_graph>(p);
+    auto r = std::make_unique<mpoa_records>();
+    e = read_gfa(p, text.data(), text.size(), g->g, r->r);
+    if (!e.empty()) return fail(MPOA_ERR_GRAPH, e);
+    *graph = g.release();
+    *records = r.release();
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 
@@ -560,6 +623,17 @@ int mpoa_graph_toposort(mpoa_graph *g, int grouped) try {
 int mpoa_graph_view_of(mpoa_graph *g, mpoa_graph_view *out) try {
     if (!g || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = g->g.view();
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+int mpoa_graph_path(const mpoa_graph *g, uint64_t k, uint64_t *seq_id, uint64_t *ids, uint64_t cap,
+                    uint64_t *out_len) try {
+    if (!g || k >= g->g.paths().size()) return fail(MPOA_ERR_INVALID_ARGUMENT, "path out of range");
+    const auto &pa = g->g.paths()[k];
+    if (seq_id) *seq_id = pa.first;
+    if (out_len) *out_len = pa.second.size();
+    if (pa.second.size() > cap) return fail(MPOA_ERR_BUFFER_TOO_SMALL, "output buffer is too small");
+    for (size_t t = 0; t < pa.second.size(); t++) ids[t] = pa.second[t];
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 

@@ -275,7 +275,13 @@ EngineStats Engine::stats() {
         st.run.fill_ms += x.fill_ms;
     }
     st.run.retries = retries_.load();
-    st.host_ms = double(task_ns_.load()) / 1e6 / double(std::max(1u, tasks_.size()));
+    const double nw = double(std::max(1u, tasks_.size()));
+    st.host_ms = double(task_ns_.load()) / 1e6 / nw;
+    st.plan_ms = double(plan_ns_.load()) / 1e6 / nw;
+    st.consume_ms = double(consume_ns_.load()) / 1e6 / nw;
+    st.fuse_ms = double(fuse_ns_.load()) / 1e6 / nw;
+    st.render_ms = double(render_ns_.load()) / 1e6 / nw;
+    st.pack_ms = double(pack_ns_.load()) / 1e6 / double(std::max<size_t>(1, slots_.size()));
     return st;
 }
 
@@ -600,18 +606,32 @@ void Engine::advance(Job &J, bool have) {
         }
         guide_order(p_, seqs.data(), lens.data(), J.in.nreads, J.order);
     }
+    using clk = std::chrono::steady_clock;
+    auto ns_since = [](clk::time_point t) {
+        return std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t).count();
+    };
     for (;;) {
         if (have) {
             have = false;
-            if (!consume(J, again)) return;
+            auto t = clk::now();
+            const bool ok = consume(J, again);
+            consume_ns_ += ns_since(t);
+            if (!ok) return;
             if (!again.empty()) {
                 requeue(J, again);
                 return;
             }
-            if (!fuse(J)) return;
+            t = clk::now();
+            const bool fused = fuse(J);
+            fuse_ns_ += ns_since(t);
+            if (!fused) return;
         }
         if (J.step >= J.in.nreads) {
-            if (o_.render) render(J);
+            if (o_.render) {
+                const auto t = clk::now();
+                render(J);
+                render_ns_ += ns_since(t);
+            }
             finish(J);
             return;
         }
@@ -624,7 +644,10 @@ void Engine::advance(Job &J, bool have) {
             if (!fuse(J)) return;
             continue;
         }
-        if (!plan_step(J, again)) return;
+        const auto tp = clk::now();
+        const bool planned = plan_step(J, again);
+        plan_ns_ += ns_since(tp);
+        if (!planned) return;
         if (again.empty()) {  // every DP failed at planning
             have = true;
             continue;
@@ -768,6 +791,7 @@ void Engine::slot_loop(size_t si) {
         RunStats rs;
         int r = launch_begin(c, p_, plans, batch[0]->plan.use_i64, false, li);
         const double t1 = now_ms();
+        pack_ns_ += int64_t((t1 - t0) * 1e6);
         if (r == MPOA_OK) r = launch_finish(c, li, &rs);
         if (r != MPOA_OK) {
             std::lock_guard<std::mutex> lk(mu_);

@@ -97,6 +97,11 @@ struct EngineOpts {
 struct EngineStats {
     RunStats run;
     double host_ms = 0;  // host task time divided by the number of workers
+    // Parts of the host task time (also per worker): planning the next DP
+    // (head/tail distances, predecessor lists), consuming results (path
+    // assembly), fusion (add_path + toposort) and rendering the outputs;
+    // and the launch slots' packing of input blobs (per slot).
+    double plan_ms = 0, consume_ms = 0, fuse_ms = 0, render_ms = 0, pack_ms = 0;
 };
 
 class Engine {
@@ -164,6 +169,7 @@ private:
     std::vector<RunStats> sst_;
     std::atomic<uint64_t> retries_{0};
     std::atomic<int64_t> task_ns_{0};
+    std::atomic<int64_t> plan_ns_{0}, consume_ns_{0}, fuse_ns_{0}, render_ns_{0}, pack_ns_{0};
     std::vector<std::thread> threads_;
 
     // MPOA_TRACE=<file>: one CSV line per launch, written at shutdown.

@@ -143,6 +143,35 @@ void Graph::add_edge(uint32_t from, uint32_t to) {
     t.coverage += 1;
 }
 
+void Graph::link(uint32_t from, uint32_t to, uint32_t w) {
+    if (from == to) return;
+    Node &f = nodes_[from];
+    int k = f.out.find(to);
+    if (k >= 0)
+        f.out[size_t(k)].w += w;
+    else
+        f.out.push(to, w);
+    Node &t = nodes_[to];
+    k = t.in.find(from);
+    if (k >= 0)
+        t.in[size_t(k)].w += w;
+    else
+        t.in.push(from, w);
+    t.coverage += w;
+}
+
+void Graph::weigh_unused_edges() {
+    for (Node &n : nodes_) {
+        for (uint32_t k = 0; k < n.out.size(); k++)
+            if (n.out[k].w == 0) n.out[k].w = 1;
+        for (uint32_t k = 0; k < n.in.size(); k++)
+            if (n.in[k].w == 0) {
+                n.in[k].w = 1;
+                n.coverage += 1;
+            }
+    }
+}
+
 // graph.rs:82-122: reuse the column node of `aligned`'s parent for the base
 // or create it; the path runs from the source and ends with an edge to the sink.
 std::string Graph::add_path(uint64_t seq_id, const AlignedBase *a, size_t len) {

@@ -116,6 +116,11 @@ public:
 
     uint32_t add_node(uint32_t base);                                  // graph.rs:42-61
     void add_edge(uint32_t from, uint32_t to);                         // graph.rs:63-80
+    // add_edge with weight w (0 allowed: the edge exists before any read
+    // traverses it); used to load graphs from GFA links.
+    void link(uint32_t from, uint32_t to, uint32_t w);
+    // Every edge of weight 0 gets weight 1 (links that no read traversed).
+    void weigh_unused_edges();
     std::string add_path(uint64_t seq_id, const AlignedBase *a, size_t len);  // graph.rs:82-122
     std::string toposort(bool grouped);                                // graph.rs:124-178
     // graph.rs:180-237: ids of the heaviest path (source/sink excluded).

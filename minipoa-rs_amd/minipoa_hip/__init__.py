@@ -81,6 +81,8 @@ class _Stats(C.Structure):
         ("i64_calls", C.c_uint64), ("pred_cells", C.c_uint64), ("fill_launches", C.c_uint64),
         ("device_ms", C.c_double), ("fill_ms", C.c_double),
         ("host_ms", C.c_double), ("wall_ms", C.c_double),
+        ("plan_ms", C.c_double), ("consume_ms", C.c_double), ("fuse_ms", C.c_double),
+        ("render_ms", C.c_double), ("pack_ms", C.c_double),
     ]
 
 
@@ -133,11 +135,18 @@ SIGNATURES = {
     "mpoa_graph_toposort": (C.c_int, [_vp, C.c_int]),
     "mpoa_graph_view_of": (C.c_int, [_vp, _V]),
     "mpoa_graph_n_paths": (_u64, [_vp]),
+    "mpoa_graph_path": (C.c_int, [_vp, _u64, U64P, U64P, _u64, U64P]),
     "mpoa_graph_consensus": (_u64, [_vp, _P, C.POINTER(C.c_uint8), U64P, U64P, _u64]),
     "mpoa_graph_msa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), _u64,
                                  C.c_char_p, _u64, U64P]),
     "mpoa_graph_gfa": (C.c_int, [_vp, _P, C.POINTER(C.c_char_p), _u64, C.c_char_p, _u64, U64P]),
     "mpoa_params_validate": (C.c_int, [_P]),
+    "mpoa_read_sequences": (C.c_int, [C.c_char_p, _u64, C.POINTER(_vp)]),
+    "mpoa_records_len": (_u64, [_vp]),
+    "mpoa_record_get": (C.c_int, [_vp, _u64, C.POINTER(C.c_void_p), U64P, C.POINTER(C.c_void_p),
+                                  U64P, C.POINTER(C.c_void_p), U64P, C.POINTER(C.c_void_p), U64P]),
+    "mpoa_records_free": (None, [_vp]),
+    "mpoa_read_gfa": (C.c_int, [_P, C.c_char_p, _u64, C.POINTER(_vp), C.POINTER(_vp)]),
     "mpoa_sketch": (C.c_int, [C.c_char_p, _u64, _u64, _u64, U64P, U64P, _u64, U64P]),
     "mpoa_collect_anchors": (C.c_int, [_P, C.c_char_p, _u64, C.c_char_p, _u64, C.POINTER(_Anchor),
                                        _u64, U64P]),
@@ -370,6 +379,21 @@ class Graph:
         v = self.view()
         return list(v.order[:v.n_order])
 
+    @property
+    def paths(self):
+        """graph.paths: [(seq_id, [node ids])] in alignment order."""
+        out = []
+        L = lib()
+        for k in range(L.mpoa_graph_n_paths(self._h)):
+            sid, n = C.c_uint64(), C.c_uint64()
+            rc = L.mpoa_graph_path(self._h, k, C.byref(sid), None, 0, C.byref(n))
+            if rc not in (0, BUFFER_TOO_SMALL):
+                _check(rc)
+            ids = (C.c_uint64 * max(1, n.value))()
+            _check(L.mpoa_graph_path(self._h, k, C.byref(sid), ids, n.value, C.byref(n)))
+            out.append((sid.value, list(ids)[:n.value]))
+        return out
+
     def consensus(self, params):
         n = self.n_nodes
         seq = (C.c_uint8 * max(1, n))()
@@ -477,6 +501,50 @@ def result_or_error(fn, *args, **kw):
         return ("ok", fn(*args, **kw))
     except HipPoaError as e:
         return ("err", str(e))
+
+
+def _take_records(h):
+    """Copies an mpoa_records handle into Sequence records and frees it."""
+    L = lib()
+    out = []
+    try:
+        for k in range(L.mpoa_records_len(h)):
+            ptr = [C.c_void_p() for _ in range(4)]
+            ln = [C.c_uint64() for _ in range(4)]
+            _check(L.mpoa_record_get(h, k, C.byref(ptr[0]), C.byref(ln[0]), C.byref(ptr[1]),
+                                     C.byref(ln[1]), C.byref(ptr[2]), C.byref(ln[2]),
+                                     C.byref(ptr[3]), C.byref(ln[3])))
+            f = [C.string_at(p, n.value) if n.value else b"" for p, n in zip(ptr, ln)]
+            out.append(Sequence(name=f[0].decode(), comment=f[1].decode(), seq=f[2], qual=f[3]))
+    finally:
+        L.mpoa_records_free(h)
+    return out
+
+
+def parse_sequences(data):
+    """`io::parse_sequences` / `io::read_sequences` on bytes (FASTA, FASTQ,
+    gzip-compressed either): a list of Sequence records."""
+    data = bytes(data)
+    h = C.c_void_p()
+    _check(lib().mpoa_read_sequences(data, len(data), C.byref(h)))
+    return _take_records(h)
+
+
+def read_sequences(path):
+    """`io::read_sequences` (the --input file)."""
+    return parse_sequences(Path(path).read_bytes())
+
+
+def read_gfa(params, data):
+    """`io::read_gfa` (the -i graph) on GFA text (bytes or str): (Graph,
+    records), one record per path in file order."""
+    if isinstance(data, str):
+        data = data.encode()
+    data = bytes(data)
+    cp = CParams(params)
+    g, r = C.c_void_p(), C.c_void_p()
+    _check(lib().mpoa_read_gfa(cp.ref, data, len(data), C.byref(g), C.byref(r)))
+    return Graph(params, _handle=g.value), _take_records(r)
 
 
 def load_matrix(params, path):
@@ -746,6 +814,7 @@ def batch_run(ctx, params, job_off, read_off, bases):
 
 __all__ = [
     "AlignedBase", "Params", "Sequence", "records", "default_matrix", "HipPoaError", "Context",
+    "parse_sequences", "read_sequences", "read_gfa",
     "Graph", "GraphView", "view_from_graph", "poa", "poa_inner", "poa_kernel", "align",
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
     "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server", "load_matrix",

@@ -82,6 +82,10 @@ def lib():
         L.o_last_error.restype = C.c_char_p
         L.og_new.argtypes = [P]; L.og_new.restype = G
         L.og_free.argtypes = [G]
+        u64p_ = C.POINTER(C.c_uint64)
+        L.og_from_arrays.argtypes = [P, sz] + [u64p_] * 9 + [C.POINTER(C.c_int64), sz, u64p_,
+                                                             sz, u64p_, u64p_, u64p_]
+        L.og_from_arrays.restype = G
         L.og_add_node.argtypes = [G, sz]; L.og_add_node.restype = sz
         L.og_add_edge.argtypes = [G, sz, sz]
         L.og_add_path.argtypes = [G, sz, B, sz]; L.og_add_path.restype = C.c_int
@@ -185,6 +189,42 @@ def encode(params, base):
 def validate(params):
     if lib().o_validate(_P(params).ref):
         raise _err()
+
+
+def graph_from_arrays(params, nodes, aligned, order, paths):
+    """A graph copied from plain data: nodes = list of dicts with rank,
+    parent, base, incoming, in_weights, outgoing, out_weights; aligned[i] =
+    list of alphabet_len entries (None or node id); order = rank order;
+    paths = [(seq_id, [node ids])]."""
+    import numpy as np
+
+    def u64(a):
+        return np.ascontiguousarray(np.array(a, dtype=np.uint64).reshape(-1))
+
+    n = len(nodes)
+    cols = {}
+    for key in ("rank", "parent", "base"):
+        cols[key] = u64([nd[key] for nd in nodes])
+    in_off = u64([0] + list(np.cumsum([len(nd["incoming"]) for nd in nodes])))
+    out_off = u64([0] + list(np.cumsum([len(nd["outgoing"]) for nd in nodes])))
+    in_node = u64([x for nd in nodes for x in nd["incoming"]] or [0])
+    in_w = u64([x for nd in nodes for x in nd["in_weights"]] or [0])
+    out_node = u64([x for nd in nodes for x in nd["outgoing"]] or [0])
+    out_w = u64([x for nd in nodes for x in nd["out_weights"]] or [0])
+    al = np.ascontiguousarray(np.array([[-1 if x is None else x for x in row] for row in aligned],
+                                       dtype=np.int64).reshape(-1))
+    order_a = u64(order or [0])
+    pseq = u64([s for s, _ in paths] or [0])
+    poff = u64([0] + list(np.cumsum([len(ids) for _, ids in paths])))
+    pids = u64([x for _, ids in paths for x in ids] or [0])
+    p64 = C.POINTER(C.c_uint64)
+    ptr = lambda a: a.ctypes.data_as(p64)
+    h = lib().og_from_arrays(
+        _P(params).ref, n, ptr(cols["rank"]), ptr(cols["parent"]), ptr(cols["base"]),
+        ptr(in_off), ptr(in_node), ptr(in_w), ptr(out_off), ptr(out_node), ptr(out_w),
+        al.ctypes.data_as(C.POINTER(C.c_int64)), len(order), ptr(order_a), len(paths), ptr(pseq),
+        ptr(poff), ptr(pids))
+    return Graph(_handle=h)
 
 
 class Graph:

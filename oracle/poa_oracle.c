@@ -232,6 +232,50 @@ size_t og_add_node(og_graph *g, size_t base) {
     return id;
 }
 
+/* Test infrastructure: a copy of a graph given as flat arrays (node fields,
+ * weighted adjacency in list order, rank order, paths), so that a graph
+ * built elsewhere (for instance loaded from GFA by the library under test)
+ * can be continued by the oracle. Node coverage is the sum of in-weights,
+ * which is what add_edge leaves (src/graph.rs:63-80). */
+og_graph *og_from_arrays(const op_params *p, size_t n_nodes, const uint64_t *rank,
+                         const uint64_t *parent, const uint64_t *base, const uint64_t *in_off,
+                         const uint64_t *in_node, const uint64_t *in_w, const uint64_t *out_off,
+                         const uint64_t *out_node, const uint64_t *out_w, const int64_t *aligned,
+                         size_t n_order, const uint64_t *order, size_t n_paths,
+                         const uint64_t *path_seq, const uint64_t *path_off,
+                         const uint64_t *path_ids) {
+    og_graph *g = xcalloc(1, sizeof *g);
+    g->alphabet_size = p->alphabet_len;
+    for (size_t i = 0; i < n_nodes; i++) {
+        size_t id = og_add_node(g, base[i]);
+        onode *nd = &g->nodes[id];
+        nd->rank = rank[i];
+        nd->parent = parent[i];
+        for (size_t a = 0; a < g->alphabet_size; a++) nd->aligned[a] = aligned[i * g->alphabet_size + a];
+        for (uint64_t k = in_off[i]; k < in_off[i + 1]; k++) {
+            vz_push(&nd->incoming, in_node[k]);
+            vz_push(&nd->in_w, in_w[k]);
+            nd->coverage += in_w[k];
+        }
+        for (uint64_t k = out_off[i]; k < out_off[i + 1]; k++) {
+            vz_push(&nd->outgoing, out_node[k]);
+            vz_push(&nd->out_w, out_w[k]);
+        }
+    }
+    for (size_t r = 0; r < n_order; r++) vz_push(&g->rank, order[r]);
+    for (size_t k = 0; k < n_paths; k++) {
+        if (g->n_paths == g->cap_paths) {
+            g->cap_paths = g->cap_paths ? 2 * g->cap_paths : 16;
+            g->paths = xrealloc(g->paths, g->cap_paths * sizeof(opath));
+        }
+        opath *pa = &g->paths[g->n_paths++];
+        memset(pa, 0, sizeof *pa);
+        pa->seq_id = path_seq[k];
+        for (uint64_t t = path_off[k]; t < path_off[k + 1]; t++) vz_push(&pa->ids, path_ids[t]);
+    }
+    return g;
+}
+
 /* src/graph.rs:30-40 */
 og_graph *og_new(const op_params *p) {
     og_graph *g = xcalloc(1, sizeof *g);

@@ -48,6 +48,16 @@ const char *o_last_error(void);
 
 /* src/graph.rs:30-178 */
 og_graph *og_new(const op_params *p);
+/* Test infrastructure: a graph copied from flat arrays (node-indexed fields,
+ * adjacency in list order with weights, rank order, paths as seq id + node
+ * ids); coverage = sum of in-weights. */
+og_graph *og_from_arrays(const op_params *p, size_t n_nodes, const uint64_t *rank,
+                         const uint64_t *parent, const uint64_t *base, const uint64_t *in_off,
+                         const uint64_t *in_node, const uint64_t *in_w, const uint64_t *out_off,
+                         const uint64_t *out_node, const uint64_t *out_w, const int64_t *aligned,
+                         size_t n_order, const uint64_t *order, size_t n_paths,
+                         const uint64_t *path_seq, const uint64_t *path_off,
+                         const uint64_t *path_ids);
 void og_free(og_graph *g);
 size_t og_add_node(og_graph *g, size_t base);
 void og_add_edge(og_graph *g, size_t from, size_t to);
