@@ -305,6 +305,9 @@ typedef struct {
      * fusion (add_path + toposort) and rendering, per host worker; and the
      * launch slots' input packing, per slot. */
     double plan_ms, consume_ms, fuse_ms, render_ms, pack_ms;
+    /* DPs whose adaptive rows outgrew their arena and were re-run with
+     * full-width rows (the same DP, so the same result). */
+    uint64_t arena_retries;
 } mpoa_batch_stats;
 
 typedef struct mpoa_batch mpoa_batch;

@@ -293,6 +293,7 @@ void fill_stats(const EngineStats &es, mpoa_batch_stats &S) {
     S.fuse_ms = es.fuse_ms;
     S.render_ms = es.render_ms;
     S.pack_ms = es.pack_ms;
+    S.arena_retries = es.run.arena_retries;
 }
 
 }  // namespace

@@ -273,6 +273,7 @@ EngineStats Engine::stats() {
         st.run.fill_launches += x.fill_launches;
         st.run.device_ms += x.device_ms;
         st.run.fill_ms += x.fill_ms;
+        st.run.arena_retries += x.arena_retries;
     }
     st.run.retries = retries_.load();
     const double nw = double(std::max(1u, tasks_.size()));
@@ -820,6 +821,7 @@ void Engine::slot_loop(size_t si) {
             x.fill_launches += rs.fill_launches;
             x.device_ms += rs.device_ms;
             x.fill_ms += rs.fill_ms;
+            x.arena_retries += rs.arena_retries;
             if (trace_path_) {
                 TraceRec t{int(si), t0, t1, now_ms(), li.fill_ms, li.all_ms, li.h2d_ms, li.d2h_ms, li.grown, li.arena_bytes,
                            c.d_arena.cap, batch.size(), 0, 0, -1, li.ring_bytes, li.width, li.shape, li.rows};

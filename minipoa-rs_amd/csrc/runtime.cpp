@@ -283,6 +283,9 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     struct Off {
         uint64_t rows, preds, seq, arena, cap, meta, adapt, tb;
     };
+    // MPOA_ARENA_SQUEEZE (tests): private arena regions of 64 elements and a
+    // one-chunk pool, so that DPs overflow and take the full-arena relaunch.
+    const bool squeeze = getenv("MPOA_ARENA_SQUEEZE") != nullptr;
     std::vector<Off> off(N);
     uint64_t rows = 0, aux = 0, preds = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
              tbn = 0;
@@ -294,7 +297,8 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         o.preds = preds;
         o.seq = seq;
         o.cap = align_up((full_arena || J.full_arena) ? arena_full(J)
-                                                      : std::max<uint64_t>(J.arena_est, 64),
+                         : squeeze                  ? 64
+                                                    : std::max<uint64_t>(J.arena_est, 64),
                          64);
         o.arena = arena;
         o.meta = meta;
@@ -315,7 +319,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     // their estimate take chunks of at least three full-width rows).
     const uint64_t chunk = align_up(std::max<uint64_t>(65536, 3ull * maxpad), 64);
     const uint64_t pool_beg = arena;
-    uint64_t pool_len = std::max<uint64_t>(arena / 2, 4 * chunk);
+    uint64_t pool_len = squeeze ? chunk : std::max<uint64_t>(arena / 2, 4 * chunk);
     const size_t alpha = p.alpha();
     size_t b_ctr = 0;  // the pool's next-chunk counter
     size_t b_jobs = 256;
@@ -343,7 +347,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
             set_error(buf);
             return MPOA_ERR_SCRATCH_LIMIT;
         }
-        pool_len = (fx->bytes - s_arena) / tsz - arena;
+        if (!squeeze) pool_len = (fx->bytes - s_arena) / tsz - arena;
     }
     arena += pool_len;
 
@@ -572,7 +576,10 @@ int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
         st->device_ms += all_ms;
         st->fill_launches += 1;
         for (size_t k = 0; k < li.n; k++) {
-            if (hr[k].status == kArenaOverflow) continue;
+            if (hr[k].status == kArenaOverflow) {
+                st->arena_retries += 1;
+                continue;
+            }
             st->cells += hr[k].cells;
             st->pred_cells += hr[k].pred_cells;
             st->dp_calls += 1;

@@ -62,6 +62,7 @@ struct PinBuf {
 struct RunStats {
     uint64_t cells = 0, dp_calls = 0, retries = 0, i64_calls = 0, pred_cells = 0;
     uint64_t fill_launches = 0;
+    uint64_t arena_retries = 0;  // DPs re-run with full-width arena rows after an overflow
     double device_ms = 0, fill_ms = 0, host_ms = 0;
 };
 

@@ -82,7 +82,7 @@ class _Stats(C.Structure):
         ("device_ms", C.c_double), ("fill_ms", C.c_double),
         ("host_ms", C.c_double), ("wall_ms", C.c_double),
         ("plan_ms", C.c_double), ("consume_ms", C.c_double), ("fuse_ms", C.c_double),
-        ("render_ms", C.c_double), ("pack_ms", C.c_double),
+        ("render_ms", C.c_double), ("pack_ms", C.c_double), ("arena_retries", C.c_uint64),
     ]
 
 
@@ -373,6 +373,23 @@ class Graph:
         v = self.view()
         x = v.aligned[i * v.alphabet_len + base]
         return None if x < 0 else x
+
+    def snapshot(self):
+        """Every node (as node(i) gives it), the aligned table and the rank
+        order from one view: (nodes, aligned, order)."""
+        v = self.view()
+        n, a = v.n_nodes, v.alphabet_len
+        io, oo = v.in_off[:n + 1], v.out_off[:n + 1]
+        ein, ew = v.in_node[:io[n]], v.in_weight[:io[n]]
+        eout, eow = v.out_node[:oo[n]], v.out_weight[:oo[n]]
+        rank, parent, base = v.rank[:n], v.parent[:n], v.base[:n]
+        al = v.aligned[:n * a]
+        nodes = [dict(id=i, rank=rank[i], parent=parent[i], base=base[i],
+                      incoming=ein[io[i]:io[i + 1]], in_weights=ew[io[i]:io[i + 1]],
+                      outgoing=eout[oo[i]:oo[i + 1]], out_weights=eow[oo[i]:oo[i + 1]])
+                 for i in range(n)]
+        aligned = [[None if x < 0 else x for x in al[i * a:(i + 1) * a]] for i in range(n)]
+        return nodes, aligned, list(v.order[:v.n_order])
 
     @property
     def rank(self):

@@ -192,6 +192,5 @@ def test_oracle_copy_of_a_loaded_graph_is_the_same_graph(oracle):
 
 
 def oracle_copy(oracle, params, g):
-    nodes = [g.node(i) for i in range(g.n_nodes)]
-    aligned = [[g.aligned(i, b) for b in range(len(params.alphabet))] for i in range(g.n_nodes)]
-    return oracle.graph_from_arrays(params, nodes, aligned, g.rank, g.paths)
+    nodes, aligned, order = g.snapshot()
+    return oracle.graph_from_arrays(params, nodes, aligned, order, g.paths)
