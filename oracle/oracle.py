@@ -20,6 +20,10 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_poa.so"
+# Mutation tests load a mutant build of the same source (ORACLE_MUTANT=k)
+# through this variable; everything else uses the default build.
+if os.environ.get("ORACLE_LIB"):
+    LIB_PATH = Path(os.environ["ORACLE_LIB"])
 
 AlignedBase = namedtuple("AlignedBase", "node base aligned")
 

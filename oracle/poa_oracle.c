@@ -19,10 +19,17 @@
  *                         merge, chain monotonicity, drift split, minimum
  *                         chain score) and its seeded-alignment invariants
  *                         (src/align.rs:1525-1559, tests/avx512.rs:145-180)
- * Parity unpinned: minimizer positions (see the seeding section),
- * exact traceback tie-breaks, adaptive band evolution,
- * head/tail support ties and toposort order beyond what the KATs reach are
- * pinned only by this restatement being line-faithful.
+ *   ordering rules        hand-derived known answers (tests/rule_kats.py):
+ *                         choose ties and weights, the sum(w)/10 filter,
+ *                         open-before-extend for deletions and insertions,
+ *                         head/tail support ties, streak-30 recentring,
+ *                         extra growth and its 5 x band cap, LIFO toposort,
+ *                         consensus ties; each fails when its rule is
+ *                         mutated here (ORACLE_MUTANT, tests/test_oracle_mutants.py)
+ * Parity unpinned: minimizer positions (see the seeding section); the reset
+ * of `extra` on recentring and its zero-propagation to successors, and the
+ * sharing of `support` between the head and tail passes, are pinned only by
+ * this restatement being line-faithful.
  */
 #include "poa_oracle.h"
 
@@ -36,6 +43,15 @@
 #include <string.h>
 
 #include <immintrin.h>
+
+/* Mutation testing: built with -DORACLE_MUTANT=k (tests/test_oracle_mutants.py
+ * does so in a scratch directory), one ordering rule below is replaced by a
+ * plausible wrong one, so that the tests can show their known answers fail
+ * on it. The default, 0, is the faithful restatement. */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUTANT(k) (ORACLE_MUTANT == (k))
 
 /* src/align.rs:16-23, 216-220 */
 #define NEG (-(INT64_C(1) << 50))
@@ -393,8 +409,9 @@ int og_toposort(og_graph *g, int grouped) {
     g->rank.n = 0;
     size_t visited = 0;
     int rc = 0;
-    while (stack.n > 0) {
-        size_t id = stack.v[--stack.n];
+    size_t fifo_head = 0; /* MUTANT(16): first in, first out */
+    while (MUTANT(16) ? fifo_head < stack.n : stack.n > 0) {
+        size_t id = MUTANT(16) ? stack.v[fifo_head++] : stack.v[--stack.n];
         size_t rank = g->rank.n;
         vz_push(&g->rank, id);
         size_t members[256];
@@ -486,9 +503,9 @@ size_t og_consensus(const og_graph *g, const op_params *p, uint8_t *seq, uint64_
                 if (node->in_w.v[bi] != node->in_w.v[k])
                     acc_greater = node->in_w.v[bi] > node->in_w.v[k];
                 else if (score[a] != score[b])
-                    acc_greater = score[a] > score[b];
+                    acc_greater = MUTANT(18) ? score[a] < score[b] : score[a] > score[b];
                 else
-                    acc_greater = k > bi;
+                    acc_greater = MUTANT(17) ? k < bi : k > bi;
                 if (!acc_greater) bi = k;
             }
             size_t pr = node->incoming.v[bi];
@@ -718,7 +735,9 @@ static int dp_setup(const op_params *p, const og_graph *g, size_t beg_id, size_t
             size_t pr = rank - beg;
             if (s->head[pr] == NEG) continue;
             size_t w = node->in_w.v[k];
-            if (!have || w > bw2 || (w == bw2 && s->support[pr] > s->support[bp])) {
+            const int better = MUTANT(9) ? s->support[pr] < s->support[bp]
+                                         : s->support[pr] > s->support[bp];
+            if (!have || w > bw2 || (w == bw2 && better)) {
                 have = 1;
                 bp = pr;
                 bw2 = w;
@@ -741,7 +760,9 @@ static int dp_setup(const op_params *p, const og_graph *g, size_t beg_id, size_t
             size_t sc = rank - beg;
             if (s->tail[sc] == NEG) continue;
             size_t w = node->out_w.v[k];
-            if (!have || w > bw2 || (w == bw2 && s->support[sc] > s->support[bs])) {
+            const int better = MUTANT(10) ? s->support[sc] < s->support[bs]
+                                          : s->support[sc] > s->support[bs];
+            if (!have || w > bw2 || (w == bw2 && better)) {
                 have = 1;
                 bs = sc;
                 bw2 = w;
@@ -811,11 +832,13 @@ static void adaptive_update(const og_graph *g, dpsetup *s, size_t i, size_t star
             }
         }
     }
-    if (!is_match && max_streak < 30) s->extra[i] = min64(s->extra[i] + max_streak + 1, 5 * s->band);
-    if (s->streak[i] >= 30) {
+    if (!is_match && max_streak < 30)
+        s->extra[i] = min64(s->extra[i] + max_streak + (MUTANT(13) ? 0 : 1),
+                            (MUTANT(14) ? 4 : 5) * s->band);
+    if (!MUTANT(11) && s->streak[i] >= (MUTANT(12) ? 31 : 30)) {
         s->offset_left[i] = maximum - s->head[i];
         s->offset_right[i] = maximum - s->head[i];
-        s->extra[i] = 0;
+        if (!MUTANT(15)) s->extra[i] = 0;
         for (size_t k = 0; k < node->outgoing.n; k++) {
             size_t rank = g->nodes[node->outgoing.v[k]].rank;
             if (rank > s->beg + i && rank <= s->end) {
@@ -833,7 +856,7 @@ static void adaptive_update(const og_graph *g, dpsetup *s, size_t i, size_t star
         s->offset_right[sc] = max64(s->offset_right[sc], s->offset_right[i]);
         s->peak_left[sc] = min64(s->peak_left[sc], maximum);
         s->peak_right[sc] = max64(s->peak_right[sc], maximum + 1);
-        s->extra[sc] = s->extra[i] != 0 ? max64(s->extra[sc], s->extra[i]) : 0;
+        s->extra[sc] = (MUTANT(19) || s->extra[i] != 0) ? max64(s->extra[sc], s->extra[i]) : 0;
     }
 }
 
@@ -905,7 +928,10 @@ static int64_t tb_choose(const og_graph *g, const dpsetup *s, const omat *x, con
         if (s->head[pr] == NEG || !x->contains(x, pr, pj)) continue;
         int64_t source = state == DELETE ? x->d(x, pr, pj) : x->m(x, pr, pj);
         int64_t target = state == MATCH ? x->m(x, i, j) : x->d(x, i, j);
-        if (target == source + penalty && (best < 0 || node->in_w.v[k] > weight)) {
+        const int heavier = MUTANT(1) ? node->in_w.v[k] >= weight
+                            : MUTANT(2) ? 0
+                                        : node->in_w.v[k] > weight;
+        if (target == source + penalty && (best < 0 || heavier)) {
             best = (int64_t)k;
             weight = node->in_w.v[k];
         }
@@ -935,7 +961,10 @@ static int traceback(const op_params *p, const og_graph *g, const dpsetup *s, co
         if (pref >= 0) {
             size_t tot = 0;
             for (size_t k = 0; k < node->in_w.n; k++) tot += node->in_w.v[k];
-            if (!(node->in_w.v[pref] >= tot / 10)) pref = -1;
+            const int keep = MUTANT(3) ? 1
+                             : MUTANT(4) ? node->in_w.v[pref] > tot / 10
+                                         : node->in_w.v[pref] >= tot / 10;
+            if (!keep) pref = -1;
         }
         if (pref >= 0) {
             push_diagonal(result, g, i, n, j, m, s->seq[j], node->id, base);
@@ -945,10 +974,19 @@ static int traceback(const op_params *p, const og_graph *g, const dpsetup *s, co
             continue;
         }
         if ((op & DELETE) && (op == DELETE || x->m(x, i, j) == x->d(x, i, j))) {
-            int64_t k = tb_choose(g, s, x, node, i, j, INSERT, gap_open, j);
+            int64_t k;
+            if (MUTANT(5)) {
+                k = tb_choose(g, s, x, node, i, j, DELETE, gap_ext, j);
+                if (k >= 0) {
+                    i = g->nodes[node->incoming.v[k]].rank - beg;
+                    op = DELETE;
+                    continue;
+                }
+            }
+            k = tb_choose(g, s, x, node, i, j, INSERT, gap_open, j);
             if (k >= 0) {
                 i = g->nodes[node->incoming.v[k]].rank - beg;
-                op = p->gap_open == 0 ? ALL : (MATCH | INSERT);
+                op = (p->gap_open == 0 && !MUTANT(6)) ? ALL : (MATCH | INSERT);
                 continue;
             }
             k = tb_choose(g, s, x, node, i, j, DELETE, gap_ext, j);
@@ -960,10 +998,16 @@ static int traceback(const op_params *p, const og_graph *g, const dpsetup *s, co
         }
         if ((op & INSERT) && j > 0 && x->contains(x, i, j - 1) &&
             (op == INSERT || x->m(x, i, j) == x->ins(x, i, j))) {
+            if (MUTANT(8) && x->ins(x, i, j) == x->ins(x, i, j - 1) + gap_ext) {
+                ob_push(result, -1, (int64_t)s->seq[j], -1);
+                j -= 1;
+                op = INSERT;
+                continue;
+            }
             if (x->ins(x, i, j) == x->m(x, i, j - 1) + gap_open) {
                 ob_push(result, -1, (int64_t)s->seq[j], -1);
                 j -= 1;
-                op = p->gap_open == 0 ? ALL : (MATCH | DELETE);
+                op = (p->gap_open == 0 && !MUTANT(7)) ? ALL : (MATCH | DELETE);
                 continue;
             }
             if (x->ins(x, i, j) == x->ins(x, i, j - 1) + gap_ext) {
