@@ -9,6 +9,7 @@ library or device raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from collections import namedtuple
 from pathlib import Path
 
@@ -16,6 +17,10 @@ from .params import AlignedBase, Params, Sequence, default_matrix, records
 
 PKG = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG / "libminipoa_hip.so"
+# MPOA_LIB names another build of the library (for side-by-side comparisons
+# of build options); the default is the in-tree build.
+if os.environ.get("MPOA_LIB"):
+    LIB_PATH = Path(os.environ["MPOA_LIB"])
 
 STATUS = {
     0: "MPOA_OK", 1: "MPOA_ERR_ANCHOR_NOT_IN_GRAPH", 2: "MPOA_ERR_ANCHORS_NOT_ORDERED",
