@@ -25,7 +25,7 @@ struct RowIn {
     uint32_t pred_beg;  // into the job's predecessor list
     int32_t head;       // most-supported distance from beg (kHeadNone = NEG)
     int32_t tail;       // most-supported distance to end (kHeadNone = NEG)
-    uint32_t pad_;
+    uint32_t last_use;  // last row that has this row as a predecessor (0: none)
 };
 
 struct RowAux {
@@ -33,14 +33,40 @@ struct RowAux {
     uint32_t base;      // node base (alphabet index)
 };
 
-// In-range predecessors of a row (head != NEG), sorted by row: the adaptive
-// band state is folded from them in processing order. `ord` is the index in
-// the node's `incoming` list, which breaks traceback weight ties.
+// In-range predecessors of a row (head != NEG), in the traceback's `choose`
+// order (heaviest edge first, then the earliest in `incoming`,
+// src/align.rs:1201-1225): among tied candidates the winner is the one of
+// lowest position, and records name winners by position. The adaptive band
+// state is folded from the same entries in rank order, which rperm gives:
+// entry rperm of the list is the t-th predecessor by rank, for the t-th
+// entry.
 struct PredE {
     uint32_t row;       // predecessor row (rank - beg)
     uint32_t weight;    // in_weight of that edge
     uint32_t ord;       // position in `incoming`
+    uint32_t rperm;     // list position of the t-th predecessor in rank order
 };
+
+// Traceback record of one cell (the fill writes it, the traceback walks it):
+//   bit 0  R    M > NEG/2 (the cell can be on a path)
+//   bit 1  MD   M == D
+//   bit 2  MI   M == I
+//   bit 3  NO   I of the next column takes the open term:  I[j+1] == M[j] + o
+//   bit 4  NE   I of the next column takes the extend term: I[j+1] == I[j] + e
+//   then three fields of b bits: the list position of the `choose` winner
+//   for the diagonal (M[p][j-1] + score == M), the deletion open
+//   (M[p][j] + o == D) and the deletion extension (D[p][j] + e == D), or all
+//   ones for none.
+// b and the record's width follow the row's predecessor count np.
+constexpr uint32_t kRecFlagR = 1, kRecFlagMD = 2, kRecFlagMI = 4, kRecFlagNO = 8, kRecFlagNE = 16;
+constexpr uint32_t kMaxRecPreds = 65534;  // predecessors of one row a record can name
+// (constexpr: usable in host and device code)
+constexpr uint32_t rec_field_bits(uint32_t np) {
+    return np <= 1 ? 1u : np <= 7 ? 3u : np <= 511 ? 9u : 16u;
+}
+constexpr uint32_t rec_bytes(uint32_t np) {
+    return np <= 1 ? 1u : np <= 7 ? 2u : np <= 511 ? 4u : 8u;
+}
 
 // Final adaptive band state of a computed row (src/align.rs:829-897): its
 // M argmax column, match streak, band offsets and extra width. Successors
@@ -57,10 +83,13 @@ constexpr uint64_t kMaxSpan = uint64_t(1) << 29;  // n + padded of a device DP
 constexpr int32_t kBandCap = int32_t(1) << 30;
 
 // Row geometry written by the fill kernel, read by traceback.
+constexpr uint64_t kNoValues = ~uint64_t(0);
 struct RowMeta {
     uint32_t start;     // first column (multiple of 8)
     uint32_t len;       // width (multiple of 8); 0 for skipped rows
-    uint64_t off;       // element offset of [M len][D len][I len] in the arena
+    uint64_t off;       // byte offset of the row's len traceback records in the arena
+    uint64_t voff;      // byte offset of [M len][D len] scores, or kNoValues
+    int64_t mlast;      // M of the row's last column (start + len - 1)
 };
 
 struct DevJob {
@@ -68,8 +97,8 @@ struct DevJob {
     uint64_t preds;     // PredE index of the job's first entry
     uint64_t seq;       // byte offset of the encoded, padded query
     uint64_t matrix;    // int32 index of the alpha x alpha matrix
-    uint64_t arena;     // element offset of the job's own arena region
-    uint64_t arena_cap; // elements in that region (more come from the pool)
+    uint64_t arena;     // byte offset of the job's own arena region
+    uint64_t arena_cap; // bytes in that region (more come from the pool)
     uint64_t rowmeta;   // RowMeta index of row 0
     uint64_t adapt;     // RowRec index of row 0 (adaptive jobs)
     uint64_t tb;        // uint32 index of the m-entry traceback record
@@ -92,6 +121,8 @@ enum DevStatus : int32_t {
     kTbNoAdvance = 8,
     kTbNoPredecessor = 9,
     kArenaOverflow = 100,
+    kTooManyPreds = 101,  // a row with more than kMaxRecPreds predecessors
+    kLostScores = 102,    // a predecessor's scores were neither in LDS nor kept in HBM
 };
 
 struct DevResult {
@@ -100,7 +131,7 @@ struct DevResult {
     uint32_t pad_;
     int64_t score;      // M at (n-1, m-1)
     uint64_t cells;     // sum of computed row widths
-    uint64_t arena_used;
+    uint64_t arena_used;  // arena bytes the DP took
     uint64_t pred_cells;  // sum over rows and predecessors of the band overlap
 };
 

@@ -111,7 +111,7 @@ struct Engine::Job {
     int seed_rc = MPOA_OK;
     std::string seed_err;
     uint32_t pending = 0;    // DPs of the step on the device (under the engine lock)
-    uint64_t prev_used = 0, prev_n = 0;
+    uint64_t prev_used = 0, prev_cells = 0, prev_n = 0;
     uint64_t cost = 0;       // expected cells of the step
     uint64_t rem = ~0ull;    // cells left when last queued (0 once finished; under the engine lock)
     uint64_t footprint = 0;  // bytes counted in active_bytes_ (under the engine lock)
@@ -501,6 +501,7 @@ bool Engine::consume(Job &J, std::vector<Dp *> &again) {
         if (D.status != MPOA_OK) continue;
         if (!D.retry && J.win.empty()) {
             J.prev_used = D.res.arena_used;
+            J.prev_cells = D.res.cells;
             J.prev_n = D.plan.n;
         }
         J.out.cells += D.res.cells;
@@ -581,9 +582,9 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
         // scaled to the grown graph.
         if (J.win.empty() && (P.flags & kAdaptive) && J.prev_n > 0) {
             uint64_t hint = J.prev_used * P.n / J.prev_n;
-            hint += hint / 8 + 3ull * P.padded * 8;
+            hint += hint / 8 + uint64_t(P.padded) * 8 * (P.use_i64 ? 18 : 10);
             P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
-            P.width_hint = uint32_t(J.prev_used / (3 * J.prev_n));
+            P.width_hint = uint32_t(J.prev_cells / J.prev_n);
             D.cost = uint64_t(P.width_hint) * P.n;
         }
         J.cost += D.cost;

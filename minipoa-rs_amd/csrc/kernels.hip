@@ -19,11 +19,14 @@
 //    Kogge-Stone scan (integer max-plus is associative, so this equals the
 //    serial recurrence exactly, as src/align.rs:1568-1601 requires);
 //  * the first-argmax of M for adaptive bands is two DPP reductions.
-// M, D and I of every cell are written to the global arena for traceback.
+// What leaves the chip per cell is a traceback record of 1-8 bytes (the
+// cell's flags and the `choose` winners among its predecessors, resolved
+// while they are folded in); M and D go to HBM only for rows read by a row
+// other than the next.
 //
-// Traceback: one lane per job replays the weighted deterministic traceback
-// (src/align.rs:1164-1313) and writes, per query column, the row of its
-// diagonal step or an insertion mark.
+// Traceback: one wave per job replays the weighted deterministic traceback
+// (src/align.rs:1164-1313) over those records and writes, per query column,
+// the row of its diagonal step or an insertion mark.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -156,7 +159,8 @@ struct LMeta {
     uint32_t start, len;
     uint32_t slot;   // byte offset in the ring, or kNoSlot
     uint32_t wpos;   // ring write position (monotonic bytes) of the slot
-    uint64_t off;    // arena element offset of the row
+    uint64_t voff;   // arena byte offset of the row's kept M and D, or kNoValues
+    int64_t mlast;   // M of the row's last column
 };
 
 // Stores a lane's C contiguous values; whole runs use the widest aligned
@@ -253,6 +257,42 @@ template <int W> __device__ __forceinline__ void sync_global() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Running maximum of one kind of candidate with its `choose` winner: the
+// predecessors are folded in list order, which is `choose` order
+// (src/align.rs:1201-1225: the heaviest edge, then the first in
+// `incoming`), so among tied candidates the first one stays.
+template <typename T>
+__device__ __forceinline__ void track(T &v, uint32_t &w, T c, uint32_t idx) {
+    w = c > v ? idx : w;
+    v = tmax(v, c);
+}
+
+// Stores a lane's C traceback records of rb (1, 2, 4 or 8) bytes each. Row
+// records start 16-byte aligned and a lane's run at a multiple of C records,
+// so every group of four records is one aligned vector store; `valid` is a
+// multiple of four (the 8-column band grid).
+template <int C>
+__device__ __forceinline__ void store_recs(uint8_t *dst, const uint64_t (&r)[C], uint32_t rb, uint32_t valid) {
+#pragma unroll
+    for (int g = 0; g < C; g += 4) {
+        if (uint32_t(g) >= valid) break;
+        uint8_t *d = dst + size_t(g) * rb;
+        if (rb == 1) {
+            *reinterpret_cast<uint32_t *>(d) = uint32_t(r[g]) | uint32_t(r[g + 1]) << 8 |
+                                               uint32_t(r[g + 2]) << 16 | uint32_t(r[g + 3]) << 24;
+        } else if (rb == 2) {
+            *reinterpret_cast<uint2 *>(d) = make_uint2(uint32_t(r[g]) | uint32_t(r[g + 1]) << 16,
+                                                       uint32_t(r[g + 2]) | uint32_t(r[g + 3]) << 16);
+        } else if (rb == 4) {
+            *reinterpret_cast<uint4 *>(d) = make_uint4(uint32_t(r[g]), uint32_t(r[g + 1]),
+                                                       uint32_t(r[g + 2]), uint32_t(r[g + 3]));
+        } else {
+            *reinterpret_cast<ulonglong2 *>(d) = make_ulonglong2(r[g], r[g + 1]);
+            *reinterpret_cast<ulonglong2 *>(d + 16) = make_ulonglong2(r[g + 2], r[g + 3]);
+        }
+    }
+}
+
 // Fill: a workgroup of W waves per job. The row is covered in stripes of
 // 64*C*W columns; in each stripe wave w owns the 64*C columns from
 // w*64*C and lane l of it the C columns from l*C. Every wave computes the
@@ -261,14 +301,25 @@ template <int W> __device__ __forceinline__ void sync_global() {
 // across waves (per stripe) and to reduce the row's first argmax.
 // LT: the score matrix and the query codes are staged in LDS (every launch
 // whose alphabet and queries fit); otherwise they are read from global memory.
-// amdgpu_waves_per_eu(4) for the i32 shapes that fit 128 VGPRs without
-// spilling (8x1, 4x2, 4x4): four waves share each SIMD (three at the 147
-// VGPRs the compiler picks for 4x4 by itself). Rows are latency chains, so
-// throughput comes from the number of resident jobs. The other kernels
-// would spill in the row loop under that limit, and on gfx9 a scratch
-// reload waits for the wave's outstanding stores; they keep the default.
+//
+// What a row leaves in HBM: one traceback record per cell (bit layout in
+// dp_types.hpp) -- for the cell's own M, D and I, which term the next
+// column's I takes, and the `choose` winners of the diagonal, the deletion
+// open and the deletion extension, tracked while the predecessors are folded
+// in -- and, only for a row that a later row other than the next one reads
+// (its last use, from the host) or that has no ring slot, its M and D
+// scores. The next row always finds its immediate predecessor in the ring:
+// a row whose placement would overwrite the previous row's slot gets no slot
+// (and keeps its scores in HBM if any row reads it).
+//
+// amdgpu_waves_per_eu(4) for the i32 shapes with four columns per lane
+// (4x2, 4x4): four waves share each SIMD. Rows are latency chains, so
+// throughput comes from the number of resident jobs. One-wave shapes are
+// held to few jobs per CU by their LDS already, and wider lanes would spill
+// under the limit (on gfx9 a scratch reload waits for the wave's outstanding
+// stores); they keep the default.
 template <typename T, int C, int W, bool LT> constexpr int fill_waves_per_eu() {
-    return sizeof(T) == 4 && LT && (C == 4 || (C == 8 && W == 1)) ? 4 : 1;
+    return sizeof(T) == 4 && LT && C == 4 && W > 1 ? 4 : 1;
 }
 template <typename T, int C, int W, bool LT>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT>()))) void poa_fill(DevPools P) {
@@ -284,6 +335,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     __shared__ T xbest[W];            // per-wave row maximum of M
     __shared__ uint32_t xbestc[W];    // its first column
     __shared__ uint64_t xpool;        // arena chunk taken by thread 0
+    __shared__ int64_t xscore;        // M at (n-1, m-1)
     __shared__ __align__(16) T negrow[16];  // NEG: reads outside a predecessor's band
     const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
     T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
@@ -302,7 +354,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const int32_t *gmat = P.matrix + J.matrix;
-    T *arena = reinterpret_cast<T *>(P.arena);  // RowMeta offsets are absolute
+    uint8_t *arena = static_cast<uint8_t *>(P.arena);  // RowMeta offsets are absolute bytes
     RowMeta *meta = P.meta + J.rowmeta;
     RowRec *grec = P.adapt + J.adapt;
     const bool adaptive = (J.flags & kAdaptive) != 0;
@@ -312,18 +364,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     const T a64 = T(64) * a;   // decay over one wave's columns
     const T d15 = T((lane & 15) + 1) * a, d31 = T((lane & 31) + 1) * a;
     constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
+    constexpr T HALF = NEG / 2;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
     const uint32_t alpha = J.alpha;
     constexpr bool mat_lds = LT, seq_lds = LT;
     if (tid < 16) negrow[tid] = NEG;
+    if (leader) xscore = int64_t(NEG);
     if (mat_lds)
         for (uint32_t k = tid; k < alpha * alpha; k += 64u * W) smat[k] = glb(gmat)[k];
     if (seq_lds)
         for (uint32_t k = tid; k < J.padded; k += 64u * W) sseq[k] = glb(seq)[k];
 
-    uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region
+    uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region (bytes)
     uint64_t used = 0, cells = 0, pcells = 0;
     uint32_t wpos = 0;      // ring bytes handed out so far
     uint32_t ring_at = 0;   // wpos modulo the ring size
+    uint32_t prev_slot = kNoSlot;  // ring slot of row i - 1 (kNoSlot: none)
     uint32_t pb0 = 0, pcnt = 0;
     int32_t status = kOk;
     // Record of the previous row, kept in registers: the leader writes it
@@ -349,28 +405,40 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         if (ri.head == kHeadNone || ri.tail == kHeadNone) {
             // Skipped row (src/align.rs:708-710): no band, never pushes state.
             if (leader) {
-                hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0, 0};
+                hmeta[i % kHist] = LMeta{0, 0, kNoSlot, 0, kNoValues, int64_t(NEG)};
                 hrec[i % kHist] = RowRec{0, 0, 0, 0, 0};
-                meta[i] = RowMeta{0, 0, 0};
+                meta[i] = RowMeta{0, 0, 0, kNoValues, int64_t(NEG)};
                 if (adaptive) grec[i] = RowRec{0, 0, 0, 0, 0};
             }
             last_rec = RowRec{0, 0, 0, 0, 0};
+            prev_slot = kNoSlot;
             sync_lds<W>();
             continue;
         }
         const uint32_t np = pe - pb;
+        if (np > kMaxRecPreds) {
+            status = kTooManyPreds;
+            break;
+        }
+        const uint32_t rb = rec_bytes(np), fb = rec_field_bits(np);
+        const uint64_t fnone = (uint64_t(1) << fb) - 1u;
+        const uint32_t ffn = uint32_t(fnone);
+        const bool one = np == 1;
 
         // Gather the predecessors in parallel, lane q <-> predecessor q (in
-        // groups of 64): row, geometry, ring slot and adaptive record.
-        // Rows older than the LDS history come from global memory.
-        uint32_t g_row = 0;
-        LMeta g_meta{0, 0, kNoSlot, 0, 0};
+        // groups of 64): row, geometry, ring slot, adaptive record, list
+        // position and rank permutation. Rows older than the LDS history come
+        // from global memory.
+        uint32_t g_row = 0, g_key = 0, g_rperm = 0;
+        LMeta g_meta{0, 0, kNoSlot, 0, kNoValues, 0};
         RowRec g_rec{0, 0, 0, 0, 0};
         auto gather = [&](uint32_t g0) {
             const uint32_t q = g0 + uint32_t(lane);
-            g_meta = LMeta{0, 0, kNoSlot, 0, 0};
+            g_meta = LMeta{0, 0, kNoSlot, 0, kNoValues, 0};
             g_rec = RowRec{0, 0, 0, 0, 0};
             g_row = 0;
+            g_key = 0;
+            g_rperm = 0;
             // The global fallbacks below wait for their loads where they are
             // issued: a load still counted at a control-flow join makes the
             // compiler put vmcnt waits on the hot path, and on gfx9 vmcnt
@@ -385,6 +453,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 }
                 g_row = pq.row;
+                g_key = q - pb;
+                g_rperm = pq.rperm;
             }
             const bool far = q < pe && i - g_row >= kHist;
             if (q < pe) {
@@ -393,7 +463,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                     if (adaptive) g_rec = g_row + 1 == i ? last_rec : hrec[g_row % kHist];
                 } else {
                     const RowMeta gm = ldg(meta + g_row);
-                    g_meta = LMeta{gm.start, gm.len, kNoSlot, 0, gm.off};
+                    g_meta = LMeta{gm.start, gm.len, kNoSlot, 0, gm.voff, gm.mlast};
                     if (adaptive) g_rec = ldg(grec + g_row);
                     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 }
@@ -411,26 +481,40 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         // Adaptive state of row i: the reference's pushes from its computed
         // predecessors (rows >= 1), replayed in rank order.
         int32_t ofl = 0, ofr = 0, pkl = int32_t(m), pkr = 0, extra = 0;
-        if (adaptive) {
-            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
-                if (np > 64) gather(g0);
-                const uint32_t cnt = tmin(64u, pe - g0);
-                for (uint32_t q = 0; q < cnt; q++) {
-                    const uint32_t p = rl32(g_row, q);
-                    if (p == 0 || rl32(g_meta.len, q) == 0) continue;
-                    const int32_t r_streak = rli(g_rec.streak, q), r_ofl = rli(g_rec.ofl, q);
-                    const int32_t r_ofr = rli(g_rec.ofr, q), r_peak = rli(g_rec.peak, q);
-                    const int32_t r_extra = rli(g_rec.extra, q);
-                    if (r_streak >= 30) {
-                        ofl = r_ofl;
-                        ofr = r_ofr;
-                    }
-                    ofl = tmin(ofl, r_ofl);
-                    ofr = tmax(ofr, r_ofr);
-                    pkl = tmin(pkl, r_peak);
-                    pkr = tmax(pkr, r_peak + 1);
-                    extra = r_extra != 0 ? tmax(extra, r_extra) : 0;
+        auto push = [&](const RowRec &r) {
+            if (r.streak >= 30) {
+                ofl = r.ofl;
+                ofr = r.ofr;
+            }
+            ofl = tmin(ofl, r.ofl);
+            ofr = tmax(ofr, r.ofr);
+            pkl = tmin(pkl, r.peak);
+            pkr = tmax(pkr, r.peak + 1);
+            extra = r.extra != 0 ? tmax(extra, r.extra) : 0;
+        };
+        if (adaptive && np <= 64) {
+            for (uint32_t t = 0; t < np; t++) {
+                const uint32_t q = rl32(g_rperm, t);
+                if (rl32(g_row, q) == 0 || rl32(g_meta.len, q) == 0) continue;
+                push(RowRec{rli(g_rec.peak, q), rli(g_rec.streak, q), rli(g_rec.ofl, q),
+                            rli(g_rec.ofr, q), rli(g_rec.extra, q)});
+            }
+        } else if (adaptive) {
+            // More than 64 predecessors: one at a time, uniform loads.
+            for (uint32_t t = 0; t < np; t++) {
+                const uint32_t q = ldg(pr + pb + t).rperm;
+                const uint32_t p = ldg(pr + pb + q).row;
+                uint32_t plen;
+                RowRec r;
+                if (i - p < kHist) {
+                    plen = hmeta[p % kHist].len;
+                    r = p + 1 == i ? last_rec : hrec[p % kHist];
+                } else {
+                    plen = ldg(meta + p).len;
+                    r = ldg(grec + p);
+                    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 }
+                if (p != 0 && plen != 0) push(r);
             }
         }
 
@@ -458,7 +542,29 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         const uint32_t start = uint32_t(left) / 8 * 8;
         const uint32_t stop = (uint32_t(right) / 8 + 1) * 8;
         const uint32_t len = stop - start;
-        const uint64_t need = 3ull * len;
+
+        // Ring slot for this row's M and D: never over the previous row's
+        // slot, which this row reads.
+        const uint32_t bytes = 2u * len * uint32_t(sizeof(T));
+        uint32_t slot = kNoSlot, swpos = 0;
+        if (bytes <= kRingBytes / 2) {
+            const bool wrap = ring_at + bytes > kRingBytes;
+            if (!wrap || prev_slot == kNoSlot || bytes <= prev_slot) {
+                if (wrap) {
+                    wpos += kRingBytes - ring_at;
+                    ring_at = 0;
+                }
+                slot = ring_at;
+                swpos = wpos;
+                wpos += bytes;
+                ring_at += bytes;
+            }
+        }
+        // Scores stay in HBM for a row that a later row other than the next
+        // one reads, or that the next row cannot find in the ring.
+        const bool keep = ri.last_use > i + 1 || (ri.last_use > i && slot == kNoSlot);
+        const uint64_t rec_sz = (uint64_t(len) * rb + 15u) / 16u * 16u;
+        const uint64_t need = rec_sz + (keep ? uint64_t(bytes) : 0u);
         if (cur + need > rend) {
             // Own region exhausted: take a chunk from the launch's pool. The
             // leader's result reaches its wave by readlane and other waves
@@ -481,42 +587,30 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
             rend = got + P.chunk;
         }
         const uint64_t off = cur;
+        const uint64_t voff = keep ? off + rec_sz : kNoValues;
         cur += need;
         used += need;
         cells += len;
-        T *Mr = arena + off;
-        T *Dr = Mr + len;
-        T *Ir = Dr + len;
-
-        // Ring slot for this row's M and D.
-        const uint32_t bytes = 2u * len * uint32_t(sizeof(T));
-        uint32_t slot = kNoSlot, swpos = 0;
-        if (bytes <= kRingBytes / 2) {
-            if (ring_at + bytes > kRingBytes) {
-                wpos += kRingBytes - ring_at;
-                ring_at = 0;
-            }
-            slot = ring_at;
-            swpos = wpos;
-            wpos += bytes;
-            ring_at += bytes;
-        }
+        uint8_t *Rr = arena + off;
+        T *Mk = keep ? reinterpret_cast<T *>(arena + voff) : nullptr;
+        T *Dk = keep ? Mk + len : nullptr;
         const uint32_t sm = (slot == kNoSlot ? 0u : slot) / uint32_t(sizeof(T));
         const uint32_t sd = sm + len;
-        if (leader) {
-            meta[i] = RowMeta{start, len, off};
-            // Read by later rows only after this row's first barrier.
-            hmeta[i % kHist] = LMeta{start, len, slot, swpos, off};
-        }
+        // The lane owning the row's last column writes its RowMeta and the
+        // mlast of this LDS copy once the row is computed; later rows read
+        // the copy only after this row's first barrier (mlast after its last).
+        if (leader) hmeta[i % kHist] = LMeta{start, len, slot, swpos, voff, int64_t(NEG)};
 
         if (i == 0) {
             // Source row (src/align.rs:737-750): M[0] = score(N,N), then
-            // insertions only; D stays NEG.
+            // insertions only; D stays NEG; no predecessors, so every winner
+            // field is none.
             if (start != 0) {
                 status = kBandExcludesSource;
                 break;
             }
             if (leader) {
+                const uint64_t wnone = fnone << 5 | fnone << (5 + fb) | fnone << (5 + 2 * fb);
                 T mprev = T(J.score_nn), iprev = NEG;
                 for (uint32_t j = 0; j < len; j++) {
                     T mj = mprev, ins = iprev;
@@ -524,20 +618,29 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                         ins = tmax(iprev + e, mprev + o);
                         mj = tmax(NEG, ins);
                     }
-                    Mr[j] = mj;
-                    Dr[j] = NEG;
-                    Ir[j] = ins;
+                    const uint64_t f = uint64_t(mj > HALF) | uint64_t(mj == NEG) << 1 |
+                                       uint64_t(mj == ins) << 2 | uint64_t(mj + o >= ins + e) << 3 |
+                                       uint64_t(ins + e >= mj + o) << 4;
+                    Rr[j] = uint8_t(f | wnone);  // one predecessor or none: 1-byte records
+                    if (keep) {
+                        Mk[j] = mj;
+                        Dk[j] = NEG;
+                    }
                     if (slot != kNoSlot) {
                         ring[sm + j] = mj;
                         ring[sd + j] = NEG;
                     }
+                    if (n == 1 && j == m - 1) xscore = int64_t(mj);
                     iprev = ins;
                     mprev = mj;
                 }
+                meta[0] = RowMeta{start, len, off, voff, int64_t(mprev)};
+                hmeta[0].mlast = int64_t(mprev);
                 hrec[0] = RowRec{0, 0, 0, 0, 0};
                 if (adaptive) grec[0] = RowRec{0, 0, 0, 0, 0};
             }
             last_rec = RowRec{0, 0, 0, 0, 0};
+            prev_slot = slot;
             sync_global<W>();  // the source row's arena values too
             continue;
         }
@@ -558,23 +661,49 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         T best = NI;
         uint32_t bestc = 0xFFFFFFFFu;
         uint32_t parity = 0;
+        bool lost = false;  // a predecessor's scores were in neither ring nor HBM
         for (uint32_t seg = start; seg < stop; seg += 64u * C * W, parity ^= 1u) {
             const uint32_t c0 = seg + wave * 64u * C + uint32_t(lane) * C;
             // A wave whose columns all lie past the row's end only joins the
             // carry exchange: its issue slots go to other workgroups.
             const bool live = seg + wave * 64u * C < stop;
-            T Mv[C], Dv[C], md[C], Lk[C];
+            // Per column: the diagonal maximum and the two deletion maxima
+            // with their `choose` winners (keys), then D, the profile score,
+            // md = max(diagonal + profile, D) and the in-lane insertion prefix.
+            T Mdg[C], Do[C], De[C], Dv[C], pf[C], md[C], Lk[C];
+            uint32_t wm[C], wo[C], we[C];
             T S = NI, Sp = NI;
             if (live) {
 #pragma unroll
                 for (int k = 0; k < C; k++) {
-                    Mv[k] = NEG;
-                    Dv[k] = NEG;
+                    Mdg[k] = NEG;
+                    Do[k] = NEG;
+                    De[k] = NEG;
+                    wm[k] = wo[k] = we[k] = kNone;
                 }
                 // Predecessor transitions on the band overlap (src/align.rs:751-797).
                 // Bands and lane runs sit on the same 8-column grid, so each
                 // piece of G columns is wholly inside or outside an overlap; the
-                // diagonal into the predecessor's first column reads NEG.
+                // diagonal into the predecessor's first column reads NEG. Pieces
+                // outside an overlap fold the NEG row: such a candidate is far
+                // below any cell a path can reach, so the winner it may become
+                // is never read. A row with one predecessor needs no tie rule.
+                auto fold = [&](int sg, const T (&pm)[G], const T (&pd)[G], T dg0, uint32_t key) {
+#pragma unroll
+                    for (int k = 0; k < G; k++) {
+                        const T dg = k == 0 ? dg0 : pm[k - 1];
+                        if (one) {
+                            Mdg[sg + k] = dg;
+                            Do[sg + k] = pm[k] + o;
+                            De[sg + k] = pd[k] + e;
+                            wm[sg + k] = wo[sg + k] = we[sg + k] = key;
+                        } else {
+                            track(Mdg[sg + k], wm[sg + k], dg, key);
+                            track(Do[sg + k], wo[sg + k], T(pm[k] + o), key);
+                            track(De[sg + k], we[sg + k], T(pd[k] + e), key);
+                        }
+                    }
+                };
                 if (!anymiss) {
                     // Every predecessor row is in the LDS ring: this loop has no
                     // global loads, so nothing in it waits on vmcnt (which on
@@ -585,10 +714,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                         if (ovs >= ove) continue;
                         if (seg == start) pcells += ove - ovs;
                         const uint32_t pmo = rl32(g_meta.slot, q) / uint32_t(sizeof(T));
+                        const uint32_t key = rl32(g_key, q);
 #pragma unroll
                         for (int sg = 0; sg < C; sg += G) {
-                            // Pieces outside the overlap read the NEG row: max-plus
-                            // with NEG leaves M and D as they are.
                             const uint32_t cs = c0 + sg;
                             const bool in = cs >= ovs && cs < ove;
                             const uint32_t rel = cs - ps;
@@ -598,13 +726,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                             T pm[G], pd[G];
                             loadg<T, G>(mp, pm);
                             loadg<T, G>(dp, pd);
-                            const T dg0 = *gp0;
-#pragma unroll
-                            for (int k = 0; k < G; k++) {
-                                const T dg = k == 0 ? dg0 : pm[k - 1];
-                                Mv[sg + k] = tmax(Mv[sg + k], dg);
-                                Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
-                            }
+                            fold(sg, pm, pd, *gp0, key);
                         }
                     }
                 } else {
@@ -617,12 +739,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                         if (ovs >= ove) continue;
                         if (seg == start) pcells += ove - ovs;
                         const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
+                        const uint32_t key = rl32(g_key, q);
                         const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
-                        const uint64_t poff = in_ring ? 0 : uint64_t(rl64(int64_t(g_meta.off), q));
                         if (in_ring) {
-                            // Pieces outside the overlap read the NEG row: max-plus
-                            // with NEG leaves M and D as they are, so the ring path
-                            // has no per-piece branch.
                             const uint32_t pmo = pslot / uint32_t(sizeof(T));
 #pragma unroll
                             for (int sg = 0; sg < C; sg += G) {
@@ -635,23 +754,26 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                                 T pm[G], pd[G];
                                 loadg<T, G>(mp, pm);
                                 loadg<T, G>(dp, pd);
-                                const T dg0 = *gp0;
-#pragma unroll
-                                for (int k = 0; k < G; k++) {
-                                    const T dg = k == 0 ? dg0 : pm[k - 1];
-                                    Mv[sg + k] = tmax(Mv[sg + k], dg);
-                                    Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
-                                }
+                                fold(sg, pm, pd, *gp0, key);
                             }
                             continue;
                         }
+                        const uint64_t pvoff = uint64_t(rl64(int64_t(g_meta.voff), q));
+                        if (pvoff == kNoValues) {
+                            // Every row read by a row other than the next one
+                            // keeps its scores, and the next row always finds
+                            // its predecessor in the ring; refuse rather than
+                            // read outside the arena if that ever fails.
+                            lost = true;
+                            continue;
+                        }
+                        const gptr<const T> PM = glb(reinterpret_cast<const T *>(arena + pvoff));
 #pragma unroll
                         for (int sg = 0; sg < C; sg += G) {
                             const uint32_t cs = c0 + sg;
                             if (cs < ovs || cs >= ove) continue;
                             const uint32_t rel = cs - ps;
                             T pm[G], pd[G], dg0;
-                            const gptr<const T> PM = glb(static_cast<const T *>(arena + poff));
                             loadg<T, G>(PM + rel, pm);
                             loadg<T, G>(PM + pl + rel, pd);
                             dg0 = rel > 0 ? PM[rel - 1] : NEG;
@@ -660,17 +782,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                             // the ring path gets vmcnt waits, which also wait for
                             // every store of the wave.
                             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-#pragma unroll
-                            for (int k = 0; k < G; k++) {
-                                const T dg = k == 0 ? dg0 : pm[k - 1];
-                                Mv[sg + k] = tmax(Mv[sg + k], dg);
-                                Dv[sg + k] = tmax(Dv[sg + k], tmax(pd[k] + e, pm[k] + o));
-                            }
+                            fold(sg, pm, pd, dg0, key);
                         }
                     }
                 }
                 }
-                // Profile + in-lane insertion prefix without carry.
+                // D = max(open, extend); a deletion winner stands only where its
+                // kind gives D.
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    Dv[k] = tmax(Do[k], De[k]);
+                    if (Do[k] < De[k]) wo[k] = kNone;
+                    if (De[k] < Do[k]) we[k] = kNone;
+                }
                 T L = NI;
 #pragma unroll
                 for (int sg = 0; sg < C; sg += G) {
@@ -691,26 +815,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                             for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
                         }
 #pragma unroll
-                        for (int k = 0; k < G; k++) {
-                            const T prof = T(smat[srow + codes[k]]);
-                            const T v = piece_live ? tmax(Mv[sg + k] + prof, Dv[sg + k]) : NI;
-                            md[sg + k] = v;
-                            Lk[sg + k] = L;
-                            L = tmax(L + e, v + o);
-                        }
+                        for (int k = 0; k < G; k++) pf[sg + k] = T(smat[srow + codes[k]]);
                     } else {
                         if (piece_live) {
 #pragma unroll
                             for (int k = 0; k < G; k++) codes[k] = glb(seq)[cs + k];
                         }
 #pragma unroll
-                        for (int k = 0; k < G; k++) {
-                            T v = NI;
-                            if (piece_live) v = tmax(Mv[sg + k] + T(glb(gmat)[srow + codes[k]]), Dv[sg + k]);
-                            md[sg + k] = v;
-                            Lk[sg + k] = L;
-                            L = tmax(L + e, v + o);
-                        }
+                        for (int k = 0; k < G; k++) pf[sg + k] = piece_live ? T(glb(gmat)[srow + codes[k]]) : T(0);
+                    }
+#pragma unroll
+                    for (int k = 0; k < G; k++) {
+                        const T dgp = Mdg[sg + k] + pf[sg + k];
+                        // The diagonal winner stands only where the diagonal
+                        // gives M: here against D, after the scan against I.
+                        if (dgp < Dv[sg + k]) wm[sg + k] = kNone;
+                        const T v = piece_live ? tmax(dgp, Dv[sg + k]) : NI;
+                        md[sg + k] = v;
+                        Lk[sg + k] = L;
+                        L = tmax(L + e, v + o);
                     }
                 }
                 // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
@@ -737,7 +860,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
             if (!live) continue;
             const T X = tmax(cw + T(lane) * a, Sp);
             const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
-            T Iv[C];
+            T Iv[C], Mv[C];
 #pragma unroll
             for (int k = 0; k < C; k++) {
                 Iv[k] = tmax(X + T(k) * e, Lk[k]);
@@ -746,18 +869,62 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                     best = Mv[k];
                     bestc = c0 + k;
                 }
+                if (Iv[k] > md[k]) wm[k] = kNone;
+            }
+            // The diagonal `choose` also sees a predecessor whose band ends
+            // at the column before (its last column is j - 1, so the fill's
+            // overlap leaves j out): it names that predecessor where its last
+            // M plus the profile score equals M.
+            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                if (np > 64) gather(g0);
+                const uint32_t cnt = tmin(64u, pe - g0);
+                for (uint32_t q = 0; q < cnt; q++) {
+                    const uint32_t pl = rl32(g_meta.len, q), bc = rl32(g_meta.start, q) + pl;
+                    if (pl == 0 || bc < start || bc >= stop) continue;
+                    const T ml = T(rl64(g_meta.mlast, q));
+                    const uint32_t key = rl32(g_key, q);
+#pragma unroll
+                    for (int k = 0; k < C; k++)
+                        if (c0 + k == bc && Mv[k] == ml + pf[k]) wm[k] = tmin(wm[k], key);
+                }
             }
             if (valid > 0) {
+                uint64_t rec[C];
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    const uint32_t f = uint32_t(Mv[k] > HALF) | uint32_t(Mv[k] == Dv[k]) << 1 |
+                                       uint32_t(Mv[k] == Iv[k]) << 2 | uint32_t(Mv[k] + o >= Iv[k] + e) << 3 |
+                                       uint32_t(Iv[k] + e >= Mv[k] + o) << 4;
+                    rec[k] = uint64_t(f) | uint64_t(tmin(wm[k], ffn)) << 5 | uint64_t(tmin(wo[k], ffn)) << (5 + fb) |
+                             uint64_t(tmin(we[k], ffn)) << (5 + 2 * fb);
+                }
                 const uint32_t rel = c0 - start;
-                store_run<T, C>(Mr + rel, Mv, valid);
-                store_run<T, C>(Dr + rel, Dv, valid);
-                store_run<T, C>(Ir + rel, Iv, valid);
+                store_recs<C>(Rr + size_t(rel) * rb, rec, rb, valid);
+                if (keep) {
+                    store_run<T, C>(Mk + rel, Mv, valid);
+                    store_run<T, C>(Dk + rel, Dv, valid);
+                }
                 if (slot != kNoSlot) {
                     store_run<T, C>(ring + sm + rel, Mv, valid);
                     store_run<T, C>(ring + sd + rel, Dv, valid);
                 }
+                // The row's geometry with its last M (read by successors'
+                // diagonal choice), and the DP's final score.
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    if (c0 + k == stop - 1) {
+                        hmeta[i % kHist].mlast = int64_t(Mv[k]);
+                        meta[i] = RowMeta{start, len, off, voff, int64_t(Mv[k])};
+                    }
+                    if (i == n - 1 && c0 + k == m - 1) xscore = int64_t(Mv[k]);
+                }
             }
         }
+        if (lost) {
+            status = kLostScores;
+            break;
+        }
+        prev_slot = slot;
 
         // One wave: order this row's LDS writes (ring, geometry) before the
         // next row's reads. Several waves meet at the barrier below instead.
@@ -833,51 +1000,59 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         }
         last_rec = rec;
     }
-    sync_global<W>();  // every wave's stores, for the score read below
+    sync_global<W>();  // every wave's stores and the score in LDS
     if (leader) {
         DevResult R{};
         R.status = status;
         R.cells = cells;
         R.pred_cells = pcells;
         R.arena_used = used;
-        R.score = int64_t(NEG);
-        if (status == kOk) {
-            const RowMeta rm = ldg(meta + (n - 1));
-            if (m - 1 >= rm.start && m - 1 - rm.start < rm.len)
-                R.score = int64_t(glb(arena)[rm.off + (m - 1 - rm.start)]);
-        }
+        R.score = status == kOk ? xscore : int64_t(NEG);
         P.res[blockIdx.x] = R;
     }
 }
 
-// Weighted deterministic traceback (src/align.rs:1164-1313), one wave per job.
-// The walk is serial, so its cost is the latency of each step's reads. The
-// wave therefore stages a window of the matrix in LDS -- kTbRows rows ending
-// at the current row (geometry, predecessor lists and M/D/I over kTbCols
-// columns ending at the current column), loaded with all lanes and all loads
-// in flight at once -- and walks inside it, reloading when the path nears
-// the window's top-left edge. Predecessors older than the window, or rows
-// with more than kTbPred predecessors, read global memory directly.
-// The query codes and the score matrix are staged in LDS once (LT), and the
-// per-column records collect in LDS and go out as one coalesced store per 64
-// columns: on gfx9 a global load's vmcnt wait also waits for the wave's
-// stores, so a step that stored to and loaded from global memory would pay
-// a full memory round trip.
-constexpr uint32_t kTbRows = 64, kTbPred = 8, kTbReach = 16;
+// Weighted deterministic traceback (src/align.rs:1164-1313), one wave per job,
+// over the fill's per-cell records: a step reads the record of its cell (and,
+// for an insertion, of the cell to its left), which already names the
+// `choose` winners, so no predecessor scores are read. The walk is serial and
+// its cost is the latency of those reads, so the wave stages a window in LDS
+// -- kTbRows rows ending at the current row (geometry, predecessor rows and
+// weights) and their records over kTbCols columns ending at the current
+// column -- loaded with all lanes and all loads in flight, and reloads when
+// the path leaves it. Records of rows with more than 511 predecessors (8
+// bytes) and predecessor lists longer than kTbPred are read from global
+// memory. The query codes and the score matrix are staged in LDS once (LT),
+// and the per-column results collect in LDS and go out as one coalesced
+// store per 64 columns: on gfx9 a global load's vmcnt wait also waits for the
+// wave's stores, so a step that stored to and loaded from global memory
+// would pay a full memory round trip.
+constexpr uint32_t kTbRows = 64, kTbCols = 64, kTbPred = 8;
 
 struct WRow {
     uint32_t start, len;
-    uint64_t off;
+    uint64_t off;   // byte offset of the row's records
     uint32_t pb, np, thresh, base;
 };
+struct WPred {
+    uint32_t row, weight;
+};
 
-template <typename T, bool LT>
+// One traceback record from global memory (rb = 1, 2, 4 or 8 bytes).
+__device__ __forceinline__ uint64_t load_rec(const uint8_t *p, uint32_t rb) {
+    if (rb == 8) return *glb(reinterpret_cast<const uint64_t *>(p));
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t w = *glb(reinterpret_cast<const uint32_t *>(a & ~uint64_t(3)));
+    const uint32_t v = w >> ((a & 3u) * 8u);
+    return rb == 4 ? v : (v & ((1u << (8u * rb)) - 1u));
+}
+
+template <bool LT>
 __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) {
-    constexpr uint32_t kTbCols = sizeof(T) == 4 ? 32 : 16;
     __shared__ WRow wrow[kTbRows];
-    __shared__ PredE wpred[kTbRows * kTbPred];
-    __shared__ T wm[kTbRows * kTbCols], wd[kTbRows * kTbCols], wi[kTbRows * kTbCols];
-    __shared__ uint32_t tbw[64];  // records of the columns of the current 64-column chunk
+    __shared__ WPred wpred[kTbRows * kTbPred];
+    __shared__ uint32_t wrec[kTbRows * kTbCols];
+    __shared__ uint32_t tbw[64];  // results of the columns of the current 64-column chunk
     extern __shared__ __align__(16) char tb_lds[];  // LT: query codes, then the matrix
     const uint32_t jid = blockIdx.x;
     if (jid >= njobs) return;
@@ -891,26 +1066,17 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     const RowAux *aux = P.aux + J.rows;
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
-    const int32_t *mat = P.matrix + J.matrix;
-    const T *arena = reinterpret_cast<const T *>(P.arena);
+    const uint8_t *arena = static_cast<const uint8_t *>(P.arena);
     const RowMeta *meta = P.meta + J.rowmeta;
     uint32_t *tb = P.tb + J.tb;
-    const int64_t o = J.o, e = J.e;
-    const int64_t half = int64_t(Lim<T>::neg) / 2;
     const bool gap0 = (J.flags & kGapOpenZero) != 0;
     uint8_t *sseq = reinterpret_cast<uint8_t *>(tb_lds);
-    int32_t *smat = reinterpret_cast<int32_t *>(tb_lds + ((P.seq_lds + 15u) & ~15u));
     if constexpr (LT) {
-        for (uint32_t k = uint32_t(lane); k < J.alpha * J.alpha; k += 64u) smat[k] = glb(mat)[k];
         for (uint32_t k = uint32_t(lane); k < m; k += 64u) sseq[k] = glb(seq)[k];
     }
     auto qcode = [&](uint32_t c) -> uint32_t {
         if constexpr (LT) return sseq[c];
         else return glb(seq)[c];
-    };
-    auto score_of = [&](uint32_t a, uint32_t b) -> int64_t {
-        if constexpr (LT) return smat[a * J.alpha + b];
-        else return glb(mat)[size_t(a) * J.alpha + b];
     };
     // Columns are consumed in decreasing order, each exactly once, so a
     // chunk is complete when the walk moves below it.
@@ -946,170 +1112,146 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
             const RowMeta rm = ldg(meta + ri);
             const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
             const RowAux x = ldg(aux + ri);
-            const uint32_t np = tmin(b.pred_beg - a.pred_beg, kTbPred);
+            const uint32_t np = b.pred_beg - a.pred_beg;
+            const uint32_t nq = tmin(np, kTbPred);
             PredE pe[kTbPred];
 #pragma unroll
-            for (uint32_t q = 0; q < kTbPred; q++) pe[q] = ldg(pr + a.pred_beg + (q < np ? q : 0u));
+            for (uint32_t q = 0; q < kTbPred; q++)
+                pe[q] = ldg(nq ? pr + a.pred_beg + (q < nq ? q : 0u) : P.preds);
             if (r <= i0) {
-                wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
+                wrow[r] = WRow{rm.start, rm.len, rm.off, a.pred_beg, np, x.thresh, x.base};
 #pragma unroll
                 for (uint32_t q = 0; q < kTbPred; q++)
-                    if (q < np) wpred[r * kTbPred + q] = pe[q];
+                    if (q < nq) wpred[r * kTbPred + q] = WPred{pe[q].row, pe[q].weight};
             }
         }
         sync_lds<1>();
-        // M, D and I: lane k holds column cw0 + k; 16 rows per batch, every
-        // load of a batch in flight before the first is used.
+        // Records: lane k holds column cw0 + k; 16 rows per batch, every load
+        // of a batch in flight before the first is used. A record of 1, 2 or
+        // 4 bytes lies inside one aligned word (records are rb-aligned from a
+        // 16-byte aligned row start), so every row loads that word.
         const uint32_t c = cw0 + uint32_t(lane);
         const bool col_ok = uint32_t(lane) < kTbCols && c <= j0;
-        const gptr<const T> A = glb(static_cast<const T *>(arena));
+        const gptr<const uint32_t> A = glb(reinterpret_cast<const uint32_t *>(arena));
         constexpr uint32_t kB = 16;
         for (uint32_t r0 = 0; r0 < kTbRows && r0 <= i0; r0 += kB) {
-            T vm[kB], vd[kB], vi[kB];
+            uint32_t vw[kB], sh[kB];
             bool ok[kB];
 #pragma unroll
             for (uint32_t u = 0; u < kB; u++) {
                 const uint32_t r = r0 + u;
                 const WRow g = wrow[r <= i0 ? r : 0u];
-                ok[u] = col_ok && r <= i0 && c >= g.start && c - g.start < g.len;
-                const uint64_t k = ok[u] ? g.off + (c - g.start) : 0ull;
-                const uint64_t l = ok[u] ? uint64_t(g.len) : 0ull;
-                vm[u] = A[k];
-                vd[u] = A[k + l];
-                vi[u] = A[k + 2ull * l];
+                const uint32_t rb = rec_bytes(g.np);
+                ok[u] = col_ok && r <= i0 && rb <= 4u && c >= g.start && c - g.start < g.len;
+                const uint64_t a = ok[u] ? g.off + uint64_t(c - g.start) * rb : 0ull;
+                vw[u] = A[a >> 2];
+                sh[u] = (uint32_t(a) & 3u) * 8u | (rb << 8);
             }
 #pragma unroll
             for (uint32_t u = 0; u < kB; u++) {
                 if (ok[u]) {
-                    const uint32_t w = (r0 + u) * kTbCols + uint32_t(lane);
-                    wm[w] = vm[u];
-                    wd[w] = vd[u];
-                    wi[w] = vi[u];
+                    const uint32_t rb = sh[u] >> 8;
+                    const uint32_t v = vw[u] >> (sh[u] & 31u);
+                    wrec[(r0 + u) * kTbCols + uint32_t(lane)] = rb == 4 ? v : (v & ((1u << (8u * rb)) - 1u));
                 }
             }
         }
         sync_lds<1>();
     };
-    auto row_in_win = [&](uint32_t ri) { return have && ri <= i0 && i0 - ri < kTbRows; };
-    // Geometry of a row: window copy, else global.
-    auto geo = [&](uint32_t ri) -> WRow {
-        if (row_in_win(ri)) return wrow[i0 - ri];
-        const RowMeta rm = ldg(meta + ri);
-        const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
-        return WRow{rm.start, rm.len, rm.off, a.pred_beg, b.pred_beg - a.pred_beg, 0, 0};
-    };
     auto inside = [](const WRow &r, uint32_t c) { return c >= r.start && c - r.start < r.len; };
-    // State s (0 = M, 1 = D, 2 = I) of row ri at an in-band column c.
-    auto val = [&](uint32_t ri, const WRow &g, uint32_t c, int s) -> int64_t {
-        if (row_in_win(ri) && c >= cw0 && c <= j0) {
-            const uint32_t k = (i0 - ri) * kTbCols + (c - cw0);
-            if (s == 0) return int64_t(wm[k]);
-            if (s == 1) return int64_t(wd[k]);
-            return int64_t(wi[k]);
-        }
-        return int64_t(glb(arena)[g.off + uint64_t(s) * g.len + (c - g.start)]);
-    };
-    auto pred = [&](uint32_t ri, const WRow &g, uint32_t q) -> PredE {
-        if (row_in_win(ri) && g.np <= kTbPred) return wpred[(i0 - ri) * kTbPred + q];
-        return ldg(pr + g.pb + q);
-    };
 
     uint32_t i = n - 1, j = m - 1, op = kAll;
+    uint32_t prev_i = n - 1;
     uint64_t steps = 0;
     int32_t status = kOk;
     while (i > 0 || j > 0) {
-        if (++steps > uint64_t(n) + m) {
+        if (++steps > uint64_t(n) + m || i > prev_i) {
             status = kTbNoAdvance;
             break;
         }
-        if (!have || i0 - i + kTbReach > kTbRows || (j > 0 && j - 1 < cw0)) load_window(i, j);
-        const WRow rg = geo(i);
+        prev_i = i;
+        // The step reads row i at columns j and j - 1.
+        if (!have || i0 - i >= kTbRows || (j > 0 ? j - 1 : 0u) < cw0) load_window(i, j);
+        const uint32_t wr = i0 - i;
+        const WRow rg = wrow[wr];
         if (!inside(rg, j)) {
             status = kTbLeftBand;
             break;
         }
-        const int64_t Mij = val(i, rg, j, 0);
-        if (Mij <= half) {
+        const uint32_t rb = rec_bytes(rg.np), fb = rec_field_bits(rg.np);
+        const uint64_t fnone = (uint64_t(1) << fb) - 1u;
+        auto rec_at = [&](uint32_t c) -> uint64_t {
+            if (rb <= 4u) return wrec[wr * kTbCols + (c - cw0)];
+            return load_rec(arena + rg.off + uint64_t(c - rg.start) * rb, rb);
+        };
+        const uint64_t rec = rec_at(j);
+        if (!(rec & kRecFlagR)) {
             status = kTbLeftBand;
             break;
         }
-        const int64_t Dij = val(i, rg, j, 1);
-        const uint32_t base = (i == n - 1) ? J.unknown : rg.base;
-        const uint32_t qj = qcode(j);
-        const int64_t score = score_of(base, qj);
-        // `choose` (src/align.rs:1201-1225): among in-range predecessors whose
-        // state plus penalty equals the target, the largest in_weight; ties go
-        // to the earliest in `incoming` order. Returns the predecessor row.
-        auto choose = [&](uint32_t state, int64_t penalty, uint32_t pj, uint32_t &w_out) -> int64_t {
-            const int64_t target = state == kMatch ? Mij : Dij;
-            int64_t best = -1;
-            uint32_t w = 0, bord = 0;
-            for (uint32_t q = 0; q < rg.np; q++) {
-                const PredE pq = pred(i, rg, q);
-                const WRow pg = geo(pq.row);
-                if (!inside(pg, pj)) continue;
-                const int64_t src = val(pq.row, pg, pj, state == kDelete ? 1 : 0);
-                if (target == src + penalty &&
-                    (best < 0 || pq.weight > w || (pq.weight == w && pq.ord < bord))) {
-                    best = pq.row;
-                    w = pq.weight;
-                    bord = pq.ord;
-                }
-            }
-            w_out = w;
-            return best;
+        const uint32_t win_m = uint32_t((rec >> 5) & fnone);
+        const uint32_t win_o = uint32_t((rec >> (5 + fb)) & fnone);
+        const uint32_t win_e = uint32_t((rec >> (5 + 2 * fb)) & fnone);
+        // A winner names a position in the row's list; the walk only moves
+        // to earlier rows.
+        if ((win_m != fnone && win_m >= rg.np) || (win_o != fnone && win_o >= rg.np) ||
+            (win_e != fnone && win_e >= rg.np)) {
+            status = kTbNoPredecessor;
+            break;
+        }
+        // Predecessor `idx` of row i (its position in the row's list).
+        auto pred = [&](uint32_t idx) -> WPred {
+            if (rg.np <= kTbPred) return wpred[wr * kTbPred + idx];
+            const PredE pq = ldg(pr + rg.pb + idx);
+            return WPred{pq.row, pq.weight};
         };
-        uint32_t w = 0;
-        int64_t pref = -1;
-        if ((op & kMatch) && j > 0 && base == qj) pref = choose(kMatch, score, j - 1, w);
-        if (pref >= 0 && w < rg.thresh) pref = -1;
-        if (pref >= 0) {
-            put(j, i);
-            i = uint32_t(pref);
-            j -= 1;
-            op = kAll;
-            continue;
-        }
-        if ((op & kDelete) && (op == kDelete || Mij == Dij)) {
-            int64_t k = choose(kInsert, o, j, w);
-            if (k >= 0) {
-                i = uint32_t(k);
-                op = gap0 ? kAll : (kMatch | kInsert);
-                continue;
-            }
-            k = choose(kDelete, e, j, w);
-            if (k >= 0) {
-                i = uint32_t(k);
-                op = kDelete;
-                continue;
-            }
-        }
-        if ((op & kInsert) && j > 0 && inside(rg, j - 1)) {
-            const int64_t Iij = val(i, rg, j, 2);
-            if (op == kInsert || Mij == Iij) {
-                if (Iij == val(i, rg, j - 1, 0) + o) {
-                    put(j, kTbInsert);
-                    j -= 1;
-                    op = gap0 ? kAll : (kMatch | kDelete);
-                    continue;
-                }
-                if (Iij == val(i, rg, j - 1, 2) + e) {
-                    put(j, kTbInsert);
-                    j -= 1;
-                    op = kInsert;
-                    continue;
-                }
-            }
-        }
-        if ((op & kMatch) && j > 0) {
-            const int64_t k = choose(kMatch, score, j - 1, w);
-            if (k >= 0) {
+        const uint32_t base = (i == n - 1) ? J.unknown : rg.base;
+        // Preferred match: the diagonal winner, if the query base equals the
+        // node's and its edge carries at least a tenth of the node's support.
+        if ((op & kMatch) && j > 0 && win_m != fnone && base == qcode(j)) {
+            const WPred p = pred(win_m);
+            if (p.weight >= rg.thresh) {
                 put(j, i);
-                i = uint32_t(k);
+                i = p.row;
                 j -= 1;
                 op = kAll;
                 continue;
             }
+        }
+        if ((op & kDelete) && (op == kDelete || (rec & kRecFlagMD))) {
+            if (win_o != fnone) {
+                i = pred(win_o).row;
+                op = gap0 ? kAll : (kMatch | kInsert);
+                continue;
+            }
+            if (win_e != fnone) {
+                i = pred(win_e).row;
+                op = kDelete;
+                continue;
+            }
+        }
+        if ((op & kInsert) && j > 0 && inside(rg, j - 1) && (op == kInsert || (rec & kRecFlagMI))) {
+            // The record of column j - 1 says which term I at column j took.
+            const uint64_t left = rec_at(j - 1);
+            if (left & kRecFlagNO) {
+                put(j, kTbInsert);
+                j -= 1;
+                op = gap0 ? kAll : (kMatch | kDelete);
+                continue;
+            }
+            if (left & kRecFlagNE) {
+                put(j, kTbInsert);
+                j -= 1;
+                op = kInsert;
+                continue;
+            }
+        }
+        if ((op & kMatch) && j > 0 && win_m != fnone) {
+            put(j, i);
+            i = pred(win_m).row;
+            j -= 1;
+            op = kAll;
+            continue;
         }
         status = kTbNoPredecessor;
         break;
@@ -1180,28 +1322,19 @@ int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStr
     return i64 ? launch_fill_s<int64_t>(P, njobs, f, s) : launch_fill_s<int32_t>(P, njobs, f, s);
 }
 
-// Static LDS of the traceback kernel (window rows, predecessor lists, M/D/I
-// window, column records); the query codes and matrix come on top when LT.
-template <typename T> constexpr uint32_t tb_static_lds() {
-    constexpr uint32_t cols = sizeof(T) == 4 ? 32 : 16;
-    return uint32_t(sizeof(WRow) * kTbRows + sizeof(PredE) * kTbRows * kTbPred +
-                    3 * kTbRows * cols * sizeof(T) + 64 * sizeof(uint32_t));
-}
+// Static LDS of the traceback kernel (window rows, predecessor lists,
+// records, column results); the query codes come on top when LT.
+constexpr uint32_t kTbStaticLds =
+    uint32_t(sizeof(WRow) * kTbRows + sizeof(WPred) * kTbRows * kTbPred +
+             sizeof(uint32_t) * kTbRows * kTbCols + 64 * sizeof(uint32_t));
 
-void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s) {
+void launch_traceback(const DevPools &P, uint32_t njobs, bool, hipStream_t s) {
     if (njobs == 0) return;
     dim3 grid(njobs);
-    const uint32_t dyn = fill_dynamic_lds(0, P.seq_lds, P.mat_lds);
-    const uint32_t stat = i64 ? tb_static_lds<int64_t>() : tb_static_lds<int32_t>();
-    const bool lt = P.seq_lds != 0 && P.mat_lds != 0 && stat + dyn <= 65536u;
-    const uint32_t lds = lt ? dyn : 0;
-    if (i64) {
-        if (lt) hipLaunchKernelGGL((poa_traceback<int64_t, true>), grid, dim3(64), lds, s, P, njobs);
-        else hipLaunchKernelGGL((poa_traceback<int64_t, false>), grid, dim3(64), 0, s, P, njobs);
-    } else {
-        if (lt) hipLaunchKernelGGL((poa_traceback<int32_t, true>), grid, dim3(64), lds, s, P, njobs);
-        else hipLaunchKernelGGL((poa_traceback<int32_t, false>), grid, dim3(64), 0, s, P, njobs);
-    }
+    const uint32_t dyn = (P.seq_lds + 15u) & ~15u;
+    const bool lt = P.seq_lds != 0 && kTbStaticLds + dyn <= 65536u;
+    if (lt) hipLaunchKernelGGL((poa_traceback<true>), grid, dim3(64), dyn, s, P, njobs);
+    else hipLaunchKernelGGL((poa_traceback<false>), grid, dim3(64), 0, s, P, njobs);
 }
 
 // Device scratch of the no-allocation entry point: one fixed array per GPU,

@@ -18,7 +18,7 @@ struct DevPools {
     const PredE *preds;
     const uint8_t *seq;
     const int32_t *matrix;
-    void *arena;
+    void *arena;         // byte-addressed: traceback records and kept scores
     RowMeta *meta;
     RowRec *adapt;
     uint32_t *tb;
@@ -26,7 +26,7 @@ struct DevPools {
     // takes further chunks from *pool_next (element offsets) up to pool_end.
     unsigned long long *pool_next;
     uint64_t pool_end;
-    uint32_t chunk;      // elements per chunk (>= 3 x widest padded row)
+    uint64_t chunk;      // bytes per chunk (>= 3 widest padded rows with scores)
     uint32_t seq_lds;    // bytes of LDS for the query codes (0 = global)
     uint32_t mat_lds;    // bytes of LDS for the score matrix (0 = global)
     uint32_t ring_bytes; // LDS ring of recent rows (multiple of 64 bytes)

@@ -59,7 +59,8 @@ struct JobPlan {
     int64_t band = 0;
     int32_t o = 0, e = 0, score_nn = 0;
     uint64_t static_cells = 0;  // cells of the rows under the static/full band
-    uint64_t arena_est = 0;     // elements requested for the arena
+    uint64_t arena_est = 0;     // arena bytes requested (records, and scores of kept rows)
+    uint64_t arena_max = 0;     // arena bytes of every row at full width, scores kept
     uint32_t max_width = 0;     // widest static row (chooses columns per lane)
     uint32_t width_hint = 0;    // mean adaptive row width seen on this job's last DP
     bool full_arena = false;    // size the arena for full-width rows (after an overflow)
@@ -78,7 +79,7 @@ struct JobPlan {
         flags = unknown = 0;
         band = 0;
         o = e = score_nn = 0;
-        static_cells = arena_est = 0;
+        static_cells = arena_est = arena_max = 0;
         max_width = width_hint = 0;
         full_arena = false;
         rows.clear();
@@ -181,14 +182,14 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
     // distances (src/align.rs:623-650: heaviest edge, ties to larger
     // support) and the row's predecessor list. Both keep only in-range
     // predecessors with head != NEG, and a predecessor's head is final
-    // before its successors are visited. Each entry keeps its position in
-    // `incoming` for traceback ties; lists are sorted by row.
+    // before its successors are visited.
     thread_local std::vector<PathDist> dist;
     dist.assign(n, PathDist{INT64_MIN, INT64_MIN, 0});
     J.row_node.resize(n);
     for (uint64_t i = 0; i < n; i++) J.row_node[i] = uint32_t(g.order(beg + i));
-    J.rows.resize(n + 1);
+    J.rows.assign(n + 1, RowIn{0, 0, 0, 0});
     J.aux.resize(n);
+    uint32_t max_np = 0;
     dist[0].head = 0;
     for (uint64_t i = 0; i < n; i++) {
         const uint64_t id = J.row_node[i];
@@ -203,7 +204,7 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
             if (rk < beg || rk >= beg + i) return;
             uint64_t pr = rk - beg;
             if (dist[pr].head == INT64_MIN) return;
-            J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX)), k});
+            J.preds.push_back(PredE{uint32_t(pr), uint32_t(std::min<uint64_t>(w, UINT32_MAX)), k, 0});
             if (!have || w > bwt || (w == bwt && dist[pr].support > dist[bp].support)) {
                 have = true;
                 bp = pr;
@@ -214,15 +215,24 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
             dist[i].head = dist[bp].head + 1;
             dist[i].support = dist[bp].support + int64_t(bwt);
         }
+        // The list in `choose` order (src/align.rs:1201-1225): heaviest edge
+        // first, then the earliest in `incoming` (entries arrive in `incoming`
+        // order, so a stable sort on weight gives it). rperm walks the same
+        // entries in rank order, which the adaptive band fold needs.
         PredE *pe = J.preds.data();
-        for (uint32_t a = pbeg + 1; a < J.preds.size(); a++) {
-            PredE v = pe[a];
-            uint32_t b = a;
-            for (; b > pbeg && pe[b - 1].row > v.row; b--) pe[b] = pe[b - 1];
-            pe[b] = v;
+        const uint32_t pend = uint32_t(J.preds.size());
+        std::stable_sort(pe + pbeg, pe + pend, [](const PredE &x, const PredE &y) { return x.weight > y.weight; });
+        thread_local std::vector<uint32_t> by_row;
+        by_row.resize(pend - pbeg);
+        for (uint32_t a = 0; a < pend - pbeg; a++) by_row[a] = a;
+        std::sort(by_row.begin(), by_row.end(),
+                  [&](uint32_t x, uint32_t y) { return pe[pbeg + x].row < pe[pbeg + y].row; });
+        for (uint32_t a = pbeg; a < pend; a++) {
+            pe[a].rperm = by_row[a - pbeg];
+            J.rows[pe[a].row].last_use = uint32_t(i);
         }
+        max_np = std::max(max_np, pend - pbeg);
         J.rows[i].pred_beg = pbeg;
-        J.rows[i].pad_ = 0;
         J.aux[i].thresh = uint32_t(std::min<uint64_t>(tot / 10, UINT32_MAX));
         J.aux[i].base = g.base(id);
     }
@@ -259,11 +269,27 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
     }
     J.rows[n].pred_beg = uint32_t(J.preds.size());
     J.rows[n].head = J.rows[n].tail = kHeadNone;
-    J.rows[n].pad_ = 0;
+    J.rows[n].last_use = 0;
+    if (max_np > kMaxRecPreds) {
+        J.status = MPOA_ERR_UNSUPPORTED;
+        J.err = "a graph node has more predecessors than a device traceback record can name";
+        return;
+    }
 
-    // Arena: exact for static/full bands; adaptive rows start from the static
-    // width plus two band widths and grow to the full width on overflow.
-    uint64_t cells = 0, est = 0;
+    // Arena bytes: every computed row stores one traceback record per cell
+    // (rec_bytes of its predecessor count), rounded to 16 bytes; a row that a
+    // row other than the next one reads also stores its M and D scores.
+    // Exact for static/full bands; adaptive rows start from the static width
+    // plus two band widths, and rows wider than the LDS ring (which must keep
+    // their scores) take pool chunks; an overflow relaunches with arena_max.
+    const uint64_t tsz = J.use_i64 ? 8 : 4;
+    auto row_bytes = [&](uint32_t i, uint64_t w, bool all_scores) {
+        const uint32_t np = J.rows[i + 1].pred_beg - J.rows[i].pred_beg;
+        uint64_t b = (w * rec_bytes(np) + 15) / 16 * 16;
+        if (all_scores || J.rows[i].last_use > i + 1) b += 2 * w * tsz;
+        return b;
+    };
+    uint64_t cells = 0, est = 0, full = 0;
     uint32_t maxw = 0;
     for (uint32_t i = 0; i < n; i++) {
         if (J.rows[i].head == kHeadNone || J.rows[i].tail == kHeadNone) continue;
@@ -275,18 +301,19 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
             w = J.padded;
         cells += w;
         maxw = std::max<uint32_t>(maxw, uint32_t(w));
-        if (J.flags & kAdaptive)
-            est += std::min<uint64_t>(J.padded, w + 2 * uint64_t(J.band) + 16);
-        else
-            est += w;
+        const uint64_t we = (J.flags & kAdaptive) ? std::min<uint64_t>(J.padded, w + 2 * uint64_t(J.band) + 16) : w;
+        est += row_bytes(i, we, false);
+        full += row_bytes(i, J.padded, true);
     }
     J.static_cells = cells;
     J.max_width = maxw;
-    J.arena_est = 3 * est;
+    J.arena_est = est;
+    J.arena_max = full;
 }
 
-// Full-width arena bound: every row is at most `padded` wide.
-inline uint64_t arena_full(const JobPlan &J) { return 3ull * J.n * J.padded; }
+// Full-width arena bound in bytes: every row at most `padded` wide, with its
+// scores kept.
+inline uint64_t arena_full(const JobPlan &J) { return J.arena_max; }
 
 // Device traceback record -> AlignedBase list (src/align.rs:1293-1347): the
 // record holds, per column j = 1 .. m-1, the row of the diagonal step that

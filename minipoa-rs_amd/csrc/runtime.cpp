@@ -221,6 +221,10 @@ int device_status(const DevResult &r, std::string &msg) {
             return MPOA_ERR_TRACEBACK_NO_PREDECESSOR;
         }
         case kArenaOverflow: msg = "alignment arena overflow"; return MPOA_ERR_ALLOC;
+        case kTooManyPreds:
+            msg = "a graph node has more predecessors than a device traceback record can name";
+            return MPOA_ERR_UNSUPPORTED;
+        case kLostScores: msg = "device DP lost a predecessor row's scores"; return MPOA_ERR_DEVICE;
         default: msg = "device DP returned an unknown status"; return MPOA_ERR_DEVICE;
     }
 }
@@ -283,8 +287,9 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     struct Off {
         uint64_t rows, preds, seq, arena, cap, meta, adapt, tb;
     };
-    // MPOA_ARENA_SQUEEZE (tests): private arena regions of 64 elements and a
+    // MPOA_ARENA_SQUEEZE (tests): private arena regions of 256 bytes and a
     // one-chunk pool, so that DPs overflow and take the full-arena relaunch.
+    // Arena offsets and sizes are in bytes.
     const bool squeeze = getenv("MPOA_ARENA_SQUEEZE") != nullptr;
     std::vector<Off> off(N);
     uint64_t rows = 0, aux = 0, preds = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
@@ -297,9 +302,9 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         o.preds = preds;
         o.seq = seq;
         o.cap = align_up((full_arena || J.full_arena) ? arena_full(J)
-                         : squeeze                  ? 64
-                                                    : std::max<uint64_t>(J.arena_est, 64),
-                         64);
+                         : squeeze                  ? 256
+                                                    : std::max<uint64_t>(J.arena_est, 256),
+                         256);
         o.arena = arena;
         o.meta = meta;
         o.adapt = adapt;
@@ -308,16 +313,17 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         aux += J.n + 1;
         preds += J.preds.size();
         seq += J.padded;
-        arena += align_up(o.cap, 64);
+        arena += o.cap;
         meta += J.n;
         if (J.flags & kAdaptive) adapt += J.n;
         tbn += J.m;
         maxw = std::max(maxw, J.max_width);
         maxpad = std::max(maxpad, J.padded);
     }
-    // Arena chunk pool after the private regions (adaptive rows that outgrow
-    // their estimate take chunks of at least three full-width rows).
-    const uint64_t chunk = align_up(std::max<uint64_t>(65536, 3ull * maxpad), 64);
+    // Arena chunk pool after the private regions (rows that outgrow their
+    // estimate take chunks of at least three full-width rows, each with
+    // 8-byte records and its scores).
+    const uint64_t chunk = align_up(std::max<uint64_t>(262144, 3ull * maxpad * (8 + 2 * tsz)), 256);
     const uint64_t pool_beg = arena;
     uint64_t pool_len = squeeze ? chunk : std::max<uint64_t>(arena / 2, 4 * chunk);
     const size_t alpha = p.alpha();
@@ -338,7 +344,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     const uint64_t s_tb = align_up(s_adapt + std::max<uint64_t>(adapt, 1) * sizeof(RowRec), 256);
     const uint64_t s_arena = align_up(s_tb + std::max<uint64_t>(tbn, 1) * sizeof(uint32_t), 256);
     if (fx) {
-        const uint64_t need = s_arena + arena * tsz;
+        const uint64_t need = s_arena + arena;
         if (need > fx->bytes) {
             char buf[200];
             snprintf(buf, sizeof buf,
@@ -347,7 +353,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
             set_error(buf);
             return MPOA_ERR_SCRATCH_LIMIT;
         }
-        if (!squeeze) pool_len = (fx->bytes - s_arena) / tsz - arena;
+        if (!squeeze) pool_len = (fx->bytes - s_arena - arena) / 256 * 256;
     }
     arena += pool_len;
 
@@ -429,7 +435,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         HIP_TRY(grow(ctx.d_meta, std::max<uint64_t>(meta, 1) * sizeof(RowMeta)));
         HIP_TRY(grow(ctx.d_adapt, std::max<uint64_t>(adapt, 1) * sizeof(RowRec)));
         HIP_TRY(grow(ctx.d_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
-        HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 1) * tsz));
+        HIP_TRY(grow(ctx.d_arena, std::max<uint64_t>(arena, 256)));
         HIP_TRY(grow(ctx.h_res, N * sizeof(DevResult)));
         HIP_TRY(grow(ctx.h_tb, std::max<uint64_t>(tbn, 1) * sizeof(uint32_t)));
         li.hres = ctx.h_res.p;
@@ -441,7 +447,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         d_tb = ctx.d_tb.p;
         d_arena = ctx.d_arena.p;
     }
-    li.arena_bytes = std::max<uint64_t>(arena, 1) * tsz;
+    li.arena_bytes = std::max<uint64_t>(arena, 256);
     li.rows = meta;
 
     DevPools P;
@@ -458,7 +464,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     P.tb = static_cast<uint32_t *>(d_tb);
     P.pool_next = reinterpret_cast<unsigned long long *>(d + b_ctr);
     P.pool_end = pool_beg + pool_len;
-    P.chunk = uint32_t(chunk);
+    P.chunk = chunk;
     // Query codes up to 32 KB stage in LDS (the ring shrinks to fit below);
     // longer queries read the global-table kernel.
     P.seq_lds = maxpad <= 32768 ? maxpad : 0;
@@ -508,7 +514,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
     {
         // Stay within 64 KB of LDS per workgroup with the fill's static arrays.
-        const uint32_t static_lds = 9216, dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
+        const uint32_t static_lds = 10240, dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
         const uint32_t room = 65536u - static_lds - dyn_other;
         if (P.ring_bytes > room) P.ring_bytes = room & ~63u;
     }
@@ -544,9 +550,9 @@ bool launch_done(Ctx &ctx) {
     return true;
 }
 
-uint64_t job_arena_bytes(const JobPlan &J, bool i64) {
-    const uint64_t cap = align_up(J.full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 64), 64);
-    return (cap + cap / 2) * (i64 ? 8 : 4);
+uint64_t job_arena_bytes(const JobPlan &J, bool) {
+    const uint64_t cap = align_up(J.full_arena ? arena_full(J) : std::max<uint64_t>(J.arena_est, 256), 256);
+    return cap + cap / 2;
 }
 
 uint64_t device_free_bytes(int device) {
