@@ -111,6 +111,24 @@ const char *mpoa_last_error(void);
  * use one context per host thread. */
 int mpoa_ctx_create(int device, mpoa_ctx **out);
 void mpoa_ctx_destroy(mpoa_ctx *ctx);
+/* No counterpart in the reference (whose DP matrix lives in host memory):
+ * how a context's DPs lay out their traceback data in device memory.
+ * VALUES keeps M, D and I of every cell (12 bytes per i32 cell; the fill
+ * does the least work per cell, the fastest on the batched workloads
+ * measured); RECORDS keeps a 1-8 byte traceback record per cell with the
+ * `choose` winners already resolved, plus M and D of rows read beyond the
+ * next (about 1.5-2 bytes per cell, at half again the fill's integer work).
+ * AUTO (the default) takes VALUES, and RECORDS in mpoa_poa_noalloc, whose
+ * fixed scratch then holds several times larger DPs. Results are identical
+ * in every layout. The environment variable MPOA_LAYOUT (values, records)
+ * sets the default of contexts created afterwards. Fails while a batch or a
+ * server holds the context. */
+enum {
+    MPOA_LAYOUT_AUTO = 0,
+    MPOA_LAYOUT_VALUES = 1,
+    MPOA_LAYOUT_RECORDS = 2
+};
+int mpoa_ctx_set_layout(mpoa_ctx *ctx, int layout);
 /* Bytes of device memory the context currently holds. */
 uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx);
 

@@ -85,6 +85,7 @@ int poa_once(Ctx &ctx, const Params &p, const G &g, uint64_t beg, uint64_t end, 
     JobPlan J;
     plan_job(p, g, beg, end, q, qlen, adaptive, band_b, force_bits, J);
     if (J.status != MPOA_OK) return fail(J.status, J.err);
+    use_layout(J, fx != nullptr || ctx.layout == kLayoutRecords);
     std::vector<JobPlan *> plans{&J};
     std::vector<DevResult> res;
     std::vector<uint32_t> tb;
@@ -141,6 +142,7 @@ int run_windows(Ctx &ctx, const Params &p, const G &g, const uint8_t *q,
                 msg[k] = plan[k].err;
                 continue;
             }
+            use_layout(plan[k], ctx.layout == kLayoutRecords);
             launch.push_back(&plan[k]);
             who.push_back(k);
         }
@@ -247,6 +249,7 @@ int ensure_engine(mpoa_ctx *ctx, int slots) {
         Ctx *c = nullptr;
         int rc = ctx_create(ctx->c->device, &c, 2);  // 2 threads pack each launch
         if (rc != MPOA_OK) return rc;
+        c->layout = ctx->c->layout;
         ctx->lanes.emplace_back(c);
     }
     if (!ctx->tasks) ctx->tasks = std::make_unique<TaskQueue>(std::max(1u, default_threads() - 1));
@@ -309,9 +312,26 @@ int mpoa_ctx_create(int device, mpoa_ctx **out) try {
     Ctx *c = nullptr;
     int rc = ctx_create(device, &c);
     if (rc != MPOA_OK) return rc;
+    // MPOA_LAYOUT (values / records) sets the DP layout of every context the
+    // process creates, for side-by-side runs; mpoa_ctx_set_layout overrides it.
+    if (const char *lay = getenv("MPOA_LAYOUT")) {
+        if (!strcmp(lay, "values")) c->layout = kLayoutValues;
+        if (!strcmp(lay, "records")) c->layout = kLayoutRecords;
+    }
     auto *h = new mpoa_ctx();
     h->c = c;
     *out = h;
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+int mpoa_ctx_set_layout(mpoa_ctx *ctx, int layout) try {
+    if (!ctx) return fail(MPOA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (layout != MPOA_LAYOUT_AUTO && layout != MPOA_LAYOUT_VALUES && layout != MPOA_LAYOUT_RECORDS)
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown DP layout");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->engine_busy) return fail(MPOA_ERR_INVALID_ARGUMENT, "the context's launch slots are in use by a batch or a server");
+    ctx->c->layout = layout;
+    for (auto &c : ctx->lanes) c->layout = layout;
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 

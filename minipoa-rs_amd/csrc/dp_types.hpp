@@ -60,6 +60,7 @@ struct PredE {
 // b and the record's width follow the row's predecessor count np.
 constexpr uint32_t kRecFlagR = 1, kRecFlagMD = 2, kRecFlagMI = 4, kRecFlagNO = 8, kRecFlagNE = 16;
 constexpr uint32_t kMaxRecPreds = 65534;  // predecessors of one row a record can name
+constexpr uint32_t kMaxNarrowPreds = 511;  // at most this many: records of 1, 2 or 4 bytes
 // (constexpr: usable in host and device code)
 constexpr uint32_t rec_field_bits(uint32_t np) {
     return np <= 1 ? 1u : np <= 7 ? 3u : np <= 511 ? 9u : 16u;

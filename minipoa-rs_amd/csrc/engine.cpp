@@ -479,9 +479,11 @@ bool Engine::consume(Job &J, std::vector<Dp *> &again) {
             retries_++;
             if (J.win.empty()) {
                 plan_job(full_, GraphAcc{g}, 0, 1, q, qlen, false, p_.b, 0, D.plan);
+                use_layout(D.plan, slots_[0]->layout == kLayoutRecords);
             } else {
                 const SeedWindow &w = J.win[k];
                 plan_job(full_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, false, w.band, 0, D.plan);
+                use_layout(D.plan, slots_[0]->layout == kLayoutRecords);
             }
             if (D.plan.status == MPOA_OK) {
                 D.cost = D.plan.static_cells;
@@ -562,9 +564,11 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
         D.err.clear();
         if (J.win.empty()) {
             plan_job(p_, GraphAcc{g}, 0, 1, q, qlen, p_.adaptive, p_.b, 0, D.plan);
+            use_layout(D.plan, slots_[0]->layout == kLayoutRecords);
         } else {
             const SeedWindow &w = J.win[k];
             plan_job(p_, GraphAcc{g}, w.beg, w.end, q + w.qbeg, w.qend - w.qbeg, w.adaptive, w.band, 0, D.plan);
+            use_layout(D.plan, slots_[0]->layout == kLayoutRecords);
         }
     };
     // Windows plan independently (each writes its own plan only).
@@ -582,7 +586,9 @@ bool Engine::plan_step(Job &J, std::vector<Dp *> &launch) {
         // scaled to the grown graph.
         if (J.win.empty() && (P.flags & kAdaptive) && J.prev_n > 0) {
             uint64_t hint = J.prev_used * P.n / J.prev_n;
-            hint += hint / 8 + uint64_t(P.padded) * 8 * (P.use_i64 ? 18 : 10);
+            // Slack: eight full-width rows (records with scores, or M/D/I).
+            const uint64_t tsz = P.use_i64 ? 8 : 4;
+            hint += hint / 8 + uint64_t(P.padded) * 8 * (P.records ? 2 + 2 * tsz : 3 * tsz);
             P.arena_est = std::min(std::max(P.arena_est, hint), arena_full(P));
             P.width_hint = uint32_t(J.prev_cells / J.prev_n);
             D.cost = uint64_t(P.width_hint) * P.n;

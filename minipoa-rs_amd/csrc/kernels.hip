@@ -31,6 +31,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <type_traits>
 
 #include "dp_types.hpp"
 #include "kernels.hpp"
@@ -257,22 +258,38 @@ template <int W> __device__ __forceinline__ void sync_global() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Running maximum of one kind of candidate with its `choose` winner: the
-// predecessors are folded in list order, which is `choose` order
-// (src/align.rs:1201-1225: the heaviest edge, then the first in
-// `incoming`), so among tied candidates the first one stays.
-template <typename T>
-__device__ __forceinline__ void track(T &v, uint32_t &w, T c, uint32_t idx) {
-    w = c > v ? idx : w;
+// Running maximum of one kind of candidate with its `choose` winner, kept
+// as a field of the cell's packed record p. The predecessors are folded in
+// list order, which is `choose` order (src/align.rs:1201-1225: the heaviest
+// edge, then the first in `incoming`), so among tied candidates the first
+// one stays. `mask` is the field, `ins` the candidate's list position
+// shifted into it.
+template <typename T, typename R>
+__device__ __forceinline__ void track(T &v, R &p, T c, R mask, R ins) {
+    p = c > v ? ((p & ~mask) | ins) : p;
     v = tmax(v, c);
+}
+
+// 1 if a < b, from the sign of a - b: exact while |a - b| < 2^(bits-1),
+// which holds for every pair of DP values compared here (they lie within
+// the score guard's range around NEG). Plain VALU arithmetic: no compare
+// mask is held in scalar registers.
+template <typename T> __device__ __forceinline__ uint32_t lt01(T a, T b) {
+    using U = typename std::make_unsigned<T>::type;
+    return uint32_t((U(a) - U(b)) >> (8 * sizeof(T) - 1));
+}
+// All ones if a < b, else zero.
+template <typename R, typename T> __device__ __forceinline__ R ltmask(T a, T b) {
+    return R(0) - R(lt01(a, b));
 }
 
 // Stores a lane's C traceback records of rb (1, 2, 4 or 8) bytes each. Row
 // records start 16-byte aligned and a lane's run at a multiple of C records,
 // so every group of four records is one aligned vector store; `valid` is a
-// multiple of four (the 8-column band grid).
-template <int C>
-__device__ __forceinline__ void store_recs(uint8_t *dst, const uint64_t (&r)[C], uint32_t rb, uint32_t valid) {
+// multiple of four (the 8-column band grid). 8-byte records come only from
+// the wide-record kernel (R = uint64_t).
+template <int C, typename R>
+__device__ __forceinline__ void store_recs(uint8_t *dst, const R (&r)[C], uint32_t rb, uint32_t valid) {
 #pragma unroll
     for (int g = 0; g < C; g += 4) {
         if (uint32_t(g) >= valid) break;
@@ -283,7 +300,7 @@ __device__ __forceinline__ void store_recs(uint8_t *dst, const uint64_t (&r)[C],
         } else if (rb == 2) {
             *reinterpret_cast<uint2 *>(d) = make_uint2(uint32_t(r[g]) | uint32_t(r[g + 1]) << 16,
                                                        uint32_t(r[g + 2]) | uint32_t(r[g + 3]) << 16);
-        } else if (rb == 4) {
+        } else if (sizeof(R) == 4 || rb == 4) {
             *reinterpret_cast<uint4 *>(d) = make_uint4(uint32_t(r[g]), uint32_t(r[g + 1]),
                                                        uint32_t(r[g + 2]), uint32_t(r[g + 3]));
         } else {
@@ -302,27 +319,37 @@ __device__ __forceinline__ void store_recs(uint8_t *dst, const uint64_t (&r)[C],
 // LT: the score matrix and the query codes are staged in LDS (every launch
 // whose alphabet and queries fit); otherwise they are read from global memory.
 //
-// What a row leaves in HBM: one traceback record per cell (bit layout in
-// dp_types.hpp) -- for the cell's own M, D and I, which term the next
-// column's I takes, and the `choose` winners of the diagonal, the deletion
-// open and the deletion extension, tracked while the predecessors are folded
-// in -- and, only for a row that a later row other than the next one reads
-// (its last use, from the host) or that has no ring slot, its M and D
-// scores. The next row always finds its immediate predecessor in the ring:
-// a row whose placement would overwrite the previous row's slot gets no slot
-// (and keeps its scores in HBM if any row reads it).
+// What a row leaves in HBM depends on the launch's layout (REC):
+//  * values (REC false): M, D and I of every cell, [M len][D len][I len]
+//    (12 bytes per i32 cell); poa_traceback_values re-evaluates `choose`
+//    from them. The fill does the least work per cell.
+//  * records (REC true): one traceback record per cell (bit layout in
+//    dp_types.hpp) -- for the cell's own M, D and I, which term the next
+//    column's I takes, and the `choose` winners of the diagonal, the
+//    deletion open and the deletion extension, tracked while the
+//    predecessors are folded in -- and, only for a row that a later row
+//    other than the next one reads (its last use, from the host) or that
+//    has no ring slot, its M and D scores: 1-2 bytes per cell, at about
+//    half again the fill's VALU work. The next row always finds its
+//    immediate predecessor in the ring: a row whose placement would
+//    overwrite the previous row's slot gets no slot (and keeps its scores in
+//    HBM if any row reads it).
 //
-// amdgpu_waves_per_eu(4) for the i32 shapes with four columns per lane
-// (4x2, 4x4): four waves share each SIMD. Rows are latency chains, so
-// throughput comes from the number of resident jobs. One-wave shapes are
-// held to few jobs per CU by their LDS already, and wider lanes would spill
-// under the limit (on gfx9 a scratch reload waits for the wave's outstanding
-// stores); they keep the default.
-template <typename T, int C, int W, bool LT> constexpr int fill_waves_per_eu() {
-    return sizeof(T) == 4 && LT && C == 4 && W > 1 ? 4 : 1;
+// R is the register type of a packed record: uint32_t when every row has at
+// most kMaxNarrowPreds predecessors (records of 1, 2 or 4 bytes), uint64_t
+// for the wide-record kernel a launch takes when some row has more.
+//
+// amdgpu_waves_per_eu(4) for the i32 shapes that fit 128 VGPRs: four waves
+// share each SIMD. Rows are latency chains, so throughput comes from the
+// number of resident jobs. Values: 8x1, 4x2 and 4x4; records: 4x2 and 4x4
+// (8x1 would spill). Other kernels would spill in the row loop under the
+// limit (on gfx9 a scratch reload waits for the wave's outstanding stores)
+// and keep the default.
+template <typename T, int C, int W, bool LT, bool REC> constexpr int fill_waves_per_eu() {
+    return sizeof(T) == 4 && LT && (C == 4 || (!REC && C == 8 && W == 1)) ? 4 : 1;
 }
-template <typename T, int C, int W, bool LT>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT>()))) void poa_fill(DevPools P) {
+template <typename T, int C, int W, bool LT, typename R, bool REC>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT, REC>()))) void poa_fill(DevPools P) {
     static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
     constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
     extern __shared__ __align__(16) char dyn_lds[];
@@ -365,7 +392,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     const T d15 = T((lane & 15) + 1) * a, d31 = T((lane & 31) + 1) * a;
     constexpr T NEG = Lim<T>::neg, NI = Lim<T>::ninf;
     constexpr T HALF = NEG / 2;
-    constexpr uint32_t kNone = 0xFFFFFFFFu;
     const uint32_t alpha = J.alpha;
     constexpr bool mat_lds = LT, seq_lds = LT;
     if (tid < 16) negrow[tid] = NEG;
@@ -416,20 +442,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
             continue;
         }
         const uint32_t np = pe - pb;
-        if (np > kMaxRecPreds) {
+        if (REC && np > (sizeof(R) == 4 ? kMaxNarrowPreds : kMaxRecPreds)) {
             status = kTooManyPreds;
             break;
         }
+        // Record layout of the row: flags in bits 0-4, then the diagonal,
+        // deletion-open and deletion-extend winner fields of fb bits each.
         const uint32_t rb = rec_bytes(np), fb = rec_field_bits(np);
-        const uint64_t fnone = (uint64_t(1) << fb) - 1u;
-        const uint32_t ffn = uint32_t(fnone);
-        const bool one = np == 1;
+        const uint32_t ffn = (1u << fb) - 1u;  // "no winner"
+        const uint32_t so = 5 + fb, se = 5 + 2 * fb;
+        const R Mm = R(ffn) << 5, Mo = R(ffn) << so, Me = R(ffn) << se;
 
         // Gather the predecessors in parallel, lane q <-> predecessor q (in
         // groups of 64): row, geometry, ring slot, adaptive record, list
         // position and rank permutation. Rows older than the LDS history come
         // from global memory.
-        uint32_t g_row = 0, g_key = 0, g_rperm = 0;
+        uint32_t g_row = 0, g_rperm = 0;
         LMeta g_meta{0, 0, kNoSlot, 0, kNoValues, 0};
         RowRec g_rec{0, 0, 0, 0, 0};
         auto gather = [&](uint32_t g0) {
@@ -437,7 +465,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
             g_meta = LMeta{0, 0, kNoSlot, 0, kNoValues, 0};
             g_rec = RowRec{0, 0, 0, 0, 0};
             g_row = 0;
-            g_key = 0;
             g_rperm = 0;
             // The global fallbacks below wait for their loads where they are
             // issued: a load still counted at a control-flow join makes the
@@ -453,7 +480,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 }
                 g_row = pq.row;
-                g_key = q - pb;
                 g_rperm = pq.rperm;
             }
             const bool far = q < pe && i - g_row >= kHist;
@@ -560,11 +586,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                 ring_at += bytes;
             }
         }
-        // Scores stay in HBM for a row that a later row other than the next
-        // one reads, or that the next row cannot find in the ring.
-        const bool keep = ri.last_use > i + 1 || (ri.last_use > i && slot == kNoSlot);
-        const uint64_t rec_sz = (uint64_t(len) * rb + 15u) / 16u * 16u;
-        const uint64_t need = rec_sz + (keep ? uint64_t(bytes) : 0u);
+        // Records: scores stay in HBM for a row that a later row other than
+        // the next one reads, or that the next row cannot find in the ring.
+        // Values: [M][D][I] of every row.
+        const bool keep = !REC || ri.last_use > i + 1 || (ri.last_use > i && slot == kNoSlot);
+        const uint64_t rec_sz = REC ? (uint64_t(len) * rb + 15u) / 16u * 16u : 0u;
+        const uint64_t need = rec_sz + (keep ? uint64_t(bytes) : 0u) + (REC ? 0u : uint64_t(bytes) / 2);
         if (cur + need > rend) {
             // Own region exhausted: take a chunk from the launch's pool. The
             // leader's result reaches its wave by readlane and other waves
@@ -594,6 +621,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         uint8_t *Rr = arena + off;
         T *Mk = keep ? reinterpret_cast<T *>(arena + voff) : nullptr;
         T *Dk = keep ? Mk + len : nullptr;
+        T *Ik = REC ? nullptr : Mk + 2 * len;
         const uint32_t sm = (slot == kNoSlot ? 0u : slot) / uint32_t(sizeof(T));
         const uint32_t sd = sm + len;
         // The lane owning the row's last column writes its RowMeta and the
@@ -610,7 +638,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                 break;
             }
             if (leader) {
-                const uint64_t wnone = fnone << 5 | fnone << (5 + fb) | fnone << (5 + 2 * fb);
+                const uint32_t wnone = ffn << 5 | ffn << so | ffn << se;
                 T mprev = T(J.score_nn), iprev = NEG;
                 for (uint32_t j = 0; j < len; j++) {
                     T mj = mprev, ins = iprev;
@@ -618,14 +646,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                         ins = tmax(iprev + e, mprev + o);
                         mj = tmax(NEG, ins);
                     }
-                    const uint64_t f = uint64_t(mj > HALF) | uint64_t(mj == NEG) << 1 |
-                                       uint64_t(mj == ins) << 2 | uint64_t(mj + o >= ins + e) << 3 |
-                                       uint64_t(ins + e >= mj + o) << 4;
-                    Rr[j] = uint8_t(f | wnone);  // one predecessor or none: 1-byte records
+                    const uint32_t f = uint32_t(mj > HALF) | uint32_t(mj == NEG) << 1 |
+                                       uint32_t(mj == ins) << 2 | uint32_t(mj + o >= ins + e) << 3 |
+                                       uint32_t(ins + e >= mj + o) << 4;
+                    if constexpr (REC) Rr[j] = uint8_t(f | wnone);  // no predecessors: 1-byte records
                     if (keep) {
                         Mk[j] = mj;
                         Dk[j] = NEG;
                     }
+                    if constexpr (!REC) Ik[j] = ins;
                     if (slot != kNoSlot) {
                         ring[sm + j] = mj;
                         ring[sd + j] = NEG;
@@ -662,264 +691,296 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         uint32_t bestc = 0xFFFFFFFFu;
         uint32_t parity = 0;
         bool lost = false;  // a predecessor's scores were in neither ring nor HBM
-        for (uint32_t seg = start; seg < stop; seg += 64u * C * W, parity ^= 1u) {
-            const uint32_t c0 = seg + wave * 64u * C + uint32_t(lane) * C;
-            // A wave whose columns all lie past the row's end only joins the
-            // carry exchange: its issue slots go to other workgroups.
-            const bool live = seg + wave * 64u * C < stop;
-            // Per column: the diagonal maximum and the two deletion maxima
-            // with their `choose` winners (keys), then D, the profile score,
-            // md = max(diagonal + profile, D) and the in-lane insertion prefix.
-            T Mdg[C], Do[C], De[C], Dv[C], pf[C], md[C], Lk[C];
-            uint32_t wm[C], wo[C], we[C];
-            T S = NI, Sp = NI;
-            if (live) {
+        // The stripes of the row, compiled separately for rows with one
+        // predecessor (no tie rule: its fields hold position 0 until a gate
+        // clears them) and with several.
+        auto stripes = [&](auto one_c) {
+            constexpr bool one = decltype(one_c)::value;
+            for (uint32_t seg = start; seg < stop; seg += 64u * C * W, parity ^= 1u) {
+                const uint32_t c0 = seg + wave * 64u * C + uint32_t(lane) * C;
+                // A wave whose columns all lie past the row's end only joins the
+                // carry exchange: its issue slots go to other workgroups.
+                const bool live = seg + wave * 64u * C < stop;
+                // Per column: the diagonal maximum and the two deletion maxima,
+                // the packed record p (winner fields, then flags), D,
+                // md = max(diagonal + profile score, D) and the in-lane insertion
+                // prefix.
+                T Mdg[C], Do[C], De[C], Dv[C], md[C], Lk[C];
+                R pk[C];
+                T S = NI, Sp = NI;
+                const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
+                const uint32_t rel = c0 - start;
+                if (live) {
 #pragma unroll
-                for (int k = 0; k < C; k++) {
-                    Mdg[k] = NEG;
-                    Do[k] = NEG;
-                    De[k] = NEG;
-                    wm[k] = wo[k] = we[k] = kNone;
-                }
-                // Predecessor transitions on the band overlap (src/align.rs:751-797).
-                // Bands and lane runs sit on the same 8-column grid, so each
-                // piece of G columns is wholly inside or outside an overlap; the
-                // diagonal into the predecessor's first column reads NEG. Pieces
-                // outside an overlap fold the NEG row: such a candidate is far
-                // below any cell a path can reach, so the winner it may become
-                // is never read. A row with one predecessor needs no tie rule.
-                auto fold = [&](int sg, const T (&pm)[G], const T (&pd)[G], T dg0, uint32_t key) {
-#pragma unroll
-                    for (int k = 0; k < G; k++) {
-                        const T dg = k == 0 ? dg0 : pm[k - 1];
-                        if (one) {
-                            Mdg[sg + k] = dg;
-                            Do[sg + k] = pm[k] + o;
-                            De[sg + k] = pd[k] + e;
-                            wm[sg + k] = wo[sg + k] = we[sg + k] = key;
-                        } else {
-                            track(Mdg[sg + k], wm[sg + k], dg, key);
-                            track(Do[sg + k], wo[sg + k], T(pm[k] + o), key);
-                            track(De[sg + k], we[sg + k], T(pd[k] + e), key);
-                        }
+                    for (int k = 0; k < C; k++) {
+                        Mdg[k] = NEG;
+                        Do[k] = NEG;
+                        De[k] = NEG;
+                        // One predecessor: its fields hold position 0 until a gate
+                        // clears them.
+                        pk[k] = one ? R(0) : (Mm | Mo | Me);
                     }
-                };
-                if (!anymiss) {
-                    // Every predecessor row is in the LDS ring: this loop has no
-                    // global loads, so nothing in it waits on vmcnt (which on
-                    // gfx9 also counts the wave's outstanding stores).
-                    for (uint32_t q = 0; q < np; q++) {
-                        const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
-                        const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
-                        if (ovs >= ove) continue;
-                        if (seg == start) pcells += ove - ovs;
-                        const uint32_t pmo = rl32(g_meta.slot, q) / uint32_t(sizeof(T));
-                        const uint32_t key = rl32(g_key, q);
+                    // Predecessor transitions on the band overlap (src/align.rs:751-797).
+                    // Bands and lane runs sit on the same 8-column grid, so each
+                    // piece of G columns is wholly inside or outside an overlap; the
+                    // diagonal into the predecessor's first column reads NEG. Pieces
+                    // outside an overlap fold the NEG row: such a candidate is far
+                    // below any cell a path can reach, so the winner it may become
+                    // is never read. A row with one predecessor needs no tie rule.
+                    auto fold = [&](int sg, const T (&pm)[G], const T (&pd)[G], T dg0, uint32_t idx) {
+                        const R im = R(idx) << 5, io = R(idx) << so, ie = R(idx) << se;
 #pragma unroll
-                        for (int sg = 0; sg < C; sg += G) {
-                            const uint32_t cs = c0 + sg;
-                            const bool in = cs >= ovs && cs < ove;
-                            const uint32_t rel = cs - ps;
-                            const T *mp = in ? ring + pmo + rel : negrow;
-                            const T *dp = in ? ring + pmo + pl + rel : negrow;
-                            const T *gp0 = in && rel > 0 ? ring + pmo + rel - 1 : negrow;
-                            T pm[G], pd[G];
-                            loadg<T, G>(mp, pm);
-                            loadg<T, G>(dp, pd);
-                            fold(sg, pm, pd, *gp0, key);
+                        for (int k = 0; k < G; k++) {
+                            const T dg = k == 0 ? dg0 : pm[k - 1];
+                            if constexpr (one) {
+                                Mdg[sg + k] = dg;
+                                Do[sg + k] = pm[k] + o;
+                                De[sg + k] = pd[k] + e;
+                            } else if constexpr (!REC) {
+                                Mdg[sg + k] = tmax(Mdg[sg + k], dg);
+                                Do[sg + k] = tmax(Do[sg + k], T(pm[k] + o));
+                                De[sg + k] = tmax(De[sg + k], T(pd[k] + e));
+                            } else {
+                                track(Mdg[sg + k], pk[sg + k], dg, Mm, im);
+                                track(Do[sg + k], pk[sg + k], T(pm[k] + o), Mo, io);
+                                track(De[sg + k], pk[sg + k], T(pd[k] + e), Me, ie);
+                            }
                         }
-                    }
-                } else {
-                for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
-                    if (np > 64) gather(g0);
-                    const uint32_t cnt = tmin(64u, pe - g0);
-                    for (uint32_t q = 0; q < cnt; q++) {
-                        const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
-                        const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
-                        if (ovs >= ove) continue;
-                        if (seg == start) pcells += ove - ovs;
-                        const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
-                        const uint32_t key = rl32(g_key, q);
-                        const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
-                        if (in_ring) {
-                            const uint32_t pmo = pslot / uint32_t(sizeof(T));
+                    };
+                    if (!anymiss) {
+                        // Every predecessor row is in the LDS ring: this loop has no
+                        // global loads, so nothing in it waits on vmcnt (which on
+                        // gfx9 also counts the wave's outstanding stores).
+                        for (uint32_t q = 0; q < np; q++) {
+                            const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
+                            const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
+                            if (ovs >= ove) continue;
+                            if (seg == start) pcells += ove - ovs;
+                            const uint32_t pmo = rl32(g_meta.slot, q) / uint32_t(sizeof(T));
 #pragma unroll
                             for (int sg = 0; sg < C; sg += G) {
                                 const uint32_t cs = c0 + sg;
                                 const bool in = cs >= ovs && cs < ove;
-                                const uint32_t rel = cs - ps;
-                                const T *mp = in ? ring + pmo + rel : negrow;
-                                const T *dp = in ? ring + pmo + pl + rel : negrow;
-                                const T *gp0 = in && rel > 0 ? ring + pmo + rel - 1 : negrow;
+                                const uint32_t prel = cs - ps;
+                                const T *mp = in ? ring + pmo + prel : negrow;
+                                const T *dp = in ? ring + pmo + pl + prel : negrow;
+                                const T *gp0 = in && prel > 0 ? ring + pmo + prel - 1 : negrow;
                                 T pm[G], pd[G];
                                 loadg<T, G>(mp, pm);
                                 loadg<T, G>(dp, pd);
-                                fold(sg, pm, pd, *gp0, key);
+                                fold(sg, pm, pd, *gp0, q);
                             }
-                            continue;
                         }
-                        const uint64_t pvoff = uint64_t(rl64(int64_t(g_meta.voff), q));
-                        if (pvoff == kNoValues) {
-                            // Every row read by a row other than the next one
-                            // keeps its scores, and the next row always finds
-                            // its predecessor in the ring; refuse rather than
-                            // read outside the arena if that ever fails.
-                            lost = true;
-                            continue;
-                        }
-                        const gptr<const T> PM = glb(reinterpret_cast<const T *>(arena + pvoff));
-#pragma unroll
-                        for (int sg = 0; sg < C; sg += G) {
-                            const uint32_t cs = c0 + sg;
-                            if (cs < ovs || cs >= ove) continue;
-                            const uint32_t rel = cs - ps;
-                            T pm[G], pd[G], dg0;
-                            loadg<T, G>(PM + rel, pm);
-                            loadg<T, G>(PM + pl + rel, pd);
-                            dg0 = rel > 0 ? PM[rel - 1] : NEG;
-                            // Retire these loads inside the branch. Otherwise the
-                            // registers reach the join with loads in flight and
-                            // the ring path gets vmcnt waits, which also wait for
-                            // every store of the wave.
-                            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-                            fold(sg, pm, pd, dg0, key);
-                        }
-                    }
-                }
-                }
-                // D = max(open, extend); a deletion winner stands only where its
-                // kind gives D.
-#pragma unroll
-                for (int k = 0; k < C; k++) {
-                    Dv[k] = tmax(Do[k], De[k]);
-                    if (Do[k] < De[k]) wo[k] = kNone;
-                    if (De[k] < Do[k]) we[k] = kNone;
-                }
-                T L = NI;
-#pragma unroll
-                for (int sg = 0; sg < C; sg += G) {
-                    const uint32_t cs = c0 + sg;
-                    const bool piece_live = cs < stop;
-                    uint32_t codes[G];
-                    if constexpr (LT) {
-                        // Dead pieces read column 0's codes and are discarded.
-                        const uint32_t cr = piece_live ? cs : 0u;
-                        if constexpr (G == 8) {
-                            const uint2 w2 = *reinterpret_cast<const uint2 *>(sseq + cr);
-#pragma unroll
-                            for (int k = 0; k < 8; k++)
-                                codes[k] = ((k < 4 ? w2.x : w2.y) >> (8 * (k & 3))) & 0xFFu;
-                        } else {
-                            const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cr);
-#pragma unroll
-                            for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
-                        }
-#pragma unroll
-                        for (int k = 0; k < G; k++) pf[sg + k] = T(smat[srow + codes[k]]);
                     } else {
-                        if (piece_live) {
+                    for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
+                        if (np > 64) gather(g0);
+                        const uint32_t cnt = tmin(64u, pe - g0);
+                        for (uint32_t q = 0; q < cnt; q++) {
+                            const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
+                            const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
+                            if (ovs >= ove) continue;
+                            if (seg == start) pcells += ove - ovs;
+                            const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
+                            const uint32_t idx = g0 - pb + q;
+                            const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
+                            if (in_ring) {
+                                const uint32_t pmo = pslot / uint32_t(sizeof(T));
 #pragma unroll
-                            for (int k = 0; k < G; k++) codes[k] = glb(seq)[cs + k];
+                                for (int sg = 0; sg < C; sg += G) {
+                                    const uint32_t cs = c0 + sg;
+                                    const bool in = cs >= ovs && cs < ove;
+                                    const uint32_t prel = cs - ps;
+                                    const T *mp = in ? ring + pmo + prel : negrow;
+                                    const T *dp = in ? ring + pmo + pl + prel : negrow;
+                                    const T *gp0 = in && prel > 0 ? ring + pmo + prel - 1 : negrow;
+                                    T pm[G], pd[G];
+                                    loadg<T, G>(mp, pm);
+                                    loadg<T, G>(dp, pd);
+                                    fold(sg, pm, pd, *gp0, idx);
+                                }
+                                continue;
+                            }
+                            const uint64_t pvoff = uint64_t(rl64(int64_t(g_meta.voff), q));
+                            if (pvoff == kNoValues) {
+                                // Every row read by a row other than the next one
+                                // keeps its scores, and the next row always finds
+                                // its predecessor in the ring; refuse rather than
+                                // read outside the arena if that ever fails.
+                                lost = true;
+                                continue;
+                            }
+                            const gptr<const T> PM = glb(reinterpret_cast<const T *>(arena + pvoff));
+#pragma unroll
+                            for (int sg = 0; sg < C; sg += G) {
+                                const uint32_t cs = c0 + sg;
+                                if (cs < ovs || cs >= ove) continue;
+                                const uint32_t prel = cs - ps;
+                                T pm[G], pd[G], dg0;
+                                loadg<T, G>(PM + prel, pm);
+                                loadg<T, G>(PM + pl + prel, pd);
+                                dg0 = prel > 0 ? PM[prel - 1] : NEG;
+                                // Retire these loads inside the branch. Otherwise the
+                                // registers reach the join with loads in flight and
+                                // the ring path gets vmcnt waits, which also wait for
+                                // every store of the wave.
+                                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                                fold(sg, pm, pd, dg0, idx);
+                            }
+                        }
+                    }
+                    }
+                    // D = max(open, extend), final here: a deletion winner stands
+                    // only where its kind gives D. D goes out now.
+#pragma unroll
+                    for (int k = 0; k < C; k++) {
+                        Dv[k] = tmax(Do[k], De[k]);
+                        if constexpr (REC) pk[k] |= (ltmask<R>(Do[k], De[k]) & Mo) | (ltmask<R>(De[k], Do[k]) & Me);
+                    }
+                    if (valid > 0) {
+                        if (keep) store_run<T, C>(Dk + rel, Dv, valid);
+                        if (slot != kNoSlot) store_run<T, C>(ring + sd + rel, Dv, valid);
+                    }
+                    T L = NI;
+#pragma unroll
+                    for (int sg = 0; sg < C; sg += G) {
+                        const uint32_t cs = c0 + sg;
+                        const bool piece_live = cs < stop;
+                        uint32_t codes[G];
+                        T pf[G];
+                        if constexpr (LT) {
+                            // Dead pieces read column 0's codes and are discarded.
+                            const uint32_t cr = piece_live ? cs : 0u;
+                            if constexpr (G == 8) {
+                                const uint2 w2 = *reinterpret_cast<const uint2 *>(sseq + cr);
+#pragma unroll
+                                for (int k = 0; k < 8; k++)
+                                    codes[k] = ((k < 4 ? w2.x : w2.y) >> (8 * (k & 3))) & 0xFFu;
+                            } else {
+                                const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cr);
+#pragma unroll
+                                for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
+                            }
+#pragma unroll
+                            for (int k = 0; k < G; k++) pf[k] = T(smat[srow + codes[k]]);
+                        } else {
+                            if (piece_live) {
+#pragma unroll
+                                for (int k = 0; k < G; k++) codes[k] = glb(seq)[cs + k];
+                            }
+#pragma unroll
+                            for (int k = 0; k < G; k++) pf[k] = piece_live ? T(glb(gmat)[srow + codes[k]]) : T(0);
                         }
 #pragma unroll
-                        for (int k = 0; k < G; k++) pf[sg + k] = piece_live ? T(glb(gmat)[srow + codes[k]]) : T(0);
+                        for (int k = 0; k < G; k++) {
+                            const T dgp = Mdg[sg + k] + pf[k];
+                            const T v = piece_live ? tmax(dgp, Dv[sg + k]) : NI;
+                            // The diagonal winner stands only where the diagonal
+                            // gives M: here against D, after the scan against I.
+                            // M == D so far where D reaches md (md >= D).
+                            if constexpr (REC)
+                                pk[sg + k] |= (ltmask<R>(dgp, Dv[sg + k]) & Mm) | R(1u - lt01(Dv[sg + k], v)) << 1;
+                            md[sg + k] = v;
+                            Lk[sg + k] = L;
+                            L = tmax(L + e, v + o);
+                        }
                     }
+                    // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
+                    // enters wave w: c_0 = carry, c_{w+1} = max(c_w + 64a, U_w) with
+                    // U_w the wave's inclusive scan total (integer max-plus is
+                    // associative, so this is the serial recurrence exactly).
+                    S = maxplus_scan<T>(L, a, d15, d31);
+                    Sp = dpp<kWaveShr1, 0xF>(NI, S);
+                }
+                T cw = carry;
+                if constexpr (W > 1) {
+                    if (lane == 63) xcar[parity][wave] = S;
+                    sync_lds<W>();
 #pragma unroll
-                    for (int k = 0; k < G; k++) {
-                        const T dgp = Mdg[sg + k] + pf[sg + k];
-                        // The diagonal winner stands only where the diagonal
-                        // gives M: here against D, after the scan against I.
-                        if (dgp < Dv[sg + k]) wm[sg + k] = kNone;
-                        const T v = piece_live ? tmax(dgp, Dv[sg + k]) : NI;
-                        md[sg + k] = v;
-                        Lk[sg + k] = L;
-                        L = tmax(L + e, v + o);
+                    for (uint32_t w = 0; w + 1 < W; w++) {
+                        const T u = xcar[parity][w];
+                        if (w < wave) cw = tmax(cw + a64, u);
+                        carry = tmax(carry + a64, u);
                     }
+                    carry = tmax(carry + a64, xcar[parity][W - 1]);
+                } else {
+                    carry = tmax(carry + a64, lane63(S));
                 }
-                // Carry into each lane: X_l = max(c_w + l*a, S_{l-1}), where c_w
-                // enters wave w: c_0 = carry, c_{w+1} = max(c_w + 64a, U_w) with
-                // U_w the wave's inclusive scan total (integer max-plus is
-                // associative, so this is the serial recurrence exactly).
-                S = maxplus_scan<T>(L, a, d15, d31);
-                Sp = dpp<kWaveShr1, 0xF>(NI, S);
-            }
-            T cw = carry;
-            if constexpr (W > 1) {
-                if (lane == 63) xcar[parity][wave] = S;
-                sync_lds<W>();
-#pragma unroll
-                for (uint32_t w = 0; w + 1 < W; w++) {
-                    const T u = xcar[parity][w];
-                    if (w < wave) cw = tmax(cw + a64, u);
-                    carry = tmax(carry + a64, u);
-                }
-                carry = tmax(carry + a64, xcar[parity][W - 1]);
-            } else {
-                carry = tmax(carry + a64, lane63(S));
-            }
-            if (!live) continue;
-            const T X = tmax(cw + T(lane) * a, Sp);
-            const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
-            T Iv[C], Mv[C];
-#pragma unroll
-            for (int k = 0; k < C; k++) {
-                Iv[k] = tmax(X + T(k) * e, Lk[k]);
-                Mv[k] = tmax(md[k], Iv[k]);
-                if (uint32_t(k) < valid && Mv[k] > best) {
-                    best = Mv[k];
-                    bestc = c0 + k;
-                }
-                if (Iv[k] > md[k]) wm[k] = kNone;
-            }
-            // The diagonal `choose` also sees a predecessor whose band ends
-            // at the column before (its last column is j - 1, so the fill's
-            // overlap leaves j out): it names that predecessor where its last
-            // M plus the profile score equals M.
-            for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
-                if (np > 64) gather(g0);
-                const uint32_t cnt = tmin(64u, pe - g0);
-                for (uint32_t q = 0; q < cnt; q++) {
-                    const uint32_t pl = rl32(g_meta.len, q), bc = rl32(g_meta.start, q) + pl;
-                    if (pl == 0 || bc < start || bc >= stop) continue;
-                    const T ml = T(rl64(g_meta.mlast, q));
-                    const uint32_t key = rl32(g_key, q);
-#pragma unroll
-                    for (int k = 0; k < C; k++)
-                        if (c0 + k == bc && Mv[k] == ml + pf[k]) wm[k] = tmin(wm[k], key);
-                }
-            }
-            if (valid > 0) {
-                uint64_t rec[C];
+                if (!live) continue;
+                const T X = tmax(cw + T(lane) * a, Sp);
+                T Mv[C], Ivs[C];
 #pragma unroll
                 for (int k = 0; k < C; k++) {
-                    const uint32_t f = uint32_t(Mv[k] > HALF) | uint32_t(Mv[k] == Dv[k]) << 1 |
-                                       uint32_t(Mv[k] == Iv[k]) << 2 | uint32_t(Mv[k] + o >= Iv[k] + e) << 3 |
-                                       uint32_t(Iv[k] + e >= Mv[k] + o) << 4;
-                    rec[k] = uint64_t(f) | uint64_t(tmin(wm[k], ffn)) << 5 | uint64_t(tmin(wo[k], ffn)) << (5 + fb) |
-                             uint64_t(tmin(we[k], ffn)) << (5 + 2 * fb);
-                }
-                const uint32_t rel = c0 - start;
-                store_recs<C>(Rr + size_t(rel) * rb, rec, rb, valid);
-                if (keep) {
-                    store_run<T, C>(Mk + rel, Mv, valid);
-                    store_run<T, C>(Dk + rel, Dv, valid);
-                }
-                if (slot != kNoSlot) {
-                    store_run<T, C>(ring + sm + rel, Mv, valid);
-                    store_run<T, C>(ring + sd + rel, Dv, valid);
-                }
-                // The row's geometry with its last M (read by successors'
-                // diagonal choice), and the DP's final score.
-#pragma unroll
-                for (int k = 0; k < C; k++) {
-                    if (c0 + k == stop - 1) {
-                        hmeta[i % kHist].mlast = int64_t(Mv[k]);
-                        meta[i] = RowMeta{start, len, off, voff, int64_t(Mv[k])};
+                    const T Iv = tmax(X + T(k) * e, Lk[k]);
+                    Mv[k] = tmax(md[k], Iv);
+                    if (uint32_t(k) < valid && Mv[k] > best) {
+                        best = Mv[k];
+                        bestc = c0 + k;
                     }
-                    if (i == n - 1 && c0 + k == m - 1) xscore = int64_t(Mv[k]);
+                    if constexpr (!REC) {
+                        Ivs[k] = Iv;
+                        continue;
+                    }
+                    // Where I exceeds md, M comes from I: no diagonal winner, and
+                    // M is above D. Then the flags: reachable, M == I, and which
+                    // term I of the next column takes (I' = max(I + e, M + o)).
+                    const R over = ltmask<R>(md[k], Iv);
+                    const uint32_t f = lt01(HALF, Mv[k]) | (1u - lt01(Iv, md[k])) << 2 |
+                                       (1u - lt01(T(Mv[k] + o), T(Iv + e))) << 3 |
+                                       (1u - lt01(T(Iv + e), T(Mv[k] + o))) << 4;
+                    pk[k] = ((pk[k] | (over & Mm)) & ~(over & R(2))) | R(f);
+                }
+                // The diagonal `choose` also sees a predecessor whose band ends
+                // at the column before (its last column is j - 1, so the fill's
+                // overlap leaves j out): it names that predecessor where its last
+                // M plus the profile score equals M, if it comes first in list
+                // order.
+                for (uint32_t g0 = pb; REC && g0 < pe; g0 += 64) {
+                    if (np > 64) gather(g0);
+                    const uint32_t cnt = tmin(64u, pe - g0);
+                    for (uint32_t q = 0; q < cnt; q++) {
+                        const uint32_t pl = rl32(g_meta.len, q), bc = rl32(g_meta.start, q) + pl;
+                        if (pl == 0 || bc < start || bc >= stop) continue;
+                        const T ml = T(rl64(g_meta.mlast, q));
+                        const uint32_t idx = g0 - pb + q;
+                        if (bc >= c0 && bc < c0 + uint32_t(C)) {
+                            // Only the lane holding column bc: its profile score
+                            // again, rather than keeping C of them live.
+                            T pfb;
+                            if constexpr (LT) pfb = T(smat[srow + sseq[bc]]);
+                            else pfb = T(glb(gmat)[srow + glb(seq)[bc]]);
+#pragma unroll
+                            for (int k = 0; k < C; k++) {
+                                if (c0 + k == bc && Mv[k] == ml + pfb && idx < (uint32_t(pk[k] >> 5) & ffn))
+                                    pk[k] = (pk[k] & ~Mm) | (R(idx) << 5);
+                            }
+                        }
+                    }
+                }
+                if (valid > 0) {
+                    if constexpr (REC) {
+                        store_recs<C, R>(Rr + size_t(rel) * rb, pk, rb, valid);
+                    } else {
+                        store_run<T, C>(Ik + rel, Ivs, valid);
+                    }
+                    if (keep) store_run<T, C>(Mk + rel, Mv, valid);
+                    if (slot != kNoSlot) store_run<T, C>(ring + sm + rel, Mv, valid);
+                    // The row's geometry with its last M (read by successors'
+                    // diagonal choice), and the DP's final score.
+#pragma unroll
+                    for (int k = 0; k < C; k++) {
+                        if (c0 + k == stop - 1) {
+                            hmeta[i % kHist].mlast = int64_t(Mv[k]);
+                            meta[i] = RowMeta{start, len, off, voff, int64_t(Mv[k])};
+                        }
+                        if (i == n - 1 && c0 + k == m - 1) xscore = int64_t(Mv[k]);
+                    }
                 }
             }
-        }
+        };
+        if (np == 1)
+            stripes(std::true_type{});
+        else
+            stripes(std::false_type{});
         if (lost) {
             status = kLostScores;
             break;
@@ -1002,13 +1063,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     }
     sync_global<W>();  // every wave's stores and the score in LDS
     if (leader) {
-        DevResult R{};
-        R.status = status;
-        R.cells = cells;
-        R.pred_cells = pcells;
-        R.arena_used = used;
-        R.score = status == kOk ? xscore : int64_t(NEG);
-        P.res[blockIdx.x] = R;
+        DevResult res{};
+        res.status = status;
+        res.cells = cells;
+        res.pred_cells = pcells;
+        res.arena_used = used;
+        res.score = status == kOk ? xscore : int64_t(NEG);
+        P.res[blockIdx.x] = res;
     }
 }
 
@@ -1265,29 +1326,308 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     }
 }
 
-template <typename T, int C, int W, bool LT>
-static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
-    const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
-    hipLaunchKernelGGL((poa_fill<T, C, W, LT>), dim3(njobs), dim3(64 * W), lds, s, P);
+// Weighted deterministic traceback over the values layout (src/align.rs:
+// 1164-1313), one wave per job: `choose` is re-evaluated from the stored M
+// and D of the predecessors, and the insertion rules from I. The walk is
+// serial, so its cost is the latency of each step's reads; the wave stages a
+// window of the matrix in LDS -- kTbRows rows ending at the current row
+// (geometry, predecessor lists and M/D/I over kVCols columns ending at the
+// current column), loaded with all lanes and all loads in flight at once --
+// and walks inside it, reloading when the path nears the window's top-left
+// edge. Predecessors older than the window, or rows with more than kTbPred
+// predecessors, read global memory directly. Query codes, score matrix and
+// the per-column results use LDS as in poa_traceback.
+constexpr uint32_t kTbReach = 16;
+
+template <typename T, bool LT>
+__global__ __launch_bounds__(64) void poa_traceback_values(DevPools P, uint32_t njobs) {
+    constexpr uint32_t kVCols = sizeof(T) == 4 ? 32 : 16;
+    __shared__ WRow wrow[kTbRows];
+    __shared__ PredE wpred[kTbRows * kTbPred];
+    __shared__ T wm[kTbRows * kVCols], wd[kTbRows * kVCols], wi[kTbRows * kVCols];
+    __shared__ uint32_t tbw[64];  // records of the columns of the current 64-column chunk
+    extern __shared__ __align__(16) char tb_lds[];  // LT: query codes, then the matrix
+    const uint32_t jid = blockIdx.x;
+    if (jid >= njobs) return;
+    if (P.prio) __builtin_amdgcn_s_setprio(3);
+    DevResult *R = P.res + jid;
+    if (R->status != kOk) return;
+    const DevJob J = P.jobs[jid];
+    const int lane = threadIdx.x;
+    const uint32_t n = J.n, m = J.m;
+    const RowIn *rin = P.rows + J.rows;
+    const RowAux *aux = P.aux + J.rows;
+    const PredE *pr = P.preds + J.preds;
+    const uint8_t *seq = P.seq + J.seq;
+    const int32_t *mat = P.matrix + J.matrix;
+    const T *arena = reinterpret_cast<const T *>(P.arena);
+    const RowMeta *meta = P.meta + J.rowmeta;
+    uint32_t *tb = P.tb + J.tb;
+    const int64_t o = J.o, e = J.e;
+    const int64_t half = int64_t(Lim<T>::neg) / 2;
+    const bool gap0 = (J.flags & kGapOpenZero) != 0;
+    uint8_t *sseq = reinterpret_cast<uint8_t *>(tb_lds);
+    int32_t *smat = reinterpret_cast<int32_t *>(tb_lds + ((P.seq_lds + 15u) & ~15u));
+    if constexpr (LT) {
+        for (uint32_t k = uint32_t(lane); k < J.alpha * J.alpha; k += 64u) smat[k] = glb(mat)[k];
+        for (uint32_t k = uint32_t(lane); k < m; k += 64u) sseq[k] = glb(seq)[k];
+    }
+    auto qcode = [&](uint32_t c) -> uint32_t {
+        if constexpr (LT) return sseq[c];
+        else return glb(seq)[c];
+    };
+    auto score_of = [&](uint32_t a, uint32_t b) -> int64_t {
+        if constexpr (LT) return smat[a * J.alpha + b];
+        else return glb(mat)[size_t(a) * J.alpha + b];
+    };
+    // Columns are consumed in decreasing order, each exactly once, so a
+    // chunk is complete when the walk moves below it.
+    uint32_t chunk = (m - 1) / 64u;
+    auto flush = [&]() {
+        sync_lds<1>();
+        const uint32_t c = chunk * 64u + uint32_t(lane);
+        if (c >= 1 && c < m) tb[c] = tbw[lane];
+        sync_lds<1>();
+    };
+    auto put = [&](uint32_t col, uint32_t v) {
+        if (col / 64u != chunk) {
+            flush();
+            chunk = col / 64u;
+        }
+        if (lane == 0) tbw[col & 63u] = v;
+    };
+
+    uint32_t i0 = 0, j0 = 0, cw0 = 0;  // window: rows (i0-kTbRows, i0], columns [cw0, j0]
+    bool have = false;
+    auto load_window = [&](uint32_t ai, uint32_t aj) {
+        sync_lds<1>();
+        i0 = ai;
+        j0 = aj;
+        cw0 = aj + 1 > kVCols ? aj + 1 - kVCols : 0;
+        have = true;
+        // Geometry and predecessor lists: lane r holds row i0 - r. Lanes past
+        // row 0 load row 0 again and store nothing, so no load sits behind a
+        // branch (a load in flight at a join costs a wait on every path).
+        {
+            const uint32_t r = uint32_t(lane);
+            const uint32_t ri = r <= i0 ? i0 - r : 0u;
+            const RowMeta rm = ldg(meta + ri);
+            const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
+            const RowAux x = ldg(aux + ri);
+            const uint32_t np = tmin(b.pred_beg - a.pred_beg, kTbPred);
+            PredE pe[kTbPred];
+#pragma unroll
+            for (uint32_t q = 0; q < kTbPred; q++) pe[q] = ldg(np ? pr + a.pred_beg + (q < np ? q : 0u) : P.preds);
+            if (r <= i0) {
+                wrow[r] = WRow{rm.start, rm.len, rm.voff / sizeof(T), a.pred_beg, b.pred_beg - a.pred_beg, x.thresh, x.base};
+#pragma unroll
+                for (uint32_t q = 0; q < kTbPred; q++)
+                    if (q < np) wpred[r * kTbPred + q] = pe[q];
+            }
+        }
+        sync_lds<1>();
+        // M, D and I: lane k holds column cw0 + k; 16 rows per batch, every
+        // load of a batch in flight before the first is used.
+        const uint32_t c = cw0 + uint32_t(lane);
+        const bool col_ok = uint32_t(lane) < kVCols && c <= j0;
+        const gptr<const T> A = glb(static_cast<const T *>(arena));
+        constexpr uint32_t kB = 16;
+        for (uint32_t r0 = 0; r0 < kTbRows && r0 <= i0; r0 += kB) {
+            T vm[kB], vd[kB], vi[kB];
+            bool ok[kB];
+#pragma unroll
+            for (uint32_t u = 0; u < kB; u++) {
+                const uint32_t r = r0 + u;
+                const WRow g = wrow[r <= i0 ? r : 0u];
+                ok[u] = col_ok && r <= i0 && c >= g.start && c - g.start < g.len;
+                const uint64_t k = ok[u] ? g.off + (c - g.start) : 0ull;
+                const uint64_t l = ok[u] ? uint64_t(g.len) : 0ull;
+                vm[u] = A[k];
+                vd[u] = A[k + l];
+                vi[u] = A[k + 2ull * l];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kB; u++) {
+                if (ok[u]) {
+                    const uint32_t w = (r0 + u) * kVCols + uint32_t(lane);
+                    wm[w] = vm[u];
+                    wd[w] = vd[u];
+                    wi[w] = vi[u];
+                }
+            }
+        }
+        sync_lds<1>();
+    };
+    auto row_in_win = [&](uint32_t ri) { return have && ri <= i0 && i0 - ri < kTbRows; };
+    // Geometry of a row: window copy, else global.
+    auto geo = [&](uint32_t ri) -> WRow {
+        if (row_in_win(ri)) return wrow[i0 - ri];
+        const RowMeta rm = ldg(meta + ri);
+        const RowIn a = ldg(rin + ri), b = ldg(rin + ri + 1);
+        return WRow{rm.start, rm.len, rm.voff / sizeof(T), a.pred_beg, b.pred_beg - a.pred_beg, 0, 0};
+    };
+    auto inside = [](const WRow &r, uint32_t c) { return c >= r.start && c - r.start < r.len; };
+    // State s (0 = M, 1 = D, 2 = I) of row ri at an in-band column c.
+    auto val = [&](uint32_t ri, const WRow &g, uint32_t c, int s) -> int64_t {
+        if (row_in_win(ri) && c >= cw0 && c <= j0) {
+            const uint32_t k = (i0 - ri) * kVCols + (c - cw0);
+            if (s == 0) return int64_t(wm[k]);
+            if (s == 1) return int64_t(wd[k]);
+            return int64_t(wi[k]);
+        }
+        return int64_t(glb(arena)[g.off + uint64_t(s) * g.len + (c - g.start)]);
+    };
+    auto pred = [&](uint32_t ri, const WRow &g, uint32_t q) -> PredE {
+        if (row_in_win(ri) && g.np <= kTbPred) return wpred[(i0 - ri) * kTbPred + q];
+        return ldg(pr + g.pb + q);
+    };
+
+    uint32_t i = n - 1, j = m - 1, op = kAll;
+    uint64_t steps = 0;
+    int32_t status = kOk;
+    while (i > 0 || j > 0) {
+        if (++steps > uint64_t(n) + m) {
+            status = kTbNoAdvance;
+            break;
+        }
+        if (!have || i0 - i + kTbReach > kTbRows || (j > 0 && j - 1 < cw0)) load_window(i, j);
+        const WRow rg = geo(i);
+        if (!inside(rg, j)) {
+            status = kTbLeftBand;
+            break;
+        }
+        const int64_t Mij = val(i, rg, j, 0);
+        if (Mij <= half) {
+            status = kTbLeftBand;
+            break;
+        }
+        const int64_t Dij = val(i, rg, j, 1);
+        const uint32_t base = (i == n - 1) ? J.unknown : rg.base;
+        const uint32_t qj = qcode(j);
+        const int64_t score = score_of(base, qj);
+        // `choose` (src/align.rs:1201-1225): among in-range predecessors whose
+        // state plus penalty equals the target, the largest in_weight; ties go
+        // to the earliest in `incoming` order. Returns the predecessor row.
+        auto choose = [&](uint32_t state, int64_t penalty, uint32_t pj, uint32_t &w_out) -> int64_t {
+            const int64_t target = state == kMatch ? Mij : Dij;
+            int64_t best = -1;
+            uint32_t w = 0, bord = 0;
+            for (uint32_t q = 0; q < rg.np; q++) {
+                const PredE pq = pred(i, rg, q);
+                const WRow pg = geo(pq.row);
+                if (!inside(pg, pj)) continue;
+                const int64_t src = val(pq.row, pg, pj, state == kDelete ? 1 : 0);
+                if (target == src + penalty &&
+                    (best < 0 || pq.weight > w || (pq.weight == w && pq.ord < bord))) {
+                    best = pq.row;
+                    w = pq.weight;
+                    bord = pq.ord;
+                }
+            }
+            w_out = w;
+            return best;
+        };
+        uint32_t w = 0;
+        int64_t pref = -1;
+        if ((op & kMatch) && j > 0 && base == qj) pref = choose(kMatch, score, j - 1, w);
+        if (pref >= 0 && w < rg.thresh) pref = -1;
+        if (pref >= 0) {
+            put(j, i);
+            i = uint32_t(pref);
+            j -= 1;
+            op = kAll;
+            continue;
+        }
+        if ((op & kDelete) && (op == kDelete || Mij == Dij)) {
+            int64_t k = choose(kInsert, o, j, w);
+            if (k >= 0) {
+                i = uint32_t(k);
+                op = gap0 ? kAll : (kMatch | kInsert);
+                continue;
+            }
+            k = choose(kDelete, e, j, w);
+            if (k >= 0) {
+                i = uint32_t(k);
+                op = kDelete;
+                continue;
+            }
+        }
+        if ((op & kInsert) && j > 0 && inside(rg, j - 1)) {
+            const int64_t Iij = val(i, rg, j, 2);
+            if (op == kInsert || Mij == Iij) {
+                if (Iij == val(i, rg, j - 1, 0) + o) {
+                    put(j, kTbInsert);
+                    j -= 1;
+                    op = gap0 ? kAll : (kMatch | kDelete);
+                    continue;
+                }
+                if (Iij == val(i, rg, j - 1, 2) + e) {
+                    put(j, kTbInsert);
+                    j -= 1;
+                    op = kInsert;
+                    continue;
+                }
+            }
+        }
+        if ((op & kMatch) && j > 0) {
+            const int64_t k = choose(kMatch, score, j - 1, w);
+            if (k >= 0) {
+                put(j, i);
+                i = uint32_t(k);
+                j -= 1;
+                op = kAll;
+                continue;
+            }
+        }
+        status = kTbNoPredecessor;
+        break;
+    }
+    if (status == kOk) {
+        flush();
+    } else if (lane == 0) {
+        R->status = status;
+        R->err_i = i;
+        R->err_j = j;
+    }
 }
 
-template <typename T>
-static int launch_fill_s(const DevPools &P, uint32_t njobs, FillShape f, hipStream_t s) {
-    if (P.seq_lds == 0 || P.mat_lds == 0) {
-        // Tables in global memory: one shape only.
+template <typename T, int C, int W, bool LT, bool REC, typename R = uint32_t>
+static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
+    const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
+    hipLaunchKernelGGL((poa_fill<T, C, W, LT, R, REC>), dim3(njobs), dim3(64 * W), lds, s, P);
+}
+
+template <typename T, bool REC>
+static int launch_fill_l(const DevPools &P, uint32_t njobs, bool wide, FillShape f, hipStream_t s) {
+    const bool lt = P.seq_lds != 0 && P.mat_lds != 0;
+    if (!lt || wide) {
+        // Tables in global memory, or rows with more than kMaxNarrowPreds
+        // predecessors in the records layout (8-byte records): one shape.
         if (f.cols != 8 || f.waves != 1) return -1;
-        launch_fill_t<T, 8, 1, false>(P, njobs, s);
+        if constexpr (REC) {
+            if (wide && lt) launch_fill_t<T, 8, 1, true, true, uint64_t>(P, njobs, s);
+            else if (wide) launch_fill_t<T, 8, 1, false, true, uint64_t>(P, njobs, s);
+            else launch_fill_t<T, 8, 1, false, true>(P, njobs, s);
+        } else {
+            if (wide) return -1;
+            launch_fill_t<T, 8, 1, false, false>(P, njobs, s);
+        }
         return 0;
     }
     switch (f.cols * 8 + f.waves) {
-        case 8 * 8 + 1: launch_fill_t<T, 8, 1, true>(P, njobs, s); return 0;
-        case 16 * 8 + 1: launch_fill_t<T, 16, 1, true>(P, njobs, s); return 0;
-        case 4 * 8 + 2: launch_fill_t<T, 4, 2, true>(P, njobs, s); return 0;
-        case 8 * 8 + 2: launch_fill_t<T, 8, 2, true>(P, njobs, s); return 0;
-        case 4 * 8 + 4: launch_fill_t<T, 4, 4, true>(P, njobs, s); return 0;
-        case 8 * 8 + 4: launch_fill_t<T, 8, 4, true>(P, njobs, s); return 0;
+        case 8 * 8 + 1: launch_fill_t<T, 8, 1, true, REC>(P, njobs, s); return 0;
+        case 16 * 8 + 1: launch_fill_t<T, 16, 1, true, REC>(P, njobs, s); return 0;
+        case 4 * 8 + 2: launch_fill_t<T, 4, 2, true, REC>(P, njobs, s); return 0;
+        case 8 * 8 + 2: launch_fill_t<T, 8, 2, true, REC>(P, njobs, s); return 0;
+        case 4 * 8 + 4: launch_fill_t<T, 4, 4, true, REC>(P, njobs, s); return 0;
+        case 8 * 8 + 4: launch_fill_t<T, 8, 4, true, REC>(P, njobs, s); return 0;
         default: return -1;
     }
+}
+
+template <typename T>
+static int launch_fill_s(const DevPools &P, uint32_t njobs, bool rec, bool wide, FillShape f, hipStream_t s) {
+    return rec ? launch_fill_l<T, true>(P, njobs, wide, f, s) : launch_fill_l<T, false>(P, njobs, wide, f, s);
 }
 
 bool fill_shape_ok(FillShape f) {
@@ -1317,24 +1657,46 @@ FillShape fill_shape_for_width(uint32_t width) {
     return FillShape{8, 4};
 }
 
-int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStream_t s) {
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, bool rec, bool wide, FillShape f, hipStream_t s) {
     if (njobs == 0) return 0;
-    return i64 ? launch_fill_s<int64_t>(P, njobs, f, s) : launch_fill_s<int32_t>(P, njobs, f, s);
+    return i64 ? launch_fill_s<int64_t>(P, njobs, rec, wide, f, s)
+               : launch_fill_s<int32_t>(P, njobs, rec, wide, f, s);
 }
 
-// Static LDS of the traceback kernel (window rows, predecessor lists,
-// records, column results); the query codes come on top when LT.
+// Static LDS of the records traceback (window rows, predecessor lists,
+// records, column results) and of the values traceback (window rows,
+// predecessor lists, M/D/I window, column results); the query codes (and
+// for values the score matrix) come on top when LT.
 constexpr uint32_t kTbStaticLds =
     uint32_t(sizeof(WRow) * kTbRows + sizeof(WPred) * kTbRows * kTbPred +
              sizeof(uint32_t) * kTbRows * kTbCols + 64 * sizeof(uint32_t));
+template <typename T> constexpr uint32_t tb_values_static_lds() {
+    constexpr uint32_t cols = sizeof(T) == 4 ? 32 : 16;
+    return uint32_t(sizeof(WRow) * kTbRows + sizeof(PredE) * kTbRows * kTbPred +
+                    3 * kTbRows * cols * sizeof(T) + 64 * sizeof(uint32_t));
+}
 
-void launch_traceback(const DevPools &P, uint32_t njobs, bool, hipStream_t s) {
+void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hipStream_t s) {
     if (njobs == 0) return;
     dim3 grid(njobs);
-    const uint32_t dyn = (P.seq_lds + 15u) & ~15u;
-    const bool lt = P.seq_lds != 0 && kTbStaticLds + dyn <= 65536u;
-    if (lt) hipLaunchKernelGGL((poa_traceback<true>), grid, dim3(64), dyn, s, P, njobs);
-    else hipLaunchKernelGGL((poa_traceback<false>), grid, dim3(64), 0, s, P, njobs);
+    if (rec) {
+        const uint32_t dyn = (P.seq_lds + 15u) & ~15u;
+        const bool lt = P.seq_lds != 0 && kTbStaticLds + dyn <= 65536u;
+        if (lt) hipLaunchKernelGGL((poa_traceback<true>), grid, dim3(64), dyn, s, P, njobs);
+        else hipLaunchKernelGGL((poa_traceback<false>), grid, dim3(64), 0, s, P, njobs);
+        return;
+    }
+    const uint32_t dyn = fill_dynamic_lds(0, P.seq_lds, P.mat_lds);
+    const uint32_t stat = i64 ? tb_values_static_lds<int64_t>() : tb_values_static_lds<int32_t>();
+    const bool lt = P.seq_lds != 0 && P.mat_lds != 0 && stat + dyn <= 65536u;
+    const uint32_t lds = lt ? dyn : 0;
+    if (i64) {
+        if (lt) hipLaunchKernelGGL((poa_traceback_values<int64_t, true>), grid, dim3(64), lds, s, P, njobs);
+        else hipLaunchKernelGGL((poa_traceback_values<int64_t, false>), grid, dim3(64), 0, s, P, njobs);
+    } else {
+        if (lt) hipLaunchKernelGGL((poa_traceback_values<int32_t, true>), grid, dim3(64), lds, s, P, njobs);
+        else hipLaunchKernelGGL((poa_traceback_values<int32_t, false>), grid, dim3(64), 0, s, P, njobs);
+    }
 }
 
 // Device scratch of the no-allocation entry point: one fixed array per GPU,

@@ -49,9 +49,12 @@ bool fill_shape_ok(FillShape f);
 FillShape fill_shape_for_width(uint32_t width);
 // Default ring bytes for rows of a given width (about three rows, 16-48 KB).
 uint32_t ring_bytes_for_width(uint32_t width, bool i64);
-// Returns -1 for a shape without a compiled kernel.
-int launch_fill(const DevPools &P, uint32_t njobs, bool i64, FillShape f, hipStream_t s);
-void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, hipStream_t s);
+// wide: some row of the launch has more than kMaxNarrowPreds predecessors
+// (8-byte records; shape (8,1) only). Returns -1 for a shape without a
+// compiled kernel.
+// rec: the records layout, else values.
+int launch_fill(const DevPools &P, uint32_t njobs, bool i64, bool rec, bool wide, FillShape f, hipStream_t s);
+void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hipStream_t s);
 
 // Fixed device scratch of the no-allocation entry point (64 MiB per GPU,
 // a static device array of kernels.hip). scratch_init looks up its address

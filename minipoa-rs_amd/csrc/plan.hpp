@@ -59,9 +59,16 @@ struct JobPlan {
     int64_t band = 0;
     int32_t o = 0, e = 0, score_nn = 0;
     uint64_t static_cells = 0;  // cells of the rows under the static/full band
-    uint64_t arena_est = 0;     // arena bytes requested (records, and scores of kept rows)
-    uint64_t arena_max = 0;     // arena bytes of every row at full width, scores kept
+    // Arena bytes of the DP in each layout: values ([M][D][I] of every
+    // row) and records (records of every row, M and D of rows read beyond
+    // the next); est as planned, max with every row at full width and its
+    // scores kept. use_layout() picks one into arena_est / arena_max.
+    uint64_t est_val = 0, max_val = 0, est_rec = 0, max_rec = 0;
+    bool records = false;       // the layout the arena is sized for
+    uint64_t arena_est = 0;     // arena bytes requested
+    uint64_t arena_max = 0;     // arena bytes of every row at full width
     uint32_t max_width = 0;     // widest static row (chooses columns per lane)
+    uint32_t max_np = 0;        // most predecessors of one row (record width)
     uint32_t width_hint = 0;    // mean adaptive row width seen on this job's last DP
     bool full_arena = false;    // size the arena for full-width rows (after an overflow)
     std::vector<RowIn> rows;    // n + 1
@@ -80,7 +87,9 @@ struct JobPlan {
         band = 0;
         o = e = score_nn = 0;
         static_cells = arena_est = arena_max = 0;
-        max_width = width_hint = 0;
+        est_val = max_val = est_rec = max_rec = 0;
+        records = false;
+        max_width = width_hint = max_np = 0;
         full_arena = false;
         rows.clear();
         aux.clear();
@@ -94,6 +103,17 @@ struct JobPlan {
 struct PathDist {
     int64_t head, tail, support;
 };
+
+// Sizes the DP's arena for a layout: records (rec) or values. Records
+// name at most kMaxRecPreds predecessors of a row; past that the DP keeps
+// values. Returns the layout taken.
+inline bool use_layout(JobPlan &J, bool rec) {
+    rec = rec && J.max_np <= kMaxRecPreds;
+    J.records = rec;
+    J.arena_est = rec ? J.est_rec : J.est_val;
+    J.arena_max = rec ? J.max_rec : J.max_val;
+    return rec;
+}
 
 constexpr uint64_t kWidth = 8;
 constexpr uint64_t kScoreLimit = uint64_t(1) << 26;  // src/align.rs:217
@@ -220,18 +240,32 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
         // order, so a stable sort on weight gives it). rperm walks the same
         // entries in rank order, which the adaptive band fold needs.
         PredE *pe = J.preds.data();
-        const uint32_t pend = uint32_t(J.preds.size());
-        std::stable_sort(pe + pbeg, pe + pend, [](const PredE &x, const PredE &y) { return x.weight > y.weight; });
-        thread_local std::vector<uint32_t> by_row;
-        by_row.resize(pend - pbeg);
-        for (uint32_t a = 0; a < pend - pbeg; a++) by_row[a] = a;
-        std::sort(by_row.begin(), by_row.end(),
-                  [&](uint32_t x, uint32_t y) { return pe[pbeg + x].row < pe[pbeg + y].row; });
-        for (uint32_t a = pbeg; a < pend; a++) {
-            pe[a].rperm = by_row[a - pbeg];
-            J.rows[pe[a].row].last_use = uint32_t(i);
+        const uint32_t pend = uint32_t(J.preds.size()), np = pend - pbeg;
+        if (np <= 16) {
+            // Short lists (nearly all): insertion sort, which is stable, and
+            // rank positions by counting.
+            for (uint32_t a = pbeg + 1; a < pend; a++) {
+                const PredE v = pe[a];
+                uint32_t b = a;
+                for (; b > pbeg && pe[b - 1].weight < v.weight; b--) pe[b] = pe[b - 1];
+                pe[b] = v;
+            }
+            for (uint32_t a = pbeg; a < pend; a++) {
+                uint32_t t = 0;
+                for (uint32_t b = pbeg; b < pend; b++) t += pe[b].row < pe[a].row;
+                pe[pbeg + t].rperm = a - pbeg;
+            }
+        } else {
+            std::stable_sort(pe + pbeg, pe + pend, [](const PredE &x, const PredE &y) { return x.weight > y.weight; });
+            thread_local std::vector<uint32_t> by_row;
+            by_row.resize(np);
+            for (uint32_t a = 0; a < np; a++) by_row[a] = a;
+            std::sort(by_row.begin(), by_row.end(),
+                      [&](uint32_t x, uint32_t y) { return pe[pbeg + x].row < pe[pbeg + y].row; });
+            for (uint32_t a = 0; a < np; a++) pe[pbeg + a].rperm = by_row[a];
         }
-        max_np = std::max(max_np, pend - pbeg);
+        for (uint32_t a = pbeg; a < pend; a++) J.rows[pe[a].row].last_use = uint32_t(i);
+        max_np = std::max(max_np, np);
         J.rows[i].pred_beg = pbeg;
         J.aux[i].thresh = uint32_t(std::min<uint64_t>(tot / 10, UINT32_MAX));
         J.aux[i].base = g.base(id);
@@ -270,49 +304,54 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
     J.rows[n].pred_beg = uint32_t(J.preds.size());
     J.rows[n].head = J.rows[n].tail = kHeadNone;
     J.rows[n].last_use = 0;
-    if (max_np > kMaxRecPreds) {
-        J.status = MPOA_ERR_UNSUPPORTED;
-        J.err = "a graph node has more predecessors than a device traceback record can name";
-        return;
-    }
+    J.max_np = max_np;
 
-    // Arena bytes: every computed row stores one traceback record per cell
-    // (rec_bytes of its predecessor count), rounded to 16 bytes; a row that a
-    // row other than the next one reads also stores its M and D scores.
-    // Exact for static/full bands; adaptive rows start from the static width
-    // plus two band widths, and rows wider than the LDS ring (which must keep
-    // their scores) take pool chunks; an overflow relaunches with arena_max.
+    // Arena bytes. Values: M, D and I of every cell. Records: one record
+    // per cell (rec_bytes of the row's predecessor count), rounded to 16
+    // bytes, and M and D of a row that a row other than the next one reads
+    // (or, when it is wider than the smallest LDS ring's half, that any row
+    // reads: it gets no ring slot). Exact for static/full bands; adaptive
+    // rows start from the static width plus two band widths and take pool
+    // chunks beyond; an overflow relaunches with the full-width bound.
     const uint64_t tsz = J.use_i64 ? 8 : 4;
-    auto row_bytes = [&](uint32_t i, uint64_t w, bool all_scores) {
+    auto rec_row = [&](uint32_t i, uint64_t w, bool all_scores) {
         const uint32_t np = J.rows[i + 1].pred_beg - J.rows[i].pred_beg;
         uint64_t b = (w * rec_bytes(np) + 15) / 16 * 16;
-        if (all_scores || J.rows[i].last_use > i + 1) b += 2 * w * tsz;
+        const uint32_t lu = J.rows[i].last_use;
+        if (all_scores || lu > i + 1 || (lu > i && 2 * w * tsz > 8192)) b += 2 * w * tsz;
         return b;
     };
-    uint64_t cells = 0, est = 0, full = 0;
+    uint64_t cells = 0, est_r = 0, est_v = 0;
     uint32_t maxw = 0;
+    uint64_t nrows = 0;
     for (uint32_t i = 0; i < n; i++) {
         if (J.rows[i].head == kHeadNone || J.rows[i].tail == kHeadNone) continue;
-        uint64_t s, t;
+        nrows++;
+        uint64_t s0, t0;
         uint64_t w;
-        if (static_band(J, i, s, t))
-            w = t - s;
+        if (static_band(J, i, s0, t0))
+            w = t0 - s0;
         else
             w = J.padded;
         cells += w;
         maxw = std::max<uint32_t>(maxw, uint32_t(w));
         const uint64_t we = (J.flags & kAdaptive) ? std::min<uint64_t>(J.padded, w + 2 * uint64_t(J.band) + 16) : w;
-        est += row_bytes(i, we, false);
-        full += row_bytes(i, J.padded, true);
+        est_r += rec_row(i, we, false);
+        est_v += 3 * we * tsz;
     }
     J.static_cells = cells;
     J.max_width = maxw;
-    J.arena_est = est;
-    J.arena_max = full;
+    J.est_rec = est_r;
+    J.est_val = est_v;
+    J.max_val = 3 * nrows * J.padded * tsz;
+    uint64_t full_r = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if (J.rows[i].head != kHeadNone && J.rows[i].tail != kHeadNone) full_r += rec_row(i, J.padded, true);
+    J.max_rec = full_r;
+    use_layout(J, false);
 }
 
-// Full-width arena bound in bytes: every row at most `padded` wide, with its
-// scores kept.
+// Full-width arena bound of the chosen layout, in bytes.
 inline uint64_t arena_full(const JobPlan &J) { return J.arena_max; }
 
 // Device traceback record -> AlignedBase list (src/align.rs:1293-1347): the

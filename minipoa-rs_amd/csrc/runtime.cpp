@@ -291,6 +291,13 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     // one-chunk pool, so that DPs overflow and take the full-arena relaunch.
     // Arena offsets and sizes are in bytes.
     const bool squeeze = getenv("MPOA_ARENA_SQUEEZE") != nullptr;
+    // One layout per launch: records only if every DP of it was sized for
+    // records (a row past kMaxRecPreds predecessors keeps a DP in values).
+    bool rec = N > 0;
+    for (size_t k = 0; k < N; k++) rec = rec && plans[k]->records;
+    for (size_t k = 0; k < N; k++)
+        if (!rec && plans[k]->records) use_layout(*plans[k], false);
+    li.records = rec;
     std::vector<Off> off(N);
     uint64_t rows = 0, aux = 0, preds = 0, seq = 0, arena = 0, meta = 0, adapt = 0,
              tbn = 0;
@@ -321,9 +328,10 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         maxpad = std::max(maxpad, J.padded);
     }
     // Arena chunk pool after the private regions (rows that outgrow their
-    // estimate take chunks of at least three full-width rows, each with
-    // 8-byte records and its scores).
-    const uint64_t chunk = align_up(std::max<uint64_t>(262144, 3ull * maxpad * (8 + 2 * tsz)), 256);
+    // estimate take chunks of at least three full-width rows: M, D and I, or
+    // 8-byte records and M and D).
+    const uint64_t row_max = rec ? 8 + 2 * tsz : 3 * tsz;
+    const uint64_t chunk = align_up(std::max<uint64_t>(262144, 3ull * maxpad * row_max), 256);
     const uint64_t pool_beg = arena;
     uint64_t pool_len = squeeze ? chunk : std::max<uint64_t>(arena / 2, 4 * chunk);
     const size_t alpha = p.alpha();
@@ -510,7 +518,11 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         FillShape f{0, 0};
         if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
     }
-    if (P.seq_lds == 0) shape = FillShape{8, 1};  // the global-table kernel has one shape
+    // Rows with more than kMaxNarrowPreds predecessors take the wide-record
+    // kernel, which like the global-table kernel has one shape.
+    bool wide = false;
+    for (size_t k = 0; k < N; k++) wide = wide || (rec && plans[k]->max_np > kMaxNarrowPreds);
+    if (P.seq_lds == 0 || wide) shape = FillShape{8, 1};
     if (const char *e = getenv("MPOA_RING_KB")) P.ring_bytes = uint32_t(atoi(e)) * 1024u;
     {
         // Stay within 64 KB of LDS per workgroup with the fill's static arrays.
@@ -525,13 +537,13 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
     li.ring_bytes = P.ring_bytes;
     li.shape = shape;
     li.width = wexp;
-    if (launch_fill(P, uint32_t(N), i64, shape, ctx.stream) != 0) {
+    if (launch_fill(P, uint32_t(N), i64, rec, wide, shape, ctx.stream) != 0) {
         set_error("no fill kernel for the requested launch shape");
         return MPOA_ERR_DEVICE;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx.ev[1], ctx.stream));
-    launch_traceback(P, uint32_t(N), i64, ctx.stream);
+    launch_traceback(P, uint32_t(N), i64, rec, ctx.stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx.ev[2], ctx.stream));
     HIP_TRY(hipMemcpyAsync(const_cast<void *>(li.hres), P.res, N * sizeof(DevResult),

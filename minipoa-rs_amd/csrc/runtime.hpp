@@ -66,6 +66,8 @@ struct RunStats {
     double device_ms = 0, fill_ms = 0, host_ms = 0;
 };
 
+constexpr int kLayoutAuto = 0, kLayoutValues = 1, kLayoutRecords = 2;
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -78,6 +80,10 @@ struct Ctx {
     // Launches of this context run their waves at raised issue priority
     // (s_setprio): the engine's priority slot, whose jobs bound the makespan.
     bool prio = false;
+    // DP layout in HBM (kLayout*, the C ABI's MPOA_LAYOUT_*): auto takes
+    // values, except for the no-allocation entry point, whose fixed scratch
+    // holds several times more cells as records.
+    int layout = 0;
     ~Ctx();
 };
 
@@ -118,6 +124,7 @@ struct LaunchInfo {
     uint64_t rows = 0;            // DP rows over the launch's jobs
     uint32_t grown = 0;           // bit mask of the buffers this launch reallocated
     uint64_t arena_bytes = 0;     // score arena the launch asked for
+    bool records = false;         // the launch's DP layout (else values)
     float fill_ms = 0, all_ms = 0;  // device time of the fill, and of the fill and traceback
     float h2d_ms = 0, d2h_ms = 0;   // device time of the input and result copies
     const void *hres = nullptr;     // host copies of the results and traceback records

@@ -35,6 +35,7 @@ STATUS = {
 BUFFER_TOO_SMALL = 16
 INVALID_ARGUMENT = 17
 SCRATCH_LIMIT = 20
+LAYOUT_AUTO, LAYOUT_VALUES, LAYOUT_RECORDS = 0, 1, 2  # mpoa_ctx_set_layout
 
 
 class HipPoaError(RuntimeError):
@@ -121,6 +122,7 @@ SIGNATURES = {
     "mpoa_last_error": (C.c_char_p, []),
     "mpoa_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "mpoa_ctx_destroy": (None, [_vp]),
+    "mpoa_ctx_set_layout": (C.c_int, [_vp, C.c_int]),
     "mpoa_ctx_device_bytes": (_u64, [_vp]),
     "mpoa_poa": (C.c_int, [_vp, _P, _V, _u64, _u64, C.c_char_p, _u64, C.c_int, _u64, _B, _u64,
                            U64P, C.POINTER(C.c_int64)]),
@@ -248,6 +250,11 @@ class Context:
         h = C.c_void_p()
         _check(lib().mpoa_ctx_create(device, C.byref(h)))
         self._h = h
+
+    def set_layout(self, layout):
+        """DP layout in device memory: LAYOUT_AUTO, LAYOUT_VALUES or
+        LAYOUT_RECORDS (mpoa_ctx_set_layout); results are the same."""
+        _check(lib().mpoa_ctx_set_layout(self._h, int(layout)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -926,6 +926,9 @@ static int64_t tb_choose(const og_graph *g, const dpsetup *s, const omat *x, con
         if (rank < s->beg || rank >= s->beg + i) continue;
         size_t pr = rank - s->beg;
         if (s->head[pr] == NEG || !x->contains(x, pr, pj)) continue;
+        /* MUTANT(20): the diagonal only from predecessors whose band also
+         * holds column j (the fill's overlap), not one ending at j - 1 */
+        if (MUTANT(20) && state == MATCH && !x->contains(x, pr, j)) continue;
         int64_t source = state == DELETE ? x->d(x, pr, pj) : x->m(x, pr, pj);
         int64_t target = state == MATCH ? x->m(x, i, j) : x->d(x, i, j);
         const int heavier = MUTANT(1) ? node->in_w.v[k] >= weight

@@ -19,3 +19,16 @@ def oracle():
 
     o.lib()
     return o
+
+
+@pytest.fixture(scope="module", params=["values", "records"])
+def ctx(request):
+    """A device context per DP layout: every GPU test module that takes
+    `ctx` runs once with M/D/I values and once with traceback records in
+    device memory (mpoa_ctx_set_layout); results must not differ."""
+    import minipoa_hip as mh
+
+    c = mh.Context(0)
+    c.set_layout({"values": mh.LAYOUT_VALUES, "records": mh.LAYOUT_RECORDS}[request.param])
+    yield c
+    c.close()

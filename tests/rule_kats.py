@@ -159,6 +159,38 @@ insertion mode at (T, 2), insert the second A and give T the first.)
 Path: source, insertion A, T as A, insertion C."""
 
 
+
+# ---- the diagonal from a predecessor whose band ends one column before
+# (src/align.rs:764-794 fill overlap, :1201-1225 `choose` over in-band j - 1)
+
+BAND_EDGE_DIAGONAL = Case(
+    "diagonal_from_band_ending_one_column_before",
+    Params(gap_open=-2, gap_ext=-2, match_score=2, mismatch=-1, b=0, f=1000).rebuild_matrix(),
+    [T, G], [(0, 2), (0, 3), (2, 1), (2, 1), (3, 1)], b"TCTGATA",
+    path=[ab(0, N), ab(None, T), ab(None, C), ab(None, T), ab(3, G), ab(None, A), ab(None, T),
+          ab(None, A)], cells=40)
+BAND_EDGE_DIAGONAL.__doc__ = """Source -> X (T, 2) -> sink (weight 2) and source ->
+Y (G, 3) -> sink (weight 1); query "TCTGATA" (m = 9); gap_open -2, gap_ext
+-2 (o = -4, e = -2), match 2, mismatch -1; b = 0 (f = 1000 adds nothing).
+Bands (head, m - tail): source (0, 6) and X, Y (1, 7) give [0, 8); the sink
+(2, 8) gives [0, 16). So the sink's column 8 is in no predecessor's band:
+its fill has no diagonal and no deletion there.
+Rows (M by column 0..7):
+  source  2 -2 -4 -6  -8 -10 -12 -14
+  X      -2  4  0 -2  -4  -6  -8 -10
+  Y      -2  1 -3 -5  -4  -8 -10 -12   (I[Y][5..7] = -8 -10 -12)
+  sink   -4  0  4  0  -2  -4  -6  -8, and M[sink][8] = I = -8 - 2 = -10.
+At (sink, 8) op = ALL: N == N and `choose` looks at column 7 of every
+predecessor whose band holds column 7, which both do: X gives -10 + 2 = -8,
+Y gives -12 + 2 = -10 = M. So Y is the diagonal (weight 1 >= 3/10), though
+the fill never took that diagonal. At (Y, 7) M == I == I[Y][6] + e: A, T are
+insertions (extend), and at (Y, 5) I == M[Y][4] + o: A is an insertion
+(open, op = MATCH|DELETE). At (Y, 4) G == G and M = -4 = M[0][3] + 2: Y
+takes the G. The source inserts T, C, T. (A `choose` that also required
+column 8 in the predecessor's band would find no diagonal at (sink, 8),
+insert there and give the first T to X.)"""
+
+
 # ---- head/tail: heaviest edge, ties to larger support (src/align.rs:623-678)
 
 HEAD_TAIL_SUPPORT = Case(
@@ -277,5 +309,6 @@ score Z = 1, score Y2 = 2: the larger score wins over the lower index:
 "GT"."""
 
 PATH_CASES = [CHOOSE_TIE, CHOOSE_HEAVIER, FILTER_LIGHT, FILTER_EDGE, OPEN_BEFORE_EXTEND,
-              INSERT_OPEN_BEFORE_EXTEND, HEAD_TAIL_SUPPORT, ADAPTIVE_RECENTRE, ADAPTIVE_EXTRA]
+              INSERT_OPEN_BEFORE_EXTEND, BAND_EDGE_DIAGONAL, HEAD_TAIL_SUPPORT, ADAPTIVE_RECENTRE,
+              ADAPTIVE_EXTRA]
 GRAPH_CASES = [TOPOSORT_LIFO, CONSENSUS_INDEX_TIE, CONSENSUS_SCORE_TIE]

@@ -44,6 +44,11 @@ def test_context_creation_fails_cleanly_without_device():
         ctx.close()
 
 
+def test_layout_setter_rejects_null_context():
+    """mpoa_ctx_set_layout without a context is an argument error."""
+    assert mh.lib().mpoa_ctx_set_layout(None, mh.LAYOUT_RECORDS) == mh.INVALID_ARGUMENT
+
+
 def test_server_entry_points_reject_null_arguments():
     """The serving entry points fail cleanly, with no device and no server."""
     import ctypes as C

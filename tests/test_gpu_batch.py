@@ -9,13 +9,6 @@ from minipoa_hip import Params, records, synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def fnv1a(data, h=14695981039346656037):
     for b in data:
         h ^= b
@@ -61,3 +54,21 @@ def test_batched_workload_sample_matches_oracle_hashes(ctx, oracle):
     assert (want != 0).all()
     assert (got == want).all(), np.nonzero(got != want)
     assert res.stats()["cells"] == cells
+
+
+def test_records_layout_needs_less_device_memory_same_bytes(oracle):
+    """The same batch on a values context and on a records context: equal
+    outputs, and the records context's device buffers (arena included) are
+    smaller."""
+    jo, ro, b = synth.jobs(seed=31, njobs=24, lmin=300, lmax=1500, rmin=4, rmax=12)
+    p = Params(adaptive=True)
+    out, size = {}, {}
+    for name, lay in (("values", mh.LAYOUT_VALUES), ("records", mh.LAYOUT_RECORDS)):
+        c = mh.Context(0)
+        c.set_layout(lay)
+        res = mh.batch_run(c, p, jo, ro, b)
+        out[name] = [(res.output(j, 0), res.output(j, 1)) for j in range(24)]
+        size[name] = c.device_bytes()
+        c.close()
+    assert out["values"] == out["records"]
+    assert size["records"] < size["values"], size

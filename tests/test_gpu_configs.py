@@ -22,13 +22,6 @@ pytestmark = pytest.mark.gpu
 CORES = min(16, len(os.sched_getaffinity(0)))
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def fnv1a(data, h=14695981039346656037):
     for x in data:
         h ^= x

@@ -26,13 +26,6 @@ HOXD70 = [91, -114, -31, -123, 0,
           0, 0, 0, 0, 100]
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def linear(oracle, params, reference):
     g = oracle.Graph(params)
     g.add_path(0, oracle.align(params, g, reference))

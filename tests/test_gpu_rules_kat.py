@@ -11,13 +11,6 @@ import rule_kats as K
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 @pytest.mark.parametrize("case", K.PATH_CASES, ids=lambda c: c.name)
 @pytest.mark.parametrize("bits", [32, 64])
 def test_device_path_known_answers(ctx, case, bits):

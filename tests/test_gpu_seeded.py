@@ -19,13 +19,6 @@ from minipoa_hip import Params, records, synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def fnv1a(data, h=14695981039346656037):
     for b in data:
         h ^= b

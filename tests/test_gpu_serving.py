@@ -14,13 +14,6 @@ from minipoa_hip import Params, records, synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def small_stream(seed, njobs):
     rng = np.random.Generator(np.random.PCG64(seed))
     jobs = []

@@ -22,13 +22,6 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(8, 1), (16, 1), (4, 2), (8, 2), (4, 4), (8, 4)]
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 @pytest.fixture
 def shape(request):
     cols, waves = request.param

@@ -34,6 +34,8 @@ MUTANTS = {
     16: ("toposort pops first in, first out", ["case:toposort_lifo"]),
     17: ("consensus full tie keeps the highest incoming index", ["case:consensus_lowest_index_tie"]),
     18: ("consensus weight tie keeps the smaller path score", ["case:consensus_score_breaks_weight_tie"]),
+    20: ("choose skips a diagonal predecessor whose band ends at j - 1",
+         ["case:diagonal_from_band_ending_one_column_before"]),
 }
 
 RUNNER = r"""

@@ -10,13 +10,6 @@ from minipoa_hip import Params, records, synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = mh.Context(0)
-    yield c
-    c.close()
-
-
 def _graph(ctx, params, reads):
     return mh.align_sequences(ctx, params, records(reads))
 
@@ -63,9 +56,10 @@ def test_noalloc_band_failure_retry_and_i64_route(ctx, oracle):
 
 
 def test_noalloc_rejects_a_dp_beyond_the_scratch_then_runs_again(ctx):
-    """A full-DP matrix of ~6,000 x 6,000 cells (~430 MB of scores) exceeds
-    the 64 MiB scratch: a clear status, and the next smaller call is still
-    equal to mpoa_poa."""
+    """A full-DP matrix of ~6,000 x 6,000 cells exceeds the 64 MiB scratch
+    (in the records layout its rows, too wide for the LDS ring, keep M and D:
+    ~330 MB): a clear status, and the next smaller call is still equal to
+    mpoa_poa."""
     reads = synth.c1(length=6000, nreads=2)
     p = Params(f=0)
     g = _graph(ctx, p, reads[:1])
