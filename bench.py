@@ -25,7 +25,10 @@ every one checked byte for byte against the oracle:
   c5  serving: Poisson arrivals of DNA jobs (template log-uniform in
       [300, 20000] bp, 5-64 reads at 5-15% error; 10% extend a 50-read graph
       loaded from GFA text (-i) by 20 reads), admitted by device footprint;
-      p50/p99 job latency.
+      p50/p99 job latency;
+  rec the headline workload once more with the DP in the records layout
+      (1-8 byte traceback records instead of M/D/I values): its step time,
+      traceback bytes per cell, and its outputs against the values run.
 """
 import argparse
 import json
@@ -56,7 +59,7 @@ sys.path[:0] = [str(ROOT / "minipoa-rs_amd"), str(ROOT / "oracle")]
 METRIC = ("aggregate GCUPS + POA jobs/sec (batched vs plain, bit-exact); "
           "p50/p99 job latency")
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = 12  # M, D and I (i32) stored per cell for the traceback
+BYTES_PER_CELL = 12  # M, D and I (i32) stored per cell for the traceback (values layout)
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s, nominal
 
 
@@ -163,7 +166,7 @@ def main():
                     help="hardware queues per process (read before HIP starts; default 8)")
     ap.add_argument("--no-valu-probe", dest="valu_probe", action="store_false",
                     help="skip the int-VALU rate probe (the roofline then uses the nominal peak)")
-    ap.add_argument("--legs", default="c1,c3,c4,c5",
+    ap.add_argument("--legs", default="rec,c1,c3,c4,c5",
                     help="other workloads to report once each (c1, c3, c4, c5; empty = none)")
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
@@ -225,7 +228,9 @@ def main():
     fill_s = st["fill_ms"] / 1e3
     launches = st["fill_launches"]
     wall_s = total / len(times)
-    alg_bytes = BYTES_PER_CELL * cells
+    # The fills report the traceback bytes they wrote (12 per i32 cell in
+    # the values layout, which the headline runs in).
+    alg_bytes = st["arena_bytes"] or BYTES_PER_CELL * cells
     int_ops = 6 * cells + 5 * st["pred_cells"] + 2 * cells
     achieved_gbs = alg_bytes / fill_s / 1e9
     traffic = None
@@ -245,7 +250,7 @@ def main():
         "kernel": "poa_fill", "launches": launches,
         "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3),
         "bytes_per_launch": alg_bytes // max(1, launches),
-        "bytes_per_cell": BYTES_PER_CELL,
+        "bytes_per_cell": round(alg_bytes / max(1, cells), 3),
         # Over the wall: the step's algorithmic bytes per wall second (the
         # per-launch figure divides one launch's bytes by its own time, and
         # launches of the engine's streams overlap).
@@ -320,6 +325,10 @@ def main():
         # 40% slower (C4 690 ms instead of 400-450 ms).
         if "c1" in legs:
             out["legs"]["c1"] = leg_c1(ctx, oracle)
+        # On a context of its own, after the legs on the main context (its
+        # streams would share the hardware queues with theirs).
+        if "rec" in legs:
+            out["legs"]["rec"] = leg_records(params, job_off, read_off, bases, res, gcups, total / len(times))
     if args.plain_jobs > 0:
         # Plain path: the same jobs, one DP per launch (mpoa_align_sequences).
         n = args.plain_jobs
@@ -351,6 +360,40 @@ def hashes_of(get_text, n):
 
     return np.concatenate([fnv1a_many([get_text(j).encode() for j in range(c, min(n, c + 512))])
                            for c in range(0, n, 512)]) if n else np.zeros(0, dtype=np.uint64)
+
+
+def leg_records(params, job_off, read_off, bases, values_res, values_gcups, values_s):
+    """The headline workload in the records layout (one warmup, one timed
+    run) on a context of its own: step time against the values layout's,
+    traceback bytes per cell, and every job's consensus + MSA against the
+    values run's."""
+    import numpy as np
+
+    import minipoa_hip as mh
+
+    c = mh.Context(0)
+    c.set_layout(mh.LAYOUT_RECORDS)
+    mh.batch_run(c, params, job_off, read_off, bases)
+    t = time.perf_counter()
+    r = mh.batch_run(c, params, job_off, read_off, bases)
+    dt = time.perf_counter() - t
+    st = r.stats()
+    n = len(job_off) - 1
+    want = hashes_of(lambda j: values_res.output(j, 0) + values_res.output(j, 1), n)
+    got = hashes_of(lambda j: r.output(j, 0) + r.output(j, 1), n)
+    same = int((got == want).sum())
+    out = {"ms_per_step": round(dt * 1e3, 1), "gcups": round(st["cells"] / dt / 1e9, 3),
+           "vs_values": round(st["cells"] / dt / 1e9 / values_gcups, 3),
+           "values_ms_per_step": round(values_s * 1e3, 1),
+           "fill_ms": round(st["fill_ms"], 1), "record_launches": st["record_launches"],
+           "fill_launches": st["fill_launches"],
+           "bytes_per_cell": round(st["arena_bytes"] / max(1, st["cells"]), 3),
+           "identical_to_values": same, "jobs": n}
+    del r
+    c.close()
+    log(f"rec: records layout {dt*1e3:.0f} ms ({out['vs_values']:.2f}x values), "
+        f"{out['bytes_per_cell']} B/cell, identical {same}/{n}")
+    return out
 
 
 def leg_c1(ctx, oracle):

@@ -297,6 +297,8 @@ void fill_stats(const EngineStats &es, mpoa_batch_stats &S) {
     S.render_ms = es.render_ms;
     S.pack_ms = es.pack_ms;
     S.arena_retries = es.run.arena_retries;
+    S.arena_bytes = es.run.arena_bytes;
+    S.record_launches = es.run.record_launches;
 }
 
 }  // namespace

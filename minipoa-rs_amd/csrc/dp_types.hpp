@@ -81,6 +81,9 @@ struct RowRec {
 };
 
 constexpr uint64_t kMaxSpan = uint64_t(1) << 29;  // n + padded of a device DP
+// Rows of geometry (and adaptive records) the fill keeps in LDS; a row read
+// from further back is found through global memory.
+constexpr uint32_t kHistRows = 64;
 constexpr int32_t kBandCap = int32_t(1) << 30;
 
 // Row geometry written by the fill kernel, read by traceback.

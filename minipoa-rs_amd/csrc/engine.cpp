@@ -274,6 +274,8 @@ EngineStats Engine::stats() {
         st.run.device_ms += x.device_ms;
         st.run.fill_ms += x.fill_ms;
         st.run.arena_retries += x.arena_retries;
+        st.run.arena_bytes += x.arena_bytes;
+        st.run.record_launches += x.record_launches;
     }
     st.run.retries = retries_.load();
     const double nw = double(std::max(1u, tasks_.size()));
@@ -829,6 +831,8 @@ void Engine::slot_loop(size_t si) {
             x.device_ms += rs.device_ms;
             x.fill_ms += rs.fill_ms;
             x.arena_retries += rs.arena_retries;
+            x.arena_bytes += rs.arena_bytes;
+            x.record_launches += rs.record_launches;
             if (trace_path_) {
                 TraceRec t{int(si), t0, t1, now_ms(), li.fill_ms, li.all_ms, li.h2d_ms, li.d2h_ms, li.grown, li.arena_bytes,
                            c.d_arena.cap, batch.size(), 0, 0, -1, li.ring_bytes, li.width, li.shape, li.rows};

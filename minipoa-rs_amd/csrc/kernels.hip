@@ -149,7 +149,7 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // live in dynamic LDS sized per launch).
 constexpr uint32_t kBlk = 64;           // rows per descriptor block
 constexpr uint32_t kPredCache = 256;    // predecessor entries cached per block
-constexpr uint32_t kHist = 64;          // rows of metadata/records kept in LDS
+constexpr uint32_t kHist = kHistRows;   // rows of metadata/records kept in LDS
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 uint32_t fill_dynamic_lds(uint32_t ring_bytes, uint32_t seq_lds, uint32_t mat_lds) {
@@ -162,6 +162,7 @@ struct LMeta {
     uint32_t wpos;   // ring write position (monotonic bytes) of the slot
     uint64_t voff;   // arena byte offset of the row's kept M and D, or kNoValues
     int64_t mlast;   // M of the row's last column
+    uint32_t last_use;  // last row that reads this row (0: none)
 };
 
 // Stores a lane's C contiguous values; whole runs use the widest aligned
@@ -586,16 +587,21 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
                 ring_at += bytes;
             }
         }
-        // Records: scores stay in HBM for a row that a later row other than
-        // the next one reads, or that the next row cannot find in the ring.
-        // Values: [M][D][I] of every row.
-        const bool keep = !REC || ri.last_use > i + 1 || (ri.last_use > i && slot == kNoSlot);
+        // Records: M and D go to HBM at once for a row that some later row
+        // reads and that has no ring slot, or whose last reader is beyond the
+        // LDS history (it reads the row's geometry from global memory); a
+        // row read only within the history is copied out of the ring when a
+        // later row's slot would overwrite it before its last reader (the
+        // spill below). Values: [M][D][I] of every row.
+        const bool keep = !REC || (ri.last_use > i && slot == kNoSlot) || ri.last_use >= i + kHist;
         const uint64_t rec_sz = REC ? (uint64_t(len) * rb + 15u) / 16u * 16u : 0u;
         const uint64_t need = rec_sz + (keep ? uint64_t(bytes) : 0u) + (REC ? 0u : uint64_t(bytes) / 2);
-        if (cur + need > rend) {
-            // Own region exhausted: take a chunk from the launch's pool. The
-            // leader's result reaches its wave by readlane and other waves
-            // through LDS behind a barrier.
+        // `want` arena bytes at cur, taking a chunk from the launch's pool
+        // when the current region is exhausted (uniform across the
+        // workgroup). The leader's atomic result reaches its wave by readlane
+        // and other waves through LDS behind a barrier.
+        auto take = [&](uint64_t want) -> bool {
+            if (cur + want <= rend) return true;
             uint64_t got = 0;
             if (leader) got = atomicAdd(P.pool_next, (unsigned long long)P.chunk);
             if constexpr (W > 1) {
@@ -606,12 +612,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
             } else {
                 got = uint64_t(rl64(int64_t(got), 0));
             }
-            if (need > P.chunk || got + P.chunk > P.pool_end) {
-                status = kArenaOverflow;
-                break;
-            }
+            if (want > P.chunk || got + P.chunk > P.pool_end) return false;
             cur = got;
             rend = got + P.chunk;
+            return true;
+        };
+        if (!take(need)) {
+            status = kArenaOverflow;
+            break;
         }
         const uint64_t off = cur;
         const uint64_t voff = keep ? off + rec_sz : kNoValues;
@@ -627,7 +635,59 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         // The lane owning the row's last column writes its RowMeta and the
         // mlast of this LDS copy once the row is computed; later rows read
         // the copy only after this row's first barrier (mlast after its last).
-        if (leader) hmeta[i % kHist] = LMeta{start, len, slot, swpos, voff, int64_t(NEG)};
+        if (leader) hmeta[i % kHist] = LMeta{start, len, slot, swpos, voff, int64_t(NEG), ri.last_use};
+
+        if constexpr (REC) {
+            // Spill: rows of the history whose ring bytes this row's slot
+            // overwrites (the readers' own test: wpos - slot's wpos > ring)
+            // and that a row from this one on still reads get M and D
+            // copied from the ring to the arena, by every thread. Lane q
+            // checks row i - 1 - q; every wave finds the same rows.
+            bool spilled = false;
+            if (i > 0) {
+                const uint32_t q = uint32_t(lane);
+                bool sp = false;
+                if (q < tmin(i, kHist - 1)) {  // row i - kHist shares row i's entry
+                    const LMeta h = hmeta[(i - 1 - q) % kHist];
+                    sp = h.len != 0 && h.slot != kNoSlot && h.voff == kNoValues && h.last_use >= i &&
+                         wpos - h.wpos > kRingBytes;
+                }
+                uint64_t mask = __builtin_amdgcn_ballot_w64(sp);
+                spilled = mask != 0;
+                // Every wave reads the history before the leader updates it
+                // below, so all waves spill the same rows (and meet the same
+                // barriers in take()).
+                if (spilled) sync_lds<W>();
+                while (mask) {
+                    const uint32_t qq = uint32_t(__builtin_ctzll(mask));
+                    mask &= mask - 1;
+                    const uint32_t r = i - 1 - qq;
+                    const LMeta h = hmeta[r % kHist];
+                    const uint64_t sb = 2ull * h.len * sizeof(T);
+                    if (!take(sb)) {
+                        status = kArenaOverflow;
+                        break;
+                    }
+                    const uint64_t dst = cur;
+                    cur += sb;
+                    used += sb;
+                    const int4 *src4 = reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ring) + h.slot);
+                    int4 *dst4 = reinterpret_cast<int4 *>(arena + dst);
+                    for (uint32_t k = tid; k < uint32_t(sb / 16); k += 64u * W) dst4[k] = src4[k];
+                    // Read by the next rows only behind the barrier below.
+                    if (leader) hmeta[r % kHist].voff = dst;
+                }
+                if (status != kOk) break;
+            }
+            if (spilled) {
+                // The copies' ring reads before this row's ring writes, and the
+                // new offsets before any wave reads them.
+                sync_lds<W>();
+                const uint32_t q = pb + uint32_t(lane);
+                if (np <= 64 && q < pe && g_meta.len != 0 && i - g_row < kHist)
+                    g_meta.voff = hmeta[g_row % kHist].voff;
+            }
+        }
 
         if (i == 0) {
             // Source row (src/align.rs:737-750): M[0] = score(N,N), then

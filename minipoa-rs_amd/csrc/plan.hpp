@@ -308,17 +308,18 @@ void plan_job(const Params &p, const G &g, uint64_t beg_id, uint64_t end_id, con
 
     // Arena bytes. Values: M, D and I of every cell. Records: one record
     // per cell (rec_bytes of the row's predecessor count), rounded to 16
-    // bytes, and M and D of a row that a row other than the next one reads
-    // (or, when it is wider than the smallest LDS ring's half, that any row
-    // reads: it gets no ring slot). Exact for static/full bands; adaptive
-    // rows start from the static width plus two band widths and take pool
-    // chunks beyond; an overflow relaunches with the full-width bound.
+    // bytes, and M and D of a row read beyond the fill's LDS history, or
+    // read later and wider than the smallest LDS ring's half (no ring slot),
+    // or -- an estimate of the fill's spills -- read more than four rows
+    // later. Static/full bands are exact in values; adaptive rows start from
+    // the static width plus two band widths; rows beyond the estimate take
+    // pool chunks, and an overflow relaunches with the full-width bound.
     const uint64_t tsz = J.use_i64 ? 8 : 4;
     auto rec_row = [&](uint32_t i, uint64_t w, bool all_scores) {
         const uint32_t np = J.rows[i + 1].pred_beg - J.rows[i].pred_beg;
         uint64_t b = (w * rec_bytes(np) + 15) / 16 * 16;
         const uint32_t lu = J.rows[i].last_use;
-        if (all_scores || lu > i + 1 || (lu > i && 2 * w * tsz > 8192)) b += 2 * w * tsz;
+        if (all_scores || lu >= i + kHistRows || lu > i + 4 || (lu > i && 2 * w * tsz > 8192)) b += 2 * w * tsz;
         return b;
     };
     uint64_t cells = 0, est_r = 0, est_v = 0;

@@ -593,6 +593,7 @@ int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
         st->fill_ms += fill_ms;
         st->device_ms += all_ms;
         st->fill_launches += 1;
+        if (li.records) st->record_launches += 1;
         for (size_t k = 0; k < li.n; k++) {
             if (hr[k].status == kArenaOverflow) {
                 st->arena_retries += 1;
@@ -600,6 +601,7 @@ int launch_finish(Ctx &ctx, LaunchInfo &li, RunStats *st) {
             }
             st->cells += hr[k].cells;
             st->pred_cells += hr[k].pred_cells;
+            st->arena_bytes += hr[k].arena_used;
             st->dp_calls += 1;
             if (li.i64) st->i64_calls += 1;
         }

@@ -63,6 +63,8 @@ struct RunStats {
     uint64_t cells = 0, dp_calls = 0, retries = 0, i64_calls = 0, pred_cells = 0;
     uint64_t fill_launches = 0;
     uint64_t arena_retries = 0;  // DPs re-run with full-width arena rows after an overflow
+    uint64_t arena_bytes = 0;    // traceback data the fills wrote (arena bytes of completed DPs)
+    uint64_t record_launches = 0;  // fill launches in the records layout
     double device_ms = 0, fill_ms = 0, host_ms = 0;
 };
 
