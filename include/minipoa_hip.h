@@ -326,6 +326,10 @@ typedef struct {
     /* DPs whose adaptive rows outgrew their arena and were re-run with
      * full-width rows (the same DP, so the same result). */
     uint64_t arena_retries;
+    /* Device bytes of traceback data the fills wrote (completed DPs), and
+     * the fill launches that used the records layout (mpoa_ctx_set_layout). */
+    uint64_t arena_bytes;
+    uint64_t record_launches;
 } mpoa_batch_stats;
 
 typedef struct mpoa_batch mpoa_batch;

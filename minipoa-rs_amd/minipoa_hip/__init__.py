@@ -89,6 +89,7 @@ class _Stats(C.Structure):
         ("host_ms", C.c_double), ("wall_ms", C.c_double),
         ("plan_ms", C.c_double), ("consume_ms", C.c_double), ("fuse_ms", C.c_double),
         ("render_ms", C.c_double), ("pack_ms", C.c_double), ("arena_retries", C.c_uint64),
+        ("arena_bytes", C.c_uint64), ("record_launches", C.c_uint64),
     ]
 
 
