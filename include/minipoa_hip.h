@@ -330,6 +330,13 @@ typedef struct {
      * the fill launches that used the records layout (mpoa_ctx_set_layout). */
     uint64_t arena_bytes;
     uint64_t record_launches;
+    /* mpoa_batch_run_resident only: jobs that finished on the device, jobs
+     * that left it and finished on the host engine, captured step-loop
+     * replays, length buckets, DP placements postponed to a later step,
+     * wall time of the device-resident part and its step loops' device
+     * time (HIP events). Zero for mpoa_batch_run. */
+    uint64_t resident_jobs, evicted_jobs, step_replays, buckets, deferred;
+    double resident_ms, resident_device_ms;
 } mpoa_batch_stats;
 
 typedef struct mpoa_batch mpoa_batch;
@@ -337,6 +344,21 @@ typedef struct mpoa_batch mpoa_batch;
 int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
                    const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
                    mpoa_batch **out);
+/* mpoa_batch_run with every job kept on the device between reads (no
+ * reference counterpart; the same per-job semantics, src/lib.rs:184-204):
+ * each job's graph, its next DP's descriptors and its traceback live in
+ * device memory; one progressive step of a length bucket of jobs -- path
+ * fusion (src/graph.rs:82-122), the LIFO re-rank (src/graph.rs:124-178)
+ * and the next read's planning (src/align.rs:587-688), then fill and
+ * traceback -- is one captured HIP graph, replayed until the bucket's jobs
+ * finish; only final graphs come back. Jobs outside it (seeding, the -p
+ * guide order, scores that need the i64 kernel, reads over 31,998 bases,
+ * alphabets over 25 letters, no reads) and jobs whose DP outgrows the
+ * device arena continue on the host-driven engine from the read where they
+ * left. Outputs are byte-identical to mpoa_batch_run's. */
+int mpoa_batch_run_resident(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
+                            const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
+                            mpoa_batch **out);
 void mpoa_batch_free(mpoa_batch *b);
 int mpoa_batch_stats_get(const mpoa_batch *b, mpoa_batch_stats *out);
 /* Per-job status (MPOA_OK or the job's first error). */
@@ -346,6 +368,18 @@ int mpoa_batch_job_status(const mpoa_batch *b, uint64_t job);
  * would print them. */
 int mpoa_batch_job_output(const mpoa_batch *b, uint64_t job, int mode, char *out, uint64_t cap,
                           uint64_t *out_len);
+
+/* Self-check of the device-resident step's code (no reference counterpart):
+ * the step's graph fusion, re-rank and planning templates run on one host
+ * thread beside this library's host graph and planner (src/graph.rs:82-178,
+ * src/align.rs:587-688), applying read r's path paths[path_off[r] ..
+ * path_off[r+1]) in turn, and compare nodes, adjacency order, ranks, paths
+ * and every read's DP plan (banded and full). lds_bytes: the step's LDS
+ * staging (0 = global arrays). MPOA_OK when equal; otherwise the first
+ * difference is written to report (cap bytes, NUL-terminated). */
+int mpoa_resident_check(const mpoa_params *params, const uint8_t *bases, const uint64_t *read_off,
+                        uint64_t nreads, const mpoa_aligned_base *paths, const uint64_t *path_off,
+                        uint32_t lds_bytes, char *report, uint64_t cap);
 
 /* ---- serving (additive; no reference counterpart) ----
  * A server admits POA jobs as they arrive and runs them in the same

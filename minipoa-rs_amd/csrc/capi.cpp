@@ -20,6 +20,7 @@
 #include "io.hpp"
 #include "minipoa_hip.h"
 #include "plan.hpp"
+#include "resident_host.hpp"
 #include "runtime.hpp"
 #include "seed.hpp"
 
@@ -797,6 +798,72 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     S.wall_ms = ms_since(t_all);
     *out = B.release();
     return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+int mpoa_batch_run_resident(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
+                            const uint64_t *job_off, const uint64_t *read_off, const uint8_t *bases,
+                            mpoa_batch **out) try {
+    if (!ctx || !out || (njobs && (!job_off || !read_off || !bases)))
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    auto t_all = std::chrono::steady_clock::now();
+    auto B = std::make_unique<mpoa_batch>();
+    B->params = p;
+    B->status.assign(njobs, MPOA_OK);
+    B->err.assign(njobs, "");
+    B->out0.assign(njobs, "");
+    B->out1.assign(njobs, "");
+    B->graphs.resize(njobs);
+    B->nreads.resize(njobs);
+    for (uint64_t j = 0; j < njobs; j++) B->nreads[j] = uint32_t(job_off[j + 1] - job_off[j]);
+    const int nslots = engine_slots(0);
+    rc = ensure_engine(ctx, nslots);
+    if (rc != MPOA_OK) return rc;
+    std::vector<Ctx *> slots;
+    for (int k = 0; k < nslots; k++) slots.push_back(ctx->lanes[size_t(k)].get());
+    BatchInput in{njobs, job_off, read_off, bases, true};
+    BatchOutput eo{&B->status, &B->err, &B->out0, &B->out1, &B->graphs};
+    EngineStats es;
+    ResidentStats rs;
+    rc = run_resident(slots, *ctx->tasks, p, in, eo, es, rs);
+    release_engine(ctx);
+    if (rc != MPOA_OK) return rc;
+    for (uint64_t j = 0; j < njobs; j++)
+        if (!B->graphs[j]) B->graphs[j] = std::make_unique<Graph>(p);
+    mpoa_batch_stats &S = B->stats;
+    fill_stats(es, S);
+    S.resident_jobs = rs.resident_done;
+    S.evicted_jobs = rs.evicted;
+    S.step_replays = rs.replays;
+    S.buckets = rs.buckets;
+    S.deferred = rs.deferred;
+    S.resident_ms = rs.resident_ms;
+    S.resident_device_ms = rs.device_ms;
+    S.wall_ms = ms_since(t_all);
+    *out = B.release();
+    return MPOA_OK;
+} catch (...) { return exception_status(); }
+
+int mpoa_resident_check(const mpoa_params *params, const uint8_t *bases, const uint64_t *read_off,
+                        uint64_t nreads, const mpoa_aligned_base *paths, const uint64_t *path_off,
+                        uint32_t lds_bytes, char *report, uint64_t cap) try {
+    if (!params || !read_off || !path_off || (nreads && (!bases || !paths)) || nreads > UINT32_MAX)
+        return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
+    Params p;
+    int rc = MPOA_OK;
+    if (!check_params(params, p, rc)) return rc;
+    std::string rep;
+    const int r = resident_check(p, bases, read_off, uint32_t(nreads), paths, path_off, lds_bytes, rep);
+    if (report && cap) {
+        const size_t n = std::min<size_t>(rep.size(), size_t(cap - 1));
+        memcpy(report, rep.data(), n);
+        report[n] = 0;
+    }
+    if (r == 0) return MPOA_OK;
+    return fail(r == 2 ? MPOA_ERR_GRAPH : MPOA_ERR_DEVICE, rep);
 } catch (...) { return exception_status(); }
 
 void mpoa_batch_free(mpoa_batch *b) { delete b; }

@@ -127,6 +127,7 @@ enum DevStatus : int32_t {
     kArenaOverflow = 100,
     kTooManyPreds = 101,  // a row with more than kMaxRecPreds predecessors
     kLostScores = 102,    // a predecessor's scores were neither in LDS nor kept in HBM
+    kSkipped = 103,       // no DP in this launch for the job (n == 0: resident step loops)
 };
 
 struct DevResult {

@@ -493,4 +493,35 @@ mpoa_graph_view Graph::view() {
     return v;
 }
 
+void Graph::assign_packed(uint32_t nn, const uint8_t *base, const uint32_t *parent, const uint32_t *rank,
+                          const uint32_t *cover, const uint32_t *out_head, const uint32_t *in_head,
+                          const uint32_t *edges5, const uint32_t *path_off, const uint32_t *paths,
+                          uint32_t nread, uint32_t first) {
+    constexpr uint32_t nil = 0xFFFFFFFFu;
+    nodes_.clear();
+    nodes_.resize(nn);
+    aligned_.assign(size_t(nn) * alpha_, -1);
+    rank_.assign(nn, 0);
+    rank_flat_.assign(nn, 0);
+    for (uint32_t id = 0; id < nn; id++) {
+        Node &nd = nodes_[id];
+        nd.base = base[id];
+        nd.parent = parent[id];
+        nd.rank = rank[id];
+        nd.coverage = cover[id];
+        for (uint32_t e = out_head[id]; e != nil; e = edges5[size_t(e) * 5 + 3])
+            nd.out.push(edges5[size_t(e) * 5 + 1], edges5[size_t(e) * 5 + 2]);
+        for (uint32_t e = in_head[id]; e != nil; e = edges5[size_t(e) * 5 + 4])
+            nd.in.push(edges5[size_t(e) * 5 + 0], edges5[size_t(e) * 5 + 2]);
+        aligned_[size_t(id) * alpha_ + nd.base] = id;
+        if (nd.parent != id) aligned_[size_t(nd.parent) * alpha_ + nd.base] = id;
+        rank_flat_[id] = rank[id];
+        if (rank[id] < nn) rank_[rank[id]] = id;
+    }
+    paths_.clear();
+    for (uint32_t k = 0; k < nread; k++)
+        paths_.emplace_back(uint64_t(first) + k,
+                            std::vector<uint32_t>(paths + path_off[k], paths + path_off[k + 1]));
+}
+
 }  // namespace mpoa

@@ -135,6 +135,20 @@ public:
     // Flat node-indexed view (mpoa_graph_view) rebuilt on demand.
     mpoa_graph_view view();
 
+    // Replaces the graph with a device-resident job's packed arrays: per node
+    // its base, parent, rank, coverage and the first records of its out- and
+    // in-list; edge records of five words (from, to, weight, next out, next
+    // in; 0xFFFFFFFF ends a list) linking each list in insertion order; and
+    // nread paths (path k: ids path_off[k] .. path_off[k + 1]) with sequence
+    // ids first + k. `aligned` follows from parents and bases: every node
+    // holds itself under its base, and a node whose parent is another node
+    // is that column's member for its base (the only writes of
+    // src/graph.rs:42-61, 95-104).
+    void assign_packed(uint32_t nn, const uint8_t *base, const uint32_t *parent, const uint32_t *rank,
+                       const uint32_t *cover, const uint32_t *out_head, const uint32_t *in_head,
+                       const uint32_t *edges5, const uint32_t *path_off, const uint32_t *paths,
+                       uint32_t nread, uint32_t first);
+
 private:
     size_t alpha_;
     std::vector<Node> nodes_;

@@ -372,6 +372,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
 
     if (P.prio) __builtin_amdgcn_s_setprio(3);
     const DevJob J = ldg(P.jobs + blockIdx.x);
+    if (J.n == 0) {
+        // No DP for this job in this launch (a resident step loop replays
+        // one launch shape over every job of its batch): uniform exit.
+        if (threadIdx.x == 0) P.res[blockIdx.x] = DevResult{kSkipped, 0, 0, 0, 0, 0, 0, 0};
+        return;
+    }
     const uint32_t tid = threadIdx.x;
     const int lane = int(tid & 63u);
     const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));

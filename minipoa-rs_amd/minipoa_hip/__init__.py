@@ -90,6 +90,9 @@ class _Stats(C.Structure):
         ("plan_ms", C.c_double), ("consume_ms", C.c_double), ("fuse_ms", C.c_double),
         ("render_ms", C.c_double), ("pack_ms", C.c_double), ("arena_retries", C.c_uint64),
         ("arena_bytes", C.c_uint64), ("record_launches", C.c_uint64),
+        ("resident_jobs", C.c_uint64), ("evicted_jobs", C.c_uint64), ("step_replays", C.c_uint64),
+        ("buckets", C.c_uint64), ("deferred", C.c_uint64), ("resident_ms", C.c_double),
+        ("resident_device_ms", C.c_double),
     ]
 
 
@@ -170,6 +173,8 @@ SIGNATURES = {
                                        C.POINTER(_vp)]),
     "mpoa_extend_graph": (C.c_int, [_vp, _P, _vp, C.POINTER(C.c_char_p), U64P, _u64, _u64]),
     "mpoa_batch_run": (C.c_int, [_vp, _P, _u64, U64P, U64P, _vp, C.POINTER(_vp)]),
+    "mpoa_batch_run_resident": (C.c_int, [_vp, _P, _u64, U64P, U64P, _vp, C.POINTER(_vp)]),
+    "mpoa_resident_check": (C.c_int, [_P, _vp, U64P, _u64, _B, U64P, C.c_uint32, C.c_char_p, _u64]),
     "mpoa_batch_free": (None, [_vp]),
     "mpoa_batch_stats_get": (C.c_int, [_vp, C.POINTER(_Stats)]),
     "mpoa_batch_job_status": (C.c_int, [_vp, _u64]),
@@ -827,8 +832,13 @@ class Server:
         return lib().mpoa_server_now_ms(self._h)
 
 
-def batch_run(ctx, params, job_off, read_off, bases):
-    """Batched progressive POA over numpy job/read offsets and a base buffer."""
+def batch_run(ctx, params, job_off, read_off, bases, resident=False):
+    """Batched progressive POA over numpy job/read offsets and a base buffer.
+
+    resident: keep every job's graph and DP descriptors on the device between
+    reads, with graph fusion and planning on the device and each length
+    bucket's step loop a replayed HIP graph (mpoa_batch_run_resident); the
+    outputs are the same."""
     import numpy as np
 
     job_off = np.ascontiguousarray(job_off, dtype=np.uint64)
@@ -836,10 +846,37 @@ def batch_run(ctx, params, job_off, read_off, bases):
     bases = np.ascontiguousarray(bases, dtype=np.uint8)
     cp = CParams(params)
     h = C.c_void_p()
-    _check(lib().mpoa_batch_run(ctx._h, cp.ref, len(job_off) - 1, job_off.ctypes.data_as(U64P),
-                                read_off.ctypes.data_as(U64P), bases.ctypes.data_as(C.c_void_p),
-                                C.byref(h)))
+    fn = lib().mpoa_batch_run_resident if resident else lib().mpoa_batch_run
+    _check(fn(ctx._h, cp.ref, len(job_off) - 1, job_off.ctypes.data_as(U64P),
+              read_off.ctypes.data_as(U64P), bases.ctypes.data_as(C.c_void_p), C.byref(h)))
     return BatchResult(h, len(job_off) - 1)
+
+
+def resident_check(params, reads, paths, lds_bytes=0):
+    """Runs the device-resident step's graph fusion, re-rank and planning code
+    on one host thread beside the host graph and planner, applying paths[r]
+    (AlignedBase tuples (node, base, aligned), None for absent) after
+    planning read r. Returns "" when every graph and plan matched, else the
+    first difference."""
+    import numpy as np
+
+    cp = CParams(params)
+    raw = [bytes(r) for r in reads]
+    read_off = np.zeros(len(raw) + 1, dtype=np.uint64)
+    read_off[1:] = np.cumsum([len(r) for r in raw])
+    blob = b"".join(raw) or b"\0"
+    path_off = np.zeros(len(paths) + 1, dtype=np.uint64)
+    path_off[1:] = np.cumsum([len(p) for p in paths])
+    flat = [e for p in paths for e in p]
+    arr = (_Base * max(1, len(flat)))()
+    for k, (node, base, aligned) in enumerate(flat):
+        arr[k] = _Base(-1 if node is None else node, base, -1 if aligned is None else aligned)
+    rep = C.create_string_buffer(512)
+    rc = lib().mpoa_resident_check(cp.ref, blob, read_off.ctypes.data_as(U64P), len(raw), arr,
+                                   path_off.ctypes.data_as(U64P), lds_bytes, rep, 512)
+    if rc == 0:
+        return ""
+    return rep.value.decode() or last_error()
 
 
 __all__ = [
@@ -849,5 +886,5 @@ __all__ = [
     "align_sequences", "extend_graph", "batch_run", "backend_name", "i32_scores_safe",
     "validate", "result_or_error", "lib", "last_error", "SIGNATURES", "Server", "load_matrix",
     "Minimizer", "Anchor", "sketch", "collect_anchors", "chain", "partition_windows",
-    "anchor_windows", "guide_order",
+    "anchor_windows", "guide_order", "resident_check",
 ]
