@@ -28,7 +28,11 @@ every one checked byte for byte against the oracle:
       p50/p99 job latency;
   rec the headline workload once more with the DP in the records layout
       (1-8 byte traceback records instead of M/D/I values): its step time,
-      traceback bytes per cell, and its outputs against the values run.
+      traceback bytes per cell, and its outputs against the values run;
+  res the headline workload with every job device-resident: graph fusion,
+      re-ranking and planning on the device, each length bucket's step loop
+      a replayed HIP graph; its step time and outputs against the host-
+      driven engine's.
 """
 import argparse
 import json
@@ -166,8 +170,8 @@ def main():
                     help="hardware queues per process (read before HIP starts; default 8)")
     ap.add_argument("--no-valu-probe", dest="valu_probe", action="store_false",
                     help="skip the int-VALU rate probe (the roofline then uses the nominal peak)")
-    ap.add_argument("--legs", default="rec,c1,c3,c4,c5",
-                    help="other workloads to report once each (c1, c3, c4, c5; empty = none)")
+    ap.add_argument("--legs", default="rec,res,c1,c3,c4,c5",
+                    help="other workloads to report once each (rec, res, c1, c3, c4, c5; empty = none)")
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=0, help="serving launch slots (0 = library default)")
@@ -320,6 +324,9 @@ def main():
                                        args.serve_slots)
         if "c3" in legs:
             out["legs"]["c3"] = leg_c3(ctx, oracle, cores)
+        if "res" in legs:
+            out["legs"]["res"] = leg_resident(ctx, params, job_off, read_off, bases, res, gcups,
+                                              total / len(times))
         # One DP per launch on the context's own stream last: after that
         # stream has run, the batched legs on the same context measured
         # 40% slower (C4 690 ms instead of 400-450 ms).
@@ -393,6 +400,43 @@ def leg_records(params, job_off, read_off, bases, values_res, values_gcups, valu
     c.close()
     log(f"rec: records layout {dt*1e3:.0f} ms ({out['vs_values']:.2f}x values), "
         f"{out['bytes_per_cell']} B/cell, identical {same}/{n}")
+    return out
+
+
+def leg_resident(ctx, params, job_off, read_off, bases, values_res, values_gcups, values_s):
+    """The headline workload with every job device-resident (one warmup, two
+    timed runs): graph fusion, re-ranking and DP planning on the device, one
+    captured step graph per length bucket replayed until its jobs finish.
+    Step time against the host-driven engine, and every job's consensus +
+    MSA against the values run's."""
+    import minipoa_hip as mh
+
+    mh.batch_run(ctx, params, job_off, read_off, bases, resident=True)
+    times = []
+    for _ in range(2):
+        t = time.perf_counter()
+        r = mh.batch_run(ctx, params, job_off, read_off, bases, resident=True)
+        times.append(time.perf_counter() - t)
+        st = r.stats()
+        if len(times) < 2:
+            del r
+    dt = min(times)
+    n = len(job_off) - 1
+    want = hashes_of(lambda j: values_res.output(j, 0) + values_res.output(j, 1), n)
+    got = hashes_of(lambda j: r.output(j, 0) + r.output(j, 1), n)
+    same = int((got == want).sum())
+    out = {"ms_per_step": round(dt * 1e3, 1), "ms_runs": [round(x * 1e3, 1) for x in times],
+           "gcups": round(st["cells"] / dt / 1e9, 3),
+           "vs_host_engine": round(st["cells"] / dt / 1e9 / values_gcups, 3),
+           "host_engine_ms_per_step": round(values_s * 1e3, 1),
+           "resident_jobs": st["resident_jobs"], "evicted_jobs": st["evicted_jobs"],
+           "buckets": st["buckets"], "step_replays": st["step_replays"],
+           "step_loop_device_ms": round(st["resident_device_ms"], 1),
+           "host_ms": round(st["host_ms"], 1),
+           "identical_to_host_engine": same, "jobs": n}
+    del r
+    log(f"res: device-resident {dt*1e3:.0f} ms ({out['vs_host_engine']:.2f}x host engine), "
+        f"identical {same}/{n}")
     return out
 
 

@@ -32,6 +32,7 @@ struct mpoa_ctx {
     std::vector<std::unique_ptr<Ctx>> lanes;  // batched-engine launch slots
     std::unique_ptr<TaskQueue> tasks;         // batched-engine host workers
     bool engine_busy = false;                 // a batch or a server holds the slots (under mu)
+    ResidentCache *resident = nullptr;        // resident batches' per-slot buffers
 };
 struct mpoa_graph {
     Graph g;
@@ -340,6 +341,7 @@ int mpoa_ctx_set_layout(mpoa_ctx *ctx, int layout) try {
 
 void mpoa_ctx_destroy(mpoa_ctx *ctx) {
     if (!ctx) return;
+    resident_cache_free(ctx->resident);
     ctx->lanes.clear();
     delete ctx->c;
     delete ctx;
@@ -353,7 +355,7 @@ uint64_t mpoa_ctx_device_bytes(const mpoa_ctx *ctx) try {
     };
     add(*ctx->c);
     for (const auto &l : ctx->lanes) add(*l);
-    return total;
+    return total + resident_cache_bytes(ctx->resident);
 } catch (...) {
     (void)exception_status();
     return 0;
@@ -828,7 +830,8 @@ int mpoa_batch_run_resident(mpoa_ctx *ctx, const mpoa_params *params, uint64_t n
     BatchOutput eo{&B->status, &B->err, &B->out0, &B->out1, &B->graphs};
     EngineStats es;
     ResidentStats rs;
-    rc = run_resident(slots, *ctx->tasks, p, in, eo, es, rs);
+    if (!ctx->resident) ctx->resident = resident_cache_new();
+    rc = run_resident(slots, *ctx->tasks, *ctx->resident, p, in, eo, es, rs);
     release_engine(ctx);
     if (rc != MPOA_OK) return rc;
     for (uint64_t j = 0; j < njobs; j++)

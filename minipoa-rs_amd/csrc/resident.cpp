@@ -116,7 +116,7 @@ RCaps job_caps(const uint64_t *roff, uint32_t nreads) {
 // Byte offsets of one job's graph arrays inside a block (each 16-aligned).
 struct RGraphLayout {
     uint64_t base, parent, rank, order, aligned, out_head, out_tail, in_head, in_tail, in_cnt, cover, mark,
-        edges, paths, path_off, csr_off, csr_adj, deg, stack, succ_beg, succ, head, tail, support, bytes;
+        edges, paths, path_off, csr_off, csr_adj, deg, stack, succ_beg, succ, head, tail, support, win, bytes;
 };
 
 RGraphLayout graph_layout(const RCaps &c, uint32_t alpha) {
@@ -152,6 +152,7 @@ RGraphLayout graph_layout(const RCaps &c, uint32_t alpha) {
     L.head = take(4 * nc);
     L.tail = take(4 * nc);
     L.support = take(4 * nc);
+    L.win = take(4 * (kRowWin + 1 + 2 * kEntWin));
     L.bytes = at;
     return L;
 }
@@ -182,6 +183,7 @@ void bind_graph(RJob &J, char *blk, const RGraphLayout &L) {
     J.head = reinterpret_cast<int32_t *>(blk + L.head);
     J.tail = reinterpret_cast<int32_t *>(blk + L.tail);
     J.support = reinterpret_cast<uint32_t *>(blk + L.support);
+    J.win = reinterpret_cast<uint32_t *>(blk + L.win);
 }
 
 // Message of a failed resident job (the reference's text for its error).
@@ -422,10 +424,9 @@ int resident_check(const Params &p, const uint8_t *bases, const uint64_t *read_o
 
 // ---------------------------------------------------------------- batch
 
-namespace {
-
 // Per launch slot: the device blocks and host staging of its buckets,
-// grown as needed and kept across buckets (hipFree synchronises the device).
+// sized before the first bucket starts and kept across buckets and runs
+// (hipMalloc and hipFree while other streams run stall them).
 struct RSlot {
     Ctx *ctx = nullptr;
     DevBuf dg, dp, da, dk;
@@ -445,6 +446,22 @@ struct RSlot {
     }
 };
 
+struct ResidentCache {
+    std::vector<std::unique_ptr<RSlot>> slots;
+    uint64_t arena_bytes = 0;
+};
+
+ResidentCache *resident_cache_new() { return new ResidentCache(); }
+void resident_cache_free(ResidentCache *c) { delete c; }
+uint64_t resident_cache_bytes(const ResidentCache *c) {
+    uint64_t t = 0;
+    if (c)
+        for (const auto &s : c->slots) t += s->dg.cap + s->dp.cap + s->da.cap + s->dk.cap;
+    return t;
+}
+
+namespace {
+
 struct RShared {
     const Params *p = nullptr;
     const BatchInput *in = nullptr;
@@ -460,6 +477,20 @@ struct RShared {
     ResidentStats st;
     int rc = MPOA_OK;
     std::string err;
+    // MPOA_RES_TRACE=<file>: one CSV line per bucket (host clock in ms from
+    // the start of the run at each phase's end), written at the end.
+    std::chrono::steady_clock::time_point t0;
+    struct Trace {
+        size_t slot;
+        uint32_t njobs, rmax;
+        uint64_t lmax, replays;
+        double t[6];
+        float dev_ms;
+    };
+    std::vector<Trace> trace;
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
 };
 
 #define RJ_TRY(expr)                                                                        \
@@ -481,7 +512,70 @@ uint32_t expected_width(const Params &p, uint64_t L) {
     return uint32_t(std::min<uint64_t>(padded, 2 * band + m / 4 + 16));
 }
 
-int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
+// Layout of one bucket: block A (parameters, counters, job states, the
+// launch's job and result arrays, reads, graphs) and block B (the launch
+// pools of the DP descriptors), in bytes.
+struct BucketLayout {
+    std::vector<RCaps> caps;
+    std::vector<RGraphLayout> gl;
+    uint32_t maxpad = 8, rmax = 0;
+    uint64_t lmax = 0, nmax = 0;
+    uint64_t a_prm, a_mat, a_ctr, a_jobs, a_dj, a_res, a_poff, a_bases, a_roff, a_graph, a_end;
+    uint64_t b_rows, b_aux, b_preds, b_seq, b_meta, b_adapt, b_tb, b_end;
+};
+
+BucketLayout layout_bucket(const Params &p, const BatchInput &in, const std::vector<uint64_t> &jobs) {
+    BucketLayout B;
+    const uint32_t N = uint32_t(jobs.size());
+    const uint32_t alpha = uint32_t(p.alpha());
+    B.caps.resize(N);
+    B.gl.resize(N);
+    uint64_t nbases = 0, noffs = 0, gbytes = 0, rows = 0, preds = 0, seqb = 0, meta = 0, tbn = 0;
+    for (uint32_t k = 0; k < N; k++) {
+        const uint64_t j = jobs[k];
+        const uint64_t *ro = in.read_off + in.job_off[j];
+        const uint32_t nr = uint32_t(in.job_off[j + 1] - in.job_off[j]);
+        B.caps[k] = job_caps(ro, nr);
+        B.gl[k] = graph_layout(B.caps[k], alpha);
+        nbases += B.caps[k].total;
+        noffs += nr + 1;
+        gbytes += B.gl[k].bytes;
+        rows += uint64_t(B.caps[k].nc) + 1;
+        preds += B.caps[k].ec;
+        seqb += B.caps[k].padm;
+        meta += B.caps[k].nc;
+        tbn += B.caps[k].maxm;
+        B.maxpad = std::max(B.maxpad, B.caps[k].padm);
+        B.rmax = std::max(B.rmax, nr);
+        B.nmax = std::max<uint64_t>(B.nmax, B.caps[k].nc);
+        if (nr) B.lmax = std::max<uint64_t>(B.lmax, ro[1] - ro[0]);
+    }
+    B.a_prm = 0;
+    B.a_mat = up256(B.a_prm + sizeof(RParams));
+    B.a_ctr = up256(B.a_mat + 4ull * alpha * alpha);
+    B.a_jobs = up256(B.a_ctr + 256);
+    B.a_dj = up256(B.a_jobs + sizeof(RJob) * N);
+    B.a_res = up256(B.a_dj + sizeof(DevJob) * N);
+    B.a_poff = up256(B.a_res + sizeof(DevResult) * N);
+    B.a_bases = up256(B.a_poff + 8ull * N);
+    B.a_roff = up256(B.a_bases + nbases);
+    B.a_graph = up256(B.a_roff + 8 * noffs);
+    B.a_end = up256(B.a_graph + gbytes);
+    B.b_rows = 0;
+    B.b_aux = up256(B.b_rows + sizeof(RowIn) * rows);
+    B.b_preds = up256(B.b_aux + sizeof(RowAux) * rows);
+    B.b_seq = up256(B.b_preds + sizeof(PredE) * std::max<uint64_t>(preds, 1));
+    B.b_meta = up256(B.b_seq + seqb + 16);
+    B.b_adapt = up256(B.b_meta + sizeof(RowMeta) * meta);
+    B.b_tb = up256(B.b_adapt + sizeof(RowRec) * meta);
+    B.b_end = up256(B.b_tb + 4 * tbn);
+    return B;
+}
+
+int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &jobs) {
+    RShared::Trace tr{};
+    tr.slot = si;
+    tr.t[0] = S.ms();
     const Params &p = *S.p;
     const BatchInput &in = *S.in;
     Ctx &ctx = *slot.ctx;
@@ -489,49 +583,16 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
     const uint32_t alpha = uint32_t(p.alpha());
     RJ_TRY(hipSetDevice(ctx.device));
     hipStream_t s = ctx.stream;
-    // Block A: header, reads, graphs. Block B: the launch pools.
-    std::vector<RCaps> caps(N);
-    std::vector<RGraphLayout> gl(N);
-    uint64_t nbases = 0, noffs = 0, gbytes = 0, rows = 0, preds = 0, seqb = 0, meta = 0, tbn = 0;
-    uint32_t maxpad = 8, rmax = 0;
-    uint64_t lmax = 0;
-    for (uint32_t k = 0; k < N; k++) {
-        const uint64_t j = jobs[k];
-        const uint64_t *ro = in.read_off + in.job_off[j];
-        const uint32_t nr = uint32_t(in.job_off[j + 1] - in.job_off[j]);
-        caps[k] = job_caps(ro, nr);
-        gl[k] = graph_layout(caps[k], alpha);
-        nbases += caps[k].total;
-        noffs += nr + 1;
-        gbytes += gl[k].bytes;
-        rows += uint64_t(caps[k].nc) + 1;
-        preds += caps[k].ec;
-        seqb += caps[k].padm;
-        meta += caps[k].nc;
-        tbn += caps[k].maxm;
-        maxpad = std::max(maxpad, caps[k].padm);
-        rmax = std::max(rmax, nr);
-        if (nr) lmax = std::max<uint64_t>(lmax, ro[1] - ro[0]);
-    }
-    const uint64_t a_prm = 0;
-    const uint64_t a_mat = up256(a_prm + sizeof(RParams));
-    const uint64_t a_ctr = up256(a_mat + 4ull * alpha * alpha);
-    const uint64_t a_jobs = up256(a_ctr + 256);
-    const uint64_t a_dj = up256(a_jobs + sizeof(RJob) * N);
-    const uint64_t a_res = up256(a_dj + sizeof(DevJob) * N);
-    const uint64_t a_poff = up256(a_res + sizeof(DevResult) * N);
-    const uint64_t a_bases = up256(a_poff + 8ull * N);
-    const uint64_t a_roff = up256(a_bases + nbases);
-    const uint64_t a_graph = up256(a_roff + 8 * noffs);
-    const uint64_t a_end = up256(a_graph + gbytes);
-    const uint64_t b_rows = 0;
-    const uint64_t b_aux = up256(b_rows + sizeof(RowIn) * rows);
-    const uint64_t b_preds = up256(b_aux + sizeof(RowAux) * rows);
-    const uint64_t b_seq = up256(b_preds + sizeof(PredE) * std::max<uint64_t>(preds, 1));
-    const uint64_t b_meta = up256(b_seq + seqb + 16);
-    const uint64_t b_adapt = up256(b_meta + sizeof(RowMeta) * meta);
-    const uint64_t b_tb = up256(b_adapt + sizeof(RowRec) * meta);
-    const uint64_t b_end = up256(b_tb + 4 * tbn);
+    const BucketLayout BL = layout_bucket(p, in, jobs);
+    const std::vector<RCaps> &caps = BL.caps;
+    const std::vector<RGraphLayout> &gl = BL.gl;
+    const uint32_t maxpad = BL.maxpad, rmax = BL.rmax;
+    const uint64_t lmax = BL.lmax;
+    const uint64_t a_prm = BL.a_prm, a_mat = BL.a_mat, a_ctr = BL.a_ctr, a_jobs = BL.a_jobs, a_dj = BL.a_dj,
+                   a_res = BL.a_res, a_poff = BL.a_poff, a_bases = BL.a_bases, a_roff = BL.a_roff,
+                   a_graph = BL.a_graph, a_end = BL.a_end;
+    const uint64_t b_rows = BL.b_rows, b_aux = BL.b_aux, b_preds = BL.b_preds, b_seq = BL.b_seq,
+                   b_meta = BL.b_meta, b_adapt = BL.b_adapt, b_tb = BL.b_tb, b_end = BL.b_end;
     RJ_TRY(slot.dg.ensure(a_end));
     RJ_TRY(slot.dp.ensure(b_end));
     RJ_TRY(slot.da.ensure(S.arena_bytes));
@@ -577,6 +638,7 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
         to += c.maxm;
     }
     RJ_TRY(hipMemcpyAsync(dA, h, a_graph, hipMemcpyHostToDevice, s));
+    tr.t[1] = S.ms();
 
     RPools R{};
     R.jobs = reinterpret_cast<RJob *>(dA + a_jobs);
@@ -593,7 +655,11 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
     R.njobs = N;
     R.alloc_limit = S.arena_bytes / 4 * 3;
     R.job_cap = R.alloc_limit / 2;
-    R.lds_bytes = 65536;
+    // The step's LDS: the serial passes' windows and per-row arrays for a
+    // graph of about three times the longest template (what the batched
+    // workload's graphs reach), at most 64 KB; larger graphs use the jobs'
+    // global arrays. Smaller buckets leave LDS to the fills beside them.
+    R.lds_bytes = uint32_t(std::min<uint64_t>(65536, std::max<uint64_t>(16384, kWinBytes + 1024 + 36 * lmax)));
 
     DevPools P{};
     P.jobs = R.dj;
@@ -651,6 +717,7 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
     if (!slot.ev[1]) RJ_TRY(hipEventCreate(&slot.ev[1]));
     RJ_TRY(slot.hoff.ensure(4096));
     uint32_t *hdone = static_cast<uint32_t *>(slot.hoff.p);
+    tr.t[2] = S.ms();
     RJ_TRY(hipEventRecord(slot.ev[0], s));
     // Every read is one replay (the first read needs none of its own, the
     // last replay fuses the last read); retries and deferred DPs add more.
@@ -689,6 +756,7 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
     RJ_TRY(hipEventSynchronize(slot.ev[1]));
     float dev_ms = 0;
     RJ_TRY(hipEventElapsedTime(&dev_ms, slot.ev[0], slot.ev[1]));
+    tr.t[3] = S.ms();
 
     // Final states, then every job's graph packed into one copy.
     RJ_TRY(slot.hjobs.ensure(sizeof(RJob) * N));
@@ -710,6 +778,7 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
     RJ_TRY(hipMemcpyAsync(slot.hpack.p, slot.dk.p, pbytes, hipMemcpyDeviceToHost, s));
     RJ_TRY(hipStreamSynchronize(s));
 
+    tr.t[4] = S.ms();
     // Graphs, outputs and the jobs that continue on the host engine.
     const char *pk = static_cast<const char *>(slot.hpack.p);
     std::vector<std::unique_ptr<Graph>> graphs(N);
@@ -759,7 +828,14 @@ int run_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs) {
             graphs[k] = std::move(g);
         }
     });
+    tr.t[5] = S.ms();
+    tr.njobs = N;
+    tr.rmax = rmax;
+    tr.lmax = lmax;
+    tr.replays = replays;
+    tr.dev_ms = dev_ms;
     std::lock_guard<std::mutex> lk(S.mu);
+    S.trace.push_back(tr);
     ResidentStats &st = S.st;
     st.buckets += 1;
     st.replays += replays;
@@ -796,8 +872,8 @@ bool resident_eligible(const Params &p, const uint64_t *roff, uint32_t nreads) {
     return total + nreads + 2 < (uint64_t(1) << 30);
 }
 
-int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, const BatchInput &in,
-                 BatchOutput &out, EngineStats &est, ResidentStats &rst) {
+int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cache, const Params &p,
+                 const BatchInput &in, BatchOutput &out, EngineStats &est, ResidentStats &rst) {
     const uint64_t N = in.njobs;
     RShared S;
     S.p = &p;
@@ -819,26 +895,59 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, c
     if (const char *e = getenv("MPOA_RES_BUCKET")) bsize = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
     for (uint64_t b0 = 0; b0 < res_jobs.size(); b0 += bsize)
         S.buckets.emplace_back(res_jobs.begin() + b0, res_jobs.begin() + std::min<uint64_t>(res_jobs.size(), b0 + bsize));
-    // Arena per slot: half the free device memory over the slots, at most
-    // 24 GiB (MPOA_RES_ARENA_MB overrides; the squeeze knob of the tests
-    // makes DPs outgrow it and leave the device).
-    uint64_t free_b = device_free_bytes(slots.empty() ? 0 : slots[0]->device);
-    uint64_t arena = std::min<uint64_t>(free_b / 2 / std::max<size_t>(1, slots.size()), uint64_t(24) << 30);
+    // Arena per slot: half the device memory over the slots, at most 8 GiB
+    // (MPOA_RES_ARENA_MB overrides; the squeeze knob of the tests makes DPs
+    // outgrow it and leave the device). Slot buffers stay with the context
+    // and are sized for the largest bucket before any bucket starts.
+    uint64_t held = 0;
+    for (const auto &x : cache.slots) held += x->da.cap;
+    const uint64_t free_b = device_free_bytes(slots.empty() ? 0 : slots[0]->device) + held;
+    uint64_t arena = std::min<uint64_t>(free_b / 2 / std::max<size_t>(1, slots.size()), uint64_t(8) << 30);
     if (const char *e = getenv("MPOA_RES_ARENA_MB")) arena = strtoull(e, nullptr, 10) << 20;
     if (getenv("MPOA_ARENA_SQUEEZE")) arena = uint64_t(1) << 20;
     S.arena_bytes = std::max<uint64_t>(arena, uint64_t(1) << 20) / 256 * 256;
-    std::vector<std::unique_ptr<RSlot>> rs;
-    for (Ctx *c : slots) {
-        rs.push_back(std::make_unique<RSlot>());
-        rs.back()->ctx = c;
+    while (cache.slots.size() < slots.size()) cache.slots.push_back(std::make_unique<RSlot>());
+    uint64_t need_a = 0, need_b = 0, need_h = 0, need_pack = 0, need_n = 0;
+    for (const auto &bk : S.buckets) {
+        const BucketLayout BL = layout_bucket(p, in, bk);
+        need_a = std::max(need_a, BL.a_end);
+        need_b = std::max(need_b, BL.b_end);
+        need_h = std::max(need_h, BL.a_graph);
+        need_n = std::max<uint64_t>(need_n, bk.size());
+        uint64_t pk = 0;
+        for (const RCaps &c : BL.caps) pk += rj_pack_layout(c.nc, c.ec, c.nreads, uint32_t(c.total)).bytes;
+        need_pack = std::max(need_pack, pk);
+    }
+    std::vector<RSlot *> rs;
+    for (size_t k = 0; k < slots.size(); k++) {
+        RSlot &x = *cache.slots[k];
+        x.ctx = slots[k];
+        if (!S.buckets.empty()) {
+            hipError_t e = hipSetDevice(x.ctx->device);
+            if (e == hipSuccess) e = x.dg.ensure(need_a);
+            if (e == hipSuccess) e = x.dp.ensure(need_b);
+            if (e == hipSuccess) e = x.da.ensure(S.arena_bytes);
+            if (e == hipSuccess) e = x.dk.ensure(std::max<uint64_t>(need_pack, 256));
+            if (e == hipSuccess) e = x.hin.ensure(need_h);
+            if (e == hipSuccess) e = x.hjobs.ensure(sizeof(RJob) * need_n);
+            if (e == hipSuccess) e = x.hpack.ensure(std::max<uint64_t>(need_pack, 256));
+            if (e == hipSuccess) e = x.hoff.ensure(8 * need_n + 4096);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                set_error(std::string("HIP error: ") + hipGetErrorString(e) + " sizing the resident buffers");
+                return MPOA_ERR_ALLOC;
+            }
+        }
+        rs.push_back(&x);
     }
     const auto t0 = std::chrono::steady_clock::now();
+    S.t0 = t0;
     std::vector<std::thread> th;
     for (size_t si = 0; si < rs.size() && !S.buckets.empty(); si++)
         th.emplace_back([&, si] {
             try {
                 for (size_t b; (b = S.next++) < S.buckets.size();) {
-                    const int rc = run_bucket(S, *rs[si], S.buckets[b]);
+                    const int rc = run_bucket(S, *rs[si], si, S.buckets[b]);
                     if (rc != MPOA_OK) {
                         std::lock_guard<std::mutex> lk(S.mu);
                         if (S.rc == MPOA_OK) {
@@ -859,7 +968,16 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, c
             }
         });
     for (auto &t : th) t.join();
-    rs.clear();
+    if (const char *tp = getenv("MPOA_RES_TRACE")) {
+        if (FILE *f = fopen(tp, "w")) {
+            fprintf(f, "slot,jobs,rmax,lmax,replays,staged,captured,looped,packed,rendered,start,dev_ms\n");
+            for (const auto &x : S.trace)
+                fprintf(f, "%zu,%u,%u,%llu,%llu,%.2f,%.2f,%.2f,%.2f,%.2f,%.2f,%.2f\n", x.slot, x.njobs, x.rmax,
+                        (unsigned long long)x.lmax, (unsigned long long)x.replays, x.t[1], x.t[2], x.t[3], x.t[4],
+                        x.t[5], x.t[0], x.dev_ms);
+            fclose(f);
+        }
+    }
     if (S.rc != MPOA_OK) {
         set_error(S.err);
         return S.rc;

@@ -84,6 +84,7 @@ struct RJob {
     RSucc *succ;
     int32_t *head, *tail;
     uint32_t *support;
+    uint32_t *win;  // the serial passes' staging windows when LDS is not used
     // the job's regions of the launch pools (DevJob offsets)
     uint64_t rows_off, preds_off, seq_off, meta_off, adapt_off, tb_off;
     // state
@@ -433,9 +434,151 @@ MPOA_HD inline bool rj_scores_safe(const RParams &P, uint64_t rows, uint64_t qle
 }
 
 // head, tail, support: per-row arrays (LDS or global) of at least n entries.
+// The serial head and tail passes read each row's list through windows
+// staged in LDS by the whole team (kRowWin rows' list offsets, kEntWin list
+// entries), and keep the previous row's distance and support in registers
+// (its predecessor or successor is usually the adjacent row).
+constexpr uint32_t kRowWin = 512, kEntWin = 1024;
+struct RWin {
+    uint32_t *rb;  // kRowWin + 1 list offsets
+    uint32_t *er;  // kEntWin entry rows
+    uint32_t *ew;  // kEntWin entry weights
+};
+constexpr uint32_t kWinBytes = (kRowWin + 1 + 3) / 4 * 16 + 8 * kEntWin;
+
+// src/align.rs:623-650: the heaviest in-range predecessor, ties to the
+// larger support, then the first in `incoming` (the choose-order list keeps
+// equal weights in incoming order). False when a listed predecessor has no
+// head distance.
+template <class T>
+MPOA_HD inline bool rj_head_pass(T &t, uint32_t n, const RowIn *rows, const PredE *preds, int32_t *head,
+                                 uint32_t *support, const RWin &W) {
+    const uint32_t ntot = rows[n].pred_beg;
+    bool unreachable = false;
+    uint32_t r0 = 0, r1 = 0, e0 = 0, e1 = 0;
+    int32_t hprev = 0;
+    uint32_t sprev = 0;
+    head[0] = 0;
+    for (uint32_t i = 1; i < n; i++) {
+        if (i + 1 > r1) {
+            r0 = i;
+            r1 = i + kRowWin < n ? i + kRowWin : n;
+            t.sync();
+            team_for(t, r1 - r0 + 1, [&](uint32_t k) { W.rb[k] = rows[r0 + k].pred_beg; });
+            t.sync();
+        }
+        const uint32_t p0 = W.rb[i - r0], p1 = W.rb[i + 1 - r0];
+        const bool direct = p1 - p0 > kEntWin;
+        if (!direct && (p0 < e0 || p1 > e1)) {
+            e0 = p0;
+            e1 = p0 + kEntWin < ntot ? p0 + kEntWin : ntot;
+            t.sync();
+            team_for(t, e1 - e0, [&](uint32_t k) {
+                const PredE pe = preds[e0 + k];
+                W.er[k] = pe.row;
+                W.ew[k] = pe.weight;
+            });
+            t.sync();
+        }
+        bool have = false;
+        uint32_t bwt = 0, sb = 0;
+        int32_t hb = 0;
+        for (uint32_t a = p0; a < p1; a++) {
+            const uint32_t row = direct ? preds[a].row : W.er[a - e0];
+            const uint32_t w = direct ? preds[a].weight : W.ew[a - e0];
+            const int32_t h = row + 1 == i ? hprev : head[row];
+            if (h == kHeadNone) {
+                unreachable = true;
+                continue;
+            }
+            const uint32_t sp = row + 1 == i ? sprev : support[row];
+            if (!have || w > bwt || (w == bwt && sp > sb)) {
+                have = true;
+                hb = h;
+                sb = sp;
+                bwt = w;
+            }
+        }
+        if (have) {
+            hprev = hb + 1;
+            sprev = sb + bwt;
+            head[i] = hprev;
+            support[i] = sprev;
+        } else {
+            hprev = kHeadNone;
+            sprev = 0;
+        }
+    }
+    t.sync();
+    return !unreachable;
+}
+
+// src/align.rs:651-678: tail[n-1] = 1, then rows in reverse, the heaviest
+// in-range successor (out-list order) with a tail distance, ties to the
+// larger support.
+template <class T>
+MPOA_HD inline void rj_tail_pass(T &t, uint32_t n, const uint32_t *sbeg, const RSucc *succ, int32_t *tail,
+                                 uint32_t *support, const RWin &W) {
+    const uint32_t ntot = sbeg[n];
+    uint32_t r0 = n, r1 = n, e0 = 0, e1 = 0;
+    tail[n - 1] = 1;
+    int32_t tprev = 1;
+    uint32_t sprev = support[n - 1];
+    for (uint32_t ii = n - 1; ii-- > 0;) {
+        if (ii < r0) {
+            r1 = ii + 1;
+            r0 = r1 >= kRowWin ? r1 - kRowWin : 0;
+            t.sync();
+            team_for(t, r1 - r0 + 1, [&](uint32_t k) { W.rb[k] = sbeg[r0 + k]; });
+            t.sync();
+        }
+        const uint32_t s0 = W.rb[ii - r0], s1 = W.rb[ii + 1 - r0];
+        const bool direct = s1 - s0 > kEntWin;
+        if (!direct && (s0 < e0 || s1 > e1)) {
+            e1 = s1;
+            e0 = s1 >= kEntWin ? s1 - kEntWin : 0;
+            (void)ntot;
+            t.sync();
+            team_for(t, e1 - e0, [&](uint32_t k) {
+                const RSucc sc = succ[e0 + k];
+                W.er[k] = sc.row;
+                W.ew[k] = sc.w;
+            });
+            t.sync();
+        }
+        bool have = false;
+        uint32_t bwt = 0, sb = 0;
+        int32_t tb = 0;
+        for (uint32_t a = s0; a < s1; a++) {
+            const uint32_t row = direct ? succ[a].row : W.er[a - e0];
+            const uint32_t w = direct ? succ[a].w : W.ew[a - e0];
+            const int32_t tl = row == ii + 1 ? tprev : tail[row];
+            if (tl == kHeadNone) continue;
+            const uint32_t sp = row == ii + 1 ? sprev : support[row];
+            if (!have || w > bwt || (w == bwt && sp > sb)) {
+                have = true;
+                tb = tl;
+                sb = sp;
+                bwt = w;
+            }
+        }
+        if (have) {
+            tprev = tb + 1;
+            sprev = sb + bwt;
+            tail[ii] = tprev;
+            support[ii] = sprev;
+        } else {
+            tprev = kHeadNone;
+            sprev = support[ii];
+        }
+    }
+    t.sync();
+}
+
 template <class T>
 MPOA_HD inline int rj_plan(T &t, const RParams &P, RJob &J, const RPlanArrays &A, const uint8_t *q,
-                           uint64_t qlen, bool full, int32_t *head, int32_t *tail, uint32_t *support) {
+                           uint64_t qlen, bool full, int32_t *head, int32_t *tail, uint32_t *support,
+                           const RWin &W) {
     const uint32_t beg = J.rank[0], end = J.rank[1];
     if (end <= beg || end >= J.nn) return MPOA_ERR_ANCHORS_NOT_ORDERED;
     const uint64_t n64 = uint64_t(end) - beg + 1, m64 = qlen + 2;
@@ -541,69 +684,21 @@ MPOA_HD inline int rj_plan(T &t, const RParams &P, RJob &J, const RPlanArrays &A
         const uint32_t p0 = A.rows[i].pred_beg, p1 = A.rows[i + 1].pred_beg;
         for (uint32_t a = p0; a < p1; a++) t.umax(&A.rows[A.preds[a].row].last_use, i);
     });
-    // Head distances (src/align.rs:623-650), uniform on every lane: the
-    // heaviest edge, ties to the larger support, then the first in
-    // `incoming` (the choose-order list has equal weights in incoming order).
-    bool unreachable = false;
+    // Head distances (src/align.rs:623-650), then tail distances
+    // (src/align.rs:651-678) sharing `support`, uniform on every lane.
     team_for(t, n, [&](uint32_t i) {
         head[i] = kHeadNone;
         tail[i] = kHeadNone;
         support[i] = 0;
     });
     t.sync();
-    head[0] = 0;
-    // Every lane runs the pass on the same values and writes the same
-    // entries, so each lane reads back its own writes and no barrier is
-    // needed per row.
-    for (uint32_t i = 1; i < n; i++) {
-        const uint32_t p0 = A.rows[i].pred_beg, p1 = A.rows[i + 1].pred_beg;
-        bool have = false;
-        uint32_t bp = 0, bwt = 0;
-        for (uint32_t a = p0; a < p1; a++) {
-            const PredE pe = A.preds[a];
-            if (head[pe.row] == kHeadNone) {
-                unreachable = true;
-                continue;
-            }
-            if (!have || pe.weight > bwt || (pe.weight == bwt && support[pe.row] > support[bp])) {
-                have = true;
-                bp = pe.row;
-                bwt = pe.weight;
-            }
-        }
-        if (have) {
-            head[i] = head[bp] + 1;
-            support[i] = support[bp] + bwt;
-        }
-    }
-    t.sync();
-    if (unreachable) {
+    if (!rj_head_pass(t, n, A.rows, A.preds, head, support, W)) {
         // The reference drops such predecessors from every list; resident
         // graphs (built from reads) never have one.
         J.evict = kEvUnreachable;
         return -1;
     }
-    // Tail distances (src/align.rs:651-678), sharing `support`.
-    tail[n - 1] = 1;
-    for (uint32_t ii = n - 1; ii-- > 0;) {
-        const uint32_t s0 = J.succ_beg[ii], s1 = J.succ_beg[ii + 1];
-        bool have = false;
-        uint32_t bs = 0, bwt = 0;
-        for (uint32_t a = s0; a < s1; a++) {
-            const RSucc sc = J.succ[a];
-            if (tail[sc.row] == kHeadNone) continue;
-            if (!have || sc.w > bwt || (sc.w == bwt && support[sc.row] > support[bs])) {
-                have = true;
-                bs = sc.row;
-                bwt = sc.w;
-            }
-        }
-        if (have) {
-            tail[ii] = tail[bs] + 1;
-            support[ii] = support[bs] + bwt;
-        }
-    }
-    t.sync();
+    rj_tail_pass(t, n, J.succ_beg, J.succ, tail, support, W);
     if (head[n - 1] == kHeadNone) return MPOA_ERR_NO_PATH;
     // Rows, and the arena bytes of the values layout (M, D, I of every
     // cell; adaptive rows sized at their static width plus two bands).
@@ -763,13 +858,24 @@ MPOA_HD inline int rj_plan_read(T &t, const RPools &R, const RParams &P, RJob &J
     const uint8_t *q = J.bases + J.read_off[J.cur];
     const uint64_t qlen = J.read_off[J.cur + 1] - J.read_off[J.cur];
     const RPlanArrays A{R.rows + J.rows_off, R.aux + J.rows_off, R.preds + J.preds_off, R.seq + J.seq_off};
-    if (uint64_t(J.nn) * 12 + 64 <= lds_bytes) {
-        int32_t *hd = reinterpret_cast<int32_t *>(lds);
+    // LDS: the passes' windows, then head, tail and support when they fit;
+    // with lds_bytes below the windows (the CPU check's global form) every
+    // array is the job's own.
+    RWin W{J.win, J.win + kRowWin + 1, J.win + kRowWin + 1 + kEntWin};
+    uint64_t used = 0;
+    if (lds_bytes >= kWinBytes) {
+        W.rb = reinterpret_cast<uint32_t *>(lds);
+        W.er = reinterpret_cast<uint32_t *>(lds + (kRowWin + 1 + 3) / 4 * 16);
+        W.ew = W.er + kEntWin;
+        used = kWinBytes;
+    }
+    if (used > 0 && used + uint64_t(J.nn) * 12 + 64 <= lds_bytes) {
+        int32_t *hd = reinterpret_cast<int32_t *>(lds + used);
         int32_t *tl = hd + J.nn;
         uint32_t *sp = reinterpret_cast<uint32_t *>(tl + J.nn);
-        return rj_plan(t, P, J, A, q, qlen, full, hd, tl, sp);
+        return rj_plan(t, P, J, A, q, qlen, full, hd, tl, sp, W);
     }
-    return rj_plan(t, P, J, A, q, qlen, full, J.head, J.tail, J.support);
+    return rj_plan(t, P, J, A, q, qlen, full, J.head, J.tail, J.support, W);
 }
 
 template <class T>

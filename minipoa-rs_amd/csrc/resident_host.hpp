@@ -17,14 +17,21 @@ struct ResidentStats {
     double resident_ms = 0;      // wall time of the device-resident part
 };
 
+// Device blocks and host staging of resident batches, one set per launch
+// slot, kept with a context across runs.
+struct ResidentCache;
+ResidentCache *resident_cache_new();
+void resident_cache_free(ResidentCache *c);
+uint64_t resident_cache_bytes(const ResidentCache *c);
+
 // Whether a job runs device-resident from its first read.
 bool resident_eligible(const Params &p, const uint64_t *roff, uint32_t nreads);
 
 // The batch of `in` with every eligible job device-resident (length
 // buckets, captured step loops on the slots' streams); the rest, and jobs
 // that leave the device, on the host-driven engine over the same slots.
-int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, const Params &p, const BatchInput &in,
-                 BatchOutput &out, EngineStats &est, ResidentStats &rst);
+int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cache, const Params &p,
+                 const BatchInput &in, BatchOutput &out, EngineStats &est, ResidentStats &rst);
 
 // Applies the reads' paths (path r: paths[path_off[r] .. path_off[r+1]])
 // to a host Graph and to the resident step's code on one host thread, and
