@@ -397,6 +397,15 @@ typedef struct {
     uint32_t slots;       /* launch slots, streams of their own (0 = 4: two take DPs of
                              6,000 rows or more first, two take only shorter DPs) */
     int32_t render;       /* nonzero: render the -r0 and -r1 text of each finished job */
+    int32_t resident;     /* nonzero: serve on device-resident graphs. A job is admitted into a
+                             slot of a step table of its size class (longest read up to 2,000,
+                             8,000 or 31,998 bases) once the table's device heap holds its graph,
+                             reads and DP descriptors; they stay on the device until it finishes,
+                             and the table's captured step graph (fusion, re-rank and planning on
+                             the device, then fill and traceback) advances every admitted job one
+                             read per replay, new jobs joining between replays. Jobs outside the
+                             resident path (seeding, -p, the i64 kernel) run host-driven. Outputs
+                             are the same either way. */
 } mpoa_server_opts;
 typedef struct {
     double submit_ms, admit_ms, done_ms; /* on the server's clock (ms since creation) */

@@ -44,7 +44,15 @@ struct mpoa_records {
 struct mpoa_server {
     mpoa_ctx *ctx = nullptr;
     Params params;
-    std::unique_ptr<Engine> engine;
+    std::unique_ptr<Engine> engine;          // host-driven serving
+    std::unique_ptr<ResidentServer> res;     // serving on device-resident graphs
+    uint64_t submitted() { return res ? res->submitted() : engine->submitted(); }
+    bool done(uint64_t t) { return res ? res->done(t) : engine->done(t); }
+    bool wait(uint64_t t) { return res ? res->wait(t) : engine->wait(t); }
+    int error() { return res ? res->error() : engine->error(); }
+    JobOut &out(uint64_t t) { return res ? res->out(t) : engine->out(t); }
+    double now_ms() { return res ? res->now_ms() : engine->now_ms(); }
+    uint64_t submit(JobIn in) { return res ? res->submit(std::move(in)) : engine->submit(std::move(in)); }
 };
 struct mpoa_batch {
     std::vector<int> status;
@@ -929,7 +937,18 @@ int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_serv
     auto S = std::make_unique<mpoa_server>();
     S->ctx = ctx;
     S->params = p;
-    S->engine = std::make_unique<Engine>(slots, *ctx->tasks, p, o);
+    if (opts && opts->resident) {
+        S->res = std::make_unique<ResidentServer>(slots, *ctx->tasks, p, o);
+        if (S->res->error() != MPOA_OK) {
+            const int e = S->res->error();
+            const std::string msg = last_error();
+            S->res.reset();
+            release_engine(ctx);
+            return fail(e, msg);
+        }
+    } else {
+        S->engine = std::make_unique<Engine>(slots, *ctx->tasks, p, o);
+    }
     *out = S.release();
     return MPOA_OK;
 } catch (...) { return exception_status(); }
@@ -953,25 +972,25 @@ int mpoa_server_submit(mpoa_server *s, const uint8_t *bases, const uint64_t *rea
     const uint64_t L = nreads ? ji.own_off[1] : 0;
     ji.work = L * nreads * nreads;
     ji.guide = s->params.progressive && existing == 0;
-    *ticket = s->engine->submit(std::move(ji));
+    *ticket = s->submit(std::move(ji));
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 
 int mpoa_server_done(mpoa_server *s, uint64_t ticket) try {
-    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
-    return s->engine->done(ticket) ? 1 : 0;
+    if (!s || ticket >= s->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    return s->done(ticket) ? 1 : 0;
 } catch (...) { return exception_status(); }
 
 int mpoa_server_wait(mpoa_server *s, uint64_t ticket) try {
-    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
-    if (!s->engine->wait(ticket)) return fail(s->engine->error(), "the engine stopped: " + std::string(last_error()));
+    if (!s || ticket >= s->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->wait(ticket)) return fail(s->error(), "the engine stopped: " + std::string(last_error()));
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 
 int mpoa_server_job_status(mpoa_server *s, uint64_t ticket) try {
-    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
-    if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
-    const JobOut &o = s->engine->out(ticket);
+    if (!s || ticket >= s->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
+    const JobOut &o = s->out(ticket);
     if (o.status != MPOA_OK) set_error(o.err);
     return o.status;
 } catch (...) { return exception_status(); }
@@ -980,7 +999,7 @@ int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out,
                            uint64_t *out_len) try {
     const int st = mpoa_server_job_status(s, ticket);
     if (st != MPOA_OK) return st;
-    JobOut &o = s->engine->out(ticket);
+    JobOut &o = s->out(ticket);
     if (mode == 0) return emit_text(o.out0, out, cap, out_len);
     if (mode == 1) return emit_text(o.out1, out, cap, out_len);
     if (mode == 2) {
@@ -994,9 +1013,9 @@ int mpoa_server_job_output(mpoa_server *s, uint64_t ticket, int mode, char *out,
 
 int mpoa_server_job_times(mpoa_server *s, uint64_t ticket, mpoa_job_times *out) try {
     if (!out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (!s || ticket >= s->engine->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
-    if (!s->engine->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
-    const JobOut &o = s->engine->out(ticket);
+    if (!s || ticket >= s->submitted()) return fail(MPOA_ERR_INVALID_ARGUMENT, "unknown ticket");
+    if (!s->done(ticket)) return fail(MPOA_ERR_PENDING, "the job has not finished");
+    const JobOut &o = s->out(ticket);
     out->submit_ms = o.t_submit;
     out->admit_ms = o.t_admit;
     out->done_ms = o.t_done;
@@ -1010,7 +1029,7 @@ int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) try
     const int st = mpoa_server_job_status(s, ticket);
     if (st != MPOA_OK) return st;
     auto *g = new mpoa_graph(s->params);
-    g->g = *s->engine->out(ticket).graph;
+    g->g = *s->out(ticket).graph;
     *out = g;
     return MPOA_OK;
 } catch (...) { return exception_status(); }
@@ -1018,17 +1037,28 @@ int mpoa_server_job_graph(mpoa_server *s, uint64_t ticket, mpoa_graph **out) try
 int mpoa_server_stats_get(mpoa_server *s, mpoa_batch_stats *out) try {
     if (!s || !out) return fail(MPOA_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = mpoa_batch_stats{};
-    fill_stats(s->engine->stats(), *out);
-    out->wall_ms = s->engine->now_ms();
+    if (s->res) {
+        ResidentStats rs;
+        fill_stats(s->res->stats(&rs), *out);
+        out->resident_jobs = rs.resident_done;
+        out->evicted_jobs = rs.evicted;
+        out->step_replays = rs.replays;
+        out->buckets = rs.buckets;
+        out->deferred = rs.deferred;
+    } else {
+        fill_stats(s->engine->stats(), *out);
+    }
+    out->wall_ms = s->now_ms();
     return MPOA_OK;
 } catch (...) { return exception_status(); }
 
-double mpoa_server_now_ms(mpoa_server *s) { return s ? s->engine->now_ms() : 0.0; }
+double mpoa_server_now_ms(mpoa_server *s) { return s ? s->now_ms() : 0.0; }
 
 void mpoa_server_free(mpoa_server *s) {
     if (!s) return;
     mpoa_ctx *ctx = s->ctx;
-    s->engine.reset();  // finishes every submitted job first
+    s->res.reset();     // finishes every submitted job first
+    s->engine.reset();
     delete s;
     release_engine(ctx);
 }

@@ -97,7 +97,8 @@ struct RowMeta {
 };
 
 struct DevJob {
-    uint64_t rows;      // RowIn/RowAux index of row 0
+    uint64_t rows;      // RowIn index of row 0
+    uint64_t aux;       // RowAux index of row 0
     uint64_t preds;     // PredE index of the job's first entry
     uint64_t seq;       // byte offset of the encoded, padded query
     uint64_t matrix;    // int32 index of the alpha x alpha matrix

@@ -496,7 +496,8 @@ mpoa_graph_view Graph::view() {
 void Graph::assign_packed(uint32_t nn, const uint8_t *base, const uint32_t *parent, const uint32_t *rank,
                           const uint32_t *cover, const uint32_t *out_head, const uint32_t *in_head,
                           const uint32_t *edges5, const uint32_t *path_off, const uint32_t *paths,
-                          uint32_t nread, uint32_t first) {
+                          uint32_t nread, uint32_t first,
+                          const std::vector<std::pair<uint64_t, std::vector<uint32_t>>> *prior) {
     constexpr uint32_t nil = 0xFFFFFFFFu;
     nodes_.clear();
     nodes_.resize(nn);
@@ -519,6 +520,7 @@ void Graph::assign_packed(uint32_t nn, const uint8_t *base, const uint32_t *pare
         if (rank[id] < nn) rank_[rank[id]] = id;
     }
     paths_.clear();
+    if (prior) paths_ = *prior;
     for (uint32_t k = 0; k < nread; k++)
         paths_.emplace_back(uint64_t(first) + k,
                             std::vector<uint32_t>(paths + path_off[k], paths + path_off[k + 1]));

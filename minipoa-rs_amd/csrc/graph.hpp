@@ -135,7 +135,8 @@ public:
     // Flat node-indexed view (mpoa_graph_view) rebuilt on demand.
     mpoa_graph_view view();
 
-    // Replaces the graph with a device-resident job's packed arrays: per node
+    // Replaces the graph with a device-resident job's packed arrays (prior:
+    // paths of sequences the graph held before, kept first): per node
     // its base, parent, rank, coverage and the first records of its out- and
     // in-list; edge records of five words (from, to, weight, next out, next
     // in; 0xFFFFFFFF ends a list) linking each list in insertion order; and
@@ -147,7 +148,8 @@ public:
     void assign_packed(uint32_t nn, const uint8_t *base, const uint32_t *parent, const uint32_t *rank,
                        const uint32_t *cover, const uint32_t *out_head, const uint32_t *in_head,
                        const uint32_t *edges5, const uint32_t *path_off, const uint32_t *paths,
-                       uint32_t nread, uint32_t first);
+                       uint32_t nread, uint32_t first,
+                       const std::vector<std::pair<uint64_t, std::vector<uint32_t>>> *prior = nullptr);
 
 private:
     size_t alpha_;

@@ -384,7 +384,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
     const bool leader = tid == 0;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
-    const RowAux *aux = P.aux + J.rows;
+    const RowAux *aux = P.aux + J.aux;
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const int32_t *gmat = P.matrix + J.matrix;
@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
     const int lane = threadIdx.x;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
-    const RowAux *aux = P.aux + J.rows;
+    const RowAux *aux = P.aux + J.aux;
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const uint8_t *arena = static_cast<const uint8_t *>(P.arena);
@@ -1422,7 +1422,7 @@ __global__ __launch_bounds__(64) void poa_traceback_values(DevPools P, uint32_t 
     const int lane = threadIdx.x;
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
-    const RowAux *aux = P.aux + J.rows;
+    const RowAux *aux = P.aux + J.aux;
     const PredE *pr = P.preds + J.preds;
     const uint8_t *seq = P.seq + J.seq;
     const int32_t *mat = P.matrix + J.matrix;

@@ -30,45 +30,9 @@ namespace mpoa {
 
 void launch_rj_step(const RPools &R, hipStream_t s);
 void launch_rj_reset(unsigned long long *ctr, hipStream_t s);
-void launch_rj_pack(const RJob *jobs, uint32_t njobs, const uint64_t *off, uint8_t *out, hipStream_t s);
+void launch_rj_pack(const RJob *jobs, const uint32_t *sel, uint32_t njobs, const uint64_t *off, uint8_t *out,
+                    hipStream_t s);
 
-namespace {
-
-// The team of resident.hpp on one host thread: every team loop is the
-// serial loop, every scan the running sum.
-struct SoloTeam {
-    static constexpr uint32_t lane = 0, size = 1;
-    void sync() {}
-    uint32_t excl_scan(uint32_t v, uint32_t &tot) {
-        tot = v;
-        return 0;
-    }
-    uint64_t reduce_add64(uint64_t v) { return v; }
-    uint32_t reduce_max(uint32_t v) { return v; }
-    bool any(bool p) { return p; }
-    uint32_t xchg(uint32_t *p, uint32_t v) {
-        const uint32_t o = *p;
-        *p = v;
-        return o;
-    }
-    int32_t cas(int32_t *p, int32_t c, int32_t v) {
-        const int32_t o = *p;
-        if (o == c) *p = v;
-        return o;
-    }
-    unsigned long long cas64(unsigned long long *p, unsigned long long c, unsigned long long v) {
-        const unsigned long long o = *p;
-        if (o == c) *p = v;
-        return o;
-    }
-    void add(uint32_t *p, uint32_t v) { *p += v; }
-    void umax(uint32_t *p, uint32_t v) {
-        if (v > *p) *p = v;
-    }
-};
-
-inline uint64_t up16(uint64_t x) { return (x + 15) / 16 * 16; }
-inline uint64_t up256(uint64_t x) { return (x + 255) / 256 * 256; }
 
 RParams make_rparams(const Params &p, const int32_t *matrix) {
     RParams r{};
@@ -91,12 +55,6 @@ RParams make_rparams(const Params &p, const int32_t *matrix) {
     return r;
 }
 
-// Capacities of one job (resident.hpp RJob).
-struct RCaps {
-    uint32_t nreads = 0, nc = 0, ec = 0, maxm = 0, padm = 0;
-    uint64_t total = 0;  // bases over all reads
-};
-
 RCaps job_caps(const uint64_t *roff, uint32_t nreads) {
     RCaps c;
     c.nreads = nreads;
@@ -112,12 +70,6 @@ RCaps job_caps(const uint64_t *roff, uint32_t nreads) {
     c.padm = (c.maxm / 8 + 1) * 8;
     return c;
 }
-
-// Byte offsets of one job's graph arrays inside a block (each 16-aligned).
-struct RGraphLayout {
-    uint64_t base, parent, rank, order, aligned, out_head, out_tail, in_head, in_tail, in_cnt, cover, mark,
-        edges, paths, path_off, csr_off, csr_adj, deg, stack, succ_beg, succ, head, tail, support, win, bytes;
-};
 
 RGraphLayout graph_layout(const RCaps &c, uint32_t alpha) {
     RGraphLayout L{};
@@ -204,6 +156,41 @@ int failure_of(const RJob &J, std::string &msg) {
     }
     return J.err;
 }
+
+namespace {
+
+// The team of resident.hpp on one host thread: every team loop is the
+// serial loop, every scan the running sum.
+struct SoloTeam {
+    static constexpr uint32_t lane = 0, size = 1;
+    void sync() {}
+    uint32_t excl_scan(uint32_t v, uint32_t &tot) {
+        tot = v;
+        return 0;
+    }
+    uint64_t reduce_add64(uint64_t v) { return v; }
+    uint32_t reduce_max(uint32_t v) { return v; }
+    bool any(bool p) { return p; }
+    uint32_t xchg(uint32_t *p, uint32_t v) {
+        const uint32_t o = *p;
+        *p = v;
+        return o;
+    }
+    int32_t cas(int32_t *p, int32_t c, int32_t v) {
+        const int32_t o = *p;
+        if (o == c) *p = v;
+        return o;
+    }
+    unsigned long long cas64(unsigned long long *p, unsigned long long c, unsigned long long v) {
+        const unsigned long long o = *p;
+        if (o == c) *p = v;
+        return o;
+    }
+    void add(uint32_t *p, uint32_t v) { *p += v; }
+    void umax(uint32_t *p, uint32_t v) {
+        if (v > *p) *p = v;
+    }
+};
 
 // ---------------------------------------------------------------- CPU check
 
@@ -619,6 +606,7 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         J.maxm = c.maxm;
         bind_graph(J, dA + a_graph + go, gl[k]);
         J.rows_off = ro_;
+        J.aux_off = ro_;
         J.preds_off = po;
         J.seq_off = so;
         J.meta_off = mo;
@@ -773,7 +761,7 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     RJ_TRY(slot.dk.ensure(std::max<uint64_t>(pbytes, 256)));
     RJ_TRY(slot.hpack.ensure(std::max<uint64_t>(pbytes, 256)));
     RJ_TRY(hipMemcpyAsync(dA + a_poff, hoff, 8ull * N, hipMemcpyHostToDevice, s));
-    launch_rj_pack(R.jobs, N, reinterpret_cast<const uint64_t *>(dA + a_poff), static_cast<uint8_t *>(slot.dk.p), s);
+    launch_rj_pack(R.jobs, nullptr, N, reinterpret_cast<const uint64_t *>(dA + a_poff), static_cast<uint8_t *>(slot.dk.p), s);
     RJ_TRY(hipGetLastError());
     RJ_TRY(hipMemcpyAsync(slot.hpack.p, slot.dk.p, pbytes, hipMemcpyDeviceToHost, s));
     RJ_TRY(hipStreamSynchronize(s));

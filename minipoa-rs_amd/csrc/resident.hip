@@ -252,11 +252,12 @@ __global__ void rj_reset_kernel(unsigned long long *ctr) {
 }
 
 // Job k's graph into out + off[k] (layout rj_pack_layout), one wave per job.
-__global__ __launch_bounds__(64) void rj_pack_kernel(const RJob *jobs, uint32_t njobs, const uint64_t *off,
-                                                     uint8_t *out) {
+// sel: the jobs to pack (k-th packed job jobs[sel[k]]), or null for all.
+__global__ __launch_bounds__(64) void rj_pack_kernel(const RJob *jobs, const uint32_t *sel, uint32_t njobs,
+                                                     const uint64_t *off, uint8_t *out) {
     const uint32_t k = blockIdx.x;
     if (k >= njobs) return;
-    const RJob &J = jobs[k];
+    const RJob &J = jobs[sel ? sel[k] : k];
     const uint32_t nn = J.nn, ne = J.ne, nr = J.nreads, np = J.npath;
     const RPackLayout L = rj_pack_layout(nn, ne, nr, np);
     uint8_t *o = out + off[k];
@@ -276,9 +277,10 @@ __global__ __launch_bounds__(64) void rj_pack_kernel(const RJob *jobs, uint32_t 
 
 }  // namespace
 
-void launch_rj_pack(const RJob *jobs, uint32_t njobs, const uint64_t *off, uint8_t *out, hipStream_t s) {
+void launch_rj_pack(const RJob *jobs, const uint32_t *sel, uint32_t njobs, const uint64_t *off, uint8_t *out,
+                    hipStream_t s) {
     if (njobs == 0) return;
-    hipLaunchKernelGGL(rj_pack_kernel, dim3(njobs), dim3(64), 0, s, jobs, njobs, off, out);
+    hipLaunchKernelGGL(rj_pack_kernel, dim3(njobs), dim3(64), 0, s, jobs, sel, njobs, off, out);
 }
 
 void launch_rj_step(const RPools &R, hipStream_t s) {

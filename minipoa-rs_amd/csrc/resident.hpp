@@ -86,7 +86,7 @@ struct RJob {
     uint32_t *support;
     uint32_t *win;  // the serial passes' staging windows when LDS is not used
     // the job's regions of the launch pools (DevJob offsets)
-    uint64_t rows_off, preds_off, seq_off, meta_off, adapt_off, tb_off;
+    uint64_t rows_off, aux_off, preds_off, seq_off, meta_off, adapt_off, tb_off;
     // state
     uint32_t nn, ne, cur, phase, state, retry, full_arena, epoch;
     int32_t err;       // MPOA status of a failed job
@@ -747,6 +747,7 @@ MPOA_HD inline int rj_plan(T &t, const RParams &P, RJob &J, const RPlanArrays &A
         DevJob &d = J.dj;
         d = DevJob{};
         d.rows = J.rows_off;
+        d.aux = J.aux_off;
         d.preds = J.preds_off;
         d.seq = J.seq_off;
         d.matrix = 0;
@@ -857,7 +858,7 @@ MPOA_HD inline int rj_plan_read(T &t, const RPools &R, const RParams &P, RJob &J
                                 uint32_t lds_bytes) {
     const uint8_t *q = J.bases + J.read_off[J.cur];
     const uint64_t qlen = J.read_off[J.cur + 1] - J.read_off[J.cur];
-    const RPlanArrays A{R.rows + J.rows_off, R.aux + J.rows_off, R.preds + J.preds_off, R.seq + J.seq_off};
+    const RPlanArrays A{R.rows + J.rows_off, R.aux + J.aux_off, R.preds + J.preds_off, R.seq + J.seq_off};
     // LDS: the passes' windows, then head, tail and support when they fit;
     // with lds_bytes below the windows (the CPU check's global form) every
     // array is the job's own.

@@ -1,9 +1,11 @@
 // Host entry points of the device-resident batch (resident.cpp).
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "engine.hpp"
+#include "resident.hpp"
 
 namespace mpoa {
 
@@ -16,6 +18,36 @@ struct ResidentStats {
     double device_ms = 0;        // step loops, summed over buckets (HIP events)
     double resident_ms = 0;      // wall time of the device-resident part
 };
+
+// Helpers shared by the resident batch and server (resident.cpp).
+inline uint64_t up16(uint64_t x) { return (x + 15) / 16 * 16; }
+inline uint64_t up256(uint64_t x) { return (x + 255) / 256 * 256; }
+
+// Scoring parameters of the step kernel; matrix: the matrix's address there.
+RParams make_rparams(const Params &p, const int32_t *matrix);
+
+// Capacities of one job: reads, nodes (2 + bases), edges (bases + reads +
+// 1), query columns (longest read + 2), padded query bytes.
+struct RCaps {
+    uint32_t nreads = 0, nc = 0, ec = 0, maxm = 0, padm = 0;
+    uint64_t total = 0;  // bases over all reads
+};
+RCaps job_caps(const uint64_t *roff, uint32_t nreads);
+
+// Byte offsets of one job's graph arrays inside a block (each 16-aligned).
+struct RGraphLayout {
+    uint64_t base, parent, rank, order, aligned, out_head, out_tail, in_head, in_tail, in_cnt, cover, mark,
+        edges, paths, path_off, csr_off, csr_adj, deg, stack, succ_beg, succ, head, tail, support, win, bytes;
+};
+RGraphLayout graph_layout(const RCaps &c, uint32_t alpha);
+void bind_graph(RJob &J, char *blk, const RGraphLayout &L);
+// Status and the reference's message of a failed resident job.
+int failure_of(const RJob &J, std::string &msg);
+
+void launch_rj_step(const RPools &R, hipStream_t s);
+void launch_rj_reset(unsigned long long *ctr, hipStream_t s);
+void launch_rj_pack(const RJob *jobs, const uint32_t *sel, uint32_t njobs, const uint64_t *off, uint8_t *out,
+                    hipStream_t s);
 
 // Device blocks and host staging of resident batches, one set per launch
 // slot, kept with a context across runs.
@@ -32,6 +64,31 @@ bool resident_eligible(const Params &p, const uint64_t *roff, uint32_t nreads);
 // that leave the device, on the host-driven engine over the same slots.
 int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cache, const Params &p,
                  const BatchInput &in, BatchOutput &out, EngineStats &est, ResidentStats &rst);
+
+// Serving on device-resident graphs (resident_server.cpp): the Engine's
+// submit / done / wait / out surface over size-class step tables whose slots
+// hold admitted jobs' graphs and DP descriptors in device memory between
+// reads; jobs outside the resident path, and jobs that leave it, run on a
+// host-driven Engine over `slots`.
+class ResidentServer {
+public:
+    ResidentServer(std::vector<Ctx *> slots, TaskQueue &tasks, const Params &p, const EngineOpts &o);
+    ~ResidentServer();  // finishes every submitted job
+    ResidentServer(const ResidentServer &) = delete;
+    ResidentServer &operator=(const ResidentServer &) = delete;
+    uint64_t submit(JobIn in);
+    bool done(uint64_t ticket);
+    bool wait(uint64_t ticket);
+    JobOut &out(uint64_t ticket);  // valid once done(ticket)
+    uint64_t submitted();
+    double now_ms() const;
+    EngineStats stats(ResidentStats *rs = nullptr);
+    int error();
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> d_;
+};
 
 // Applies the reads' paths (path r: paths[path_off[r] .. path_off[r+1]])
 // to a host Graph and to the resident step's code on one host thread, and

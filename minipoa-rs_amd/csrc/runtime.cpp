@@ -392,6 +392,7 @@ int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans,
         const Off &o = off[k];
         DevJob d{};
         d.rows = o.rows;
+        d.aux = o.rows;
         d.preds = o.preds;
         d.seq = o.seq;
         d.matrix = 0;

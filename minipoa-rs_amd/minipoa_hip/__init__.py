@@ -97,7 +97,8 @@ class _Stats(C.Structure):
 
 
 class _ServerOpts(C.Structure):
-    _fields_ = [("admit_bytes", C.c_uint64), ("slots", C.c_uint32), ("render", C.c_int32)]
+    _fields_ = [("admit_bytes", C.c_uint64), ("slots", C.c_uint32), ("render", C.c_int32),
+                ("resident", C.c_int32)]
 
 
 class _JobTimes(C.Structure):
@@ -759,12 +760,16 @@ class Server:
     submit() copies a job's reads and returns a ticket at once; the job runs
     in the continuous engine alongside whatever else is admitted. A job may
     extend a graph that already holds `existing` sequences (the -i mode of
-    src/lib.rs:212-226, which continues with `extend_graph`)."""
+    src/lib.rs:212-226, which continues with `extend_graph`).
 
-    def __init__(self, ctx, params, admit_bytes=0, slots=0, render=True):
+    resident: keep every admitted job's graph and DP descriptors on the
+    device between reads, in slots of size-class step tables whose captured
+    step graphs advance all admitted jobs one read per replay."""
+
+    def __init__(self, ctx, params, admit_bytes=0, slots=0, render=True, resident=False):
         self._cp = CParams(params)
         self.params = params
-        o = _ServerOpts(admit_bytes, slots, 1 if render else 0)
+        o = _ServerOpts(admit_bytes, slots, 1 if render else 0, 1 if resident else 0)
         h = C.c_void_p()
         _check(lib().mpoa_server_create(ctx._h, self._cp.ref, C.byref(o), C.byref(h)))
         self._h = h
