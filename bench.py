@@ -586,28 +586,33 @@ def leg_c5(ctx, oracle, cores, njobs, rate, slots=0):
             pre.wait(t)
             text = pre.graph(t).gfa(p, records(stream[k]["base"]))
             bases_of[k], recs_of[k] = mh.read_gfa(p, text)
-    srv = mh.Server(ctx, p, slots=slots)
-    tickets = []
-    t0 = time.perf_counter()
-    for k, j in enumerate(stream):
-        delay = j["t"] - (time.perf_counter() - t0)
-        if delay > 0:
-            time.sleep(delay)
-        g = bases_of.get(k)
-        tickets.append(srv.submit(j["reads"], graph=g, existing=len(recs_of.get(k, []))))
-    for t in tickets:
-        srv.wait(t)
-    wall = time.perf_counter() - t0
-    times = [srv.times(t) for t in tickets]
-    lat = np.array([x["done_ms"] - x["submit_ms"] for x in times])
-    qwait = np.array([x["admit_ms"] - x["submit_ms"] for x in times])
-    first = min(x["submit_ms"] for x in times)
-    span = (max(x["done_ms"] for x in times) - first) / 1e3
-    cells = int(sum(x["cells"] for x in times))
-    failed = sum(srv.status(t) != 0 for t in tickets)
-    got = hashes_of(lambda k: srv.output(tickets[k], 0) + srv.output(tickets[k], 1), len(tickets))
-    st = srv.stats()
-    srv.close()
+    def run_stream(**kw):
+        srv = mh.Server(ctx, p, slots=slots, **kw)
+        tickets = []
+        t0 = time.perf_counter()
+        for k, j in enumerate(stream):
+            delay = j["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            g = bases_of.get(k)
+            tickets.append(srv.submit(j["reads"], graph=g, existing=len(recs_of.get(k, []))))
+        for t in tickets:
+            srv.wait(t)
+        wall = time.perf_counter() - t0
+        times = [srv.times(t) for t in tickets]
+        lat = np.array([x["done_ms"] - x["submit_ms"] for x in times])
+        qwait = np.array([x["admit_ms"] - x["submit_ms"] for x in times])
+        first = min(x["submit_ms"] for x in times)
+        span = (max(x["done_ms"] for x in times) - first) / 1e3
+        cells = int(sum(x["cells"] for x in times))
+        failed = sum(srv.status(t) != 0 for t in tickets)
+        got = hashes_of(lambda k: srv.output(tickets[k], 0) + srv.output(tickets[k], 1), len(tickets))
+        st = srv.stats()
+        srv.close()
+        return lat, qwait, span, cells, failed, got, st, wall
+
+    lat, qwait, span, cells, failed, got, st, wall = run_stream()
+    rlat, rqwait, rspan, rcells, rfailed, rgot, rst, rwall = run_stream(resident=True)
     # Oracle: ordinary jobs from scratch (job-parallel); incremental jobs
     # continue a field-by-field copy of the graph read_gfa loaded.
     plain = [k for k, j in enumerate(stream) if not j["base"]]
@@ -641,7 +646,11 @@ def leg_c5(ctx, oracle, cores, njobs, rate, slots=0):
             want[k] = f.result()
     ct = time.perf_counter() - t
     exact = int((got == want).sum())
+    rexact = int((rgot == want).sum())
     pct = lambda a, q: round(float(np.percentile(a, q)), 1)
+    log(f"c5 resident: wall {rwall:.1f}s, p50 {pct(rlat, 50)} ms p99 {pct(rlat, 99)} ms, "
+        f"{rcells/rspan/1e9:.2f} GCUPS, bit-exact {rexact}/{len(stream)}, resident {rst['resident_jobs']} "
+        f"evicted {rst['evicted_jobs']}")
     log(f"c5: {len(stream)} jobs over {stream[-1]['t']:.1f}s of arrivals, wall {wall:.1f}s, "
         f"p50 {pct(lat, 50)} ms p99 {pct(lat, 99)} ms, {cells/span/1e9:.2f} GCUPS, bit-exact {exact}/{len(stream)} "
         f"(oracle {ct:.1f}s)")
@@ -655,7 +664,15 @@ def leg_c5(ctx, oracle, cores, njobs, rate, slots=0):
             "incremental_jobs": len(bases_of),
             "bit_exact": {"jobs": len(stream), "identical": exact,
                           "what": "FNV-1a of consensus FASTA + MSA per job, served vs oracle (from scratch, "
-                                  "or continuing the GFA-loaded graph)"}}
+                                  "or continuing the GFA-loaded graph)"},
+            # The same stream served on device-resident graphs (size-class step
+            # tables, captured step graphs; mpoa_server_opts.resident).
+            "resident": {"p50_ms": pct(rlat, 50), "p99_ms": pct(rlat, 99), "max_ms": round(float(rlat.max()), 1),
+                         "admission_wait_p99_ms": pct(rqwait, 99), "jobs_per_s": round(len(stream) / rspan, 2),
+                         "gcups": round(rcells / rspan / 1e9, 3), "span_s": round(rspan, 2),
+                         "resident_jobs": rst["resident_jobs"], "evicted_jobs": rst["evicted_jobs"],
+                         "step_replays": rst["step_replays"], "failed_jobs": int(rfailed),
+                         "bit_exact": {"jobs": len(stream), "identical": rexact}}}
 
 
 if __name__ == "__main__":
