@@ -166,6 +166,8 @@ def main():
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (counter scaling)")
+    ap.add_argument("--resident", action="store_true",
+                    help="run the headline batch device-resident (mpoa_batch_run_resident)")
     ap.add_argument("--hw-queues", type=int, default=None,
                     help="hardware queues per process (read before HIP starts; default 8)")
     ap.add_argument("--no-valu-probe", dest="valu_probe", action="store_false",
@@ -198,7 +200,7 @@ def main():
 
     for w in range(args.warmup):
         t = time.time()
-        r = mh.batch_run(ctx, params, job_off, read_off, bases)
+        r = mh.batch_run(ctx, params, job_off, read_off, bases, resident=args.resident)
         log(f"warmup {w}: {time.time() - t:.2f}s")
         del r
 
@@ -206,7 +208,7 @@ def main():
     for s in range(args.steps):
         res = None  # free the previous step's graphs outside the timed region
         t = time.perf_counter()
-        res = mh.batch_run(ctx, params, job_off, read_off, bases)
+        res = mh.batch_run(ctx, params, job_off, read_off, bases, resident=args.resident)
         dt = time.perf_counter() - t
         times.append(dt)
         stats.append(res.stats())
@@ -236,7 +238,12 @@ def main():
     # the values layout, which the headline runs in).
     alg_bytes = st["arena_bytes"] or BYTES_PER_CELL * cells
     int_ops = 6 * cells + 5 * st["pred_cells"] + 2 * cells
-    achieved_gbs = alg_bytes / fill_s / 1e9
+    # A device-resident run's fills are nodes of replayed HIP graphs, which
+    # events cannot bracket one by one: its per-launch figures come from the
+    # kernel trace (profiles/), and only the wall-based ones are reported here.
+    per_launch = fill_s > 0
+    achieved_gbs = alg_bytes / fill_s / 1e9 if per_launch else None
+    rnd = lambda x, n: None if x is None else round(x, n)
     traffic = None
     tfile = ROOT / "profiles" / "poa_fill_traffic.json"
     if tfile.exists():
@@ -248,11 +255,11 @@ def main():
                       / max(1, launches))
     peak_t = (valu_peak or VALU_PEAK_TOPS * 1e12) / 1e12
     roofline = {
-        "bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "bound": "hbm", "achieved": rnd(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": rnd(achieved_gbs and achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
         "traffic_source": "profile-derived (profiles/poa_fill_traffic.json x this run's cells)",
         "kernel": "poa_fill", "launches": launches,
-        "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3),
+        "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3) if per_launch else None,
         "bytes_per_launch": alg_bytes // max(1, launches),
         "bytes_per_cell": round(alg_bytes / max(1, cells), 3),
         # Over the wall: the step's algorithmic bytes per wall second (the
@@ -261,12 +268,12 @@ def main():
         "wall_gbs": round(alg_bytes / wall_s / 1e9, 1),
         "wall_frac": round(alg_bytes / wall_s / 1e9 / HBM_PEAK_GBS, 4),
         # The DP is integer max-plus: int VALU is the binding roof.
-        "valu": {"achieved_tops": round(int_ops / fill_s / 1e12, 3),
+        "valu": {"achieved_tops": rnd(int_ops / fill_s / 1e12 if per_launch else None, 3),
                  "peak_tops": round(peak_t, 2),
                  "peak_source": "measured on this device (add/max probe)" if valu_peak
                  else "nominal 256 CU x 64 lanes x 2.4 GHz",
                  "peak_nominal_tops": round(VALU_PEAK_TOPS, 1),
-                 "frac": round(int_ops / fill_s / 1e12 / peak_t, 4),
+                 "frac": rnd(int_ops / fill_s / 1e12 / peak_t if per_launch else None, 4),
                  "wall_tops": round(int_ops / wall_s / 1e12, 3),
                  "wall_frac": round(int_ops / wall_s / 1e12 / peak_t, 4),
                  "ops_per_cell": round(int_ops / max(1, cells), 2)},
@@ -279,10 +286,12 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "i32", "data": "synthetic",
         "config": {"workload": "batched DNA POA, adaptive band: 4096 jobs, L U[200,2000], "
                                "8-32 reads/job, 5-15% error 60/20/20, consensus+MSA",
-                   "jobs": args.jobs, "reads": int(job_off[-1]), "cells": cells},
+                   "jobs": args.jobs, "reads": int(job_off[-1]), "cells": cells,
+                   "mode": "device-resident (mpoa_batch_run_resident)" if args.resident
+                   else "host-driven engine (mpoa_batch_run)"},
         "jobs_per_s": round(jobs_per_s, 1), "failed_jobs": failed,
         "dp_calls": st["dp_calls"], "retries": st["retries"], "i64_calls": st["i64_calls"],
-        "fill_gcups": round(cells / fill_s / 1e9, 2),
+        "fill_gcups": round(cells / fill_s / 1e9, 2) if per_launch else None,
         "host_ms": round(st["host_ms"], 1), "device_ms": round(st["device_ms"], 1),
         "roofline": roofline,
         "hip_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),

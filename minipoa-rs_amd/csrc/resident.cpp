@@ -831,6 +831,8 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     for (uint32_t k = 0; k < N; k++) {
         const RJob &J = hj[k];
         st.cells += J.cells;
+        st.pred_cells += J.pred_cells;
+        st.arena_bytes += J.arena_bytes;
         st.dp_calls += J.dp_calls;
         st.retries += J.retries;
         st.arena_retries += J.arena_retries;
@@ -984,6 +986,8 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     // where they left (extend_graph semantics: read k keeps sequence id k).
     if (host_jobs.empty() && S.evicted.empty()) {
         est.run.cells = rst.cells;
+        est.run.pred_cells = rst.pred_cells;
+        est.run.arena_bytes = rst.arena_bytes;
         est.run.dp_calls = rst.dp_calls;
         est.run.retries = rst.retries;
         est.run.arena_retries = rst.arena_retries;
@@ -1032,6 +1036,8 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
         est = E.stats();
     }
     est.run.cells += rst.cells;
+    est.run.pred_cells += rst.pred_cells;
+    est.run.arena_bytes += rst.arena_bytes;
     est.run.dp_calls += rst.dp_calls;
     est.run.retries += rst.retries;
     est.run.arena_retries += rst.arena_retries;

@@ -94,7 +94,7 @@ struct RJob {
     uint32_t err_i, err_j;
     uint32_t evict, beg;  // eviction reason; rank of the planned DP's first row
     uint32_t npath, pad1_;  // path ids written (path_off[cur])
-    uint64_t cells, dp_calls, retries, arena_retries, deferred;
+    uint64_t cells, dp_calls, retries, arena_retries, deferred, pred_cells, arena_bytes;
     uint64_t prev_used, prev_cells, prev_n;  // the last completed DP (arena hint)
     uint64_t est, emax;  // arena bytes of the planned DP: estimate and full-width bound
     DevJob dj;           // the planned DP (arena fields set when it is placed)
@@ -975,6 +975,8 @@ MPOA_HD inline void rj_job_step(T &t, const RPools &R, uint32_t k, char *lds, ui
         } else {
             if (t.lane == 0) {
                 J.cells += res.cells;
+                J.pred_cells += res.pred_cells;
+                J.arena_bytes += res.arena_used;
                 J.dp_calls += 1;
                 if (!J.retry) {
                     J.prev_used = res.arena_used;

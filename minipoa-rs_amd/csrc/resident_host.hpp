@@ -11,6 +11,7 @@ namespace mpoa {
 
 struct ResidentStats {
     uint64_t buckets = 0, replays = 0, cells = 0, dp_calls = 0, retries = 0, arena_retries = 0;
+    uint64_t pred_cells = 0, arena_bytes = 0;  // band overlaps and traceback data of completed DPs
     uint64_t deferred = 0;       // DP placements postponed to a later step (arena spent)
     uint64_t evicted = 0;        // jobs that left the device and finished on the host engine
     uint64_t resident_done = 0;  // jobs that finished (or failed) on the device

@@ -485,6 +485,8 @@ int ResidentServer::Impl::collect(Table &T, const std::vector<uint32_t> &fin) {
         {
             std::lock_guard<std::mutex> lk(mu);
             st.cells += J.cells;
+            st.pred_cells += J.pred_cells;
+            st.arena_bytes += J.arena_bytes;
             st.dp_calls += J.dp_calls;
             st.retries += J.retries;
             st.arena_retries += J.arena_retries;
@@ -782,6 +784,8 @@ EngineStats ResidentServer::stats(ResidentStats *rs) {
     EngineStats es = I.host->stats();
     std::lock_guard<std::mutex> lk(I.mu);
     es.run.cells += I.st.cells;
+    es.run.pred_cells += I.st.pred_cells;
+    es.run.arena_bytes += I.st.arena_bytes;
     es.run.dp_calls += I.st.dp_calls;
     es.run.retries += I.st.retries;
     es.run.arena_retries += I.st.arena_retries;

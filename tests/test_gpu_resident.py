@@ -85,6 +85,7 @@ def test_resident_batched_workload_sample_matches_oracle_and_host(ctx, oracle):
     host = mh.batch_run(ctx, params, jo, ro, b)
     for j in range(64):
         assert res.output(j, 2) == host.output(j, 2), j
+    assert st["pred_cells"] == host.stats()["pred_cells"]
 
 
 def test_resident_many_buckets_over_all_slots(ctx, oracle, env):
