@@ -673,6 +673,9 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
     }
     P.ring_bytes = ring_bytes_for_width(wexp, false);
+    // A bucket is one length class: the tuning table's entry for its width
+    // (minipoa_hip/tune.py) decides its shape and ring when one is named.
+    if (!ctx.force_shape.cols && !getenv("MPOA_FILL_SHAPE")) (void)tuned_shape(wexp, shape, P.ring_bytes);
     {
         const uint32_t static_lds = 10240, dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
         const uint32_t room = 65536u - static_lds - dyn_other;

@@ -290,8 +290,14 @@ int ResidentServer::Impl::build_table(Table &T, uint32_t alpha) {
         const uint64_t m = T.maxlen + 2, band = p.b + m / p.f;
         wexp = uint32_t(std::min<uint64_t>(maxpad, 2 * band + m / 4 + 16));
     }
-    const FillShape shape = fill_shape_for_width(wexp);
+    FillShape shape = fill_shape_for_width(wexp);
     P.ring_bytes = ring_bytes_for_width(wexp, false);
+    if (const char *e = getenv("MPOA_FILL_SHAPE")) {
+        FillShape f{0, 0};
+        if (sscanf(e, "%d,%d", &f.cols, &f.waves) == 2 && fill_shape_ok(f)) shape = f;
+    } else {
+        (void)tuned_shape(wexp, shape, P.ring_bytes);
+    }
     {
         const uint32_t static_lds = 10240, dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
         const uint32_t room = 65536u - static_lds - dyn_other;

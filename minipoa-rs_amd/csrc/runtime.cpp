@@ -277,6 +277,14 @@ bool tuned_for(uint32_t width, TuneEntry &out) {
 
 }  // namespace
 
+bool tuned_shape(uint32_t width, FillShape &shape, uint32_t &ring_bytes) {
+    TuneEntry t{};
+    if (!tuned_for(width, t)) return false;
+    shape = t.shape;
+    ring_bytes = t.ring_bytes;
+    return true;
+}
+
 int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
                  bool full_arena, LaunchInfo &li, const FixedScratch *fx) {
     const size_t N = plans.size();

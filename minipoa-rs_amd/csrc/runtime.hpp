@@ -134,6 +134,10 @@ struct LaunchInfo {
 };
 int launch_begin(Ctx &ctx, const Params &p, const std::vector<JobPlan *> &plans, bool i64,
                  bool full_arena, LaunchInfo &li, const FixedScratch *fx = nullptr);
+// The launch shape and LDS ring the tuning table named by MPOA_TUNING
+// (minipoa_hip/tune.py) gives an expected row width; false without a table
+// or an entry.
+bool tuned_shape(uint32_t width, FillShape &shape, uint32_t &ring_bytes);
 // Score-arena bytes job J adds to a launch, its share of the overflow pool
 // included.
 uint64_t job_arena_bytes(const JobPlan &J, bool i64);

@@ -4,7 +4,9 @@ byte against the oracle, and the paths the smaller tests leave out:
   * protein families (2,000 families x 16 members, -f0, drawn matrix);
   * the serving stream: jobs of 15 kb templates with 50+ reads and
     incremental jobs whose 50-read graph is loaded from GFA text (-i), all
-    arriving while others run;
+    arriving while others run, served host-driven and on device-resident
+    graphs;
+  * the batched workload: all 4,096 jobs, host-driven and device-resident;
   * the arena-overflow recovery (full-width relaunch) in the one-call path
     and in the engine."""
 import os
@@ -123,26 +125,50 @@ def test_serving_stream_at_size_with_gfa_loaded_incremental_jobs(ctx, oracle):
     # The biggest jobs' expectations start first.
     want = {k: pool.submit(expected, k) for k in sorted(range(len(jobs)),
                                                         key=lambda k: -len(jobs[k]["reads"][0]))}
-    with mh.Server(ctx, p) as srv:
-        tickets = []
-        t0 = time.perf_counter()
-        for k, j in enumerate(jobs):
-            delay = (j["t"] - jobs[0]["t"]) / 10 - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
-            if k in loaded:
-                g, rec, _ = loaded[k]
-                tickets.append(srv.submit(j["reads"], graph=g, existing=len(rec)))
-            else:
-                tickets.append(srv.submit(j["reads"]))
-        got = []
-        for t in tickets:
-            srv.wait(t)
-            assert srv.status(t) == 0, mh.last_error()
-            got.append((srv.output(t, 0), srv.output(t, 1), srv.output(t, 2)))
-    for k in range(len(jobs)):
-        assert got[k] == want[k].result(), SERVED[k]
+    # Host-driven serving, then serving on device-resident graphs (the
+    # GFA-loaded graphs uploaded to the device with their list orders).
+    for resident in (False, True):
+        print("resident", resident, flush=True)
+        with mh.Server(ctx, p, resident=resident) as srv:
+            tickets = []
+            t0 = time.perf_counter()
+            for k, j in enumerate(jobs):
+                delay = (j["t"] - jobs[0]["t"]) / 10 - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                if k in loaded:
+                    g, rec, _ = loaded[k]
+                    tickets.append(srv.submit(j["reads"], graph=g, existing=len(rec)))
+                else:
+                    tickets.append(srv.submit(j["reads"]))
+            got = []
+            for t in tickets:
+                srv.wait(t)
+                assert srv.status(t) == 0, mh.last_error()
+                got.append((srv.output(t, 0), srv.output(t, 1), srv.output(t, 2)))
+            if resident:
+                assert srv.stats()["resident_jobs"] == len(jobs)
+        for k in range(len(jobs)):
+            assert got[k] == want[k].result(), (resident, SERVED[k])
     pool.shutdown()
+
+
+def test_batched_workload_full_size_host_and_resident_match_oracle(ctx, oracle):
+    """All 4,096 jobs of the batched workload (L in [200, 2000], 8-32 reads
+    at 5-15% error, adaptive band), host-driven and device-resident:
+    consensus FASTA + MSA hashes of every job equal the oracle's, and so do
+    the cells."""
+    jo, ro, b = synth.jobs(seed=20262, njobs=4096)
+    params = Params(adaptive=True)
+    cells, want = oracle.batch_run(params, jo, ro, b, CORES)
+    for resident in (False, True):
+        print("resident", resident, flush=True)
+        res = mh.batch_run(ctx, params, jo, ro, b, resident=resident)
+        got = np.array([fnv1a((res.output(j, 0) + res.output(j, 1)).encode()) for j in range(4096)],
+                       dtype=np.uint64)
+        assert (got == want).all(), (resident, np.nonzero(got != want)[0][:10])
+        assert res.stats()["cells"] == cells
+        del res
 
 
 def test_gfa_loaded_graph_extends_on_device_as_the_oracle(ctx, oracle):

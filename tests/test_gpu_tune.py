@@ -40,9 +40,12 @@ def test_tuned_table_is_written_and_followed_with_identical_results(ctx, tmp_pat
     try:
         for t in (path, forced):
             os.environ["MPOA_TUNING"] = str(t)
-            res = mh.batch_run(ctx, p, jo, ro, b)
-            assert [(res.output(j, 0), res.output(j, 1)) for j in range(48)] == want
-            assert res.stats()["cells"] == base.stats()["cells"]
+            # The host-driven engine's launches and the device-resident
+            # buckets (one length class each) both follow the table.
+            for resident in (False, True):
+                res = mh.batch_run(ctx, p, jo, ro, b, resident=resident)
+                assert [(res.output(j, 0), res.output(j, 1)) for j in range(48)] == want
+                assert res.stats()["cells"] == base.stats()["cells"]
     finally:
         if old is None:
             os.environ.pop("MPOA_TUNING", None)
