@@ -334,7 +334,7 @@ def main():
         if "c3" in legs:
             out["legs"]["c3"] = leg_c3(ctx, oracle, cores)
         if "res" in legs:
-            out["legs"]["res"] = leg_resident(ctx, params, job_off, read_off, bases, res, gcups,
+            out["legs"]["res"] = leg_resident(params, job_off, read_off, bases, res, gcups,
                                               total / len(times))
         # One DP per launch on the context's own stream last: after that
         # stream has run, the batched legs on the same context measured
@@ -412,7 +412,7 @@ def leg_records(params, job_off, read_off, bases, values_res, values_gcups, valu
     return out
 
 
-def leg_resident(ctx, params, job_off, read_off, bases, values_res, values_gcups, values_s):
+def leg_resident(params, job_off, read_off, bases, values_res, values_gcups, values_s):
     """The headline workload with every job device-resident (one warmup, two
     timed runs): graph fusion, re-ranking and DP planning on the device, one
     captured step graph per length bucket replayed until its jobs finish.
@@ -420,6 +420,9 @@ def leg_resident(ctx, params, job_off, read_off, bases, values_res, values_gcups
     MSA against the values run's."""
     import minipoa_hip as mh
 
+    # A context of its own, closed afterwards: its per-slot device buffers
+    # (arenas, graph blocks) do not stay beside the main context's.
+    ctx = mh.Context(0)
     mh.batch_run(ctx, params, job_off, read_off, bases, resident=True)
     times = []
     for _ in range(2):
@@ -444,6 +447,7 @@ def leg_resident(ctx, params, job_off, read_off, bases, values_res, values_gcups
            "host_ms": round(st["host_ms"], 1),
            "identical_to_host_engine": same, "jobs": n}
     del r
+    ctx.close()
     log(f"res: device-resident {dt*1e3:.0f} ms ({out['vs_host_engine']:.2f}x host engine), "
         f"identical {same}/{n}")
     return out

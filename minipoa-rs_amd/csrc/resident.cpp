@@ -582,7 +582,7 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
                    b_meta = BL.b_meta, b_adapt = BL.b_adapt, b_tb = BL.b_tb, b_end = BL.b_end;
     RJ_TRY(slot.dg.ensure(a_end));
     RJ_TRY(slot.dp.ensure(b_end));
-    RJ_TRY(slot.da.ensure(S.arena_bytes));
+    RJ_TRY(slot.da.ensure_exact(S.arena_bytes));
     RJ_TRY(slot.hin.ensure(a_graph));
     char *dA = static_cast<char *>(slot.dg.p);
     char *dB = static_cast<char *>(slot.dp.p);
@@ -895,14 +895,14 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     if (const char *e = getenv("MPOA_RES_BUCKET")) bsize = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
     for (uint64_t b0 = 0; b0 < res_jobs.size(); b0 += bsize)
         S.buckets.emplace_back(res_jobs.begin() + b0, res_jobs.begin() + std::min<uint64_t>(res_jobs.size(), b0 + bsize));
-    // Arena per slot: half the device memory over the slots, at most 8 GiB
+    // Arena per slot: half the device memory over the slots, at most 6 GiB
     // (MPOA_RES_ARENA_MB overrides; the squeeze knob of the tests makes DPs
     // outgrow it and leave the device). Slot buffers stay with the context
     // and are sized for the largest bucket before any bucket starts.
     uint64_t held = 0;
     for (const auto &x : cache.slots) held += x->da.cap;
     const uint64_t free_b = device_free_bytes(slots.empty() ? 0 : slots[0]->device) + held;
-    uint64_t arena = std::min<uint64_t>(free_b / 2 / std::max<size_t>(1, slots.size()), uint64_t(8) << 30);
+    uint64_t arena = std::min<uint64_t>(free_b / 2 / std::max<size_t>(1, slots.size()), uint64_t(6) << 30);
     if (const char *e = getenv("MPOA_RES_ARENA_MB")) arena = strtoull(e, nullptr, 10) << 20;
     if (getenv("MPOA_ARENA_SQUEEZE")) arena = uint64_t(1) << 20;
     S.arena_bytes = std::max<uint64_t>(arena, uint64_t(1) << 20) / 256 * 256;
@@ -926,7 +926,7 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
             hipError_t e = hipSetDevice(x.ctx->device);
             if (e == hipSuccess) e = x.dg.ensure(need_a);
             if (e == hipSuccess) e = x.dp.ensure(need_b);
-            if (e == hipSuccess) e = x.da.ensure(S.arena_bytes);
+            if (e == hipSuccess) e = x.da.ensure_exact(S.arena_bytes);
             if (e == hipSuccess) e = x.dk.ensure(std::max<uint64_t>(need_pack, 256));
             if (e == hipSuccess) e = x.hin.ensure(need_h);
             if (e == hipSuccess) e = x.hjobs.ensure(sizeof(RJob) * need_n);

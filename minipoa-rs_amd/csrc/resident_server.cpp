@@ -228,8 +228,8 @@ int ResidentServer::Impl::build_table(Table &T, uint32_t alpha) {
     T.b_sel = up256(T.b_poff + 8ull * K);
     const uint64_t b_end = up256(T.b_sel + 4ull * K);
     RS_TRY(T.blk.ensure(b_end));
-    RS_TRY(T.heap.ensure(T.heap_bytes));
-    RS_TRY(T.arena.ensure(T.arena_bytes));
+    RS_TRY(T.heap.ensure_exact(T.heap_bytes));
+    RS_TRY(T.arena.ensure_exact(T.arena_bytes));
     RS_TRY(T.hstage.ensure(b_end));
     RS_TRY(T.hjobs.ensure(sizeof(RJob) * K + 4096));
     RS_TRY(T.hsel.ensure(12ull * K + 4096));

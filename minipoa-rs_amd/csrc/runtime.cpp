@@ -107,6 +107,19 @@ hipError_t DevBuf::ensure(size_t bytes) {
     return hipSuccess;
 }
 
+hipError_t DevBuf::ensure_exact(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return e;
+    }
+    cap = bytes;
+    return hipSuccess;
+}
+
 void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;

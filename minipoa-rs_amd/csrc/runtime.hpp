@@ -49,6 +49,9 @@ struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
     hipError_t ensure(size_t bytes);
+    // At least `bytes`, reallocated to exactly that size when smaller (for
+    // buffers sized once, such as the resident arenas).
+    hipError_t ensure_exact(size_t bytes);
     void release();
 };
 
