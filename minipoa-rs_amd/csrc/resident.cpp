@@ -647,7 +647,7 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     // graph of about three times the longest template (what the batched
     // workload's graphs reach), at most 64 KB; larger graphs use the jobs'
     // global arrays. Smaller buckets leave LDS to the fills beside them.
-    R.lds_bytes = uint32_t(std::min<uint64_t>(65536, std::max<uint64_t>(16384, kWinBytes + 1024 + 36 * lmax)));
+    R.lds_bytes = step_lds_bytes(lmax);
 
     DevPools P{};
     P.jobs = R.dj;

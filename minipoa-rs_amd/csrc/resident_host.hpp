@@ -24,6 +24,17 @@ struct ResidentStats {
 inline uint64_t up16(uint64_t x) { return (x + 15) / 16 * 16; }
 inline uint64_t up256(uint64_t x) { return (x + 255) / 256 * 256; }
 
+// LDS of the step kernel for jobs whose longest template is lmax: its
+// windows and the per-row arrays of a graph of about 3 x lmax nodes (12 B
+// per row), between 16 KB and 64 KB. A gfx950 workgroup may hold 160 KB,
+// but a step holding more than 64 KB keeps other buckets' fills off its CU:
+// with up to 160 KB the 4,096-job batch ran 1.92-2.09 s per run instead of
+// 1.53-1.63 s, and the serving stream's latencies did not improve.
+inline uint32_t step_lds_bytes(uint64_t lmax) {
+    const uint64_t want = kWinBytes + 1024 + 36 * lmax;
+    return uint32_t(want < 16384 ? 16384 : want > 65536 ? 65536 : want);
+}
+
 // Scoring parameters of the step kernel; matrix: the matrix's address there.
 RParams make_rparams(const Params &p, const int32_t *matrix);
 

@@ -266,7 +266,7 @@ int ResidentServer::Impl::build_table(Table &T, uint32_t alpha) {
     R.njobs = K;
     R.alloc_limit = T.arena_bytes / 4 * 3;
     R.job_cap = R.alloc_limit / 2;
-    R.lds_bytes = uint32_t(std::min<uint64_t>(65536, std::max<uint64_t>(16384, kWinBytes + 1024 + 36ull * T.maxlen)));
+    R.lds_bytes = step_lds_bytes(T.maxlen);
     const uint32_t maxpad = ((T.maxlen + 2) / 8 + 1) * 8;
     DevPools P{};
     P.jobs = R.dj;
