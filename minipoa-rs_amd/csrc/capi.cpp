@@ -793,6 +793,7 @@ int mpoa_batch_run(mpoa_ctx *ctx, const mpoa_params *params, uint64_t njobs,
     // Jobs advance independently; launch slots take whichever jobs are
     // ready (engine.hpp).
     const int nslots = engine_slots(0);
+    resident_cache_trim(ctx->resident);
     rc = ensure_engine(ctx, nslots);
     if (rc != MPOA_OK) return rc;
     std::vector<Ctx *> slots;
@@ -922,6 +923,7 @@ int mpoa_server_create(mpoa_ctx *ctx, const mpoa_params *params, const mpoa_serv
     if (!check_params(params, p, rc)) return rc;
     // Four slots by default: two for long DPs, two for short ones.
     const int nslots = engine_slots(opts && opts->slots ? int(opts->slots) : 4);
+    resident_cache_trim(ctx->resident);
     rc = ensure_engine(ctx, nslots);
     if (rc != MPOA_OK) return rc;
     std::vector<Ctx *> slots;

@@ -35,6 +35,7 @@
 
 #include "dp_types.hpp"
 #include "kernels.hpp"
+#include "resident_dev.hpp"
 
 namespace mpoa {
 
@@ -349,33 +350,56 @@ __device__ __forceinline__ void store_recs(uint8_t *dst, const R (&r)[C], uint32
 template <typename T, int C, int W, bool LT, bool REC> constexpr int fill_waves_per_eu() {
     return sizeof(T) == 4 && LT && (C == 4 || (!REC && C == 8 && W == 1)) ? 4 : 1;
 }
+// The fill's fixed LDS state, at the start of the job's LDS region; the
+// ring, the query codes and the score matrix follow it.
+template <typename T, int W> struct FillLds {
+    LMeta hmeta[kHist];
+    RowRec hrec[kHist];
+    RowIn drow[kBlk + 1];
+    RowAux daux[kBlk];
+    PredE dpred[kPredCache];
+    T xcar[2][W];          // per-wave scan totals, by stripe parity
+    T xbest[W];            // per-wave row maximum of M
+    uint32_t xbestc[W];    // its first column
+    uint64_t xpool;        // arena chunk taken by thread 0
+    int64_t xscore;        // M at (n-1, m-1)
+    alignas(16) T negrow[16];  // NEG: reads outside a predecessor's band
+};
+template <typename T, int W> constexpr uint32_t fill_fixed_lds() {
+    return uint32_t((sizeof(FillLds<T, W>) + 15u) & ~size_t(15));
+}
+
+// One job's DP (the job at index b of the launch pools) by the W waves of
+// the calling workgroup, with its LDS region at `lds` (16-byte aligned,
+// fill_fixed_lds + ring + query codes + matrix bytes).
 template <typename T, int C, int W, bool LT, typename R, bool REC>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT, REC>()))) void poa_fill(DevPools P) {
+__device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *lds) {
     static_assert(C == 4 || C % 8 == 0, "lane runs tile the 8-column band grid");
     constexpr int G = C < 8 ? C : 8;  // columns per piece (inside one 8-block)
-    extern __shared__ __align__(16) char dyn_lds[];
-    __shared__ LMeta hmeta[kHist];
-    __shared__ RowRec hrec[kHist];
-    __shared__ RowIn drow[kBlk + 1];
-    __shared__ RowAux daux[kBlk];
-    __shared__ PredE dpred[kPredCache];
-    __shared__ T xcar[2][W];          // per-wave scan totals, by stripe parity
-    __shared__ T xbest[W];            // per-wave row maximum of M
-    __shared__ uint32_t xbestc[W];    // its first column
-    __shared__ uint64_t xpool;        // arena chunk taken by thread 0
-    __shared__ int64_t xscore;        // M at (n-1, m-1)
-    __shared__ __align__(16) T negrow[16];  // NEG: reads outside a predecessor's band
+    FillLds<T, W> &FL = *reinterpret_cast<FillLds<T, W> *>(lds);
+    auto &hmeta = FL.hmeta;
+    auto &hrec = FL.hrec;
+    auto &drow = FL.drow;
+    auto &daux = FL.daux;
+    auto &dpred = FL.dpred;
+    auto &xcar = FL.xcar;
+    auto &xbest = FL.xbest;
+    auto &xbestc = FL.xbestc;
+    auto &xpool = FL.xpool;
+    auto &xscore = FL.xscore;
+    auto &negrow = FL.negrow;
+    char *dyn_lds = lds + fill_fixed_lds<T, W>();
     const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
     T *ring = reinterpret_cast<T *>(dyn_lds);  // element-indexed
     uint8_t *sseq = reinterpret_cast<uint8_t *>(dyn_lds + kRingBytes);
     int32_t *smat = reinterpret_cast<int32_t *>(dyn_lds + kRingBytes + ((P.seq_lds + 15u) & ~15u));
 
     if (P.prio) __builtin_amdgcn_s_setprio(3);
-    const DevJob J = ldg(P.jobs + blockIdx.x);
+    const DevJob J = ldg(P.jobs + b);
     if (J.n == 0) {
         // No DP for this job in this launch (a resident step loop replays
         // one launch shape over every job of its batch): uniform exit.
-        if (threadIdx.x == 0) P.res[blockIdx.x] = DevResult{kSkipped, 0, 0, 0, 0, 0, 0, 0};
+        if (threadIdx.x == 0) P.res[b] = DevResult{kSkipped, 0, 0, 0, 0, 0, 0, 0};
         return;
     }
     const uint32_t tid = threadIdx.x;
@@ -1135,8 +1159,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_wav
         res.pred_cells = pcells;
         res.arena_used = used;
         res.score = status == kOk ? xscore : int64_t(NEG);
-        P.res[blockIdx.x] = res;
+        P.res[b] = res;
     }
+}
+
+template <typename T, int C, int W, bool LT, typename R, bool REC>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fill_waves_per_eu<T, C, W, LT, REC>()))) void poa_fill(DevPools P) {
+    extern __shared__ __align__(16) char fill_lds[];
+    fill_job<T, C, W, LT, R, REC>(P, blockIdx.x, fill_lds);
 }
 
 // Weighted deterministic traceback (src/align.rs:1164-1313), one wave per job,
@@ -1405,21 +1435,37 @@ __global__ __launch_bounds__(64) void poa_traceback(DevPools P, uint32_t njobs) 
 // the per-column results use LDS as in poa_traceback.
 constexpr uint32_t kTbReach = 16;
 
+template <typename T> constexpr uint32_t tbv_cols() { return sizeof(T) == 4 ? 32 : 16; }
+// The values traceback's fixed LDS state, at the start of the job's LDS
+// region; LT: the query codes and the score matrix follow it.
+template <typename T> struct TbvLds {
+    WRow wrow[kTbRows];
+    PredE wpred[kTbRows * kTbPred];
+    T wm[kTbRows * tbv_cols<T>()], wd[kTbRows * tbv_cols<T>()], wi[kTbRows * tbv_cols<T>()];
+    uint32_t tbw[64];  // records of the columns of the current 64-column chunk
+};
+template <typename T> constexpr uint32_t tbv_fixed_lds() {
+    return uint32_t((sizeof(TbvLds<T>) + 15u) & ~size_t(15));
+}
+
+// The traceback of the job at index jid by the calling wave (lanes
+// threadIdx.x & 63), with its LDS region at `lds`.
 template <typename T, bool LT>
-__global__ __launch_bounds__(64) void poa_traceback_values(DevPools P, uint32_t njobs) {
-    constexpr uint32_t kVCols = sizeof(T) == 4 ? 32 : 16;
-    __shared__ WRow wrow[kTbRows];
-    __shared__ PredE wpred[kTbRows * kTbPred];
-    __shared__ T wm[kTbRows * kVCols], wd[kTbRows * kVCols], wi[kTbRows * kVCols];
-    __shared__ uint32_t tbw[64];  // records of the columns of the current 64-column chunk
-    extern __shared__ __align__(16) char tb_lds[];  // LT: query codes, then the matrix
-    const uint32_t jid = blockIdx.x;
-    if (jid >= njobs) return;
+__device__ __forceinline__ void tbv_job(const DevPools &P, uint32_t jid, char *lds) {
+    constexpr uint32_t kVCols = tbv_cols<T>();
+    TbvLds<T> &TL = *reinterpret_cast<TbvLds<T> *>(lds);
+    auto &wrow = TL.wrow;
+    auto &wpred = TL.wpred;
+    auto &wm = TL.wm;
+    auto &wd = TL.wd;
+    auto &wi = TL.wi;
+    auto &tbw = TL.tbw;
+    char *tb_lds = lds + tbv_fixed_lds<T>();  // LT: query codes, then the matrix
     if (P.prio) __builtin_amdgcn_s_setprio(3);
     DevResult *R = P.res + jid;
     if (R->status != kOk) return;
     const DevJob J = P.jobs[jid];
-    const int lane = threadIdx.x;
+    const int lane = int(threadIdx.x & 63u);
     const uint32_t n = J.n, m = J.m;
     const RowIn *rin = P.rows + J.rows;
     const RowAux *aux = P.aux + J.aux;
@@ -1657,9 +1703,16 @@ __global__ __launch_bounds__(64) void poa_traceback_values(DevPools P, uint32_t 
     }
 }
 
+template <typename T, bool LT>
+__global__ __launch_bounds__(64) void poa_traceback_values(DevPools P, uint32_t njobs) {
+    extern __shared__ __align__(16) char tbv_lds[];
+    if (blockIdx.x >= njobs) return;
+    tbv_job<T, LT>(P, blockIdx.x, tbv_lds);
+}
+
 template <typename T, int C, int W, bool LT, bool REC, typename R = uint32_t>
 static void launch_fill_t(const DevPools &P, uint32_t njobs, hipStream_t s) {
-    const uint32_t lds = fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
+    const uint32_t lds = fill_fixed_lds<T, W>() + fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
     hipLaunchKernelGGL((poa_fill<T, C, W, LT, R, REC>), dim3(njobs), dim3(64 * W), lds, s, P);
 }
 
@@ -1736,11 +1789,6 @@ int launch_fill(const DevPools &P, uint32_t njobs, bool i64, bool rec, bool wide
 constexpr uint32_t kTbStaticLds =
     uint32_t(sizeof(WRow) * kTbRows + sizeof(WPred) * kTbRows * kTbPred +
              sizeof(uint32_t) * kTbRows * kTbCols + 64 * sizeof(uint32_t));
-template <typename T> constexpr uint32_t tb_values_static_lds() {
-    constexpr uint32_t cols = sizeof(T) == 4 ? 32 : 16;
-    return uint32_t(sizeof(WRow) * kTbRows + sizeof(PredE) * kTbRows * kTbPred +
-                    3 * kTbRows * cols * sizeof(T) + 64 * sizeof(uint32_t));
-}
 
 void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hipStream_t s) {
     if (njobs == 0) return;
@@ -1753,9 +1801,9 @@ void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hip
         return;
     }
     const uint32_t dyn = fill_dynamic_lds(0, P.seq_lds, P.mat_lds);
-    const uint32_t stat = i64 ? tb_values_static_lds<int64_t>() : tb_values_static_lds<int32_t>();
+    const uint32_t stat = i64 ? tbv_fixed_lds<int64_t>() : tbv_fixed_lds<int32_t>();
     const bool lt = P.seq_lds != 0 && P.mat_lds != 0 && stat + dyn <= 65536u;
-    const uint32_t lds = lt ? dyn : 0;
+    const uint32_t lds = stat + (lt ? dyn : 0);
     if (i64) {
         if (lt) hipLaunchKernelGGL((poa_traceback_values<int64_t, true>), grid, dim3(64), lds, s, P, njobs);
         else hipLaunchKernelGGL((poa_traceback_values<int64_t, false>), grid, dim3(64), 0, s, P, njobs);
@@ -1763,6 +1811,77 @@ void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hip
         if (lt) hipLaunchKernelGGL((poa_traceback_values<int32_t, true>), grid, dim3(64), lds, s, P, njobs);
         else hipLaunchKernelGGL((poa_traceback_values<int32_t, false>), grid, dim3(64), 0, s, P, njobs);
     }
+}
+
+// Persistent resident kernel: each workgroup takes jobs from a device queue
+// (longest first) and runs each to its end -- the resident step (consume the
+// last traceback, fuse, re-rank, plan; wave 0), the fill (all waves), the
+// traceback (wave 0), once per read -- with every DP in the workgroup's own
+// arena region. There is no launch per step and no lockstep between jobs: a
+// workgroup whose job ends starts the next one at once, so the chip stays
+// full until the queue runs dry. The three phases alias one LDS region and
+// meet at workgroup barriers; decisions that end a loop are read after a
+// barrier and made uniform, so every wave meets every barrier. Workgroups
+// never wait for one another (the queue is one atomic counter), so the
+// kernel completes whatever number of them is resident at once.
+template <int C, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void rj_persist_kernel(
+    RPools R, DevPools P, RPersist Q) {
+    extern __shared__ __align__(16) char pl_lds[];
+    __shared__ uint32_t xjob;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));
+    const uint64_t slot_off = Q.arena_off + uint64_t(blockIdx.x) * Q.slot_bytes;
+    for (;;) {
+        if (tid == 0) xjob = atomicAdd(Q.next, 1u);
+        sync_lds<W>();
+        const uint32_t k = uint32_t(__builtin_amdgcn_readfirstlane(int(xjob)));
+        sync_lds<W>();  // xjob read by every wave before the next job's write
+        if (k >= R.njobs) break;
+        // A read takes one step, plus at most two for poa's full-DP retry and
+        // a full-width arena; past this bound the job leaves for the host
+        // engine, which continues it from its next unfused read.
+        const uint32_t bound = 4u * R.jobs[k].nreads + 8u;
+        for (uint32_t it = 0;; it++) {
+            if (wave == 0) {
+                WaveTeam t{tid};
+                if (it < bound) {
+                    rj_job_step(t, R, k, pl_lds, Q.step_lds, slot_off, Q.slot_bytes);
+                } else if (tid == 0) {
+                    RJob &J = R.jobs[k];
+                    if (J.state == kRsRunning) {
+                        J.state = kRsEvicted;
+                        J.evict = kEvStalled;
+                        atomicAdd(R.done, 1u);
+                    }
+                    R.dj[k].n = 0;
+                }
+            }
+            sync_global<W>();  // the planned DP (or n == 0: the job left) for every wave
+            const uint32_t n = uint32_t(__builtin_amdgcn_readfirstlane(int(glb(&R.dj[k].n)[0])));
+            if (n == 0) break;
+            fill_job<int32_t, C, W, true, uint32_t, false>(P, k, pl_lds);
+            sync_global<W>();
+            if (wave == 0) tbv_job<int32_t, true>(P, k, pl_lds);
+            sync_global<W>();
+        }
+    }
+}
+
+uint32_t rj_persist_lds(const DevPools &P, uint32_t step_lds) {
+    const uint32_t fill = fill_fixed_lds<int32_t, 4>() + fill_dynamic_lds(P.ring_bytes, P.seq_lds, P.mat_lds);
+    const uint32_t tb = tbv_fixed_lds<int32_t>() + fill_dynamic_lds(0, P.seq_lds, P.mat_lds);
+    uint32_t m = fill > tb ? fill : tb;
+    return m > step_lds ? m : step_lds;
+}
+
+int launch_rj_persist(const RPools &R, const DevPools &P, const RPersist &Q, uint32_t grid, FillShape f,
+                      hipStream_t s) {
+    if (f.cols != 4 || f.waves != 4) return -1;
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL((rj_persist_kernel<4, 4>), dim3(grid), dim3(256), rj_persist_lds(P, Q.step_lds), s, R, P,
+                       Q);
+    return 0;
 }
 
 // Device scratch of the no-allocation entry point: one fixed array per GPU,

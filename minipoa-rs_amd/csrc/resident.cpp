@@ -440,6 +440,15 @@ struct ResidentCache {
 
 ResidentCache *resident_cache_new() { return new ResidentCache(); }
 void resident_cache_free(ResidentCache *c) { delete c; }
+void resident_cache_trim(ResidentCache *c) {
+    if (!c) return;
+    for (auto &s : c->slots) {
+        s->dg.release();
+        s->dp.release();
+        s->da.release();
+        s->dk.release();
+    }
+}
 uint64_t resident_cache_bytes(const ResidentCache *c) {
     uint64_t t = 0;
     if (c)
@@ -448,6 +457,14 @@ uint64_t resident_cache_bytes(const ResidentCache *c) {
 }
 
 namespace {
+
+// The persistent kernel's launch: workgroups, bytes of each one's arena
+// region, and the step's LDS.
+struct PersistCfg {
+    uint32_t grid = 0;
+    uint64_t slot_bytes = 0;
+    uint64_t pool_bytes = 0;  // shared by DPs that outgrow their region
+};
 
 struct RShared {
     const Params *p = nullptr;
@@ -559,7 +576,11 @@ BucketLayout layout_bucket(const Params &p, const BatchInput &in, const std::vec
     return B;
 }
 
-int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &jobs) {
+int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const BucketLayout &BL,
+                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays);
+
+int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &jobs,
+               const PersistCfg *pc = nullptr) {
     RShared::Trace tr{};
     tr.slot = si;
     tr.t[0] = S.ms();
@@ -591,6 +612,10 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     const RParams prm = make_rparams(p, reinterpret_cast<const int32_t *>(dA + a_mat));
     memcpy(h + a_prm, &prm, sizeof prm);
     memcpy(h + a_mat, p.matrix.data(), 4ull * alpha * alpha);
+    if (pc) {
+        const unsigned long long pool0 = pc->slot_bytes * pc->grid;
+        memcpy(h + a_ctr + 128, &pool0, sizeof pool0);
+    }
     uint64_t bo = 0, oo = 0, go = 0, ro_ = 0, po = 0, so = 0, mo = 0, to = 0;
     for (uint32_t k = 0; k < N; k++) {
         const uint64_t j = jobs[k];
@@ -683,6 +708,46 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     }
     P.prio = 0;
 
+    if (pc) {
+        // One launch of the persistent kernel runs every job to its end.
+        // Its DPs use their workgroup's arena region; the launch-wide chunk
+        // pool is closed (pool_end 0), so a DP that outgrows its region
+        // returns kArenaOverflow and is re-planned full-width, or leaves the
+        // device when that exceeds the region too.
+        RPersist Q{};
+        Q.next = reinterpret_cast<uint32_t *>(dA + a_ctr + 192);  // zero from the staging copy
+        Q.arena_off = 0;
+        Q.slot_bytes = pc->slot_bytes;
+        // The shared pool follows the regions: its counter starts at the
+        // pool's first byte (written with the staging copy).
+        P.pool_next = reinterpret_cast<unsigned long long *>(dA + a_ctr + 128);
+        P.pool_end = pc->slot_bytes * pc->grid + pc->pool_bytes;
+        P.ring_bytes = ring_bytes_for_width(wexp, false);
+        {
+            const uint32_t dyn_other = ((P.seq_lds + 15u) & ~15u) + P.mat_lds;
+            const uint32_t room = 65536u - 10240u - dyn_other;
+            if (P.ring_bytes > room) P.ring_bytes = room & ~63u;
+        }
+        // The step gets the LDS the fill and the traceback need anyway, so
+        // four workgroups still fit a CU.
+        Q.step_lds = rj_persist_lds(P, 0) & ~15u;
+        R.lds_bytes = Q.step_lds;
+        R.job_cap = pc->slot_bytes + pc->pool_bytes / 4;
+        if (!slot.ev[0]) RJ_TRY(hipEventCreate(&slot.ev[0]));
+        if (!slot.ev[1]) RJ_TRY(hipEventCreate(&slot.ev[1]));
+        tr.t[2] = S.ms();
+        RJ_TRY(hipEventRecord(slot.ev[0], s));
+        const uint32_t grid = std::min<uint32_t>(N, pc->grid);
+        if (launch_rj_persist(R, P, Q, grid, FillShape{4, 4}, s) != 0) {
+            set_error("no persistent kernel for the launch shape");
+            return MPOA_ERR_DEVICE;
+        }
+        RJ_TRY(hipGetLastError());
+        RJ_TRY(hipEventRecord(slot.ev[1], s));
+        RJ_TRY(hipEventSynchronize(slot.ev[1]));
+        return finish_bucket(S, slot, jobs, BL, R, dA + a_poff, tr, 1);
+    }
+
     // The step, captured once and replayed.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -745,6 +810,20 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     (void)hipGraphDestroy(graph);
     if (rc != MPOA_OK) return rc;
     RJ_TRY(hipEventSynchronize(slot.ev[1]));
+    return finish_bucket(S, slot, jobs, BL, R, dA + a_poff, tr, replays);
+}
+
+// After a bucket's device run: final job states, every graph packed into one
+// copy and rebuilt on the host, outputs of finished jobs, evicted jobs
+// queued for the host engine, statistics.
+int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const BucketLayout &BL,
+                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays) {
+    const Params &p = *S.p;
+    const BatchInput &in = *S.in;
+    hipStream_t s = slot.ctx->stream;
+    const uint32_t N = uint32_t(jobs.size());
+    const uint32_t rmax = BL.rmax;
+    const uint64_t lmax = BL.lmax;
     float dev_ms = 0;
     RJ_TRY(hipEventElapsedTime(&dev_ms, slot.ev[0], slot.ev[1]));
     tr.t[3] = S.ms();
@@ -763,8 +842,8 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     }
     RJ_TRY(slot.dk.ensure(std::max<uint64_t>(pbytes, 256)));
     RJ_TRY(slot.hpack.ensure(std::max<uint64_t>(pbytes, 256)));
-    RJ_TRY(hipMemcpyAsync(dA + a_poff, hoff, 8ull * N, hipMemcpyHostToDevice, s));
-    launch_rj_pack(R.jobs, nullptr, N, reinterpret_cast<const uint64_t *>(dA + a_poff), static_cast<uint8_t *>(slot.dk.p), s);
+    RJ_TRY(hipMemcpyAsync(d_poff, hoff, 8ull * N, hipMemcpyHostToDevice, s));
+    launch_rj_pack(R.jobs, nullptr, N, reinterpret_cast<const uint64_t *>(d_poff), static_cast<uint8_t *>(slot.dk.p), s);
     RJ_TRY(hipGetLastError());
     RJ_TRY(hipMemcpyAsync(slot.hpack.p, slot.dk.p, pbytes, hipMemcpyDeviceToHost, s));
     RJ_TRY(hipStreamSynchronize(s));
@@ -893,6 +972,15 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     uint64_t bsize = (res_jobs.size() + 2 * slots.size() - 1) / std::max<size_t>(1, 2 * slots.size());
     bsize = std::min<uint64_t>(std::max<uint64_t>(bsize, 32), 384);
     if (const char *e = getenv("MPOA_RES_BUCKET")) bsize = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+    // Persistent mode (the default; MPOA_RES_PERSIST=0 selects the captured
+    // step loops): every job in one launch of the persistent kernel (one
+    // bucket, the first slot), the longest first.
+    PersistCfg pcfg;
+    const bool persist = [] {
+        const char *e = getenv("MPOA_RES_PERSIST");
+        return !e || atoi(e) != 0;
+    }();
+    if (persist) bsize = std::max<uint64_t>(res_jobs.size(), 1);
     for (uint64_t b0 = 0; b0 < res_jobs.size(); b0 += bsize)
         S.buckets.emplace_back(res_jobs.begin() + b0, res_jobs.begin() + std::min<uint64_t>(res_jobs.size(), b0 + bsize));
     // Arena per slot: half the device memory over the slots, at most 6 GiB
@@ -906,6 +994,31 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     if (const char *e = getenv("MPOA_RES_ARENA_MB")) arena = strtoull(e, nullptr, 10) << 20;
     if (getenv("MPOA_ARENA_SQUEEZE")) arena = uint64_t(1) << 20;
     S.arena_bytes = std::max<uint64_t>(arena, uint64_t(1) << 20) / 256 * 256;
+    if (persist && !res_jobs.empty()) {
+        // Four 4x4 workgroups per CU; each region holds the largest DP the
+        // longest template is expected to reach (a graph of three times the
+        // template, rows of the expected width, 12 bytes per cell, half again
+        // as margin), within half the free memory, and a shared pool of an
+        // eighth of that takes what larger DPs need beyond their region.
+        // MPOA_RES_ARENA_MB sets the region size (the pool stays an eighth).
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, slots[0]->device) != hipSuccess ||
+            cus <= 0) {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+        pcfg.grid = uint32_t(std::min<uint64_t>(res_jobs.size(), 4ull * uint64_t(cus)));
+        const uint64_t lmax = L[res_jobs.front()];
+        uint64_t want = up256(3 * (lmax + 2) * uint64_t(expected_width(p, lmax)) * 12 * 3 / 2);
+        want = std::max<uint64_t>(want, uint64_t(16) << 20);
+        want = std::min<uint64_t>(want, free_b / 2 / pcfg.grid / 9 * 8);
+        if (const char *e = getenv("MPOA_RES_ARENA_MB")) want = strtoull(e, nullptr, 10) << 20;
+        if (getenv("MPOA_ARENA_SQUEEZE")) want = uint64_t(1) << 20;
+        pcfg.slot_bytes = std::max<uint64_t>(want, uint64_t(1) << 20) / 256 * 256;
+        pcfg.pool_bytes = pcfg.slot_bytes * pcfg.grid / 8 / 256 * 256;
+        if (const char *e = getenv("MPOA_RES_POOL_MB")) pcfg.pool_bytes = strtoull(e, nullptr, 10) << 20;
+        S.arena_bytes = pcfg.slot_bytes * pcfg.grid + pcfg.pool_bytes;
+    }
     while (cache.slots.size() < slots.size()) cache.slots.push_back(std::make_unique<RSlot>());
     uint64_t need_a = 0, need_b = 0, need_h = 0, need_pack = 0, need_n = 0;
     for (const auto &bk : S.buckets) {
@@ -922,7 +1035,10 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     for (size_t k = 0; k < slots.size(); k++) {
         RSlot &x = *cache.slots[k];
         x.ctx = slots[k];
-        if (!S.buckets.empty()) {
+        if (persist && k > 0) {
+            // Only the first slot runs the persistent launch.
+            x.da.release();
+        } else if (!S.buckets.empty()) {
             hipError_t e = hipSetDevice(x.ctx->device);
             if (e == hipSuccess) e = x.dg.ensure(need_a);
             if (e == hipSuccess) e = x.dp.ensure(need_b);
@@ -943,11 +1059,11 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
     const auto t0 = std::chrono::steady_clock::now();
     S.t0 = t0;
     std::vector<std::thread> th;
-    for (size_t si = 0; si < rs.size() && !S.buckets.empty(); si++)
+    for (size_t si = 0; si < (persist ? std::min<size_t>(1, rs.size()) : rs.size()) && !S.buckets.empty(); si++)
         th.emplace_back([&, si] {
             try {
                 for (size_t b; (b = S.next++) < S.buckets.size();) {
-                    const int rc = run_bucket(S, *rs[si], si, S.buckets[b]);
+                    const int rc = run_bucket(S, *rs[si], si, S.buckets[b], persist ? &pcfg : nullptr);
                     if (rc != MPOA_OK) {
                         std::lock_guard<std::mutex> lk(S.mu);
                         if (S.rc == MPOA_OK) {

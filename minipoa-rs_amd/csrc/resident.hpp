@@ -118,6 +118,16 @@ struct RPools {
     uint32_t njobs, lds_bytes;
 };
 
+// Arguments of the persistent resident kernel: a queue of job indices
+// taken in order (the host sorts the jobs longest first), and one arena
+// region of slot_bytes per workgroup.
+struct RPersist {
+    uint32_t *next;       // queue position: jobs taken so far
+    uint64_t arena_off;   // byte offset of workgroup 0's region in the arena
+    uint64_t slot_bytes;  // bytes per region (the largest DP arena a job may use)
+    uint32_t step_lds;    // LDS bytes of the step (the rest of the kernel's LDS aliases it)
+};
+
 // ---------------------------------------------------------------------------
 // Team primitives used below (device: resident.hip; host: resident.cpp):
 //   lane, size               this lane and the team's width
@@ -879,8 +889,15 @@ MPOA_HD inline int rj_plan_read(T &t, const RPools &R, const RParams &P, RJob &J
     return rj_plan(t, P, J, A, q, qlen, full, J.head, J.tail, J.support, W);
 }
 
+// slot_cap != 0: the DP's arena is the caller's own region [slot_off,
+// slot_off + slot_cap) (the persistent kernel, one region per workgroup),
+// which the fill extends with chunks of the launch's shared pool when it
+// outgrows it; a DP whose estimate exceeds R.job_cap, or that finds the pool
+// spent, leaves the device. Otherwise the DP is placed in the step's shared
+// arena.
 template <class T>
-MPOA_HD inline void rj_job_step(T &t, const RPools &R, uint32_t k, char *lds, uint32_t lds_bytes) {
+MPOA_HD inline void rj_job_step(T &t, const RPools &R, uint32_t k, char *lds, uint32_t lds_bytes,
+                                uint64_t slot_off = 0, uint64_t slot_cap = 0) {
     RJob &J = R.jobs[k];
     const RParams &P = *R.prm;
     DevJob &out = R.dj[k];
@@ -957,6 +974,13 @@ MPOA_HD inline void rj_job_step(T &t, const RPools &R, uint32_t k, char *lds, ui
     if (phase == kRpLaunched) {
         const DevResult res = R.res[k];
         if (res.status == kArenaOverflow) {
+            if (slot_cap) {
+                // Its region and the shared pool are spent: a larger
+                // estimate would not give it more.
+                if (t.lane == 0) J.evict = kEvArena;
+                leave(kRsEvicted);
+                return;
+            }
             // The same DP again, its arena sized for full-width rows.
             if (t.lane == 0) {
                 J.arena_retries += 1;
@@ -1021,6 +1045,16 @@ MPOA_HD inline void rj_job_step(T &t, const RPools &R, uint32_t k, char *lds, ui
     if (want > R.job_cap) {
         if (t.lane == 0) J.evict = kEvArena;
         leave(kRsEvicted);
+        return;
+    }
+    if (slot_cap) {
+        if (t.lane == 0) {
+            J.dj.arena = slot_off;
+            J.dj.arena_cap = slot_cap;
+            out = J.dj;
+            J.phase = kRpLaunched;
+        }
+        t.sync();
         return;
     }
     if (t.lane == 0) {

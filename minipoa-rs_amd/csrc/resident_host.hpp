@@ -57,6 +57,12 @@ void bind_graph(RJob &J, char *blk, const RGraphLayout &L);
 int failure_of(const RJob &J, std::string &msg);
 
 void launch_rj_step(const RPools &R, hipStream_t s);
+// The persistent resident kernel (kernels.hip): `grid` workgroups of the
+// fill shape f (4x4 only), LDS from rj_persist_lds. Returns -1 for a shape
+// without a persistent kernel.
+uint32_t rj_persist_lds(const DevPools &P, uint32_t step_lds);
+int launch_rj_persist(const RPools &R, const DevPools &P, const RPersist &Q, uint32_t grid, FillShape f,
+                      hipStream_t s);
 void launch_rj_reset(unsigned long long *ctr, hipStream_t s);
 void launch_rj_pack(const RJob *jobs, const uint32_t *sel, uint32_t njobs, const uint64_t *off, uint8_t *out,
                     hipStream_t s);
@@ -67,6 +73,10 @@ struct ResidentCache;
 ResidentCache *resident_cache_new();
 void resident_cache_free(ResidentCache *c);
 uint64_t resident_cache_bytes(const ResidentCache *c);
+// Releases the device blocks (the persistent launch's arena is large), so
+// that a host-driven run on the same context budgets its launches from the
+// whole free memory; the next resident run allocates them again.
+void resident_cache_trim(ResidentCache *c);
 
 // Whether a job runs device-resident from its first read.
 bool resident_eligible(const Params &p, const uint64_t *roff, uint32_t nreads);

@@ -1,8 +1,11 @@
 """Device-resident batches (mpoa_batch_run_resident): every job's graph and
 DP descriptors stay on the device between reads, fusion and planning run
-on the device, and each length bucket's step loop is a replayed HIP graph.
-Outputs must be byte-identical to the oracle's and to the host-driven
-batch, including for jobs that leave the device part-way."""
+on the device, either in one launch of the persistent kernel (the default:
+each workgroup runs jobs from a device queue to their end) or as length
+buckets whose step loop is a replayed HIP graph (MPOA_RES_PERSIST=0). Every
+test runs in both modes. Outputs must be byte-identical to the oracle's and
+to the host-driven batch, including for jobs that leave the device
+part-way."""
 import os
 
 import numpy as np
@@ -58,6 +61,13 @@ def env():
             os.environ[k] = v
 
 
+@pytest.fixture(params=["persistent", "captured"], autouse=True)
+def mode(request, env):
+    env("MPOA_RES_PERSIST", "1" if request.param == "persistent" else "0")
+    print("resident mode", request.param, flush=True)
+    return request.param
+
+
 def test_resident_small_batches_match_oracle(ctx, oracle):
     jo, ro, b = synth.jobs(seed=7, njobs=24, lmin=40, lmax=300, rmin=2, rmax=7)
     for params in (Params(), Params(adaptive=True), Params(f=0), Params(b=2, adaptive=True)):
@@ -88,8 +98,10 @@ def test_resident_batched_workload_sample_matches_oracle_and_host(ctx, oracle):
     assert st["pred_cells"] == host.stats()["pred_cells"]
 
 
-def test_resident_many_buckets_over_all_slots(ctx, oracle, env):
+def test_resident_many_buckets_over_all_slots(ctx, oracle, env, mode):
     """Buckets of three jobs: many step graphs, every launch slot busy."""
+    if mode == "persistent":
+        pytest.skip("one launch holds every job")
     env("MPOA_RES_BUCKET", "3")
     jo, ro, b = synth.jobs(seed=13, njobs=20, lmin=60, lmax=700, rmin=1, rmax=9)
     params = Params(adaptive=True)
@@ -142,6 +154,38 @@ def test_resident_jobs_leave_the_device_when_the_arena_is_too_small(ctx, oracle,
     st = res.stats()
     assert st["evicted_jobs"] > 0
     check_against_oracle(res, oracle, params, jo, ro, b)
+
+
+def test_resident_persistent_dps_extend_into_the_shared_pool(ctx, oracle, env, mode):
+    """Workgroup regions of 1 MiB and a 512 MiB shared pool: DPs outgrow
+    their region and take pool chunks instead of leaving the device."""
+    if mode == "captured":
+        pytest.skip("regions belong to the persistent kernel")
+    env("MPOA_RES_ARENA_MB", "1")
+    env("MPOA_RES_POOL_MB", "512")
+    jo, ro, b = synth.jobs(seed=19, njobs=12, lmin=100, lmax=900, rmin=2, rmax=8)
+    params = Params(adaptive=True)
+    res = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    st = res.stats()
+    assert st["evicted_jobs"] == 0 and st["resident_jobs"] == 12
+    check_against_oracle(res, oracle, params, jo, ro, b)
+
+
+def test_resident_modes_agree(ctx, env, mode):
+    """Persistent and captured runs of 96 workload jobs: same bytes, cells
+    and band overlaps."""
+    if mode == "captured":
+        pytest.skip("compared within the persistent case")
+    jo, ro, b = synth.jobs(seed=20263, njobs=96)
+    params = Params(adaptive=True)
+    a = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    env("MPOA_RES_PERSIST", "0")
+    c = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    for j in range(96):
+        assert (a.output(j, 0), a.output(j, 1), a.output(j, 2)) == (c.output(j, 0), c.output(j, 1), c.output(j, 2)), j
+    sa, sc = a.stats(), c.stats()
+    assert sa["cells"] == sc["cells"] and sa["pred_cells"] == sc["pred_cells"] and sa["dp_calls"] == sc["dp_calls"]
+    assert sa["buckets"] == 1 and sc["buckets"] > 1
 
 
 def test_resident_seeded_and_guide_order_jobs_run_on_the_host(ctx, oracle):
