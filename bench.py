@@ -4,10 +4,14 @@
 Workload: 4,096 independent DNA POA jobs, template length U[200, 2000],
 U[8, 32] reads per job at U[5%, 15%] error (60/20/20 sub/ins/del), default
 scores with the adaptive band, consensus + MSA output. One step = the whole
-workload (every progressive-insertion step of every job: one fill launch and
-one traceback launch per step across all jobs). Inputs are generated on the
-host before timing; the timed region covers planning, device DP, host graph
-fusion and rendering.
+workload (every progressive-insertion step of every job). By default the
+step runs device-resident (mpoa_batch_run_resident): one launch of the
+persistent kernel, in which each workgroup takes jobs from a device queue and
+runs each to its end -- graph fusion, re-ranking and DP planning, fill,
+traceback, once per read. `--engine host` runs the host-driven engine
+(mpoa_batch_run) instead: fill and traceback launches on several streams,
+planning and fusion on host threads. Inputs are generated on the host before
+timing; the timed region covers staging, every DP, fusion, and rendering.
 
 Prints one JSON line. value = aggregate GCUPS (DP cells of the reference's
 exact band geometry / wall second).
@@ -29,10 +33,10 @@ every one checked byte for byte against the oracle:
   rec the headline workload once more with the DP in the records layout
       (1-8 byte traceback records instead of M/D/I values): its step time,
       traceback bytes per cell, and its outputs against the values run;
-  res the headline workload with every job device-resident: graph fusion,
-      re-ranking and planning on the device, each length bucket's step loop
-      a replayed HIP graph; its step time and outputs against the host-
-      driven engine's.
+  host the headline workload on the host-driven engine (when the headline
+      is device-resident), on a context of its own: step time and outputs;
+  res the headline workload device-resident (when the headline is the host
+      engine).
 """
 import argparse
 import json
@@ -166,18 +170,23 @@ def main():
                     help="jobs run one DP per launch for the plain-path rate (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device-to-device copy of this size first (counter scaling)")
-    ap.add_argument("--resident", action="store_true",
-                    help="run the headline batch device-resident (mpoa_batch_run_resident)")
+    ap.add_argument("--engine", choices=["resident", "host"], default="resident",
+                    help="headline path: device-resident (mpoa_batch_run_resident, default) or the "
+                         "host-driven engine (mpoa_batch_run)")
+    ap.add_argument("--resident", dest="engine", action="store_const", const="resident",
+                    help="same as --engine resident")
     ap.add_argument("--hw-queues", type=int, default=None,
                     help="hardware queues per process (read before HIP starts; default 8)")
     ap.add_argument("--no-valu-probe", dest="valu_probe", action="store_false",
                     help="skip the int-VALU rate probe (the roofline then uses the nominal peak)")
-    ap.add_argument("--legs", default="rec,res,c1,c3,c4,c5",
-                    help="other workloads to report once each (rec, res, c1, c3, c4, c5; empty = none)")
+    ap.add_argument("--legs", default="rec,other,c1,c3,c4,c5",
+                    help="other workloads to report once each (rec, host, res, other = the path the "
+                         "headline does not take, c1, c3, c4, c5; empty = none)")
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=0, help="serving launch slots (0 = library default)")
     args = ap.parse_args()
+    args.resident = args.engine == "resident"
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(jobs would shard by HBM footprint with no collectives).", file=sys.stderr)
@@ -212,6 +221,11 @@ def main():
         dt = time.perf_counter() - t
         times.append(dt)
         stats.append(res.stats())
+        if args.resident:
+            log(f"step {s}: {dt:.3f}s cells={stats[-1]['cells']/1e9:.2f}G device-resident: "
+                f"device={stats[-1]['resident_device_ms']:.0f}ms resident={stats[-1]['resident_jobs']} "
+                f"evicted={stats[-1]['evicted_jobs']} launches={stats[-1]['step_replays']}")
+            continue
         log(f"step {s}: {dt:.3f}s cells={stats[-1]['cells']/1e9:.2f}G "
             f"fill={stats[-1]['fill_ms']:.0f}ms device={stats[-1]['device_ms']:.0f}ms "
             f"host={stats[-1]['host_ms']:.0f}ms (plan {stats[-1]['plan_ms']:.0f} "
@@ -238,15 +252,24 @@ def main():
     # the values layout, which the headline runs in).
     alg_bytes = st["arena_bytes"] or BYTES_PER_CELL * cells
     int_ops = 6 * cells + 5 * st["pred_cells"] + 2 * cells
-    # A device-resident run's fills are nodes of replayed HIP graphs, which
-    # events cannot bracket one by one: its per-launch figures come from the
-    # kernel trace (profiles/), and only the wall-based ones are reported here.
+    # The device-resident headline is one launch of the persistent kernel
+    # per step (step, fill and traceback of every job inside it), bracketed
+    # by HIP events on its own stream: that launch is the dominant kernel. A
+    # run of the captured step loops (MPOA_RES_PERSIST=0) has its fills as
+    # nodes of replayed HIP graphs, which events cannot bracket one by one:
+    # only the wall-based figures are reported for it.
+    persistent = args.resident and st["buckets"] == 1 and st["step_replays"] == 1
+    kernel = "poa_fill"
+    if persistent:
+        kernel = "rj_persist_kernel"
+        fill_s = st["resident_device_ms"] / 1e3
+        launches = 1
     per_launch = fill_s > 0
     achieved_gbs = alg_bytes / fill_s / 1e9 if per_launch else None
     rnd = lambda x, n: None if x is None else round(x, n)
     traffic = None
-    tfile = ROOT / "profiles" / "poa_fill_traffic.json"
-    if tfile.exists():
+    tfile = ROOT / "profiles" / ("rj_persist_traffic.json" if persistent else "poa_fill_traffic.json")
+    if tfile.exists() and (persistent or not args.resident):
         # HBM bytes per cell measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
         # passes over this workload (copy-kernel scaled), times this run's
         # cells per launch: profile-derived, not measured in this run.
@@ -257,9 +280,9 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": rnd(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": rnd(achieved_gbs and achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "traffic_source": "profile-derived (profiles/poa_fill_traffic.json x this run's cells)",
-        "kernel": "poa_fill", "launches": launches,
-        "avg_launch_ms": round(st["fill_ms"] / max(1, launches), 3) if per_launch else None,
+        "traffic_source": f"profile-derived (profiles/{tfile.name} x this run's cells)",
+        "kernel": kernel, "launches": launches,
+        "avg_launch_ms": round(fill_s * 1e3 / max(1, launches), 3) if per_launch else None,
         "bytes_per_launch": alg_bytes // max(1, launches),
         "bytes_per_cell": round(alg_bytes / max(1, cells), 3),
         # Over the wall: the step's algorithmic bytes per wall second (the
@@ -291,7 +314,7 @@ def main():
                    else "host-driven engine (mpoa_batch_run)"},
         "jobs_per_s": round(jobs_per_s, 1), "failed_jobs": failed,
         "dp_calls": st["dp_calls"], "retries": st["retries"], "i64_calls": st["i64_calls"],
-        "fill_gcups": round(cells / fill_s / 1e9, 2) if per_launch else None,
+        "kernel_gcups": round(cells / fill_s / 1e9, 2) if per_launch else None,
         "host_ms": round(st["host_ms"], 1), "device_ms": round(st["device_ms"], 1),
         "roofline": roofline,
         "hip_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
@@ -333,9 +356,11 @@ def main():
                                        args.serve_slots)
         if "c3" in legs:
             out["legs"]["c3"] = leg_c3(ctx, oracle, cores)
-        if "res" in legs:
-            out["legs"]["res"] = leg_resident(params, job_off, read_off, bases, res, gcups,
-                                              total / len(times))
+        other = "host" if args.resident else "res"
+        for name in ("host", "res"):
+            if name in legs or ("other" in legs and name == other):
+                out["legs"][name] = leg_path(name == "res", params, job_off, read_off, bases, res, gcups,
+                                             total / len(times))
         # One DP per launch on the context's own stream last: after that
         # stream has run, the batched legs on the same context measured
         # 40% slower (C4 690 ms instead of 400-450 ms).
@@ -379,10 +404,10 @@ def hashes_of(get_text, n):
 
 
 def leg_records(params, job_off, read_off, bases, values_res, values_gcups, values_s):
-    """The headline workload in the records layout (one warmup, one timed
-    run) on a context of its own: step time against the values layout's,
-    traceback bytes per cell, and every job's consensus + MSA against the
-    values run's."""
+    """The headline workload on the host-driven engine in the records layout
+    (one warmup, one timed run) on a context of its own: step time against
+    the headline's, traceback bytes per cell, and every job's consensus + MSA
+    against the headline run's."""
     import numpy as np
 
     import minipoa_hip as mh
@@ -398,57 +423,64 @@ def leg_records(params, job_off, read_off, bases, values_res, values_gcups, valu
     want = hashes_of(lambda j: values_res.output(j, 0) + values_res.output(j, 1), n)
     got = hashes_of(lambda j: r.output(j, 0) + r.output(j, 1), n)
     same = int((got == want).sum())
-    out = {"ms_per_step": round(dt * 1e3, 1), "gcups": round(st["cells"] / dt / 1e9, 3),
-           "vs_values": round(st["cells"] / dt / 1e9 / values_gcups, 3),
-           "values_ms_per_step": round(values_s * 1e3, 1),
+    out = {"mode": "host-driven engine, records layout", "ms_per_step": round(dt * 1e3, 1),
+           "gcups": round(st["cells"] / dt / 1e9, 3),
+           "vs_headline": round(st["cells"] / dt / 1e9 / values_gcups, 3),
+           "headline_ms_per_step": round(values_s * 1e3, 1),
            "fill_ms": round(st["fill_ms"], 1), "record_launches": st["record_launches"],
            "fill_launches": st["fill_launches"],
            "bytes_per_cell": round(st["arena_bytes"] / max(1, st["cells"]), 3),
-           "identical_to_values": same, "jobs": n}
+           "identical_to_headline": same, "jobs": n}
     del r
     c.close()
-    log(f"rec: records layout {dt*1e3:.0f} ms ({out['vs_values']:.2f}x values), "
+    log(f"rec: records layout {dt*1e3:.0f} ms ({out['vs_headline']:.2f}x headline), "
         f"{out['bytes_per_cell']} B/cell, identical {same}/{n}")
     return out
 
 
-def leg_resident(params, job_off, read_off, bases, values_res, values_gcups, values_s):
-    """The headline workload with every job device-resident (one warmup, two
-    timed runs): graph fusion, re-ranking and DP planning on the device, one
-    captured step graph per length bucket replayed until its jobs finish.
-    Step time against the host-driven engine, and every job's consensus +
-    MSA against the values run's."""
+def leg_path(resident, params, job_off, read_off, bases, head_res, head_gcups, head_s):
+    """The headline workload on the path the headline does not take (one
+    warmup, two timed runs, on a context of its own): device-resident
+    (mpoa_batch_run_resident) or the host-driven engine (mpoa_batch_run).
+    Step time against the headline's, and every job's consensus + MSA
+    against the headline run's."""
     import minipoa_hip as mh
 
-    # A context of its own, closed afterwards: its per-slot device buffers
-    # (arenas, graph blocks) do not stay beside the main context's.
+    # A context of its own, closed afterwards: its device buffers (arenas,
+    # graph blocks) do not stay beside the main context's.
     ctx = mh.Context(0)
-    mh.batch_run(ctx, params, job_off, read_off, bases, resident=True)
+    mh.batch_run(ctx, params, job_off, read_off, bases, resident=resident)
     times = []
     for _ in range(2):
         t = time.perf_counter()
-        r = mh.batch_run(ctx, params, job_off, read_off, bases, resident=True)
+        r = mh.batch_run(ctx, params, job_off, read_off, bases, resident=resident)
         times.append(time.perf_counter() - t)
         st = r.stats()
         if len(times) < 2:
             del r
     dt = min(times)
     n = len(job_off) - 1
-    want = hashes_of(lambda j: values_res.output(j, 0) + values_res.output(j, 1), n)
+    want = hashes_of(lambda j: head_res.output(j, 0) + head_res.output(j, 1), n)
     got = hashes_of(lambda j: r.output(j, 0) + r.output(j, 1), n)
     same = int((got == want).sum())
-    out = {"ms_per_step": round(dt * 1e3, 1), "ms_runs": [round(x * 1e3, 1) for x in times],
+    out = {"mode": "device-resident (mpoa_batch_run_resident)" if resident
+           else "host-driven engine (mpoa_batch_run)",
+           "ms_per_step": round(dt * 1e3, 1), "ms_runs": [round(x * 1e3, 1) for x in times],
            "gcups": round(st["cells"] / dt / 1e9, 3),
-           "vs_host_engine": round(st["cells"] / dt / 1e9 / values_gcups, 3),
-           "host_engine_ms_per_step": round(values_s * 1e3, 1),
-           "resident_jobs": st["resident_jobs"], "evicted_jobs": st["evicted_jobs"],
-           "buckets": st["buckets"], "step_replays": st["step_replays"],
-           "step_loop_device_ms": round(st["resident_device_ms"], 1),
+           "vs_headline": round(st["cells"] / dt / 1e9 / head_gcups, 3),
+           "headline_ms_per_step": round(head_s * 1e3, 1),
            "host_ms": round(st["host_ms"], 1),
-           "identical_to_host_engine": same, "jobs": n}
+           "identical_to_headline": same, "jobs": n}
+    if resident:
+        out.update({"resident_jobs": st["resident_jobs"], "evicted_jobs": st["evicted_jobs"],
+                    "buckets": st["buckets"], "step_replays": st["step_replays"],
+                    "resident_device_ms": round(st["resident_device_ms"], 1)})
+    else:
+        out.update({"fill_ms": round(st["fill_ms"], 1), "fill_launches": st["fill_launches"],
+                    "device_ms": round(st["device_ms"], 1)})
     del r
     ctx.close()
-    log(f"res: device-resident {dt*1e3:.0f} ms ({out['vs_host_engine']:.2f}x host engine), "
+    log(f"{'res' if resident else 'host'}: {out['mode']} {dt*1e3:.0f} ms ({out['vs_headline']:.2f}x headline), "
         f"identical {same}/{n}")
     return out
 

@@ -718,6 +718,8 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         Q.next = reinterpret_cast<uint32_t *>(dA + a_ctr + 192);  // zero from the staging copy
         Q.arena_off = 0;
         Q.slot_bytes = pc->slot_bytes;
+        Q.prio = getenv("MPOA_PERSIST_PRIO") ? uint32_t(atoi(getenv("MPOA_PERSIST_PRIO"))) & 3u : 3u;
+        Q.ncrit = getenv("MPOA_PERSIST_CRIT") ? uint32_t(atoi(getenv("MPOA_PERSIST_CRIT"))) : 0u;
         // The shared pool follows the regions: its counter starts at the
         // pool's first byte (written with the staging copy).
         P.pool_next = reinterpret_cast<unsigned long long *>(dA + a_ctr + 128);
@@ -731,6 +733,7 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         // The step gets the LDS the fill and the traceback need anyway, so
         // four workgroups still fit a CU.
         Q.step_lds = rj_persist_lds(P, 0) & ~15u;
+        if (const char *e = getenv("MPOA_PERSIST_STEP_LDS")) Q.step_lds = uint32_t(atoi(e)) & ~15u;
         R.lds_bytes = Q.step_lds;
         R.job_cap = pc->slot_bytes + pc->pool_bytes / 4;
         if (!slot.ev[0]) RJ_TRY(hipEventCreate(&slot.ev[0]));
@@ -980,7 +983,19 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
         const char *e = getenv("MPOA_RES_PERSIST");
         return !e || atoi(e) != 0;
     }();
-    if (persist) bsize = std::max<uint64_t>(res_jobs.size(), 1);
+    if (persist) {
+        bsize = std::max<uint64_t>(res_jobs.size(), 1);
+        if (!getenv("MPOA_PERSIST_ORDER") || atoi(getenv("MPOA_PERSIST_ORDER")) != 0) {
+            // Longest expected chain first: the reads' DPs grow with the
+            // template and, as the graph takes in reads, with their count.
+            std::vector<double> key(N, 0.0);
+            for (uint64_t j : res_jobs) {
+                const double nr = double(in.job_off[j + 1] - in.job_off[j]);
+                key[j] = double(L[j]) * nr * (1.0 + 0.05 * nr);
+            }
+            std::stable_sort(res_jobs.begin(), res_jobs.end(), [&](uint64_t a, uint64_t b) { return key[a] > key[b]; });
+        }
+    }
     for (uint64_t b0 = 0; b0 < res_jobs.size(); b0 += bsize)
         S.buckets.emplace_back(res_jobs.begin() + b0, res_jobs.begin() + std::min<uint64_t>(res_jobs.size(), b0 + bsize));
     // Arena per slot: half the device memory over the slots, at most 6 GiB
