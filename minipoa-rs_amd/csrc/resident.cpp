@@ -730,10 +730,13 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
             const uint32_t room = 65536u - 10240u - dyn_other;
             if (P.ring_bytes > room) P.ring_bytes = room & ~63u;
         }
-        // The step gets the LDS the fill and the traceback need anyway, so
-        // four workgroups still fit a CU.
-        Q.step_lds = rj_persist_lds(P, 0) & ~15u;
-        if (const char *e = getenv("MPOA_PERSIST_STEP_LDS")) Q.step_lds = uint32_t(atoi(e)) & ~15u;
+        // The step gets what four workgroups per CU leave (a gfx950 CU holds
+        // 160 KB: 40 KB each, less the kernel's 16 static bytes), or what the
+        // fill and the traceback need if that is more. Measured on the
+        // 4,096-job workload: 40 KB instead of the ~34.5 KB the other phases
+        // need, launch 998 -> 990 ms (larger graphs re-rank and plan in LDS).
+        Q.step_lds = std::max(rj_persist_lds(P, 0), 40944u) & ~15u;
+        if (const char *e = getenv("MPOA_PERSIST_STEP_LDS")) Q.step_lds = std::max(Q.step_lds, uint32_t(atoi(e)) & ~15u);
         R.lds_bytes = Q.step_lds;
         R.job_cap = pc->slot_bytes + pc->pool_bytes / 4;
         if (!slot.ev[0]) RJ_TRY(hipEventCreate(&slot.ev[0]));
