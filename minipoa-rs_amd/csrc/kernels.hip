@@ -1839,10 +1839,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
         const uint32_t k = uint32_t(__builtin_amdgcn_readfirstlane(int(xjob)));
         sync_lds<W>();  // xjob read by every wave before the next job's write
         if (k >= R.njobs) break;
-        // The jobs with the longest chains bound the launch: every wave of
-        // theirs at top issue priority for the whole job.
-        const bool crit = k < Q.ncrit;
-        if (crit) __builtin_amdgcn_s_setprio(3);
         // A read takes one step, plus at most two for poa's full-DP retry and
         // a full-width arena; past this bound the job leaves for the host
         // engine, which continues it from its next unfused read.
@@ -1853,7 +1849,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                 // The step is a serial chain on one wave while other
                 // workgroups fill on the same SIMDs: at raised priority it
                 // takes their issue slots first.
-                if (!crit && (Q.prio & 1u)) __builtin_amdgcn_s_setprio(2);
+                if (Q.prio & 1u) __builtin_amdgcn_s_setprio(2);
                 if (it < bound) {
                     rj_job_step(t, R, k, pl_lds, Q.step_lds, slot_off, Q.slot_bytes);
                 } else if (tid == 0) {
@@ -1865,7 +1861,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                     R.dj[k].n = 0;
                 }
-                if (!crit && (Q.prio & 1u)) __builtin_amdgcn_s_setprio(0);
+                if (Q.prio & 1u) __builtin_amdgcn_s_setprio(0);
             }
             sync_global<W>();  // the planned DP (or n == 0: the job left) for every wave
             const uint32_t n = uint32_t(__builtin_amdgcn_readfirstlane(int(glb(&R.dj[k].n)[0])));
@@ -1873,13 +1869,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             fill_job<int32_t, C, W, true, uint32_t, false>(P, k, pl_lds);
             sync_global<W>();
             if (wave == 0) {
-                if (!crit && (Q.prio & 2u)) __builtin_amdgcn_s_setprio(2);
+                if (Q.prio & 2u) __builtin_amdgcn_s_setprio(2);
                 tbv_job<int32_t, true>(P, k, pl_lds);
-                if (!crit && (Q.prio & 2u)) __builtin_amdgcn_s_setprio(0);
+                if (Q.prio & 2u) __builtin_amdgcn_s_setprio(0);
             }
             sync_global<W>();
         }
-        if (crit) __builtin_amdgcn_s_setprio(0);
     }
 }
 

@@ -719,7 +719,6 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         Q.arena_off = 0;
         Q.slot_bytes = pc->slot_bytes;
         Q.prio = getenv("MPOA_PERSIST_PRIO") ? uint32_t(atoi(getenv("MPOA_PERSIST_PRIO"))) & 3u : 3u;
-        Q.ncrit = getenv("MPOA_PERSIST_CRIT") ? uint32_t(atoi(getenv("MPOA_PERSIST_CRIT"))) : 0u;
         // The shared pool follows the regions: its counter starts at the
         // pool's first byte (written with the staging copy).
         P.pool_next = reinterpret_cast<unsigned long long *>(dA + a_ctr + 128);
@@ -1026,7 +1025,8 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
             cus = 256;
         }
         pcfg.grid = uint32_t(std::min<uint64_t>(res_jobs.size(), 4ull * uint64_t(cus)));
-        const uint64_t lmax = L[res_jobs.front()];
+        uint64_t lmax = 0;  // the longest template (the queue is ordered by chain length)
+        for (uint64_t j : res_jobs) lmax = std::max(lmax, L[j]);
         uint64_t want = up256(3 * (lmax + 2) * uint64_t(expected_width(p, lmax)) * 12 * 3 / 2);
         want = std::max<uint64_t>(want, uint64_t(16) << 20);
         want = std::min<uint64_t>(want, free_b / 2 / pcfg.grid / 9 * 8);

@@ -127,7 +127,6 @@ struct RPersist {
     uint64_t slot_bytes;  // bytes per region (the largest DP arena a job may use)
     uint32_t step_lds;    // LDS bytes of the step (the rest of the kernel's LDS aliases it)
     uint32_t prio;        // bit 0: the step's wave, bit 1: the traceback's, at raised issue priority
-    uint32_t ncrit;       // the first ncrit jobs of the queue (the longest chains) run at top priority
 };
 
 // ---------------------------------------------------------------------------
