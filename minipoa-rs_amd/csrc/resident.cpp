@@ -1061,6 +1061,23 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
             if (e == hipSuccess) e = x.dg.ensure(need_a);
             if (e == hipSuccess) e = x.dp.ensure(need_b);
             if (e == hipSuccess) e = x.da.ensure_exact(S.arena_bytes);
+            // An arena the device cannot give (free memory is only an
+            // estimate, other allocations may come between): smaller regions
+            // (or a smaller bucket arena), down to 16 MiB; DPs that outgrow
+            // them leave the device.
+            while (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+                (void)hipGetLastError();
+                if (persist) {
+                    if (pcfg.slot_bytes / 2 < (uint64_t(1) << 24)) break;
+                    pcfg.slot_bytes = pcfg.slot_bytes / 2 / 256 * 256;
+                    pcfg.pool_bytes = pcfg.pool_bytes / 2 / 256 * 256;
+                    S.arena_bytes = pcfg.slot_bytes * pcfg.grid + pcfg.pool_bytes;
+                } else {
+                    if (S.arena_bytes / 2 < (uint64_t(1) << 24)) break;
+                    S.arena_bytes = S.arena_bytes / 2 / 256 * 256;
+                }
+                e = x.da.ensure_exact(S.arena_bytes);
+            }
             if (e == hipSuccess) e = x.dk.ensure(std::max<uint64_t>(need_pack, 256));
             if (e == hipSuccess) e = x.hin.ensure(need_h);
             if (e == hipSuccess) e = x.hjobs.ensure(sizeof(RJob) * need_n);
