@@ -52,7 +52,8 @@ def test_seeded_long_read_job_full_depth_matches_oracle(ctx, oracle):
 def test_protein_families_full_size_match_oracle(ctx, oracle):
     """2,000 protein families (root U[100,800], 16 members at 30%
     divergence), 20-letter alphabet, drawn symmetric matrix through the -m
-    file format, full DP: every family's consensus + MSA equals the oracle's."""
+    file format, full DP: every family's consensus + MSA equals the oracle's,
+    on the host-driven engine and device-resident."""
     params, jo, ro, b = synth.protein_families()
     assert len(jo) - 1 == 2000 and all(int(jo[k + 1] - jo[k]) == 16 for k in range(2000))
     import tempfile
@@ -62,12 +63,17 @@ def test_protein_families_full_size_match_oracle(ctx, oracle):
         f = Path(d) / "protein.mtx"
         f.write_text(synth.matrix_text(params.alphabet, params.matrix))
         params = mh.load_matrix(Params(f=0), f)
-    res = mh.batch_run(ctx, params, jo, ro, b)
     cells, want = oracle.batch_run(params, jo, ro, b, CORES)
-    got = np.array([fnv1a((res.output(j, 0) + res.output(j, 1)).encode()) for j in range(2000)],
-                   dtype=np.uint64)
-    assert res.stats()["cells"] == cells
-    assert (got == want).all(), np.nonzero(got != want)
+    # The host-driven engine, then device-resident (the persistent kernel).
+    for resident in (False, True):
+        res = mh.batch_run(ctx, params, jo, ro, b, resident=resident)
+        got = np.array([fnv1a((res.output(j, 0) + res.output(j, 1)).encode()) for j in range(2000)],
+                       dtype=np.uint64)
+        assert res.stats()["cells"] == cells, resident
+        assert (got == want).all(), (resident, np.nonzero(got != want))
+        if resident:
+            assert res.stats()["resident_jobs"] == 2000
+        del res
 
 
 # Jobs of the serving stream (seed 20265): 19 (15,522 bp, 50 reads loaded

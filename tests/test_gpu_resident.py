@@ -188,6 +188,38 @@ def test_resident_modes_agree(ctx, env, mode):
     assert sa["buckets"] == 1 and sc["buckets"] > 1
 
 
+def test_resident_wide_rows_and_long_reads_match_oracle(ctx, oracle):
+    """Templates of 3,000-12,500 bases: rows wider than one 4x4 stripe (1,024
+    columns) and query codes of up to 12.5 KB in LDS, which shrink the fill's
+    ring and (persistent kernel) the workgroups per CU."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    jobs = []
+    for L, R in ((3000, 4), (6000, 3), (12500, 2)):
+        tmpl = rng.integers(0, 4, size=L).astype(np.uint8)
+        jobs.append([synth.mutate(rng, tmpl, 0.08) for _ in range(R)])
+    jo, ro, b = pack(jobs)
+    params = Params(adaptive=True)
+    res = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    st = res.stats()
+    assert st["resident_jobs"] == 3 and st["evicted_jobs"] == 0
+    check_against_oracle(res, oracle, params, jo, ro, b)
+
+
+def test_resident_mixed_small_and_long_jobs_match_oracle(ctx, oracle):
+    """Forty small jobs beside two long ones in one batch: the persistent
+    kernel's arena regions are sized by the longest template, and the queue
+    takes the long jobs first."""
+    small = synth.jobs(seed=37, njobs=40, lmin=60, lmax=300, rmin=2, rmax=6)
+    big = synth.jobs(seed=38, njobs=2, lmin=3500, lmax=4000, rmin=3, rmax=4)
+    reads = [synth.job_reads(*small, j) for j in range(40)] + [synth.job_reads(*big, j) for j in range(2)]
+    jo, ro, b = pack(reads)
+    params = Params(adaptive=True)
+    res = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    st = res.stats()
+    assert st["resident_jobs"] == 42 and st["evicted_jobs"] == 0
+    check_against_oracle(res, oracle, params, jo, ro, b, gfa=False)
+
+
 def test_resident_seeded_and_guide_order_jobs_run_on_the_host(ctx, oracle):
     """Seeding and the -p guide order stay on the host-driven engine."""
     jo, ro, b = synth.jobs(seed=23, njobs=4, lmin=300, lmax=600, rmin=3, rmax=5)
