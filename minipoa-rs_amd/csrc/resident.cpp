@@ -1024,7 +1024,9 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
             (void)hipGetLastError();
             cus = 256;
         }
-        pcfg.grid = uint32_t(std::min<uint64_t>(res_jobs.size(), 4ull * uint64_t(cus)));
+        uint64_t per_cu = 4;
+        if (const char *e = getenv("MPOA_PERSIST_WGS")) per_cu = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+        pcfg.grid = uint32_t(std::min<uint64_t>(res_jobs.size(), per_cu * uint64_t(cus)));
         uint64_t lmax = 0;  // the longest template (the queue is ordered by chain length)
         for (uint64_t j : res_jobs) lmax = std::max(lmax, L[j]);
         uint64_t want = up256(3 * (lmax + 2) * uint64_t(expected_width(p, lmax)) * 12 * 3 / 2);
