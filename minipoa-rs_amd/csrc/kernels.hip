@@ -361,6 +361,7 @@ template <typename T, int W> struct FillLds {
     T xcar[2][W];          // per-wave scan totals, by stripe parity
     T xbest[W];            // per-wave row maximum of M
     uint32_t xbestc[W];    // its first column
+    uint32_t xbestq[W];    // the query code there
     uint64_t xpool;        // arena chunk taken by thread 0
     int64_t xscore;        // M at (n-1, m-1)
     alignas(16) T negrow[16];  // NEG: reads outside a predecessor's band
@@ -385,6 +386,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
     auto &xcar = FL.xcar;
     auto &xbest = FL.xbest;
     auto &xbestc = FL.xbestc;
+    auto &xbestq = FL.xbestq;
     auto &xpool = FL.xpool;
     auto &xscore = FL.xscore;
     auto &negrow = FL.negrow;
@@ -775,10 +777,12 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
         }
         if (anymiss) sync_global<W>();
 
-        const uint32_t srow = ((i == n - 1) ? J.unknown : daux[i % kBlk].base) * alpha;
+        const uint32_t rbase = daux[i % kBlk].base;
+        const uint32_t srow = ((i == n - 1) ? J.unknown : rbase) * alpha;
         T carry = NEG;  // insertion carry into the stripe's first column
         T best = NI;
         uint32_t bestc = 0xFFFFFFFFu;
+        uint32_t bestq = 0;  // LT: the query code of column bestc
         uint32_t parity = 0;
         bool lost = false;  // a predecessor's scores were in neither ring nor HBM
         // The stripes of the row, compiled separately for rows with one
@@ -797,6 +801,11 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                 // prefix.
                 T Mdg[C], Do[C], De[C], Dv[C], md[C], Lk[C];
                 R pk[C];
+                // LT: the lane's query codes, four per word (the argmax
+                // column's code is then at hand for the adaptive update).
+                uint32_t qw[C / 4];
+#pragma unroll
+                for (int k = 0; k < C / 4; k++) qw[k] = 0;
                 T S = NI, Sp = NI;
                 const uint32_t valid = c0 < stop ? tmin(stop - c0, uint32_t(C)) : 0u;
                 const uint32_t rel = c0 - start;
@@ -945,10 +954,13 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
 #pragma unroll
                                 for (int k = 0; k < 8; k++)
                                     codes[k] = ((k < 4 ? w2.x : w2.y) >> (8 * (k & 3))) & 0xFFu;
+                                qw[sg / 4] = w2.x;
+                                qw[sg / 4 + 1] = w2.y;
                             } else {
                                 const uint32_t w1 = *reinterpret_cast<const uint32_t *>(sseq + cr);
 #pragma unroll
                                 for (int k = 0; k < 4; k++) codes[k] = (w1 >> (8 * k)) & 0xFFu;
+                                qw[sg / 4] = w1;
                             }
 #pragma unroll
                             for (int k = 0; k < G; k++) pf[k] = T(smat[srow + codes[k]]);
@@ -1005,6 +1017,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                     if (uint32_t(k) < valid && Mv[k] > best) {
                         best = Mv[k];
                         bestc = c0 + k;
+                        if constexpr (LT) bestq = (qw[k / 4] >> (8 * (k & 3))) & 0xFFu;
                     }
                     if constexpr (!REC) {
                         Ivs[k] = Iv;
@@ -1085,23 +1098,29 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
             // First argmax of M over the row (src/align.rs:830-844), then the
             // streak / extra / offset update of src/align.rs:845-865.
             T gmax = wave_max<T>(best, NI);
-            uint32_t maxc;
+            uint32_t maxc, maxq;
             if (len <= 64u * C * W) {
                 // One stripe: lanes own increasing columns, so the lowest
                 // lane holding the maximum has its first column.
-                const uint64_t hit = __builtin_amdgcn_ballot_w64(best == gmax);
-                maxc = rl32(bestc, uint32_t(__builtin_ctzll(hit)));
+                const uint32_t hl = uint32_t(__builtin_ctzll(__builtin_amdgcn_ballot_w64(best == gmax)));
+                maxc = rl32(bestc, hl);
+                maxq = rl32(bestq, hl);
             } else {
                 maxc = wave_min(best == gmax ? bestc : 0xFFFFFFFFu);
+                // Columns are unique to lanes: one lane holds maxc.
+                const uint64_t at = __builtin_amdgcn_ballot_w64(best == gmax && bestc == maxc);
+                maxq = rl32(bestq, at ? uint32_t(__builtin_ctzll(at)) : 0u);
             }
             if constexpr (W > 1) {
                 if (lane == 0) {
                     xbest[wave] = gmax;
                     xbestc[wave] = maxc;
+                    xbestq[wave] = maxq;
                 }
                 sync_lds<W>();  // also publishes the ring writes
                 gmax = xbest[0];
                 maxc = xbestc[0];
+                maxq = xbestq[0];
 #pragma unroll
                 for (uint32_t w = 1; w < W; w++) {
                     const T b = xbest[w];
@@ -1109,6 +1128,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                     if (b > gmax || (b == gmax && c < maxc)) {
                         gmax = b;
                         maxc = c;
+                        maxq = xbestq[w];
                     }
                 }
             }
@@ -1117,10 +1137,10 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
             bool is_match = false;
             uint32_t qbase;
             if constexpr (LT)
-                qbase = sseq[maxc];
+                qbase = maxq;
             else
                 qbase = glb(seq)[maxc];
-            if (daux[i % kBlk].base == qbase) {
+            if (rbase == qbase) {
                 for (uint32_t g0 = pb; g0 < pe; g0 += 64) {
                     if (np > 64) gather(g0);
                     const uint32_t cnt = tmin(64u, pe - g0);
