@@ -364,6 +364,9 @@ template <typename T, int W> struct FillLds {
     uint32_t xbestq[W];    // the query code there
     uint64_t xpool;        // arena chunk taken by thread 0
     int64_t xscore;        // M at (n-1, m-1)
+    // The DP's statistics, added by thread 0 (kept out of the registers
+    // that the row loop carries).
+    unsigned long long xcells, xpcells, xused;
     alignas(16) T negrow[16];  // NEG: reads outside a predecessor's band
 };
 template <typename T, int W> constexpr uint32_t fill_fixed_lds() {
@@ -389,6 +392,9 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
     auto &xbestq = FL.xbestq;
     auto &xpool = FL.xpool;
     auto &xscore = FL.xscore;
+    auto &xcells = FL.xcells;
+    auto &xpcells = FL.xpcells;
+    auto &xused = FL.xused;
     auto &negrow = FL.negrow;
     char *dyn_lds = lds + fill_fixed_lds<T, W>();
     const uint32_t kRingBytes = P.ring_bytes;  // recent rows' M and D
@@ -428,14 +434,18 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
     const uint32_t alpha = J.alpha;
     constexpr bool mat_lds = LT, seq_lds = LT;
     if (tid < 16) negrow[tid] = NEG;
-    if (leader) xscore = int64_t(NEG);
+    if (leader) {
+        xscore = int64_t(NEG);
+        xcells = 0;
+        xpcells = 0;
+        xused = 0;
+    }
     if (mat_lds)
         for (uint32_t k = tid; k < alpha * alpha; k += 64u * W) smat[k] = glb(gmat)[k];
     if (seq_lds)
         for (uint32_t k = tid; k < J.padded; k += 64u * W) sseq[k] = glb(seq)[k];
 
     uint64_t cur = J.arena, rend = J.arena + J.arena_cap;  // current arena region (bytes)
-    uint64_t used = 0, cells = 0, pcells = 0;
     uint32_t wpos = 0;      // ring bytes handed out so far
     uint32_t ring_at = 0;   // wpos modulo the ring size
     uint32_t prev_slot = kNoSlot;  // ring slot of row i - 1 (kNoSlot: none)
@@ -656,8 +666,10 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
         const uint64_t off = cur;
         const uint64_t voff = keep ? off + rec_sz : kNoValues;
         cur += need;
-        used += need;
-        cells += len;
+        if (leader) {
+            atomicAdd(&xused, (unsigned long long)need);
+            atomicAdd(&xcells, (unsigned long long)len);
+        }
         uint8_t *Rr = arena + off;
         T *Mk = keep ? reinterpret_cast<T *>(arena + voff) : nullptr;
         T *Dk = keep ? Mk + len : nullptr;
@@ -702,7 +714,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                     }
                     const uint64_t dst = cur;
                     cur += sb;
-                    used += sb;
+                    if (leader) atomicAdd(&xused, (unsigned long long)sb);
                     const int4 *src4 = reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ring) + h.slot);
                     int4 *dst4 = reinterpret_cast<int4 *>(arena + dst);
                     for (uint32_t k = tid; k < uint32_t(sb / 16); k += 64u * W) dst4[k] = src4[k];
@@ -854,7 +866,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                             const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
                             const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
                             if (ovs >= ove) continue;
-                            if (seg == start) pcells += ove - ovs;
+                            if (leader && seg == start) atomicAdd(&xpcells, (unsigned long long)(ove - ovs));
                             const uint32_t pmo = rl32(g_meta.slot, q) / uint32_t(sizeof(T));
 #pragma unroll
                             for (int sg = 0; sg < C; sg += G) {
@@ -878,7 +890,7 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
                             const uint32_t ps = rl32(g_meta.start, q), pl = rl32(g_meta.len, q);
                             const uint32_t ovs = tmax(start, ps), ove = tmin(stop, ps + pl);
                             if (ovs >= ove) continue;
-                            if (seg == start) pcells += ove - ovs;
+                            if (leader && seg == start) atomicAdd(&xpcells, (unsigned long long)(ove - ovs));
                             const uint32_t pslot = rl32(g_meta.slot, q), pw = rl32(g_meta.wpos, q);
                             const uint32_t idx = g0 - pb + q;
                             const bool in_ring = pslot != kNoSlot && wpos - pw <= kRingBytes;
@@ -1175,9 +1187,9 @@ __device__ __forceinline__ void fill_job(const DevPools &P, uint32_t b, char *ld
     if (leader) {
         DevResult res{};
         res.status = status;
-        res.cells = cells;
-        res.pred_cells = pcells;
-        res.arena_used = used;
+        res.cells = xcells;
+        res.pred_cells = xpcells;
+        res.arena_used = xused;
         res.score = status == kOk ? xscore : int64_t(NEG);
         P.res[b] = res;
     }
