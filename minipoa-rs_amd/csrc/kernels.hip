@@ -1862,6 +1862,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     RPools R, DevPools P, RPersist Q) {
     extern __shared__ __align__(16) char pl_lds[];
     __shared__ uint32_t xjob;
+    __shared__ unsigned long long xsoff;
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = uint32_t(__builtin_amdgcn_readfirstlane(int(tid >> 6)));
     const uint64_t slot_off = Q.arena_off + uint64_t(blockIdx.x) * Q.slot_bytes;
@@ -1906,6 +1907,33 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                 if (Q.prio & 2u) __builtin_amdgcn_s_setprio(0);
             }
             sync_global<W>();
+        }
+        if (Q.sbuf) {
+            // The job has left the loop with its final state and graph
+            // (every wave's stores published by the last barrier): pack it
+            // and publish its entry. The releases are system-wide so that a
+            // copy engine reading device memory sees the bytes before the
+            // entry.
+            const RJob &J = R.jobs[k];
+            const uint64_t bytes = rj_pack_layout(J.nn, J.ne, J.nreads, J.npath).bytes;
+            if (tid == 0) xsoff = atomicAdd(Q.scur, (unsigned long long)bytes);
+            sync_lds<W>();
+            const uint64_t off = xsoff;
+            if (off + bytes <= Q.scap) {
+                rj_pack_job(J, Q.sbuf + off, tid, 64u * W);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                sync_lds<W>();
+                if (tid == 0) {
+                    uint32_t *e = Q.sent + 4ull * k;
+                    e[1] = uint32_t(off);
+                    e[2] = uint32_t(off >> 32);
+                    e[3] = uint32_t(bytes);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                    __hip_atomic_store(e, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                }
+            }
+            sync_lds<W>();  // xsoff read by every wave before the next job's write
         }
     }
 }

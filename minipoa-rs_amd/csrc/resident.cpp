@@ -417,11 +417,15 @@ int resident_check(const Params &p, const uint8_t *bases, const uint64_t *read_o
 struct RSlot {
     Ctx *ctx = nullptr;
     DevBuf dg, dp, da, dk;
-    PinBuf hin, hjobs, hpack, hoff;
+    PinBuf hin, hjobs, hpack, hoff, hsent, hsjobs;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t s2 = nullptr;  // copies of finished jobs while the persistent launch runs
     ~RSlot() {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (s2) (void)hipStreamDestroy(s2);
+        hsent.release();
+        hsjobs.release();
         dg.release();
         dp.release();
         da.release();
@@ -473,6 +477,9 @@ struct RShared {
     BatchOutput *out = nullptr;
     uint64_t arena_bytes = 0;
     uint32_t nslots = 1;
+    // Persistent mode: the stream of a launch slot that the launch leaves
+    // idle (its own hardware queue), for copies while the launch runs.
+    hipStream_t copy_stream = nullptr;
     std::mutex mu;
     std::vector<std::vector<uint64_t>> buckets;
     std::atomic<size_t> next{0};
@@ -524,7 +531,7 @@ struct BucketLayout {
     std::vector<RGraphLayout> gl;
     uint32_t maxpad = 8, rmax = 0;
     uint64_t lmax = 0, nmax = 0;
-    uint64_t a_prm, a_mat, a_ctr, a_jobs, a_dj, a_res, a_poff, a_bases, a_roff, a_graph, a_end;
+    uint64_t a_prm, a_mat, a_ctr, a_jobs, a_dj, a_res, a_poff, a_sent, a_bases, a_roff, a_graph, a_end;
     uint64_t b_rows, b_aux, b_preds, b_seq, b_meta, b_adapt, b_tb, b_end;
 };
 
@@ -561,7 +568,8 @@ BucketLayout layout_bucket(const Params &p, const BatchInput &in, const std::vec
     B.a_dj = up256(B.a_jobs + sizeof(RJob) * N);
     B.a_res = up256(B.a_dj + sizeof(DevJob) * N);
     B.a_poff = up256(B.a_res + sizeof(DevResult) * N);
-    B.a_bases = up256(B.a_poff + 8ull * N);
+    B.a_sent = up256(B.a_poff + 8ull * N);
+    B.a_bases = up256(B.a_sent + 16ull * N);
     B.a_roff = up256(B.a_bases + nbases);
     B.a_graph = up256(B.a_roff + 8 * noffs);
     B.a_end = up256(B.a_graph + gbytes);
@@ -576,8 +584,23 @@ BucketLayout layout_bucket(const Params &p, const BatchInput &in, const std::vec
     return B;
 }
 
+// Jobs rendered while the persistent launch still runs: each finished
+// job's packed graph and record come back as soon as its workgroup
+// publishes it (RPersist::sent), and a host worker rebuilds and renders it.
+struct StreamState {
+    std::vector<uint8_t> taken;                  // job handed to a worker
+    std::vector<std::unique_ptr<Graph>> graphs;  // jobs that leave the device
+    std::vector<uint8_t> leaves;
+    std::vector<std::string> names;
+    std::atomic<uint32_t> pending{0};
+    uint64_t end = 0;  // end of the streamed bytes in the pack buffers
+    uint32_t *d_sent = nullptr;
+    hipStream_t s2 = nullptr;
+};
+
 int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const BucketLayout &BL,
-                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays);
+                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays, StreamState *ss = nullptr);
+int stream_poll(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const RPools &R, StreamState &ss);
 
 int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &jobs,
                const PersistCfg *pc = nullptr) {
@@ -740,17 +763,83 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
         R.job_cap = pc->slot_bytes + pc->pool_bytes / 4;
         if (!slot.ev[0]) RJ_TRY(hipEventCreate(&slot.ev[0]));
         if (!slot.ev[1]) RJ_TRY(hipEventCreate(&slot.ev[1]));
+        // Streaming (MPOA_PERSIST_STREAM=0 turns it off): workgroups pack
+        // the jobs they finish into the pack buffer and publish them; this
+        // thread copies them back on a second stream while the launch runs
+        // and host workers render them, so little is left after it ends.
+        const bool stream = [] {
+            const char *e = getenv("MPOA_PERSIST_STREAM");
+            return !e || atoi(e) != 0;
+        }();
+        StreamState ss;
+        if (stream) {
+            // A stream that shares the launch's hardware queue would run the
+            // copies only after the launch: another slot's stream has a
+            // queue of its own.
+            if (!S.copy_stream && !slot.s2) RJ_TRY(hipStreamCreateWithFlags(&slot.s2, hipStreamNonBlocking));
+            ss.s2 = S.copy_stream ? S.copy_stream : slot.s2;
+            RJ_TRY(slot.hsent.ensure(16ull * N));
+            RJ_TRY(slot.hsjobs.ensure(sizeof(RJob) * N));
+            ss.taken.assign(N, 0);
+            ss.graphs.resize(N);
+            ss.leaves.assign(N, 0);
+            ss.names.resize(rmax);
+            for (uint32_t k = 0; k < rmax; k++) ss.names[k] = "read" + std::to_string(k);
+            ss.d_sent = reinterpret_cast<uint32_t *>(dA + BL.a_sent);  // zero from the staging copy
+            Q.sbuf = static_cast<uint8_t *>(slot.dk.p);
+            Q.scap = std::min<uint64_t>(slot.dk.cap, slot.hpack.cap);
+            Q.scur = reinterpret_cast<unsigned long long *>(dA + a_ctr + 224);  // zero likewise
+            Q.sent = ss.d_sent;
+        }
         tr.t[2] = S.ms();
         RJ_TRY(hipEventRecord(slot.ev[0], s));
-        const uint32_t grid = std::min<uint32_t>(N, pc->grid);
+        // Device-to-host copies issued while a launch holds every workgroup
+        // slot start only as it drains (measured here). Streaming leaves
+        // eight slots free (measured on the 4,096-job workload: 4 free slots
+        // still hold the copies back, 8 let them run beside the launch for
+        // +1.2% device time, 16 cost +2.3%); MPOA_PERSIST_FREE overrides.
+        uint32_t free_slots = stream ? 8u : 0u;
+        if (const char *e = getenv("MPOA_PERSIST_FREE")) free_slots = uint32_t(atoi(e));
+        const uint32_t grid = std::min<uint32_t>(N, pc->grid > free_slots ? pc->grid - free_slots : 1u);
         if (launch_rj_persist(R, P, Q, grid, FillShape{4, 4}, s) != 0) {
             set_error("no persistent kernel for the launch shape");
             return MPOA_ERR_DEVICE;
         }
         RJ_TRY(hipGetLastError());
         RJ_TRY(hipEventRecord(slot.ev[1], s));
-        RJ_TRY(hipEventSynchronize(slot.ev[1]));
-        return finish_bucket(S, slot, jobs, BL, R, dA + a_poff, tr, 1);
+        if (stream) {
+            // Until the launch's end event completes (never on a flag alone).
+            int rc = MPOA_OK;
+            for (;;) {
+                const hipError_t q = hipEventQuery(slot.ev[1]);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) {
+                    set_error(std::string("HIP error: ") + hipGetErrorString(q) + " waiting for the persistent launch");
+                    rc = MPOA_ERR_DEVICE;
+                    break;
+                }
+                std::this_thread::sleep_for(std::chrono::milliseconds(2));
+                rc = stream_poll(S, slot, jobs, R, ss);
+                if (rc != MPOA_OK) break;
+            }
+            if (rc != MPOA_OK) {
+                (void)hipEventSynchronize(slot.ev[1]);
+                while (ss.pending.load() != 0) std::this_thread::sleep_for(std::chrono::microseconds(200));
+                return rc;
+            }
+        }
+        // Every streamed render has ended before ss goes away, whatever the
+        // outcome.
+        const hipError_t se = hipEventSynchronize(slot.ev[1]);
+        int frc = MPOA_OK;
+        if (se != hipSuccess) {
+            set_error(std::string("HIP error: ") + hipGetErrorString(se) + " waiting for the persistent launch");
+            frc = MPOA_ERR_DEVICE;
+        } else {
+            frc = finish_bucket(S, slot, jobs, BL, R, dA + a_poff, tr, 1, stream ? &ss : nullptr);
+        }
+        while (ss.pending.load() != 0) std::this_thread::sleep_for(std::chrono::microseconds(200));
+        return frc;
     }
 
     // The step, captured once and replayed.
@@ -818,13 +907,104 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
     return finish_bucket(S, slot, jobs, BL, R, dA + a_poff, tr, replays);
 }
 
-// After a bucket's device run: final job states, every graph packed into one
-// copy and rebuilt on the host, outputs of finished jobs, evicted jobs
-// queued for the host engine, statistics.
-int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const BucketLayout &BL,
-                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays) {
+// Rebuilds job k's graph from its packed bytes and renders its outputs (or
+// records its failure); a job that leaves the device keeps its graph in
+// graphs[k] for the host engine.
+void render_job(RShared &S, const std::vector<uint64_t> &jobs, uint32_t k, const RJob &J, const char *b,
+                const std::vector<std::string> &names, std::vector<std::unique_ptr<Graph>> &graphs,
+                std::vector<uint8_t> &leaves) {
     const Params &p = *S.p;
     const BatchInput &in = *S.in;
+    const uint64_t j = jobs[k];
+    const RPackLayout L = rj_pack_layout(J.nn, J.ne, J.nreads, J.npath);
+    auto g = std::make_unique<Graph>(p);
+    const uint32_t nfused = J.cur;
+    g->assign_packed(J.nn, reinterpret_cast<const uint8_t *>(b + L.base),
+                     reinterpret_cast<const uint32_t *>(b + L.parent), reinterpret_cast<const uint32_t *>(b + L.rank),
+                     reinterpret_cast<const uint32_t *>(b + L.cover),
+                     reinterpret_cast<const uint32_t *>(b + L.out_head),
+                     reinterpret_cast<const uint32_t *>(b + L.in_head), reinterpret_cast<const uint32_t *>(b + L.edges),
+                     reinterpret_cast<const uint32_t *>(b + L.path_off), reinterpret_cast<const uint32_t *>(b + L.paths),
+                     nfused, 0);
+    if (J.state == kRsDone) {
+        if (in.render) {
+            std::string out0 = ">consensus sequence\n" + g->consensus_bytes(p) + "\n";
+            std::vector<std::string> nm(names.begin(), names.begin() + J.nreads), cm(J.nreads);
+            std::string out1;
+            std::string e = g->msa(p, nm, cm, out1);
+            (*S.out->out0)[j] = std::move(out0);
+            (*S.out->out1)[j] = std::move(out1);
+            if (!e.empty()) {
+                (*S.out->status)[j] = MPOA_ERR_GRAPH;
+                (*S.out->err)[j] = e;
+            }
+        }
+        (*S.out->graphs)[j] = std::move(g);
+    } else if (J.state == kRsFailed) {
+        std::string msg;
+        (*S.out->status)[j] = failure_of(J, msg);
+        (*S.out->err)[j] = msg;
+        (*S.out->graphs)[j] = std::move(g);
+    } else {
+        leaves[k] = 1;
+        graphs[k] = std::move(g);
+    }
+}
+
+// One look at the published entries: the records and packed bytes of jobs
+// not yet taken come back on the slot's second stream (contiguous ranges in
+// one copy each), then each goes to a host worker.
+int stream_poll(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const RPools &R, StreamState &ss) {
+    const uint32_t N = uint32_t(jobs.size());
+    hipStream_t s2 = ss.s2;
+    uint32_t *hs = static_cast<uint32_t *>(slot.hsent.p);
+    RJ_TRY(hipMemcpyAsync(hs, ss.d_sent, 16ull * N, hipMemcpyDeviceToHost, s2));
+    RJ_TRY(hipStreamSynchronize(s2));
+    struct Item {
+        uint32_t k;
+        uint64_t off, bytes;
+    };
+    std::vector<Item> got;
+    for (uint32_t k = 0; k < N; k++) {
+        if (ss.taken[k] || hs[4ull * k] != 1u) continue;
+        const uint64_t off = uint64_t(hs[4ull * k + 1]) | uint64_t(hs[4ull * k + 2]) << 32;
+        got.push_back(Item{k, off, hs[4ull * k + 3]});
+    }
+    if (got.empty()) return MPOA_OK;
+    std::sort(got.begin(), got.end(), [](const Item &x, const Item &y) { return x.off < y.off; });
+    RJob *hj = static_cast<RJob *>(slot.hsjobs.p);
+    char *hp = static_cast<char *>(slot.hpack.p);
+    const char *dk = static_cast<const char *>(slot.dk.p);
+    for (size_t a = 0; a < got.size();) {
+        size_t b = a + 1;
+        uint64_t end = got[a].off + got[a].bytes;
+        while (b < got.size() && got[b].off == end) end += got[b++].bytes;
+        RJ_TRY(hipMemcpyAsync(hp + got[a].off, dk + got[a].off, end - got[a].off, hipMemcpyDeviceToHost, s2));
+        a = b;
+    }
+    for (const Item &it : got)
+        RJ_TRY(hipMemcpyAsync(hj + it.k, R.jobs + it.k, sizeof(RJob), hipMemcpyDeviceToHost, s2));
+    RJ_TRY(hipStreamSynchronize(s2));
+    for (const Item &it : got) {
+        ss.taken[it.k] = 1;
+        ss.end = std::max(ss.end, it.off + it.bytes);
+        ss.pending.fetch_add(1);
+        const RJob J = hj[it.k];
+        const char *b = hp + it.off;
+        const uint32_t k = it.k;
+        S.tasks->push([&S, &jobs, &ss, k, J, b] {
+            render_job(S, jobs, k, J, b, ss.names, ss.graphs, ss.leaves);
+            ss.pending.fetch_sub(1);
+        });
+    }
+    return MPOA_OK;
+}
+
+// After a bucket's device run: final job states, the graphs of jobs not yet
+// streamed packed into one copy and rebuilt on the host, outputs of
+// finished jobs, evicted jobs queued for the host engine, statistics.
+int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, const BucketLayout &BL,
+                  const RPools &R, char *d_poff, RShared::Trace &tr, uint64_t replays, StreamState *ss) {
     hipStream_t s = slot.ctx->stream;
     const uint32_t N = uint32_t(jobs.size());
     const uint32_t rmax = BL.rmax;
@@ -833,76 +1013,88 @@ int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, co
     RJ_TRY(hipEventElapsedTime(&dev_ms, slot.ev[0], slot.ev[1]));
     tr.t[3] = S.ms();
 
-    // Final states, then every job's graph packed into one copy.
+    StreamState local;
+    if (!ss) {
+        ss = &local;
+        ss->taken.assign(N, 0);
+        ss->graphs.resize(N);
+        ss->leaves.assign(N, 0);
+        ss->names.resize(rmax);
+        for (uint32_t k = 0; k < rmax; k++) ss->names[k] = "read" + std::to_string(k);
+    } else {
+        // What the workgroups published last.
+        const int rc = stream_poll(S, slot, jobs, R, *ss);
+        if (rc != MPOA_OK) {
+            while (ss->pending.load() != 0) std::this_thread::sleep_for(std::chrono::microseconds(200));
+            return rc;
+        }
+    }
+    auto drain = [&] {
+        while (ss->pending.load() != 0) std::this_thread::sleep_for(std::chrono::microseconds(200));
+    };
+
+    // Final states, then the remaining jobs' graphs packed into one copy,
+    // after the streamed bytes (or from the start once their renders ended,
+    // if they would not fit).
     RJ_TRY(slot.hjobs.ensure(sizeof(RJob) * N));
     RJob *hj = static_cast<RJob *>(slot.hjobs.p);
-    RJ_TRY(hipMemcpyAsync(hj, R.jobs, sizeof(RJob) * N, hipMemcpyDeviceToHost, s));
-    RJ_TRY(hipStreamSynchronize(s));
-    RJ_TRY(slot.hoff.ensure(8ull * N + 4096));
-    uint64_t *hoff = static_cast<uint64_t *>(slot.hoff.p);
-    uint64_t pbytes = 0;
-    for (uint32_t k = 0; k < N; k++) {
-        hoff[k] = pbytes;
-        pbytes += rj_pack_layout(hj[k].nn, hj[k].ne, hj[k].nreads, hj[k].npath).bytes;
+    const hipError_t je = hipMemcpyAsync(hj, R.jobs, sizeof(RJob) * N, hipMemcpyDeviceToHost, s);
+    const hipError_t js = je == hipSuccess ? hipStreamSynchronize(s) : je;
+    if (js != hipSuccess) {
+        drain();
+        RJ_TRY(js);
     }
-    RJ_TRY(slot.dk.ensure(std::max<uint64_t>(pbytes, 256)));
-    RJ_TRY(slot.hpack.ensure(std::max<uint64_t>(pbytes, 256)));
-    RJ_TRY(hipMemcpyAsync(d_poff, hoff, 8ull * N, hipMemcpyHostToDevice, s));
-    launch_rj_pack(R.jobs, nullptr, N, reinterpret_cast<const uint64_t *>(d_poff), static_cast<uint8_t *>(slot.dk.p), s);
-    RJ_TRY(hipGetLastError());
-    RJ_TRY(hipMemcpyAsync(slot.hpack.p, slot.dk.p, pbytes, hipMemcpyDeviceToHost, s));
-    RJ_TRY(hipStreamSynchronize(s));
+    std::vector<uint32_t> rest;
+    for (uint32_t k = 0; k < N; k++)
+        if (!ss->taken[k]) rest.push_back(k);
+    const uint32_t nr = uint32_t(rest.size());
+    uint64_t base = (ss->end + 255) / 256 * 256, pbytes = 0;
+    for (uint32_t k : rest) pbytes += rj_pack_layout(hj[k].nn, hj[k].ne, hj[k].nreads, hj[k].npath).bytes;
+    if (base + pbytes > std::min<uint64_t>(slot.dk.cap, slot.hpack.cap)) {
+        drain();
+        base = 0;
+    }
+    hipError_t pe = slot.hoff.ensure(8ull * N + 4096);
+    uint64_t *hoff = static_cast<uint64_t *>(slot.hoff.p);
+    if (pe == hipSuccess && nr) {
+        uint64_t at = base;
+        for (uint32_t a = 0; a < nr; a++) {
+            hoff[a] = at;
+            const uint32_t k = rest[a];
+            at += rj_pack_layout(hj[k].nn, hj[k].ne, hj[k].nreads, hj[k].npath).bytes;
+        }
+        if (base == 0) {
+            drain();
+            pe = slot.dk.ensure(std::max<uint64_t>(pbytes, 256));
+            if (pe == hipSuccess) pe = slot.hpack.ensure(std::max<uint64_t>(pbytes, 256));
+        }
+        // The selection goes where the published entries were (read above).
+        uint32_t *d_sel = reinterpret_cast<uint32_t *>(d_poff + 8ull * N);
+        d_sel = reinterpret_cast<uint32_t *>((reinterpret_cast<uintptr_t>(d_sel) + 255) / 256 * 256);
+        if (pe == hipSuccess) pe = hipMemcpyAsync(d_poff, hoff, 8ull * nr, hipMemcpyHostToDevice, s);
+        if (pe == hipSuccess) pe = hipMemcpyAsync(d_sel, rest.data(), 4ull * nr, hipMemcpyHostToDevice, s);
+        if (pe == hipSuccess) {
+            launch_rj_pack(R.jobs, d_sel, nr, reinterpret_cast<const uint64_t *>(d_poff), static_cast<uint8_t *>(slot.dk.p), s);
+            pe = hipGetLastError();
+        }
+        if (pe == hipSuccess)
+            pe = hipMemcpyAsync(static_cast<char *>(slot.hpack.p) + base, static_cast<char *>(slot.dk.p) + base, pbytes,
+                                hipMemcpyDeviceToHost, s);
+        if (pe == hipSuccess) pe = hipStreamSynchronize(s);
+    }
+    if (pe != hipSuccess) {
+        drain();
+        RJ_TRY(pe);
+    }
 
     tr.t[4] = S.ms();
     // Graphs, outputs and the jobs that continue on the host engine.
     const char *pk = static_cast<const char *>(slot.hpack.p);
-    std::vector<std::unique_ptr<Graph>> graphs(N);
-    std::vector<std::string> names = [&] {
-        std::vector<std::string> v(rmax);
-        for (uint32_t k = 0; k < rmax; k++) v[k] = "read" + std::to_string(k);
-        return v;
-    }();
-    std::vector<uint8_t> leaves(N, 0);
-    par_for(*S.tasks, N, [&](size_t k) {
-        const RJob &J = hj[k];
-        const uint64_t j = jobs[k];
-        const RPackLayout L = rj_pack_layout(J.nn, J.ne, J.nreads, J.npath);
-        const char *b = pk + hoff[k];
-        auto g = std::make_unique<Graph>(p);
-        const uint32_t nfused = J.cur;
-        g->assign_packed(J.nn, reinterpret_cast<const uint8_t *>(b + L.base),
-                         reinterpret_cast<const uint32_t *>(b + L.parent),
-                         reinterpret_cast<const uint32_t *>(b + L.rank),
-                         reinterpret_cast<const uint32_t *>(b + L.cover),
-                         reinterpret_cast<const uint32_t *>(b + L.out_head),
-                         reinterpret_cast<const uint32_t *>(b + L.in_head),
-                         reinterpret_cast<const uint32_t *>(b + L.edges),
-                         reinterpret_cast<const uint32_t *>(b + L.path_off),
-                         reinterpret_cast<const uint32_t *>(b + L.paths), nfused, 0);
-        if (J.state == kRsDone) {
-            if (in.render) {
-                std::string out0 = ">consensus sequence\n" + g->consensus_bytes(p) + "\n";
-                std::vector<std::string> nm(names.begin(), names.begin() + J.nreads), cm(J.nreads);
-                std::string out1;
-                std::string e = g->msa(p, nm, cm, out1);
-                (*S.out->out0)[j] = std::move(out0);
-                (*S.out->out1)[j] = std::move(out1);
-                if (!e.empty()) {
-                    (*S.out->status)[j] = MPOA_ERR_GRAPH;
-                    (*S.out->err)[j] = e;
-                }
-            }
-            (*S.out->graphs)[j] = std::move(g);
-        } else if (J.state == kRsFailed) {
-            std::string msg;
-            (*S.out->status)[j] = failure_of(J, msg);
-            (*S.out->err)[j] = msg;
-            (*S.out->graphs)[j] = std::move(g);
-        } else {
-            leaves[k] = 1;
-            graphs[k] = std::move(g);
-        }
+    par_for(*S.tasks, nr, [&](size_t a) {
+        const uint32_t k = rest[a];
+        render_job(S, jobs, k, hj[k], pk + hoff[a], ss->names, ss->graphs, ss->leaves);
     });
+    drain();
     tr.t[5] = S.ms();
     tr.njobs = N;
     tr.rmax = rmax;
@@ -924,9 +1116,9 @@ int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, co
         st.retries += J.retries;
         st.arena_retries += J.arena_retries;
         st.deferred += J.deferred;
-        if (leaves[k]) {
+        if (ss->leaves[k]) {
             st.evicted += 1;
-            S.evicted.emplace_back(jobs[k], J.cur, std::move(graphs[k]));
+            S.evicted.emplace_back(jobs[k], J.cur, std::move(ss->graphs[k]));
         } else {
             st.resident_done += 1;
         }
@@ -1093,6 +1285,7 @@ int run_resident(std::vector<Ctx *> &slots, TaskQueue &tasks, ResidentCache &cac
         }
         rs.push_back(&x);
     }
+    if (persist && rs.size() > 1) S.copy_stream = rs[1]->ctx->stream;
     const auto t0 = std::chrono::steady_clock::now();
     S.t0 = t0;
     std::vector<std::thread> th;

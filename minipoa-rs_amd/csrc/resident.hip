@@ -29,22 +29,7 @@ __global__ __launch_bounds__(64) void rj_pack_kernel(const RJob *jobs, const uin
                                                      const uint64_t *off, uint8_t *out) {
     const uint32_t k = blockIdx.x;
     if (k >= njobs) return;
-    const RJob &J = jobs[sel ? sel[k] : k];
-    const uint32_t nn = J.nn, ne = J.ne, nr = J.nreads, np = J.npath;
-    const RPackLayout L = rj_pack_layout(nn, ne, nr, np);
-    uint8_t *o = out + off[k];
-    for (uint32_t i = threadIdx.x; i < nn; i += 64) {
-        o[L.base + i] = J.base[i];
-        reinterpret_cast<uint32_t *>(o + L.parent)[i] = J.parent[i];
-        reinterpret_cast<uint32_t *>(o + L.rank)[i] = J.rank[i];
-        reinterpret_cast<uint32_t *>(o + L.cover)[i] = J.cover[i];
-        reinterpret_cast<uint32_t *>(o + L.out_head)[i] = J.out_head[i];
-        reinterpret_cast<uint32_t *>(o + L.in_head)[i] = J.in_head[i];
-    }
-    for (uint32_t i = threadIdx.x; i < ne; i += 64) reinterpret_cast<REdge *>(o + L.edges)[i] = J.edges[i];
-    for (uint32_t i = threadIdx.x; i <= nr; i += 64)
-        reinterpret_cast<uint32_t *>(o + L.path_off)[i] = i <= J.cur ? J.path_off[i] : 0u;
-    for (uint32_t i = threadIdx.x; i < np; i += 64) reinterpret_cast<uint32_t *>(o + L.paths)[i] = J.paths[i];
+    rj_pack_job(jobs[sel ? sel[k] : k], out + off[k], threadIdx.x, 64);
 }
 
 }  // namespace

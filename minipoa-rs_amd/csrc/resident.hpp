@@ -127,6 +127,14 @@ struct RPersist {
     uint64_t slot_bytes;  // bytes per region (the largest DP arena a job may use)
     uint32_t step_lds;    // LDS bytes of the step (the rest of the kernel's LDS aliases it)
     uint32_t prio;        // bit 0: the step's wave, bit 1: the traceback's, at raised issue priority
+    // Streaming (sbuf non-null): a workgroup packs each job it finishes into
+    // sbuf at an offset taken from *scur (while it fits in scap bytes) and
+    // then publishes sent[4k..4k+3] = {1, offset low, offset high, bytes},
+    // so the host can copy and render finished jobs while the launch runs.
+    uint8_t *sbuf;
+    unsigned long long *scur;
+    uint64_t scap;
+    uint32_t *sent;
 };
 
 // ---------------------------------------------------------------------------

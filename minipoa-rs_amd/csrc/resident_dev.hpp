@@ -61,6 +61,25 @@ __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t k) {
 }
 __device__ __forceinline__ int32_t rl(int32_t v, uint32_t k) { return __builtin_amdgcn_readlane(v, int32_t(k)); }
 
+// Job J's graph in the packed layout (rj_pack_layout) at o, by threads t0,
+// t0 + nt, ... of the caller.
+__device__ void rj_pack_job(const RJob &J, uint8_t *o, uint32_t t0, uint32_t nt) {
+    const uint32_t nn = J.nn, ne = J.ne, nr = J.nreads, np = J.npath;
+    const RPackLayout L = rj_pack_layout(nn, ne, nr, np);
+    for (uint32_t i = t0; i < nn; i += nt) {
+        o[L.base + i] = J.base[i];
+        reinterpret_cast<uint32_t *>(o + L.parent)[i] = J.parent[i];
+        reinterpret_cast<uint32_t *>(o + L.rank)[i] = J.rank[i];
+        reinterpret_cast<uint32_t *>(o + L.cover)[i] = J.cover[i];
+        reinterpret_cast<uint32_t *>(o + L.out_head)[i] = J.out_head[i];
+        reinterpret_cast<uint32_t *>(o + L.in_head)[i] = J.in_head[i];
+    }
+    for (uint32_t i = t0; i < ne; i += nt) reinterpret_cast<REdge *>(o + L.edges)[i] = J.edges[i];
+    for (uint32_t i = t0; i <= nr; i += nt)
+        reinterpret_cast<uint32_t *>(o + L.path_off)[i] = i <= J.cur ? J.path_off[i] : 0u;
+    for (uint32_t i = t0; i < np; i += nt) reinterpret_cast<uint32_t *>(o + L.paths)[i] = J.paths[i];
+}
+
 // The serial head and tail passes on one wave, chosen over the generic
 // forms of resident.hpp by overload resolution. Rows go in blocks of 64:
 // lane k first loads row B + k's list offsets and up to kRegEntries of its
