@@ -788,6 +788,9 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
             ss.d_sent = reinterpret_cast<uint32_t *>(dA + BL.a_sent);  // zero from the staging copy
             Q.sbuf = static_cast<uint8_t *>(slot.dk.p);
             Q.scap = std::min<uint64_t>(slot.dk.cap, slot.hpack.cap);
+            // Test knob: a smaller streaming share, so that later jobs are
+            // packed after the launch instead.
+            if (const char *e = getenv("MPOA_PERSIST_SCAP")) Q.scap = std::min<uint64_t>(Q.scap, strtoull(e, nullptr, 10));
             Q.scur = reinterpret_cast<unsigned long long *>(dA + a_ctr + 224);  // zero likewise
             Q.sent = ss.d_sent;
         }

@@ -188,6 +188,30 @@ def test_resident_modes_agree(ctx, env, mode):
     assert sa["buckets"] == 1 and sc["buckets"] > 1
 
 
+@pytest.mark.parametrize("scap", ["0", "2000000", "-"])
+def test_resident_streamed_and_after_launch_jobs_agree(ctx, oracle, env, mode, scap):
+    """Persistent launch: finished jobs stream back while it runs; with the
+    streaming share capped (none, about 2 MB: some jobs stream, the rest are
+    packed after the launch) or streaming off, every job's bytes equal the
+    oracle's and the uncapped run's, and the statistics agree."""
+    if mode == "captured":
+        pytest.skip("streaming belongs to the persistent launch")
+    jo, ro, b = synth.jobs(seed=20265, njobs=48, lmin=200, lmax=1200, rmin=3, rmax=12)
+    params = Params(adaptive=True)
+    a = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    if scap == "-":
+        env("MPOA_PERSIST_STREAM", "0")
+    else:
+        env("MPOA_PERSIST_SCAP", scap)
+    c = mh.batch_run(ctx, params, jo, ro, b, resident=True)
+    for j in range(48):
+        assert (a.output(j, 0), a.output(j, 1), a.output(j, 2)) == (c.output(j, 0), c.output(j, 1), c.output(j, 2)), j
+    sa, sc = a.stats(), c.stats()
+    for k in ("cells", "pred_cells", "dp_calls", "resident_jobs", "evicted_jobs"):
+        assert sa[k] == sc[k], k
+    check_against_oracle(c, oracle, params, jo, ro, b)
+
+
 def test_resident_wide_rows_and_long_reads_match_oracle(ctx, oracle):
     """Templates of 3,000-12,500 bases: rows wider than one 4x4 stripe (1,024
     columns) and query codes of up to 12.5 KB in LDS, which shrink the fill's
