@@ -969,7 +969,10 @@ int stream_poll(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, cons
     };
     std::vector<Item> got;
     for (uint32_t k = 0; k < N; k++) {
-        if (ss.taken[k] || hs[4ull * k] != 1u) continue;
+        // An entry whose flag arrived without its fields (a copy that read
+        // the entry between the kernel's two releases) waits for the next
+        // look; every packed graph has a nonzero size.
+        if (ss.taken[k] || hs[4ull * k] != 1u || hs[4ull * k + 3] == 0u) continue;
         const uint64_t off = uint64_t(hs[4ull * k + 1]) | uint64_t(hs[4ull * k + 2]) << 32;
         got.push_back(Item{k, off, hs[4ull * k + 3]});
     }
