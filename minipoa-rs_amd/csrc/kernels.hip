@@ -1570,20 +1570,23 @@ __device__ __forceinline__ void tbv_job(const DevPools &P, uint32_t jid, char *l
             }
         }
         sync_lds<1>();
-        // M, D and I: lane k holds column cw0 + k; 16 rows per batch, every
-        // load of a batch in flight before the first is used.
-        const uint32_t c = cw0 + uint32_t(lane);
-        const bool col_ok = uint32_t(lane) < kVCols && c <= j0;
+        // M, D and I: lane k holds column cw0 + (k & 31) of the even rows
+        // (k < 32) or the odd rows (k >= 32); 32 rows per batch, every load
+        // of a batch in flight before the first is used.
+        const uint32_t hl = uint32_t(lane) >> 5, cl = uint32_t(lane) & 31u;
+        const uint32_t c = cw0 + cl;
+        const bool col_ok = cl < kVCols && c <= j0;
         const gptr<const T> A = glb(static_cast<const T *>(arena));
         constexpr uint32_t kB = 16;
-        for (uint32_t r0 = 0; r0 < kTbRows && r0 <= i0; r0 += kB) {
+        for (uint32_t r0 = 0; r0 < kTbRows && r0 <= i0; r0 += 2 * kB) {
             T vm[kB], vd[kB], vi[kB];
             bool ok[kB];
 #pragma unroll
             for (uint32_t u = 0; u < kB; u++) {
-                const uint32_t r = r0 + u;
-                const WRow g = wrow[r <= i0 ? r : 0u];
-                ok[u] = col_ok && r <= i0 && c >= g.start && c - g.start < g.len;
+                const uint32_t r = r0 + 2 * u + hl;
+                const bool rin = r < kTbRows && r <= i0;
+                const WRow g = wrow[rin ? r : 0u];
+                ok[u] = col_ok && rin && c >= g.start && c - g.start < g.len;
                 const uint64_t k = ok[u] ? g.off + (c - g.start) : 0ull;
                 const uint64_t l = ok[u] ? uint64_t(g.len) : 0ull;
                 vm[u] = A[k];
@@ -1593,7 +1596,7 @@ __device__ __forceinline__ void tbv_job(const DevPools &P, uint32_t jid, char *l
 #pragma unroll
             for (uint32_t u = 0; u < kB; u++) {
                 if (ok[u]) {
-                    const uint32_t w = (r0 + u) * kVCols + uint32_t(lane);
+                    const uint32_t w = (r0 + 2 * u + hl) * kVCols + cl;
                     wm[w] = vm[u];
                     wd[w] = vd[u];
                     wi[w] = vi[u];
