@@ -1862,7 +1862,18 @@ void launch_traceback(const DevPools &P, uint32_t njobs, bool i64, bool rec, hip
 // kernel completes whatever number of them is resident at once.
 template <int C, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void rj_persist_kernel(
-    RPools R, DevPools P, RPersist Q) {
+    RPools R, DevPools Pin, RPersist Q) {
+    // The launch's pools as the fill and traceback take them; the pointers
+    // both structures hold (the host sets them equal) come from R, so the
+    // kernel keeps one copy of each.
+    DevPools P = Pin;
+    P.jobs = R.dj;
+    P.res = R.res;
+    P.rows = R.rows;
+    P.aux = R.aux;
+    P.preds = R.preds;
+    P.seq = R.seq;
+    P.tb = R.tb;
     extern __shared__ __align__(16) char pl_lds[];
     __shared__ uint32_t xjob;
     __shared__ unsigned long long xsoff;
