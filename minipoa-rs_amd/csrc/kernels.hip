@@ -1874,6 +1874,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     P.preds = R.preds;
     P.seq = R.seq;
     P.tb = R.tb;
+    P.prio = 0;  // the persistent launch sets priorities per phase (Q.prio)
     extern __shared__ __align__(16) char pl_lds[];
     __shared__ uint32_t xjob;
     __shared__ unsigned long long xsoff;
