@@ -913,9 +913,9 @@ int run_bucket(RShared &S, RSlot &slot, size_t si, const std::vector<uint64_t> &
 // Rebuilds job k's graph from its packed bytes and renders its outputs (or
 // records its failure); a job that leaves the device keeps its graph in
 // graphs[k] for the host engine.
-void render_job(RShared &S, const std::vector<uint64_t> &jobs, uint32_t k, const RJob &J, const char *b,
-                const std::vector<std::string> &names, std::vector<std::unique_ptr<Graph>> &graphs,
-                std::vector<uint8_t> &leaves) {
+void render_job_body(RShared &S, const std::vector<uint64_t> &jobs, uint32_t k, const RJob &J, const char *b,
+                     const std::vector<std::string> &names, std::vector<std::unique_ptr<Graph>> &graphs,
+                     std::vector<uint8_t> &leaves) {
     const Params &p = *S.p;
     const BatchInput &in = *S.in;
     const uint64_t j = jobs[k];
@@ -951,6 +951,26 @@ void render_job(RShared &S, const std::vector<uint64_t> &jobs, uint32_t k, const
     } else {
         leaves[k] = 1;
         graphs[k] = std::move(g);
+    }
+}
+
+// render_job_body that never throws (it runs on host workers, streamed
+// jobs while the launch is still running): a job whose rendering fails ends
+// with an error instead.
+void render_job(RShared &S, const std::vector<uint64_t> &jobs, uint32_t k, const RJob &J, const char *b,
+                const std::vector<std::string> &names, std::vector<std::unique_ptr<Graph>> &graphs,
+                std::vector<uint8_t> &leaves) {
+    const uint64_t j = jobs[k];
+    try {
+        render_job_body(S, jobs, k, J, b, names, graphs, leaves);
+    } catch (const std::bad_alloc &) {
+        leaves[k] = 0;
+        (*S.out->status)[j] = MPOA_ERR_ALLOC;
+        (*S.out->err)[j] = "host memory allocation failed";
+    } catch (const std::exception &e) {
+        leaves[k] = 0;
+        (*S.out->status)[j] = MPOA_ERR_INVALID_ARGUMENT;
+        (*S.out->err)[j] = std::string("unexpected error: ") + e.what();
     }
 }
 
