@@ -1095,8 +1095,7 @@ int finish_bucket(RShared &S, RSlot &slot, const std::vector<uint64_t> &jobs, co
             if (pe == hipSuccess) pe = slot.hpack.ensure(std::max<uint64_t>(pbytes, 256));
         }
         // The selection goes where the published entries were (read above).
-        uint32_t *d_sel = reinterpret_cast<uint32_t *>(d_poff + 8ull * N);
-        d_sel = reinterpret_cast<uint32_t *>((reinterpret_cast<uintptr_t>(d_sel) + 255) / 256 * 256);
+        uint32_t *d_sel = reinterpret_cast<uint32_t *>(d_poff + (BL.a_sent - BL.a_poff));
         if (pe == hipSuccess) pe = hipMemcpyAsync(d_poff, hoff, 8ull * nr, hipMemcpyHostToDevice, s);
         if (pe == hipSuccess) pe = hipMemcpyAsync(d_sel, rest.data(), 4ull * nr, hipMemcpyHostToDevice, s);
         if (pe == hipSuccess) {
